@@ -1,0 +1,486 @@
+"""Python face of libmsc (MI355X-native Multistep SCC / WCC).
+
+Thin ctypes layer over the C ABI in ``include/msc.h``; the work happens in
+``libmsc.so`` (gfx950 HIP kernels + C++ host runtime).  Names mirror the reference's
+functions (HPCGraphAnalysis/Connectivity ``multistep/``):
+
+* :func:`read_edge` / :func:`create_csr`  -- read_edge + create_csr
+  (multistep/scc_main.cpp:94-191), returning a :class:`Graph`;
+* :func:`run_scc`  -- run_scc (multistep/scc_main.cpp:210);
+* :func:`run_cc`   -- run_cc (multistep/cc_main.cpp:184) on the digraph's mirror;
+* :func:`scc_verify` / :func:`cc_verify` -- the reference's summary counts
+  (multistep/scc_verify.cpp:47-77, cc_verify.cpp:47-74).
+
+There is no CPU fallback: every entry point raises :class:`MscError` when the library
+is missing or no gfx950 device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = [
+    "MscError", "Graph", "DeviceGraph", "Context", "Stats", "LevelRecord",
+    "read_edge", "create_csr", "gen_rmat", "gen_grid", "gen_planted",
+    "run_scc", "run_cc", "scc_pivot", "scc_verify", "cc_verify", "verify_line",
+    "canonicalize", "write_labels", "device_count", "lib_path",
+    "POLICY_PLAIN", "POLICY_LEARNED", "LABELS_CANONICAL", "LABELS_RAW",
+    "K_TRIM", "K_LEVEL", "K_COMMIT", "K_COLOR_PROP", "K_COLOR_BW", "K_ROOTS", "K_OTHER",
+    "KERNEL_NAMES",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+OK, E_INVALID, E_OOM, E_DEVICE, E_CAPACITY, E_IO = 0, -1, -2, -3, -4, -5
+_ERRNAME = {E_INVALID: "MSC_E_INVALID", E_OOM: "MSC_E_OOM", E_DEVICE: "MSC_E_DEVICE",
+            E_CAPACITY: "MSC_E_CAPACITY", E_IO: "MSC_E_IO"}
+HOST, DEVICE = 0, 1
+POLICY_PLAIN, POLICY_LEARNED = 0, 1
+LABELS_CANONICAL, LABELS_RAW = 0, 1
+K_TRIM, K_LEVEL, K_COMMIT, K_COLOR_PROP, K_COLOR_BW, K_ROOTS, K_OTHER = range(7)
+KERNEL_NAMES = ["trim", "level", "commit", "color_prop", "color_bw", "roots", "other"]
+MLP_IN, MLP_H, MLP_OUT = 12, 32, 3
+MLP_PARAMS = MLP_IN * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H * MLP_OUT + MLP_OUT
+MLP_BLOB = MLP_PARAMS + 2 * MLP_IN
+
+
+class MscError(RuntimeError):
+    """A negative MSC_E_* status from libmsc."""
+
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {_ERRNAME.get(code, code)}")
+        self.code = code
+
+
+# ---- C structs (layout of include/msc.h) ----------------------------------------------
+class _Csr(C.Structure):
+    _fields_ = [("n", C.c_int32), ("m", C.c_int64), ("out_off", C.c_void_p), ("out_adj", C.c_void_p),
+                ("in_off", C.c_void_p), ("in_adj", C.c_void_p), ("where", C.c_int32)]
+
+
+class _HostGraph(C.Structure):
+    _fields_ = [("n", C.c_int32), ("m", C.c_int64), ("out_off", C.POINTER(C.c_uint32)),
+                ("out_adj", C.POINTER(C.c_int32)), ("in_off", C.POINTER(C.c_uint32)),
+                ("in_adj", C.POINTER(C.c_int32)), ("max_deg_vert", C.c_int32), ("avg_degree", C.c_double)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("beta", C.c_int32), ("trim_level", C.c_int32),
+                ("policy", C.c_int32), ("label_mode", C.c_int32), ("labels_where", C.c_int32),
+                ("device", C.c_int32), ("validate", C.c_int32), ("timing", C.c_int32),
+                ("record", C.c_int32), ("ctx", C.c_void_p)]
+
+
+class _KStat(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("wall_ms", C.c_double), ("pivot", C.c_int32), ("avg_degree", C.c_double),
+                ("num_trim", C.c_int32), ("num_fwbw", C.c_int32), ("num_color", C.c_int32),
+                ("color_rounds", C.c_int32), ("color_iters", C.c_int32),
+                ("fw_levels", C.c_int32), ("bw_levels", C.c_int32), ("bu_levels", C.c_int32),
+                ("cbw_levels", C.c_int32), ("host_syncs", C.c_int32), ("bytes", C.c_double),
+                ("k", _KStat * 7)]
+
+
+class _Verify(C.Structure):
+    _fields_ = [("num_components", C.c_int32), ("nontrivial", C.c_int32), ("max_size", C.c_int32),
+                ("unassigned", C.c_int32)]
+
+
+class _LevelRec(C.Structure):
+    _fields_ = [("phase", C.c_int32), ("level", C.c_int32), ("bottom_up", C.c_int32),
+                ("next_size", C.c_int32), ("q_size", C.c_uint32), ("q_edges", C.c_uint32),
+                ("next_edges", C.c_uint32), ("examined", C.c_uint32), ("inspected", C.c_uint64),
+                ("found", C.c_int64), ("n_total", C.c_int32), ("num_valid", C.c_int32),
+                ("action", C.c_int32), ("pad", C.c_int32)]
+
+
+_lib_handle = None
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libmsc.so")
+
+
+def _lib():
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    path = lib_path()
+    if not os.path.exists(path):
+        raise MscError(E_DEVICE, f"{path} is not built (run __graft_entry__.build())")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "msc_opts_default": (None, [C.POINTER(_Opts)]),
+        "msc_ctx_create": (C.c_int, [C.c_int32, P, C.POINTER(P)]),
+        "msc_ctx_destroy": (C.c_int, [P]),
+        "msc_ctx_levels": (C.c_int, [P, C.POINTER(_LevelRec), C.c_int32, C.POINTER(C.c_int32)]),
+        "msc_scc_run": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
+                                  C.POINTER(_Stats)]),
+        "msc_cc_run": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
+                                 C.POINTER(_Stats)]),
+        "msc_scc_pivot": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_double)]),
+        "msc_scc_verify": (C.c_int, [C.c_int32, P, C.POINTER(_Verify)]),
+        "msc_cc_verify": (C.c_int, [C.c_int32, P, C.POINTER(_Verify)]),
+        "msc_verify_format": (C.c_int, [C.POINTER(_Verify), C.c_int32, C.c_char_p, C.c_int32]),
+        "msc_canonicalize": (C.c_int, [C.c_int32, P]),
+        "msc_graph_read_edge_list": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(_HostGraph))]),
+        "msc_graph_from_edges": (C.c_int, [C.c_int32, C.c_int64, P, P, C.c_int32,
+                                           C.POINTER(C.POINTER(_HostGraph))]),
+        "msc_write_labels": (C.c_int, [C.c_char_p, C.c_int32, P]),
+        "msc_graph_free": (None, [C.POINTER(_HostGraph)]),
+        "msc_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                   C.c_uint64, C.c_int32, C.POINTER(C.POINTER(_HostGraph))]),
+        "msc_gen_grid": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                   C.c_uint64, C.POINTER(C.POINTER(_HostGraph))]),
+        "msc_gen_planted": (C.c_int, [C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_int64,
+                                      C.c_uint64, C.POINTER(C.POINTER(_HostGraph)), P]),
+        "msc_csr_to_device": (C.c_int, [P, C.POINTER(_Csr), C.POINTER(_Csr)]),
+        "msc_csr_device_free": (C.c_int, [P, C.POINTER(_Csr)]),
+        "msc_version": (C.c_char_p, []),
+        "msc_device_count": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib_handle = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != OK:
+        raise MscError(rc, what)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    """Visible gfx950 devices (0 when none)."""
+    return int(_lib().msc_device_count())
+
+
+# ---- graphs -----------------------------------------------------------------------------
+@dataclass
+class Graph:
+    """CSR + transposed CSR in host memory (the reference's ``graph`` struct,
+    multistep/scc_main.cpp:75-86) plus create_csr's pivot and average degree."""
+
+    n: int
+    out_off: np.ndarray  # uint32[n+1]
+    out_adj: np.ndarray  # int32[m]
+    in_off: np.ndarray   # uint32[n+1]
+    in_adj: np.ndarray   # int32[m]
+    max_deg_vert: int = -1
+    avg_degree: float = 0.0
+    _keep: object = field(default=None, repr=False)
+
+    @property
+    def m(self) -> int:
+        return int(self.out_adj.shape[0])
+
+    def csr(self) -> _Csr:
+        return _Csr(self.n, self.m, _ptr(self.out_off), _ptr(self.out_adj), _ptr(self.in_off),
+                    _ptr(self.in_adj), HOST)
+
+    def edges(self) -> tuple[np.ndarray, np.ndarray]:
+        """(src, dst) arrays in CSR order."""
+        deg = np.diff(self.out_off.astype(np.int64))
+        return np.repeat(np.arange(self.n, dtype=np.int32), deg), self.out_adj.copy()
+
+
+class _Owned:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        try:
+            _lib().msc_graph_free(self.ptr)
+        except Exception:  # interpreter shutdown
+            pass
+
+
+def _wrap(hg_ptr) -> Graph:
+    hg = hg_ptr.contents
+    n, m = int(hg.n), int(hg.m)
+    owner = _Owned(hg_ptr)
+
+    def arr(p, count, dt):
+        if count == 0:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(p, shape=(count,)).view(dt)
+
+    return Graph(n=n, out_off=arr(hg.out_off, n + 1, np.uint32), out_adj=arr(hg.out_adj, m, np.int32),
+                 in_off=arr(hg.in_off, n + 1, np.uint32), in_adj=arr(hg.in_adj, m, np.int32),
+                 max_deg_vert=int(hg.max_deg_vert), avg_degree=float(hg.avg_degree), _keep=owner)
+
+
+BUILD_DROP_SELF_LOOPS, BUILD_DEDUPE = 1, 2
+
+
+def create_csr(n: int, srcs, dsts, dedupe: bool = False, drop_self_loops: bool = False) -> Graph:
+    """create_csr (multistep/scc_main.cpp:127-191).  Without flags, adjacency keeps the input
+    order and duplicates, exactly like the reference."""
+    s = np.ascontiguousarray(srcs, dtype=np.int32)
+    d = np.ascontiguousarray(dsts, dtype=np.int32)
+    if s.shape != d.shape:
+        raise ValueError("srcs and dsts differ in length")
+    flags = (BUILD_DEDUPE if dedupe else 0) | (BUILD_DROP_SELF_LOOPS if drop_self_loops else 0)
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_graph_from_edges(int(n), int(s.shape[0]), _ptr(s), _ptr(d), flags, C.byref(out)),
+           "create_csr")
+    return _wrap(out)
+
+
+def read_edge(path: str) -> Graph:
+    """read_edge + create_csr (multistep/scc_main.cpp:94-191): "n m" then m lines "u v"."""
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_graph_read_edge_list(os.fsencode(path), C.byref(out)), f"read_edge({path})")
+    return _wrap(out)
+
+
+def gen_rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1, permute: bool = True) -> Graph:
+    """Directed R-MAT, self-loops and duplicates removed, ids permuted from the seed."""
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_gen_rmat(scale, edge_factor, a, b, c, seed, 1 if permute else 0, C.byref(out)),
+           "gen_rmat")
+    return _wrap(out)
+
+
+def gen_grid(rows: int = 4096, cols: int = 4096, p_one: float = 0.5, p_both: float = 0.1,
+             arc_frac: float = 0.005, seed: int = 1) -> Graph:
+    """rows x cols grid digraph: each grid edge kept in one random direction with p_one, in
+    both with p_both, plus round(arc_frac * n) uniform random arcs."""
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_gen_grid(rows, cols, p_one, p_both, arc_frac, seed, C.byref(out)), "gen_grid")
+    return _wrap(out)
+
+
+def gen_planted(n: int, zipf_s: float = 1.5, max_size: int = 1 << 20, extra_per_vertex: float = 1.0,
+                dag_arcs: int | None = None, seed: int = 1) -> tuple[Graph, np.ndarray]:
+    """Planted SCCs (Zipf sizes) joined by random DAG arcs; returns (graph, planted partition)."""
+    if dag_arcs is None:
+        dag_arcs = 4 * n
+    comp = np.empty(n, dtype=np.int32)
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_gen_planted(n, zipf_s, max_size, extra_per_vertex, dag_arcs, seed, C.byref(out),
+                                  _ptr(comp)), "gen_planted")
+    return _wrap(out), comp
+
+
+# ---- contexts and device graphs ------------------------------------------------------------
+class Context:
+    """A device + HIP stream + reusable workspace (msc_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(_lib().msc_ctx_create(device, None, C.byref(h)), f"msc_ctx_create(device={device})")
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            _lib().msc_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def levels(self) -> list["LevelRecord"]:
+        """Per-level records of the last run made with ``record=True``."""
+        count = C.c_int32()
+        _check(_lib().msc_ctx_levels(self.handle, None, 0, C.byref(count)), "msc_ctx_levels")
+        k = count.value
+        if k == 0:
+            return []
+        buf = (_LevelRec * k)()
+        _check(_lib().msc_ctx_levels(self.handle, buf, k, C.byref(count)), "msc_ctx_levels")
+        return [LevelRecord(**{f: getattr(r, f) for f, _ in _LevelRec._fields_ if f != "pad"}) for r in buf]
+
+
+class DeviceGraph:
+    """A Graph copied into device memory (msc_csr_to_device); runs use it in place."""
+
+    def __init__(self, ctx: Context, g: Graph):
+        self.ctx = ctx
+        self.n, self.m = g.n, g.m
+        self.max_deg_vert, self.avg_degree = g.max_deg_vert, g.avg_degree
+        self._csr = _Csr()
+        host = g.csr()
+        _check(_lib().msc_csr_to_device(ctx.handle, C.byref(host), C.byref(self._csr)), "msc_csr_to_device")
+
+    def csr(self) -> _Csr:
+        return self._csr
+
+    def free(self) -> None:
+        if self._csr.where == DEVICE and self.ctx.handle:
+            _lib().msc_csr_device_free(self.ctx.handle, C.byref(self._csr))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class LevelRecord:
+    phase: int
+    level: int
+    bottom_up: int
+    next_size: int
+    q_size: int
+    q_edges: int
+    next_edges: int
+    examined: int
+    inspected: int
+    found: int
+    n_total: int
+    num_valid: int
+    action: int
+
+
+@dataclass
+class Stats:
+    wall_ms: float
+    pivot: int
+    avg_degree: float
+    num_trim: int
+    num_fwbw: int
+    num_color: int
+    color_rounds: int
+    color_iters: int
+    fw_levels: int
+    bw_levels: int
+    bu_levels: int
+    cbw_levels: int
+    host_syncs: int
+    bytes: float
+    kernels: dict
+
+    @classmethod
+    def _from(cls, s: _Stats) -> "Stats":
+        ks = {KERNEL_NAMES[i]: {"launches": s.k[i].launches, "ms": s.k[i].ms, "bytes": s.k[i].bytes}
+              for i in range(7)}
+        vals = {f: getattr(s, f) for f, _ in _Stats._fields_ if f != "k"}
+        return cls(kernels=ks, **vals)
+
+
+def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alpha=15.0, beta=25,
+          device=0) -> _Opts:
+    o = _Opts()
+    _lib().msc_opts_default(C.byref(o))
+    o.alpha, o.beta = alpha, beta
+    o.policy = policy
+    o.label_mode = labels
+    o.labels_where = labels_where
+    o.timing = 1 if timing else 0
+    o.record = 1 if record else 0
+    o.validate = 1 if validate else 0
+    o.ctx = ctx.handle if ctx is not None else None
+    o.device = ctx.device if ctx is not None else device
+    return o
+
+
+def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
+         out):
+    n = g.n
+    if out is None:
+        out = np.empty(n, dtype=np.int32)
+    elif out.dtype != np.int32 or out.shape != (n,) or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous int32 array of length n")
+    o = _opts(policy, labels, timing, record, validate, ctx, alpha=alpha, beta=beta)
+    st = _Stats()
+    csr = g.csr()
+    piv = -1 if max_deg_vert is None else int(max_deg_vert)
+    avg = -1.0 if avg_degree is None else float(avg_degree)
+    rc = getattr(_lib(), fn_name)(C.byref(csr), piv, avg, _ptr(out) if n else None, C.byref(o), C.byref(st))
+    _check(rc, fn_name)
+    return out, Stats._from(st)
+
+
+def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, *,
+            policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
+            record: bool = False, validate: bool = False, ctx: Context | None = None,
+            alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False):
+    """SCC labels of ``g`` (a Graph or DeviceGraph) on the GPU.
+
+    run_scc (multistep/scc_main.cpp:210).  ``max_deg_vert`` / ``avg_degree`` default to
+    create_csr's rule computed on device.  Returns labels, or (labels, Stats) with
+    ``stats=True``."""
+    lab, st = _run("msc_scc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
+                   alpha, beta, out)
+    return (lab, st) if stats else lab
+
+
+def run_cc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, *,
+           policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
+           record: bool = False, validate: bool = False, ctx: Context | None = None,
+           alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False):
+    """Weakly connected components (run_cc, multistep/cc_main.cpp:184, on the mirror)."""
+    lab, st = _run("msc_cc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
+                   alpha, beta, out)
+    return (lab, st) if stats else lab
+
+
+def scc_pivot(g, ctx: Context | None = None) -> tuple[int, float]:
+    """create_csr's pivot and average degree, computed on device."""
+    o = _opts(POLICY_PLAIN, LABELS_CANONICAL, False, False, False, ctx)
+    piv, avg = C.c_int32(), C.c_double()
+    csr = g.csr()
+    _check(_lib().msc_scc_pivot(C.byref(csr), C.byref(o), C.byref(piv), C.byref(avg)), "msc_scc_pivot")
+    return piv.value, avg.value
+
+
+def _verify(fn, labels):
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    v = _Verify()
+    _check(getattr(_lib(), fn)(lab.shape[0], _ptr(lab) if lab.size else None, C.byref(v)), fn)
+    return v
+
+
+def scc_verify(labels) -> tuple[int, int, int, int]:
+    """(num SCCs, nontrivial, max SCC, unassigned) -- multistep/scc_verify.cpp:47-77."""
+    v = _verify("msc_scc_verify", labels)
+    return v.num_components, v.nontrivial, v.max_size, v.unassigned
+
+
+def cc_verify(labels) -> tuple[int, int, int]:
+    """(num CCs, max CC, unassigned) -- multistep/cc_verify.cpp:47-74."""
+    v = _verify("msc_cc_verify", labels)
+    return v.num_components, v.max_size, v.unassigned
+
+
+def verify_line(labels, kind: str = "scc") -> str:
+    """The reference's summary line for these labels."""
+    v = _verify("msc_scc_verify" if kind == "scc" else "msc_cc_verify", labels)
+    buf = C.create_string_buffer(256)
+    _check(_lib().msc_verify_format(C.byref(v), 0 if kind == "scc" else 1, buf, 256), "msc_verify_format")
+    return buf.value.decode()
+
+
+def canonicalize(labels) -> np.ndarray:
+    """Copy of ``labels`` with every component relabelled to its minimum vertex id."""
+    lab = np.array(labels, dtype=np.int32, copy=True)
+    _check(_lib().msc_canonicalize(lab.shape[0], _ptr(lab) if lab.size else None), "msc_canonicalize")
+    return lab
+
+
+def write_labels(path: str, labels) -> None:
+    """output_scc (multistep/scc_main.cpp:193-202)."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    _check(_lib().msc_write_labels(os.fsencode(path), lab.shape[0], _ptr(lab) if lab.size else None),
+           "write_labels")

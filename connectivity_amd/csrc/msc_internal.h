@@ -1,0 +1,120 @@
+// Internal declarations shared by the gfx950 kernels (msc_kernels.hip) and the host
+// runtime (msc_runtime.cpp).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/msc.h"
+
+namespace msc {
+
+constexpr int kBlock = 256;                    // threads per workgroup: 4 wave64s
+constexpr uint32_t kChunk = 2048;              // edges per edge-balanced work item
+constexpr int kPer = (int)(kChunk / kBlock);   // edges per thread per work item
+constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
+constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;
+// feature standardisation (mean, std) follows the weights
+constexpr int kMlpBlob = kMlpParams + 2 * kMlpIn;
+
+// What one level kernel does with each edge it walks.
+enum Op : int {
+  OP_FW = 0,   // forward reach:  allow = valid, seen = fw   (top-down on out-lists)
+  OP_BW = 1,   // backward reach: allow = fw,    seen = bw   (top-down on in-lists)
+  OP_CC = 2,   // undirected reach over out+in lists
+  OP_PUSH = 3, // max-label push (colour propagation) along down lists
+  OP_CBW = 4,  // per-colour backward search: claim valid in-neighbours of equal colour
+};
+
+// Double-buffered frontier: queue of vertices, exclusive prefix of their list lengths,
+// and the first queue slot of every kChunk-edge chunk of that prefix space.
+struct Frontier {
+  int32_t* q;
+  uint32_t* qo;
+  uint32_t* starts;
+};
+
+// Device-resident loop state.  Every level kernel reads it at entry; the last workgroup
+// to finish a launch (ticket) closes the level and prepares the next one, so the host
+// never needs to look at it between levels.
+struct Ctrl {
+  unsigned long long next_packed;  // (vertices enqueued << 32) | their list lengths
+  uint32_t q_size;                 // frontier to expand
+  uint32_t q_edges;                // its list lengths summed
+  uint32_t cur;                    // Frontier buffer holding it
+  uint32_t bottom_up;              // direction of the coming level
+  uint32_t switched_back;          // the one-way return to top-down happened
+  uint32_t done;
+  uint32_t level;                  // levels closed so far
+  uint32_t ticket;
+  int64_t found;                   // vertices discovered, root excluded
+  uint32_t nrec;                   // level records written
+  uint32_t phase;                  // msc_level_rec.phase of this search
+  uint32_t abandoned;              // the policy stopped the search; its result is void
+  uint32_t pad[3];
+  // statistics, on their own cache line
+  unsigned long long examined __attribute__((aligned(128)));
+  unsigned long long inspected;
+  unsigned long long raised;
+  unsigned long long bytes_op[5];  // algorithmic bytes of closed levels, per Op
+  unsigned long long bu_levels;
+};
+
+struct LevelArgs {
+  const uint32_t* down_off;   // top-down lists (first half for OP_CC / mirrored push)
+  const int32_t* down_adj;
+  const uint32_t* down_off2;  // second list for mirrored ops, else null
+  const int32_t* down_adj2;
+  const uint32_t* up_off;     // bottom-up lists
+  const int32_t* up_adj;
+  const uint32_t* up_off2;
+  const int32_t* up_adj2;
+  uint32_t* allow;            // bitmap
+  uint32_t* seen;             // bitmap
+  uint32_t* valid;            // bitmap (colour ops)
+  uint32_t* inq0;             // colour worklist membership, buffer 0 / 1
+  uint32_t* inq1;
+  int32_t* colors;
+  int32_t* labels;
+  const int32_t* verts;       // bottom-up candidates
+  const uint32_t* nverts;
+  Frontier fr0, fr1;
+  Ctrl* ctrl;
+  msc_level_rec* recs;
+  uint32_t rec_cap;
+  int32_t n_total;
+  int32_t beta;
+  int32_t policy;
+  double avg;
+  double alpha;
+  const float* mlp;           // kMlpBlob floats (learned policy)
+};
+
+struct Launch {
+  hipStream_t stream;
+  int grid;  // workgroups for grid-stride kernels
+};
+
+// ---- kernel launchers (msc_kernels.hip) ----
+void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count);
+void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off,
+                 const uint32_t* in_off, uint32_t* valid, int32_t* labels, int32_t* verts,
+                 uint32_t* nverts);
+void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
+void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);
+void launch_commit(const Launch& L, const int32_t* verts, const uint32_t* nverts,
+                   const uint32_t* member, uint32_t* valid, int32_t* labels, int32_t label,
+                   int32_t* verts_out, uint32_t* nverts_out);
+void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a);
+void launch_roots(const Launch& L, const LevelArgs& a);
+void launch_compact(const Launch& L, const int32_t* verts, const uint32_t* nverts,
+                    const uint32_t* valid, int32_t* verts_out, uint32_t* nverts_out);
+void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch);
+void launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off,
+                      const uint32_t* in_off, unsigned long long* work);
+void launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* out_off,
+                     const uint32_t* in_off, unsigned long long* work);
+void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off,
+                     const int32_t* adj, uint32_t* err);
+
+}  // namespace msc

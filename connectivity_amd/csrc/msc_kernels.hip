@@ -1,0 +1,866 @@
+// gfx950 kernels of the Multistep SCC / WCC path.
+//
+// Design (for a reader with only these sources):
+//  * Vertex state lives in bitmaps (valid / visited / worklist membership): one bit per
+//    vertex, 2 MB at 16.8M vertices, so random probes are served from L2 / MALL.  Claims
+//    are atomicOr / atomicAnd test-and-set on the containing word, so a frontier never
+//    holds a vertex twice.
+//  * Frontiers are queues built with one 64-bit atomic per wave that reserves both the
+//    queue slots and the slots' edge offsets ((count << 32) | edge sum).  The enqueuing
+//    lane also records, for every kChunk-edge chunk that begins inside its vertex's
+//    list, which queue slot owns that chunk ("starts").  The next top-down level then
+//    gives each workgroup one chunk of exactly kChunk edges: it maps every edge to its
+//    source vertex with an LDS max-scan, and reads the adjacency fully coalesced.  Hubs
+//    (the s24 pivot has ~240k out-edges) are spread over ~120 workgroups this way.
+//  * Bottom-up levels sweep the candidate list one vertex per lane and stop at the first
+//    visited parent.
+//  * Each level is one launch on a fixed grid.  The launch reads its sizes and direction
+//    from a device control block; the last workgroup to finish (ticket counter) closes
+//    the level: frontier swap, counters, the direction decision (the reference's
+//    ALPHA/BETA rule or the MLP), a per-level record.  The host only polls `done`.
+//  * No MFMA: no step is a dense contraction.  Everything is HBM / L2 bound integer work.
+#include <hip/hip_runtime.h>
+
+#include "msc_internal.h"
+
+namespace msc {
+namespace {
+
+// ------------------------------------------------------------------------------------
+// wave64 and bitmap helpers
+
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ unsigned long long lanes_below() {
+  return (1ull << lane()) - 1ull;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(x, o, 64);
+    if (lane() >= o) x += t;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(x, o, 64);
+    if (lane() >= o) x = max(x, t);
+  }
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)x, o, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
+    x += ((unsigned long long)hi << 32) | lo;
+  }
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_max64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)x, o, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
+    unsigned long long y = ((unsigned long long)hi << 32) | lo;
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int src) {
+  uint32_t lo = __shfl((uint32_t)x, src, 64);
+  uint32_t hi = __shfl((uint32_t)(x >> 32), src, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ bool bit_test(const uint32_t* b, uint32_t v) {
+  return (b[v >> 5] >> (v & 31u)) & 1u;
+}
+// true when this call turned the bit on
+__device__ __forceinline__ bool bit_set(uint32_t* b, uint32_t v) {
+  uint32_t m = 1u << (v & 31u);
+  return (atomicOr(&b[v >> 5], m) & m) == 0u;
+}
+// true when this call turned the bit off
+__device__ __forceinline__ bool bit_clear(uint32_t* b, uint32_t v) {
+  uint32_t m = 1u << (v & 31u);
+  return (atomicAnd(&b[v >> 5], ~m) & m) != 0u;
+}
+
+__device__ __forceinline__ unsigned long long load_acquire_u64(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t list_len(const uint32_t* off, const uint32_t* off2, uint32_t v) {
+  uint32_t d = off[v + 1] - off[v];
+  if (off2) d += off2[v + 1] - off2[v];
+  return d;
+}
+
+// Wave-cooperative append of the lanes with `take` to frontier `nx`.  Must be reached by
+// every lane of the wave.  One 64-bit atomic reserves queue slots and edge offsets.
+__device__ __forceinline__ void enqueue(bool take, uint32_t w, uint32_t deg, const Frontier& nx,
+                                        Ctrl* c) {
+  unsigned long long mask = __ballot(take);
+  if (mask == 0ull) return;
+  uint32_t d = take ? deg : 0u;
+  uint32_t incl = wave_incl_sum(d);
+  uint32_t total = __shfl(incl, 63, 64);
+  uint32_t cnt = (uint32_t)__popcll(mask);
+  int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0ull;
+  if (lane() == leader)
+    base = atomicAdd(&c->next_packed, ((unsigned long long)cnt << 32) | (unsigned long long)total);
+  base = bcast64(base, leader);
+  if (take) {
+    uint32_t pos = (uint32_t)(base >> 32) + (uint32_t)__popcll(mask & lanes_below());
+    uint32_t off = (uint32_t)base + incl - d;
+    nx.q[pos] = (int32_t)w;
+    nx.qo[pos] = off;
+    uint64_t end = (uint64_t)off + d;
+    for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch)
+      nx.starts[ch] = pos;
+  }
+}
+
+// Block-wide sum of a per-thread value; result valid in thread 0.
+__device__ __forceinline__ unsigned long long block_sum64(unsigned long long x,
+                                                          unsigned long long* s_red) {
+  x = wave_sum64(x);
+  int w = threadIdx.x >> 6;
+  if (lane() == 0) s_red[w] = x;
+  __syncthreads();
+  unsigned long long t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) t += s_red[i];
+  __syncthreads();
+  return t;
+}
+
+// Returns true in every thread of the workgroup that finished the launch last.
+__device__ __forceinline__ bool last_block(Ctrl* c, uint32_t* s_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    uint32_t t = atomicAdd(&c->ticket, 1u);
+    *s_flag = (t == gridDim.x - 1) ? 1u : 0u;
+  }
+  __syncthreads();
+  bool last = *s_flag != 0u;
+  if (last) __threadfence();
+  return last;
+}
+
+// ------------------------------------------------------------------------------------
+// learned policy: 12 -> 32 -> 32 -> 3 ReLU MLP, evaluated by one wave
+
+// Standardised features of the level just closed (same order as the training code).
+__device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges,
+                               bool bu, float* f) {
+  float n = (float)a.n_total > 0 ? (float)a.n_total : 1.0f;
+  float nv = a.nverts ? (float)(*a.nverts) : n;
+  float found = (float)c->found;
+  f[0] = log1pf((float)next);
+  f[1] = log1pf((float)nedges);
+  f[2] = log1pf(fmaxf(nv - found, 0.0f));
+  f[3] = (float)c->level;
+  f[4] = (float)next / n;
+  f[5] = (float)nedges / fmaxf((float)a.avg * n, 1.0f);
+  f[6] = next > 0 ? log1pf((float)nedges / (float)next) : 0.0f;
+  f[7] = log1pf((float)a.avg);
+  f[8] = bu ? 1.0f : 0.0f;
+  f[9] = (float)c->switched_back;
+  f[10] = found / fmaxf(nv, 1.0f);
+  f[11] = (float)c->phase;
+}
+
+// Wave 0 of the closing workgroup evaluates the MLP; returns argmax action in lane 0.
+__device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
+  const float* W1 = blob;
+  const float* b1 = W1 + kMlpIn * kMlpH;
+  const float* W2 = b1 + kMlpH;
+  const float* b2 = W2 + kMlpH * kMlpH;
+  const float* W3 = b2 + kMlpH;
+  const float* b3 = W3 + kMlpH * kMlpOut;
+  const float* mean = blob + kMlpParams;
+  const float* sd = mean + kMlpIn;
+  int l = lane();
+  float h = 0.0f;
+  if (l < kMlpH) {
+    float acc = b1[l];
+    for (int i = 0; i < kMlpIn; ++i) acc += W1[l * kMlpIn + i] * ((feat[i] - mean[i]) / sd[i]);
+    h = fmaxf(acc, 0.0f);
+    s_h[l] = h;
+  }
+  __builtin_amdgcn_wave_barrier();
+  float h2 = 0.0f;
+  if (l < kMlpH) {
+    float acc = b2[l];
+    for (int i = 0; i < kMlpH; ++i) acc += W2[l * kMlpH + i] * s_h[i];
+    h2 = fmaxf(acc, 0.0f);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (l < kMlpH) s_h[l] = h2;
+  __builtin_amdgcn_wave_barrier();
+  int best = 0;
+  if (l == 0) {
+    float bv = -3.0e38f;
+    for (int o = 0; o < kMlpOut; ++o) {
+      float acc = b3[o];
+      for (int i = 0; i < kMlpH; ++i) acc += W3[o * kMlpH + i] * s_h[i];
+      if (acc > bv) {
+        bv = acc;
+        best = o;
+      }
+    }
+  }
+  return best;
+}
+
+// Closes a level (called by the whole last workgroup).  `examined`/`inspected` are the
+// bottom-up counters of this launch, already summed over the grid.
+__device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_t q_edges,
+                            bool bu) {
+  Ctrl* c = a.ctrl;
+  __shared__ float s_feat[kMlpIn];
+  __shared__ float s_h[kMlpH];
+  __shared__ int s_act;
+  unsigned long long p = load_acquire_u64(&c->next_packed);
+  uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
+  bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
+  if (threadIdx.x == 0) {
+    c->found += next;
+    c->level += 1;
+    // the reference's switch rule (multistep/scc_fwbw.cpp:168-195, cc_bfs.cpp:167-194)
+    int act = bu ? 1 : 0;
+    if (reach) {
+      if (!c->switched_back && !bu) {
+        double edges_frontier = (double)next * a.avg;
+        double edges_remainder = (double)((int64_t)a.n_total - c->found) * a.avg;
+        if (edges_remainder / a.alpha < edges_frontier) act = 1;
+      } else if (!c->switched_back && (a.n_total / a.beta) > (int32_t)next) {
+        act = 0;
+        c->switched_back = 1;
+      }
+    }
+    s_act = act;
+    if (reach && a.policy == MSC_POLICY_LEARNED && a.mlp) level_features(c, a, next, nedges, bu, s_feat);
+  }
+  __syncthreads();
+  if (reach && a.policy == MSC_POLICY_LEARNED && a.mlp && threadIdx.x < 64) {
+    int act = mlp_action(a.mlp, s_feat, s_h);
+    if (threadIdx.x == 0) s_act = act;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int act = s_act;
+    unsigned long long examined = c->examined, inspected = c->inspected, raised = c->raised;
+    // algorithmic bytes of this level (unit costs documented in DESIGN.md)
+    unsigned long long bytes;
+    if (op == OP_PUSH)
+      bytes = 12ull * q_size + 9ull * q_edges + 4ull * raised + 4ull * next;
+    else if (op == OP_CBW)
+      bytes = 12ull * q_size + 9ull * q_edges + 5ull * next;
+    else if (bu)
+      bytes = (a.nverts ? 1ull * (*a.nverts) : 0ull) + 8ull * examined + 5ull * inspected + 5ull * next;
+    else
+      bytes = 12ull * q_size + 5ull * q_edges + 5ull * next;
+    c->bytes_op[op] += bytes;
+    if (bu) c->bu_levels += 1;
+    if (a.recs && c->nrec < a.rec_cap) {
+      msc_level_rec& r = a.recs[c->nrec];
+      r.phase = (int32_t)c->phase;
+      r.level = (int32_t)c->level;
+      r.bottom_up = bu ? 1 : 0;
+      r.next_size = (int32_t)next;
+      r.q_size = q_size;
+      r.q_edges = q_edges;
+      r.next_edges = nedges;
+      r.examined = (uint32_t)examined;
+      r.inspected = inspected;
+      r.found = c->found;
+      r.n_total = a.n_total;
+      r.num_valid = a.nverts ? (int32_t)(*a.nverts) : a.n_total;
+      r.action = act;
+      r.pad = 0;
+    }
+    c->nrec += 1;
+    c->q_size = next;
+    c->q_edges = nedges;
+    c->cur ^= 1u;
+    c->bottom_up = act == 1 ? 1u : 0u;
+    c->next_packed = 0ull;
+    c->examined = 0ull;
+    c->inspected = 0ull;
+    c->raised = 0ull;
+    c->done = (next == 0u || act == 2) ? 1u : 0u;
+    if (act == 2) c->abandoned = 1u;  // the host discards this search's result
+    __threadfence();
+    c->ticket = 0u;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// kernels
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t value, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// Degree trim (multistep/scc_trim.cpp:89-139; kokkos_scc/scc_trim.cpp:57-140) and the CC
+// variant (multistep/cc_trim.cpp:47-97).  One wave per 64 consecutive vertices: the ballot
+// is the two bitmap words; survivors are appended in id order within the wave.
+template <bool MIRROR>
+__global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __restrict__ oo,
+                                                 const uint32_t* __restrict__ io,
+                                                 uint32_t* __restrict__ valid,
+                                                 int32_t* __restrict__ labels,
+                                                 int32_t* __restrict__ verts,
+                                                 uint32_t* __restrict__ nverts) {
+  const int64_t nw = ((int64_t)n + 31) / 32;
+  for (int64_t base = ((int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63)); base < n;
+       base += (int64_t)gridDim.x * kBlock) {
+    int64_t v = base + lane();
+    bool keep = false;
+    if (v < n) {
+      uint32_t od = oo[v + 1] - oo[v], id = io[v + 1] - io[v];
+      keep = MIRROR ? (od + id) > 0u : (od > 0u && id > 0u);
+      labels[v] = keep ? -1 : (int32_t)v;
+    }
+    unsigned long long mask = __ballot(keep);
+    int64_t word = base >> 5;
+    if (lane() == 0) valid[word] = (uint32_t)mask;
+    if (lane() == 32 && word + 1 < nw) valid[word + 1] = (uint32_t)(mask >> 32);
+    if (mask) {
+      int leader = __ffsll((long long)mask) - 1;
+      uint32_t at = 0;
+      if (lane() == leader) at = atomicAdd(nverts, (uint32_t)__popcll(mask));
+      at = __shfl(at, leader, 64);
+      if (keep) verts[at + __popcll(mask & lanes_below())] = (int32_t)v;
+    }
+  }
+}
+
+// Starts a search from `root`: fresh control block, root claimed and queued alone.
+// Follows scc_fwbw.cpp:72-73 (queue = {root}, fw[root] = true, set even if root is invalid).
+__global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
+  Ctrl* c = a.ctrl;
+  uint32_t deg = list_len(a.down_off, a.down_off2, (uint32_t)root);
+  if (threadIdx.x == 0) {
+    c->next_packed = 0ull;
+    c->q_size = 1u;
+    c->q_edges = deg;
+    c->cur = 0u;
+    c->bottom_up = 0u;
+    c->switched_back = 0u;
+    c->done = 0u;
+    c->level = 0u;
+    c->ticket = 0u;
+    c->found = 0;
+    c->phase = (uint32_t)phase;
+    c->abandoned = 0u;
+    c->examined = c->inspected = c->raised = 0ull;
+    uint32_t m = 1u << ((uint32_t)root & 31u);
+    a.seen[(uint32_t)root >> 5] |= m;
+    a.fr0.q[0] = root;
+    a.fr0.qo[0] = 0u;
+  }
+  uint32_t nch = (deg + kChunk - 1) / kChunk;
+  for (uint32_t ch = threadIdx.x; ch < nch; ch += blockDim.x) a.fr0.starts[ch] = 0u;
+}
+
+// Resets the control block before a seeding kernel fills frontier buffer 1.
+__global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
+  if (threadIdx.x == 0) {
+    c->next_packed = 0ull;
+    c->q_size = 0u;
+    c->q_edges = 0u;
+    c->cur = 0u;
+    c->bottom_up = 0u;
+    c->switched_back = 0u;
+    c->done = 0u;
+    c->level = 0u;
+    c->ticket = 0u;
+    c->found = 0;
+    c->phase = (uint32_t)phase;
+    c->abandoned = 0u;
+    c->examined = c->inspected = c->raised = 0ull;
+  }
+}
+
+// Closes a seeding launch: the queue built in buffer 1 becomes the frontier.
+__device__ void close_seed(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    unsigned long long p = load_acquire_u64(&c->next_packed);
+    uint32_t next = (uint32_t)(p >> 32);
+    c->q_size = next;
+    c->q_edges = (uint32_t)p;
+    c->cur = 1u;
+    c->next_packed = 0ull;
+    c->done = next == 0u ? 1u : 0u;
+    __threadfence();
+    c->ticket = 0u;
+  }
+}
+
+// One level of a search or a colour sweep.  Top-down levels: edge-balanced expansion of
+// the current frontier (the "Manhattan collapse" of kokkos_scc/fwbw_manhattan_global.cpp:
+// 162-289, with an LDS max-scan instead of a per-edge binary search).  Bottom-up levels
+// (reach ops only): one candidate per lane (multistep/scc_fwbw.cpp:118-148).
+template <int OP, bool MIRROR>
+__global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
+  constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
+  constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
+  __shared__ uint32_t s_own[kChunk];
+  __shared__ uint32_t s_base[kChunk + 1];
+  __shared__ int32_t s_col[COLOR ? kChunk + 1 : 1];
+  __shared__ uint32_t s_split[MIRROR ? kChunk + 1 : 1];
+  __shared__ uint32_t s_base2[MIRROR ? kChunk + 1 : 1];
+  __shared__ uint32_t s_scan[kBlock / 64];
+  __shared__ unsigned long long s_red[kBlock / 64];
+  __shared__ uint32_t s_flag;
+
+  Ctrl* c = a.ctrl;
+  if (c->done) return;  // uniform: the search already ended
+  const uint32_t cur = c->cur;
+  const uint32_t q_size = c->q_size, q_edges = c->q_edges;
+  const bool bu = REACH && c->bottom_up != 0u;
+  const Frontier F = cur ? a.fr1 : a.fr0;
+  const Frontier N = cur ? a.fr0 : a.fr1;
+  uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
+  uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
+  const int t = threadIdx.x;
+  unsigned long long my_examined = 0, my_inspected = 0, my_raised = 0;
+
+  if (!bu) {
+    const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+      const uint32_t e0 = ch * kChunk;
+      const uint32_t len = min(kChunk, q_edges - e0);
+      const uint32_t k0 = F.starts[ch];
+      const uint32_t kend = (ch + 1 < nchunks) ? F.starts[ch + 1] : q_size - 1u;
+      // every queued vertex has a non-empty list, so at most len + 1 slots overlap the
+      // chunk; the clamp only keeps LDS writes in bounds
+      const uint32_t nk = min(kend - k0 + 1u, kChunk + 1u);
+      for (uint32_t i = t; i < len; i += kBlock) s_own[i] = 0u;
+      __syncthreads();
+      for (uint32_t j = t; j < nk; j += kBlock) {
+        uint32_t k = k0 + j;
+        uint32_t qo = F.qo[k];
+        uint32_t v = (uint32_t)F.q[k];
+        uint32_t s = qo > e0 ? qo - e0 : 0u;
+        if (s < len) {
+          atomicMax(&s_own[s], j);
+          if (OP == OP_PUSH) bit_clear(inq_cur, v);
+        }
+        uint32_t b1 = a.down_off[v];
+        s_base[j] = b1 - qo;
+        if (MIRROR) {
+          uint32_t split = qo + (a.down_off[v + 1] - b1);
+          s_split[j] = split;
+          s_base2[j] = a.down_off2[v] - split;
+        }
+        if (COLOR) s_col[j] = a.colors[v];
+      }
+      __syncthreads();
+      // inclusive max-scan of s_own[0..len): thread t owns entries [t*kPer, t*kPer+kPer)
+      {
+        uint32_t vals[kPer];
+        uint32_t run = 0;
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+          uint32_t idx = (uint32_t)t * kPer + r;
+          uint32_t x = idx < len ? s_own[idx] : 0u;
+          run = max(run, x);
+          vals[r] = run;
+        }
+        uint32_t incl = wave_incl_max(run);
+        uint32_t excl = __shfl_up(incl, 1, 64);
+        if (lane() == 0) excl = 0u;
+        if (lane() == 63) s_scan[t >> 6] = incl;
+        __syncthreads();
+        uint32_t carry = 0;
+        for (int w = 0; w < (t >> 6); ++w) carry = max(carry, s_scan[w]);
+        excl = max(excl, carry);
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+          uint32_t idx = (uint32_t)t * kPer + r;
+          if (idx < len) s_own[idx] = max(vals[r], excl);
+        }
+      }
+      __syncthreads();
+#pragma unroll 2
+      for (int r = 0; r < kPer; ++r) {
+        uint32_t i = (uint32_t)t + (uint32_t)r * kBlock;
+        bool take = false;
+        uint32_t w = 0, deg = 0;
+        if (i < len) {
+          uint32_t j = s_own[i];
+          uint32_t e = e0 + i;
+          if (MIRROR)
+            w = (uint32_t)(e < s_split[j] ? a.down_adj[s_base[j] + e] : a.down_adj2[s_base2[j] + e]);
+          else
+            w = (uint32_t)a.down_adj[s_base[j] + e];
+          if (REACH) {
+            take = bit_test(a.allow, w) && !bit_test(a.seen, w) && bit_set(a.seen, w);
+          } else if (OP == OP_PUSH) {
+            int32_t col = s_col[j];
+            if (bit_test(a.valid, w) && a.colors[w] < col) {
+              int32_t old = atomicMax(&a.colors[w], col);
+              if (old < col) {
+                ++my_raised;
+                take = bit_set(inq_nxt, w);
+              }
+            }
+          } else {  // OP_CBW
+            int32_t col = s_col[j];
+            if (bit_test(a.valid, w) && a.colors[w] == col && bit_clear(a.valid, w)) {
+              a.labels[w] = col;
+              take = true;
+            }
+          }
+          if (take) deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w);
+        }
+        enqueue(take, w, deg, N, c);
+      }
+      __syncthreads();
+    }
+  } else if (REACH) {
+    const uint32_t nv = *a.nverts;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t base = blockIdx.x * kBlock + (t & ~63); base < nv; base += stride) {
+      uint32_t i = base + (uint32_t)lane();
+      bool cand = false, hit = false;
+      uint32_t v = 0;
+      if (i < nv) {
+        v = (uint32_t)a.verts[i];
+        cand = bit_test(a.allow, v) && !bit_test(a.seen, v);
+      }
+      if (cand) {
+        ++my_examined;
+        for (uint32_t j = a.up_off[v], e = a.up_off[v + 1]; j < e; ++j) {
+          ++my_inspected;
+          uint32_t u = (uint32_t)a.up_adj[j];
+          if (bit_test(a.seen, u) && bit_test(a.allow, u)) {
+            hit = true;
+            break;
+          }
+        }
+        if (MIRROR && !hit) {
+          for (uint32_t j = a.up_off2[v], e = a.up_off2[v + 1]; j < e; ++j) {
+            ++my_inspected;
+            uint32_t u = (uint32_t)a.up_adj2[j];
+            if (bit_test(a.seen, u) && bit_test(a.allow, u)) {
+              hit = true;
+              break;
+            }
+          }
+        }
+      }
+      bool take = hit && bit_set(a.seen, v);
+      uint32_t deg = take ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v) : 0u;
+      enqueue(take, v, deg, N, c);
+    }
+  }
+
+  // grid-wide statistics, one atomic per workgroup
+  unsigned long long ex = block_sum64(my_examined, s_red);
+  unsigned long long in = block_sum64(my_inspected, s_red);
+  unsigned long long ra = block_sum64(my_raised, s_red);
+  if (t == 0) {
+    if (ex) atomicAdd(&c->examined, ex);
+    if (in) atomicAdd(&c->inspected, in);
+    if (ra) atomicAdd(&c->raised, ra);
+  }
+  if (last_block(c, &s_flag)) close_level(a, OP, q_size, q_edges, bu);
+}
+
+// FW-BW commit (multistep/scc_fwbw.cpp:412-466) and CC-BFS commit (cc_bfs.cpp:217-271):
+// members get `label` and leave the valid set; the others are compacted.
+__global__ void __launch_bounds__(kBlock) k_commit(const int32_t* __restrict__ verts,
+                                                   const uint32_t* __restrict__ nverts,
+                                                   const uint32_t* __restrict__ member,
+                                                   uint32_t* __restrict__ valid,
+                                                   int32_t* __restrict__ labels, int32_t label,
+                                                   int32_t* __restrict__ verts_out,
+                                                   uint32_t* __restrict__ nverts_out) {
+  const uint32_t nv = *nverts;
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
+       base += gridDim.x * kBlock) {
+    uint32_t i = base + (uint32_t)lane();
+    bool keep = false;
+    if (i < nv) {
+      uint32_t v = (uint32_t)verts[i];
+      if (bit_test(member, v)) {
+        labels[v] = label;
+        bit_clear(valid, v);
+      } else {
+        keep = bit_test(valid, v);
+      }
+    }
+    unsigned long long mask = __ballot(keep);
+    if (mask) {
+      int leader = __ffsll((long long)mask) - 1;
+      uint32_t at = 0;
+      if (lane() == leader) at = atomicAdd(nverts_out, (uint32_t)__popcll(mask));
+      at = __shfl(at, leader, 64);
+      if (keep) verts_out[at + __popcll(mask & lanes_below())] = verts[i];
+    }
+  }
+}
+
+// Keeps the still-valid vertices of a list (multistep/scc_color.cpp:292-337).
+__global__ void __launch_bounds__(kBlock) k_compact(const int32_t* __restrict__ verts,
+                                                    const uint32_t* __restrict__ nverts,
+                                                    const uint32_t* __restrict__ valid,
+                                                    int32_t* __restrict__ verts_out,
+                                                    uint32_t* __restrict__ nverts_out) {
+  const uint32_t nv = *nverts;
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
+       base += gridDim.x * kBlock) {
+    uint32_t i = base + (uint32_t)lane();
+    int32_t v = i < nv ? verts[i] : 0;
+    bool keep = i < nv && bit_test(valid, (uint32_t)v);
+    unsigned long long mask = __ballot(keep);
+    if (mask) {
+      int leader = __ffsll((long long)mask) - 1;
+      uint32_t at = 0;
+      if (lane() == leader) at = atomicAdd(nverts_out, (uint32_t)__popcll(mask));
+      at = __shfl(at, leader, 64);
+      if (keep) verts_out[at + __popcll(mask & lanes_below())] = v;
+    }
+  }
+}
+
+// Colour initialisation (multistep/scc_color.cpp:56-63, cc_color.cpp:62-69): colors[v] = v
+// for every valid vertex, all of them queued.  Needs k_ctrl_reset first.
+template <bool MIRROR>
+__global__ void __launch_bounds__(kBlock) k_seed_colors(LevelArgs a) {
+  __shared__ uint32_t s_flag;
+  const uint32_t nv = *a.nverts;
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
+       base += gridDim.x * kBlock) {
+    uint32_t i = base + (uint32_t)lane();
+    bool take = i < nv;
+    uint32_t v = 0, deg = 0;
+    if (take) {
+      v = (uint32_t)a.verts[i];
+      a.colors[v] = (int32_t)v;
+      deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
+    }
+    enqueue(take, v, deg, a.fr1, a.ctrl);
+  }
+  if (last_block(a.ctrl, &s_flag)) close_seed(a.ctrl);
+}
+
+// Colour roots (multistep/scc_color.cpp:165-212): colors[v] == v labels v, removes it and
+// queues it for the per-colour backward search.  Needs k_ctrl_reset first.
+__global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
+  __shared__ uint32_t s_flag;
+  const uint32_t nv = *a.nverts;
+  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
+       base += gridDim.x * kBlock) {
+    uint32_t i = base + (uint32_t)lane();
+    bool take = false;
+    uint32_t v = 0, deg = 0;
+    if (i < nv) {
+      v = (uint32_t)a.verts[i];
+      if (a.colors[v] == (int32_t)v) {
+        take = true;
+        a.labels[v] = (int32_t)v;
+        bit_clear(a.valid, v);
+        deg = a.down_off[v + 1] - a.down_off[v];
+      }
+    }
+    enqueue(take, v, deg, a.fr1, a.ctrl);
+  }
+  if (last_block(a.ctrl, &s_flag)) close_seed(a.ctrl);
+}
+
+// Canonical labels: every component relabelled to its minimum vertex id.
+__global__ void k_canon_min(int32_t n, const int32_t* __restrict__ labels, int32_t* __restrict__ mins) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int32_t l = labels[v];
+    if (l >= 0) atomicMin(&mins[l], (int32_t)v);
+  }
+}
+__global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int32_t* __restrict__ mins) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    int32_t l = labels[v];
+    if (l >= 0) labels[v] = mins[l];
+  }
+}
+
+// Pivot of create_csr (multistep/scc_main.cpp:169-183): largest (double)out*(double)in,
+// first vertex on ties.  Pass 1: max of the product's bit pattern (non-negative doubles
+// order like their bits); pass 2: smallest vertex attaining it.
+__global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t* __restrict__ oo,
+                                                      const uint32_t* __restrict__ io,
+                                                      unsigned long long* work) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long best = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    double p = (double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]);
+    unsigned long long b = (unsigned long long)__double_as_longlong(p);
+    best = b > best ? b : best;
+  }
+  best = wave_max64(best);
+  if (lane() == 0) s_red[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
+    atomicMax(&work[0], best);
+  }
+}
+__global__ void k_pivot_first(int32_t n, const uint32_t* __restrict__ oo, const uint32_t* __restrict__ io,
+                              unsigned long long* work) {
+  const unsigned long long target = work[0];
+  if (target == 0ull) return;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double p = (double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]);
+    if ((unsigned long long)__double_as_longlong(p) == target) atomicMin(&work[1], (unsigned long long)v);
+  }
+}
+// CC pivot (multistep/cc_main.cpp:148-160): largest out+in degree, first vertex on ties,
+// as one 64-bit max of (degree << 32 | ~v).
+__global__ void __launch_bounds__(kBlock) k_pivot_cc(int32_t n, const uint32_t* __restrict__ oo,
+                                                     const uint32_t* __restrict__ io,
+                                                     unsigned long long* work) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long best = 0;
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    uint32_t d = (oo[v + 1] - oo[v]) + (io[v + 1] - io[v]);
+    if (d == 0u) continue;
+    unsigned long long key = ((unsigned long long)d << 32) | (unsigned long long)(0xffffffffu - (uint32_t)v);
+    best = key > best ? key : best;
+  }
+  best = wave_max64(best);
+  if (lane() == 0) s_red[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
+    atomicMax(&work[0], best);
+  }
+}
+
+// Input check: offsets start at 0, never decrease, end at m; ids in [0, n).
+__global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ off,
+                           const int32_t* __restrict__ adj, uint32_t* err) {
+  int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  if (tid == 0) bad = off[0] != 0u || (int64_t)off[n] != m;
+  for (int64_t v = tid; v < n; v += stride) bad |= off[v + 1] < off[v];
+  for (int64_t e = tid; e < m; e += stride) bad |= (uint32_t)adj[e] >= (uint32_t)n;
+  if (bad) atomicOr(err, 1u);
+}
+
+inline dim3 grid_for(int64_t items, int cap) {
+  int64_t b = (items + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return dim3((unsigned)b);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// launchers
+
+void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_fill_u32, grid_for(count, L.grid * 4), dim3(kBlock), 0, L.stream, p, value, count);
+}
+
+void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io,
+                 uint32_t* valid, int32_t* labels, int32_t* verts, uint32_t* nverts) {
+  if (n <= 0) return;
+  dim3 g = grid_for(n, L.grid * 4);
+  if (mirror)
+    hipLaunchKernelGGL(k_trim<true>, g, dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, verts, nverts);
+  else
+    hipLaunchKernelGGL(k_trim<false>, g, dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, verts, nverts);
+}
+
+void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
+  hipLaunchKernelGGL(k_search_init, dim3(1), dim3(kBlock), 0, L.stream, a, root, phase);
+}
+
+void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
+  dim3 g((unsigned)L.grid), b(kBlock);
+  switch (op) {
+    case OP_FW: hipLaunchKernelGGL((k_level<OP_FW, false>), g, b, 0, L.stream, a); break;
+    case OP_BW: hipLaunchKernelGGL((k_level<OP_BW, false>), g, b, 0, L.stream, a); break;
+    case OP_CC: hipLaunchKernelGGL((k_level<OP_CC, true>), g, b, 0, L.stream, a); break;
+    case OP_PUSH:
+      if (mirror) hipLaunchKernelGGL((k_level<OP_PUSH, true>), g, b, 0, L.stream, a);
+      else hipLaunchKernelGGL((k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
+      break;
+    case OP_CBW: hipLaunchKernelGGL((k_level<OP_CBW, false>), g, b, 0, L.stream, a); break;
+    default: break;
+  }
+}
+
+void launch_commit(const Launch& L, const int32_t* verts, const uint32_t* nverts, const uint32_t* member,
+                   uint32_t* valid, int32_t* labels, int32_t label, int32_t* verts_out, uint32_t* nverts_out) {
+  hipLaunchKernelGGL(k_commit, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts, member, valid,
+                     labels, label, verts_out, nverts_out);
+}
+
+void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a) {
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 3);
+  if (mirror)
+    hipLaunchKernelGGL(k_seed_colors<true>, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, a);
+  else
+    hipLaunchKernelGGL(k_seed_colors<false>, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, a);
+}
+
+void launch_roots(const Launch& L, const LevelArgs& a) {
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 4);
+  hipLaunchKernelGGL(k_roots, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, a);
+}
+
+void launch_compact(const Launch& L, const int32_t* verts, const uint32_t* nverts, const uint32_t* valid,
+                    int32_t* verts_out, uint32_t* nverts_out) {
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts, valid,
+                     verts_out, nverts_out);
+}
+
+void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch) {
+  if (n <= 0) return;
+  launch_fill_u32(L, (uint32_t*)scratch, 0x7fffffffu, n);
+  dim3 g = grid_for(n, L.grid * 4);
+  hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
+  hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
+}
+
+void launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io,
+                      unsigned long long* work) {
+  dim3 g = grid_for(n, L.grid * 4);
+  hipLaunchKernelGGL(k_pivot_max, g, dim3(kBlock), 0, L.stream, n, oo, io, work);
+  hipLaunchKernelGGL(k_pivot_first, g, dim3(kBlock), 0, L.stream, n, oo, io, work);
+}
+
+void launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io,
+                     unsigned long long* work) {
+  hipLaunchKernelGGL(k_pivot_cc, grid_for(n, L.grid * 4), dim3(kBlock), 0, L.stream, n, oo, io, work);
+}
+
+void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj,
+                     uint32_t* err) {
+  hipLaunchKernelGGL(k_validate, grid_for(m > n ? m : n, L.grid * 4), dim3(kBlock), 0, L.stream, n, m, off,
+                     adj, err);
+}
+
+}  // namespace msc

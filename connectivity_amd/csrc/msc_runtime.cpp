@@ -1,0 +1,750 @@
+// Host runtime of libmsc: contexts, device workspace, the phase drivers behind
+// msc_scc_run / msc_cc_run, and per-kernel timing.
+//
+// Phase order follows run_scc (multistep/scc_main.cpp:210-317) and Kokkos do_run
+// (kokkos_scc/scc_run.cpp:48-184): trim -> FW-BW -> colouring rounds until no vertex is
+// left.  Every level loop runs on device state; the host enqueues levels in growing
+// batches and only reads a `done` word between batches (one round trip per batch, where
+// the Kokkos code copies a counter back after every level, fwbw_baseline.cpp:120-125).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "msc_internal.h"
+
+using namespace msc;
+
+namespace {
+
+constexpr uint32_t kRecCap = 1u << 16;
+
+struct Ev {
+  hipEvent_t a, b;
+  int cls;
+  bool used;
+};
+
+}  // namespace
+
+struct msc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int grid = 1024;
+  int cus = 256;
+  // device workspace
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  int64_t cap_n = -1, cap_e = -1;
+  uint32_t *valid = nullptr, *visA = nullptr, *visB = nullptr, *inq0 = nullptr, *inq1 = nullptr;
+  int32_t *labels = nullptr, *colors = nullptr, *verts0 = nullptr, *verts1 = nullptr;
+  Frontier fr0{}, fr1{};
+  uint32_t* counters = nullptr;  // [0] nverts0, [1] nverts1, [2] validate error
+  unsigned long long* work = nullptr;
+  Ctrl* ctrl = nullptr;
+  msc_level_rec* recs = nullptr;
+  float* mlp = nullptr;
+  bool has_mlp = false;
+  // staging for host-resident inputs
+  void* in_buf = nullptr;
+  size_t in_bytes = 0;
+  // host side
+  Ctrl* ctrl_host = nullptr;  // pinned
+  uint32_t* pinned_u32 = nullptr;
+  std::vector<Ev> events;
+  size_t ev_used = 0;
+  bool timing = false;
+  uint32_t last_nrec = 0;
+  std::mutex mu;
+};
+
+namespace {
+
+std::mutex g_default_mu;
+std::vector<msc_ctx*> g_default;  // one default context per device
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) {                             \
+      (void)hipGetLastError();                          \
+      return _e == hipErrorOutOfMemory ? MSC_E_OOM : MSC_E_DEVICE; \
+    }                                                   \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
+  if (n <= c->cap_n && edges <= c->cap_e) return MSC_OK;
+  int64_t nn = n > c->cap_n ? n : c->cap_n;
+  int64_t ee = edges > c->cap_e ? edges : c->cap_e;
+  if (nn < 1) nn = 1;
+  if (ee < 1) ee = 1;
+  if (c->ws) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->ws);
+    c->ws = nullptr;
+  }
+  size_t nw = (size_t)((nn + 31) / 32) + 1;
+  size_t nchunks = (size_t)((ee + kChunk - 1) / kChunk) + 2;
+  size_t sizes[] = {
+      nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,                 // bitmaps
+      (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,  // labels colors verts0 verts1
+      (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr0
+      (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr1
+      64 * 4,                                                  // counters
+      8 * 8,                                                   // work
+      sizeof(Ctrl),                                            // ctrl
+      (size_t)kRecCap * sizeof(msc_level_rec),                 // records
+      (size_t)kMlpBlob * 4,                                    // mlp
+  };
+  size_t total = 0;
+  for (size_t s : sizes) total += align256(s);
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, total));
+  c->ws = p;
+  c->ws_bytes = total;
+  char* q = (char*)p;
+  void* parts[sizeof(sizes) / sizeof(sizes[0])];
+  for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
+    parts[i] = q;
+    q += align256(sizes[i]);
+  }
+  c->valid = (uint32_t*)parts[0];
+  c->visA = (uint32_t*)parts[1];
+  c->visB = (uint32_t*)parts[2];
+  c->inq0 = (uint32_t*)parts[3];
+  c->inq1 = (uint32_t*)parts[4];
+  c->labels = (int32_t*)parts[5];
+  c->colors = (int32_t*)parts[6];
+  c->verts0 = (int32_t*)parts[7];
+  c->verts1 = (int32_t*)parts[8];
+  c->fr0 = Frontier{(int32_t*)parts[9], (uint32_t*)parts[10], (uint32_t*)parts[11]};
+  c->fr1 = Frontier{(int32_t*)parts[12], (uint32_t*)parts[13], (uint32_t*)parts[14]};
+  c->counters = (uint32_t*)parts[15];
+  c->work = (unsigned long long*)parts[16];
+  c->ctrl = (Ctrl*)parts[17];
+  c->recs = (msc_level_rec*)parts[18];
+  float* old_mlp_host = nullptr;
+  (void)old_mlp_host;
+  c->mlp = (float*)parts[19];
+  c->has_mlp = false;  // weights must be set again after a regrow
+  c->cap_n = nn;
+  c->cap_e = ee;
+  HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
+  return MSC_OK;
+}
+
+// ---- timing -----------------------------------------------------------------------
+
+int ev_begin(msc_ctx* c, int cls) {
+  if (!c->timing) return -1;
+  if (c->ev_used == c->events.size()) {
+    Ev e{};
+    if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) return -1;
+    c->events.push_back(e);
+  }
+  Ev& e = c->events[c->ev_used];
+  e.cls = cls;
+  e.used = true;
+  (void)hipEventRecord(e.a, c->stream);
+  return (int)c->ev_used++;
+}
+void ev_end(msc_ctx* c, int idx) {
+  if (idx < 0) return;
+  (void)hipEventRecord(c->events[idx].b, c->stream);
+}
+void ev_collect(msc_ctx* c, msc_stats* s) {
+  for (size_t i = 0; i < c->ev_used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->events[i].a, c->events[i].b) == hipSuccess && s) {
+      s->k[c->events[i].cls].ms += ms;
+      s->k[c->events[i].cls].launches += 1;
+    }
+  }
+  c->ev_used = 0;
+}
+
+struct Timed {
+  msc_ctx* c;
+  int idx;
+  Timed(msc_ctx* ctx, int cls) : c(ctx), idx(ev_begin(ctx, cls)) {}
+  ~Timed() { ev_end(c, idx); }
+};
+
+Launch launch_of(const msc_ctx* c) { return Launch{c->stream, c->grid}; }
+
+int read_u32(msc_ctx* c, const uint32_t* dev, uint32_t* out) {
+  HIP_TRY(hipMemcpyAsync(c->pinned_u32, dev, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *out = *c->pinned_u32;
+  return MSC_OK;
+}
+
+int read_ctrl(msc_ctx* c) {
+  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MSC_OK;
+}
+
+// Runs level launches until the device control block says done.  Every search and every
+// max-label fixpoint settles within n + 1 levels, so more launches than that mean a fault.
+int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, msc_stats* s) {
+  Launch L = launch_of(c);
+  int batch = 2;
+  const int64_t limit = (int64_t)a.n_total + 64;
+  int64_t launched = 0;
+  for (;;) {
+    if (launched > limit) return MSC_E_DEVICE;
+    launched += batch;
+    for (int i = 0; i < batch; ++i) {
+      Timed t(c, kcls);
+      launch_level(L, op, mirror, a);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c->pinned_u32, &c->ctrl->done, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (s) s->host_syncs += 1;
+    if (*c->pinned_u32) break;
+    batch = batch < 16 ? batch * 2 : 16;
+  }
+  return MSC_OK;
+}
+
+LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opts* o, double avg) {
+  LevelArgs a{};
+  a.valid = c->valid;
+  a.inq0 = c->inq0;
+  a.inq1 = c->inq1;
+  a.colors = c->colors;
+  a.labels = labels;
+  a.fr0 = c->fr0;
+  a.fr1 = c->fr1;
+  a.ctrl = c->ctrl;
+  a.recs = o->record ? c->recs : nullptr;
+  a.rec_cap = kRecCap;
+  a.n_total = g->n;
+  a.beta = o->beta;
+  a.alpha = o->alpha;
+  a.avg = avg;
+  a.policy = o->policy;
+  a.mlp = (o->policy == MSC_POLICY_LEARNED && c->has_mlp) ? c->mlp : nullptr;
+  return a;
+}
+
+// Device copies of a host graph, staged in the context.
+int stage_graph(msc_ctx* c, const msc_csr* g, msc_csr* d) {
+  if (g->where == MSC_DEVICE) {
+    *d = *g;
+    return MSC_OK;
+  }
+  size_t offb = align256(((size_t)g->n + 1) * 4), adjb = align256((size_t)(g->m > 0 ? g->m : 1) * 4);
+  size_t need = 2 * offb + 2 * adjb;
+  if (need > c->in_bytes) {
+    if (c->in_buf) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->in_buf);
+      c->in_buf = nullptr;
+      c->in_bytes = 0;
+    }
+    HIP_TRY(hipMalloc(&c->in_buf, need));
+    c->in_bytes = need;
+  }
+  char* p = (char*)c->in_buf;
+  uint32_t* oo = (uint32_t*)p;
+  uint32_t* io = (uint32_t*)(p + offb);
+  int32_t* oa = (int32_t*)(p + 2 * offb);
+  int32_t* ia = (int32_t*)(p + 2 * offb + adjb);
+  HIP_TRY(hipMemcpyAsync(oo, g->out_off, ((size_t)g->n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(io, g->in_off, ((size_t)g->n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  if (g->m > 0) {
+    HIP_TRY(hipMemcpyAsync(oa, g->out_adj, (size_t)g->m * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(ia, g->in_adj, (size_t)g->m * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  *d = msc_csr{g->n, g->m, oo, oa, io, ia, MSC_DEVICE};
+  return MSC_OK;
+}
+
+int check_args(const msc_csr* g, const int32_t* labels, const msc_opts* o, bool cc) {
+  if (!g || !o) return MSC_E_INVALID;
+  if (g->n < 0 || g->m < 0) return MSC_E_INVALID;
+  if (g->n > 0 && !labels) return MSC_E_INVALID;
+  if (g->n > 0 && (!g->out_off || !g->in_off)) return MSC_E_INVALID;
+  if (g->m > 0 && (!g->out_adj || !g->in_adj)) return MSC_E_INVALID;
+  if (g->m >= (cc ? (1LL << 31) : (1LL << 32))) return MSC_E_CAPACITY;
+  if (o->trim_level != 1) return MSC_E_INVALID;
+  if (!(o->alpha > 0.0) || o->beta <= 0) return MSC_E_INVALID;
+  if (o->policy != MSC_POLICY_PLAIN && o->policy != MSC_POLICY_LEARNED) return MSC_E_INVALID;
+  if (o->label_mode != MSC_LABELS_CANONICAL && o->label_mode != MSC_LABELS_RAW) return MSC_E_INVALID;
+  if (g->where != MSC_HOST && g->where != MSC_DEVICE) return MSC_E_INVALID;
+  if (o->labels_where != MSC_HOST && o->labels_where != MSC_DEVICE) return MSC_E_INVALID;
+  return MSC_OK;
+}
+
+int host_graph_checks(const msc_csr* g) {
+  // cheap host-side checks of host-resident inputs
+  if (g->where != MSC_HOST || g->n == 0) return MSC_OK;
+  if (g->out_off[0] != 0 || g->in_off[0] != 0) return MSC_E_INVALID;
+  if ((int64_t)g->out_off[g->n] != g->m || (int64_t)g->in_off[g->n] != g->m) return MSC_E_INVALID;
+  return MSC_OK;
+}
+
+int device_validate(msc_ctx* c, const msc_csr* d) {
+  Launch L = launch_of(c);
+  launch_fill_u32(L, c->counters + 2, 0u, 1);
+  launch_validate(L, d->n, d->m, d->out_off, d->out_adj, c->counters + 2);
+  launch_validate(L, d->n, d->m, d->in_off, d->in_adj, c->counters + 2);
+  uint32_t err = 0;
+  int rc = read_u32(c, c->counters + 2, &err);
+  if (rc) return rc;
+  return err ? MSC_E_INVALID : MSC_OK;
+}
+
+msc_ctx* default_ctx(int device, int* rc) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (device < 0) {
+    *rc = MSC_E_INVALID;
+    return nullptr;
+  }
+  if ((size_t)device >= g_default.size()) g_default.resize(device + 1, nullptr);
+  if (!g_default[device]) {
+    msc_ctx* c = nullptr;
+    *rc = msc_ctx_create(device, nullptr, &c);
+    if (*rc) return nullptr;
+    g_default[device] = c;
+  }
+  *rc = MSC_OK;
+  return g_default[device];
+}
+
+enum class Kind { SCC, CC };
+
+// The whole hot path for one graph.
+int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t* labels_out,
+             const msc_opts* o, msc_stats* s) {
+  const bool cc = kind == Kind::CC;
+  int rc = check_args(g_in, labels_out, o, cc);
+  if (rc) return rc;
+  if ((rc = host_graph_checks(g_in))) return rc;
+  if (pivot >= g_in->n || pivot < -1) return MSC_E_INVALID;
+  msc_ctx* c = o->ctx;
+  if (!c) {
+    c = default_ctx(o->device, &rc);
+    if (!c) return rc;
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  auto t0 = std::chrono::steady_clock::now();
+  if (s) std::memset(s, 0, sizeof(*s));
+  c->timing = o->timing != 0;
+  c->ev_used = 0;
+  const int32_t n = g_in->n;
+  if (n == 0) {
+    if (s) s->pivot = -1;
+    return MSC_OK;
+  }
+  const int64_t edges = cc ? 2 * g_in->m : g_in->m;
+  if ((rc = ensure_workspace(c, n, edges))) return rc;
+  msc_csr g;
+  if ((rc = stage_graph(c, g_in, &g))) return rc;
+  if (o->validate && (rc = device_validate(c, &g))) return rc;
+  Launch L = launch_of(c);
+  int32_t* labels = o->labels_where == MSC_DEVICE ? labels_out : c->labels;
+
+  // pivot / average degree (create_csr rules)
+  if (avg < 0.0) avg = cc ? 2.0 * (double)g.m / (double)n : (double)g.m / (double)n;
+  if (pivot < 0) {
+    Timed t(c, MSC_K_OTHER);
+    HIP_TRY(hipMemsetAsync(c->work, 0, 16, c->stream));
+    launch_fill_u32(L, (uint32_t*)(c->work + 1), 0xffffffffu, 2);
+    if (cc)
+      launch_pivot_cc(L, n, g.out_off, g.in_off, c->work);
+    else
+      launch_pivot_scc(L, n, g.out_off, g.in_off, c->work);
+  }
+  // reset loop state, counters
+  HIP_TRY(hipMemsetAsync(c->ctrl, 0, sizeof(Ctrl), c->stream));
+  launch_fill_u32(L, c->counters, 0u, 2);
+  const int64_t nw = ((int64_t)n + 31) / 32;
+
+  // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97)
+  {
+    Timed t(c, MSC_K_TRIM);
+    launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, labels, c->verts0, c->counters + 0);
+  }
+  HIP_TRY(hipGetLastError());
+  if (pivot < 0) {
+    HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->work, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned long long w0, w1;
+    std::memcpy(&w0, c->ctrl_host, 8);
+    std::memcpy(&w1, (char*)c->ctrl_host + 8, 8);
+    if (cc)
+      pivot = w0 ? (int32_t)(0xffffffffu - (uint32_t)(w0 & 0xffffffffull)) : -1;
+    else
+      pivot = (w0 && w1 != ~0ull) ? (int32_t)w1 : -1;
+  }
+  if (s) {
+    s->pivot = pivot;
+    s->avg_degree = avg;
+  }
+  uint32_t nv0 = 0;
+  if ((rc = read_u32(c, c->counters + 0, &nv0))) return rc;
+  if (s) {
+    s->num_trim = n - (int32_t)nv0;
+    s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n + 4.0 * nv0;
+  }
+
+  LevelArgs a = base_args(c, &g, labels, o, avg);
+  int32_t* verts = c->verts0;
+  int32_t* verts_alt = c->verts1;
+  uint32_t* nverts = c->counters + 0;
+  uint32_t* nverts_alt = c->counters + 1;
+  uint32_t nv = nv0;
+
+  // FW-BW (SCC) or the CC search from the pivot; a search the policy abandons leaves
+  // every vertex to colouring
+  bool committed = false;
+  if (pivot >= 0 && nv > 0) {
+    const uint32_t* member = nullptr;
+    if (!cc) {
+      // FW: allow = valid, seen = visA, top-down on out-lists, bottom-up on in-lists
+      launch_fill_u32(L, c->visA, 0u, nw);
+      LevelArgs fw = a;
+      fw.down_off = g.out_off; fw.down_adj = g.out_adj;
+      fw.up_off = g.in_off; fw.up_adj = g.in_adj;
+      fw.allow = c->valid; fw.seen = c->visA;
+      fw.verts = verts; fw.nverts = nverts;
+      launch_search_init(L, fw, pivot, 0);
+      if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s))) return rc;
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
+      if (!c->ctrl_host->abandoned) {
+        // BW: allow = visA, seen = visB, top-down on in-lists, bottom-up on out-lists
+        launch_fill_u32(L, c->visB, 0u, nw);
+        LevelArgs bw = a;
+        bw.down_off = g.in_off; bw.down_adj = g.in_adj;
+        bw.up_off = g.out_off; bw.up_adj = g.out_adj;
+        bw.allow = c->visA; bw.seen = c->visB;
+        bw.verts = verts; bw.nverts = nverts;
+        launch_search_init(L, bw, pivot, 1);
+        if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s))) return rc;
+        if ((rc = read_ctrl(c))) return rc;
+        if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
+        if (!c->ctrl_host->abandoned) member = c->visB;
+      }
+    } else {
+      launch_fill_u32(L, c->visA, 0u, nw);
+      LevelArgs bf = a;
+      bf.down_off = g.out_off; bf.down_adj = g.out_adj;
+      bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
+      bf.up_off = g.out_off; bf.up_adj = g.out_adj;
+      bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
+      bf.allow = c->valid; bf.seen = c->visA;
+      bf.verts = verts; bf.nverts = nverts;
+      launch_search_init(L, bf, pivot, 2);
+      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s))) return rc;
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
+      if (!c->ctrl_host->abandoned) member = c->visA;
+    }
+    if (member) {
+      Timed t(c, MSC_K_COMMIT);
+      launch_fill_u32(L, nverts_alt, 0u, 1);
+      launch_commit(L, verts, nverts, member, c->valid, labels, pivot, verts_alt, nverts_alt);
+      committed = true;
+    }
+  }
+  if (committed) {
+    uint32_t left = 0;
+    if ((rc = read_u32(c, nverts_alt, &left))) return rc;
+    if (s) {
+      s->num_fwbw = (int32_t)(nv - left);
+      s->k[MSC_K_COMMIT].bytes += 4.0 * nv + 1.0 * nv + 4.0 * (nv - left) + 4.0 * left;
+    }
+    std::swap(verts, verts_alt);
+    std::swap(nverts, nverts_alt);
+    nv = left;
+  }
+
+  // colouring
+  if (!cc) {
+    // rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677)
+    while (nv > 0) {
+      LevelArgs p = a;
+      p.down_off = g.out_off; p.down_adj = g.out_adj;
+      p.verts = verts; p.nverts = nverts;
+      {
+        Timed t(c, MSC_K_OTHER);
+        launch_seed_colors(L, false, p);
+      }
+      if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s))) return rc;
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->color_iters += (int32_t)c->ctrl_host->level;
+      LevelArgs r = a;
+      r.down_off = g.in_off; r.down_adj = g.in_adj;
+      r.verts = verts; r.nverts = nverts;
+      {
+        Timed t(c, MSC_K_ROOTS);
+        launch_roots(L, r);
+      }
+      if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s))) return rc;
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->cbw_levels += (int32_t)c->ctrl_host->level;
+      {
+        Timed t(c, MSC_K_ROOTS);
+        launch_fill_u32(L, nverts_alt, 0u, 1);
+        launch_compact(L, verts, nverts, c->valid, verts_alt, nverts_alt);
+      }
+      uint32_t left = 0;
+      if ((rc = read_u32(c, nverts_alt, &left))) return rc;
+      if (s) {
+        s->color_rounds += 1;
+        s->k[MSC_K_ROOTS].bytes += 2.0 * (9.0 * nv) + 4.0 * left;
+        s->host_syncs += 1;
+      }
+      std::swap(verts, verts_alt);
+      std::swap(nverts, nverts_alt);
+      if (left >= nv) return MSC_E_DEVICE;  // every round removes at least one SCC
+      nv = left;
+    }
+  } else if (nv > 0) {
+    // one max-label fixpoint written straight into the labels (multistep/cc_color.cpp:47-175)
+    LevelArgs p = a;
+    p.colors = labels;
+    p.down_off = g.out_off; p.down_adj = g.out_adj;
+    p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
+    p.verts = verts; p.nverts = nverts;
+    {
+      Timed t(c, MSC_K_OTHER);
+      launch_seed_colors(L, true, p);
+    }
+    if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s))) return rc;
+    if ((rc = read_ctrl(c))) return rc;
+    if (s) {
+      s->color_iters = (int32_t)c->ctrl_host->level;
+      s->color_rounds = 1;
+    }
+  }
+
+  if (o->label_mode == MSC_LABELS_CANONICAL) {
+    Timed t(c, MSC_K_OTHER);
+    launch_canon(L, n, labels, c->colors);
+  }
+  if (o->labels_where == MSC_HOST)
+    HIP_TRY(hipMemcpyAsync(labels_out, labels, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipGetLastError());
+  if ((rc = read_ctrl(c))) return rc;
+  c->last_nrec = c->ctrl_host->nrec < kRecCap ? c->ctrl_host->nrec : kRecCap;
+  auto t1 = std::chrono::steady_clock::now();
+  if (s) {
+    s->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    s->bu_levels = (int32_t)c->ctrl_host->bu_levels;
+    for (int op = 0; op < 5; ++op) {
+      int cls = op == OP_PUSH ? MSC_K_COLOR_PROP : op == OP_CBW ? MSC_K_COLOR_BW : MSC_K_LEVEL;
+      s->k[cls].bytes += (double)c->ctrl_host->bytes_op[op];
+    }
+    ev_collect(c, s);
+    s->bytes = 0;
+    for (int k = 0; k < MSC_K_COUNT; ++k) s->bytes += s->k[k].bytes;
+  } else {
+    c->ev_used = 0;
+  }
+  return MSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* msc_version(void) { return "msc 0.1 (gfx950)"; }
+
+int msc_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int ok = 0;
+  for (int d = 0; d < count; ++d) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, d) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) ++ok;
+  }
+  return ok;
+}
+
+void msc_opts_default(msc_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->alpha = 15.0;
+  o->beta = 25;
+  o->trim_level = 1;
+  o->policy = MSC_POLICY_PLAIN;
+  o->label_mode = MSC_LABELS_CANONICAL;
+  o->labels_where = MSC_HOST;
+  o->device = 0;
+}
+
+int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
+  if (!out) return MSC_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return MSC_E_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return MSC_E_DEVICE;
+  HIP_TRY(hipSetDevice(device));
+  msc_ctx* c = new (std::nothrow) msc_ctx();
+  if (!c) return MSC_E_OOM;
+  c->device = device;
+  c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  c->grid = c->cus * 4;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return MSC_E_DEVICE;
+    }
+    c->own_stream = true;
+  }
+  if (hipHostMalloc((void**)&c->ctrl_host, sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess) {
+    msc_ctx_destroy(c);
+    return MSC_E_OOM;
+  }
+  *out = c;
+  return MSC_OK;
+}
+
+int msc_ctx_destroy(msc_ctx* c) {
+  if (!c) return MSC_E_INVALID;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->in_buf) (void)hipFree(c->in_buf);
+  for (Ev& e : c->events) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
+  if (c->pinned_u32) (void)hipHostFree(c->pinned_u32);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    for (auto& d : g_default)
+      if (d == c) d = nullptr;
+  }
+  delete c;
+  return MSC_OK;
+}
+
+int msc_ctx_levels(msc_ctx* c, msc_level_rec* out, int32_t cap, int32_t* count) {
+  if (!c || !count || cap < 0 || (cap > 0 && !out)) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  uint32_t k = c->last_nrec < (uint32_t)cap ? c->last_nrec : (uint32_t)cap;
+  *count = (int32_t)c->last_nrec;
+  if (k == 0 || !c->recs) return MSC_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(out, c->recs, (size_t)k * sizeof(msc_level_rec), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MSC_OK;
+}
+
+int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
+                const msc_opts* opts, msc_stats* stats) {
+  try {
+    return run_path(Kind::SCC, g, max_deg_vert, avg_degree, labels, opts, stats);
+  } catch (const std::bad_alloc&) {
+    return MSC_E_OOM;
+  } catch (...) {
+    return MSC_E_DEVICE;
+  }
+}
+
+int msc_cc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
+               const msc_opts* opts, msc_stats* stats) {
+  try {
+    return run_path(Kind::CC, g, max_deg_vert, avg_degree, labels, opts, stats);
+  } catch (const std::bad_alloc&) {
+    return MSC_E_OOM;
+  } catch (...) {
+    return MSC_E_DEVICE;
+  }
+}
+
+int msc_scc_pivot(const msc_csr* g, const msc_opts* o, int32_t* max_deg_vert, double* avg_degree) {
+  if (!g || !o || !max_deg_vert || !avg_degree || g->n < 0) return MSC_E_INVALID;
+  if (g->n == 0) {
+    *max_deg_vert = -1;
+    *avg_degree = 0.0;
+    return MSC_OK;
+  }
+  int rc = MSC_OK;
+  msc_ctx* c = o->ctx ? o->ctx : default_ctx(o->device, &rc);
+  if (!c) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  if ((rc = ensure_workspace(c, g->n, g->m))) return rc;
+  msc_csr d;
+  if ((rc = stage_graph(c, g, &d))) return rc;
+  Launch L = launch_of(c);
+  HIP_TRY(hipMemsetAsync(c->work, 0, 16, c->stream));
+  launch_fill_u32(L, (uint32_t*)(c->work + 1), 0xffffffffu, 2);
+  launch_pivot_scc(L, g->n, d.out_off, d.in_off, c->work);
+  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->work, 16, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  unsigned long long w0, w1;
+  std::memcpy(&w0, c->ctrl_host, 8);
+  std::memcpy(&w1, (char*)c->ctrl_host + 8, 8);
+  *max_deg_vert = (w0 && w1 != ~0ull) ? (int32_t)w1 : -1;
+  *avg_degree = (double)g->m / (double)g->n;
+  return MSC_OK;
+}
+
+int msc_csr_to_device(msc_ctx* c, const msc_csr* h, msc_csr* d) {
+  if (!c || !h || !d || h->n < 0 || h->m < 0 || h->where != MSC_HOST) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  size_t offb = ((size_t)h->n + 1) * 4, adjb = (size_t)(h->m > 0 ? h->m : 1) * 4;
+  void *oo = nullptr, *io = nullptr, *oa = nullptr, *ia = nullptr;
+  if (hipMalloc(&oo, offb) != hipSuccess || hipMalloc(&io, offb) != hipSuccess ||
+      hipMalloc(&oa, adjb) != hipSuccess || hipMalloc(&ia, adjb) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(oo); (void)hipFree(io); (void)hipFree(oa); (void)hipFree(ia);
+    return MSC_E_OOM;
+  }
+  HIP_TRY(hipMemcpyAsync(oo, h->out_off, offb, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(io, h->in_off, offb, hipMemcpyHostToDevice, c->stream));
+  if (h->m > 0) {
+    HIP_TRY(hipMemcpyAsync(oa, h->out_adj, (size_t)h->m * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(ia, h->in_adj, (size_t)h->m * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *d = msc_csr{h->n, h->m, (const uint32_t*)oo, (const int32_t*)oa, (const uint32_t*)io, (const int32_t*)ia,
+               MSC_DEVICE};
+  return MSC_OK;
+}
+
+int msc_csr_device_free(msc_ctx* c, msc_csr* d) {
+  if (!c || !d || d->where != MSC_DEVICE) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree((void*)d->out_off);
+  (void)hipFree((void*)d->in_off);
+  (void)hipFree((void*)d->out_adj);
+  (void)hipFree((void*)d->in_adj);
+  std::memset(d, 0, sizeof(*d));
+  return MSC_OK;
+}
+
+}  // extern "C"

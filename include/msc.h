@@ -1,0 +1,248 @@
+/*
+ * msc — Multistep strongly / weakly connected components on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libmsc.so.  Plain pointers and sizes only; every entry point returns an int
+ * status (MSC_OK or a negative MSC_E_* code) and never throws across the boundary.
+ *
+ * The reference this replaces has no FFI: its path sits behind plain C++ functions that
+ * main() calls directly.  Each declaration below names the reference function it stands
+ * in for (file:line in HPCGraphAnalysis/Connectivity).
+ *
+ * Graph layout (same as the reference's `graph` struct, multistep/scc_main.cpp:75-86):
+ * n vertices with int32 ids, CSR offsets uint32[n+1] (`unsigned* out_degree_list`),
+ * adjacency int32[m] (`int* out_array`), and the transposed CSR (`in_degree_list`,
+ * `in_array`).  m must be below 2^32 (2^31 for msc_cc_run, which walks out+in lists).
+ */
+#ifndef MSC_H
+#define MSC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define MSC_OK 0
+#define MSC_E_INVALID (-1)  /* bad argument or malformed graph */
+#define MSC_E_OOM (-2)      /* host or device allocation failed */
+#define MSC_E_DEVICE (-3)   /* HIP runtime error or no usable gfx950 device */
+#define MSC_E_CAPACITY (-4) /* a fixed-capacity buffer would overflow */
+#define MSC_E_IO (-5)       /* file could not be read or written */
+
+/* ---- enums ----------------------------------------------------------------------- */
+enum msc_where { MSC_HOST = 0, MSC_DEVICE = 1 };
+enum msc_policy {
+  MSC_POLICY_PLAIN = 0,   /* the reference's ALPHA/BETA switch rule */
+  MSC_POLICY_LEARNED = 1  /* the MLP loaded with msc_policy_set_weights */
+};
+enum msc_label_mode {
+  MSC_LABELS_CANONICAL = 0, /* each component labelled by its minimum vertex id */
+  MSC_LABELS_RAW = 1        /* the reference's per-phase labels (see msc_scc_run) */
+};
+
+/* ---- graph ----------------------------------------------------------------------- */
+/* Reference: `graph` (multistep/scc_main.cpp:75-86); Kokkos views n/out_degree_list/
+ * out_array/in_degree_list/in_array (kokkos_scc/scc_run.cpp:48-52). */
+typedef struct msc_csr {
+  int32_t n;
+  int64_t m;
+  const uint32_t* out_off; /* n+1 */
+  const int32_t* out_adj;  /* m   */
+  const uint32_t* in_off;  /* n+1 */
+  const int32_t* in_adj;   /* m   */
+  int32_t where;           /* MSC_HOST or MSC_DEVICE: where all four arrays live */
+} msc_csr;
+
+/* Host graph owned by the library (built by the generators and the edge-list reader). */
+typedef struct msc_host_graph {
+  int32_t n;
+  int64_t m;
+  uint32_t* out_off;
+  int32_t* out_adj;
+  uint32_t* in_off;
+  int32_t* in_adj;
+  int32_t max_deg_vert; /* create_csr's pivot (multistep/scc_main.cpp:169-183), -1 if none */
+  double avg_degree;    /* m / n */
+} msc_host_graph;
+
+/* ---- options and statistics ------------------------------------------------------ */
+typedef struct msc_ctx msc_ctx; /* per-device, per-stream workspace; see msc_ctx_create */
+
+typedef struct msc_opts {
+  double alpha;         /* ALPHA = 15.0 (multistep/scc_main.cpp:64) */
+  int32_t beta;         /* BETA  = 25   (multistep/scc_main.cpp:65) */
+  int32_t trim_level;   /* TRIM_LEVEL = 1 (multistep/scc_main.cpp:61); only 1 is supported */
+  int32_t policy;       /* enum msc_policy */
+  int32_t label_mode;   /* enum msc_label_mode */
+  int32_t labels_where; /* MSC_HOST or MSC_DEVICE: where the `labels` output lives */
+  int32_t device;       /* HIP device ordinal (used when ctx is NULL) */
+  int32_t validate;     /* nonzero: check offsets and ids on device before running */
+  int32_t timing;       /* nonzero: per-kernel HIP-event timing into msc_stats.k[] */
+  int32_t record;       /* nonzero: keep per-level records (msc_ctx_levels) */
+  msc_ctx* ctx;         /* NULL: the library's default context for `device` */
+} msc_opts;
+
+/* Kernel classes timed when msc_opts.timing is set. */
+enum msc_kernel {
+  MSC_K_TRIM = 0,   /* degree trim + survivor compaction */
+  MSC_K_LEVEL,      /* one BFS level (top-down or bottom-up), FW/BW/CC */
+  MSC_K_COMMIT,     /* FW-BW / CC-BFS label commit + compaction */
+  MSC_K_COLOR_PROP, /* one max-label push sweep */
+  MSC_K_COLOR_BW,   /* one level of the per-colour backward search */
+  MSC_K_ROOTS,      /* colour roots + compaction */
+  MSC_K_OTHER,      /* init, pivot, canonicalisation, small fills */
+  MSC_K_COUNT
+};
+
+typedef struct msc_kstat {
+  int64_t launches;
+  double ms;    /* summed HIP-event durations */
+  double bytes; /* summed algorithmic bytes (unit costs in DESIGN.md) */
+} msc_kstat;
+
+typedef struct msc_stats {
+  double wall_ms;      /* host wall time of the call, stream-synchronised */
+  int32_t pivot;       /* pivot actually used (-1: FW-BW skipped) */
+  double avg_degree;   /* average degree actually used */
+  int32_t num_trim;    /* vertices removed by trim */
+  int32_t num_fwbw;    /* vertices in the pivot's component */
+  int32_t num_color;   /* components found by colouring */
+  int32_t color_rounds;
+  int32_t color_iters; /* push sweeps summed over rounds */
+  int32_t fw_levels, bw_levels, bu_levels, cbw_levels;
+  int32_t host_syncs;  /* host round trips taken by the level loops */
+  double bytes;        /* algorithmic bytes over all kernels */
+  msc_kstat k[MSC_K_COUNT];
+} msc_stats;
+
+/* Defaults equal the reference's compile-time settings. */
+void msc_opts_default(msc_opts* o);
+
+/* ---- context --------------------------------------------------------------------- */
+/* A context owns a HIP stream on `device` (or borrows `stream`, a hipStream_t, when not
+ * NULL), the device workspace (grown on demand, reused across calls) and the policy
+ * weights.  Calls on one context are serialised by the caller; distinct contexts may be
+ * used from distinct threads. */
+int msc_ctx_create(int32_t device, void* stream, msc_ctx** out);
+int msc_ctx_destroy(msc_ctx* ctx);
+/* Per-level records of the last run with opts.record set (see msc_level_rec). */
+typedef struct msc_level_rec {
+  int32_t phase;      /* 0 FW, 1 BW, 2 CC */
+  int32_t level;      /* 1-based */
+  int32_t bottom_up;  /* direction this level ran in */
+  int32_t next_size;  /* vertices discovered */
+  uint32_t q_size;    /* frontier expanded (top-down) */
+  uint32_t q_edges;   /* its list lengths summed */
+  uint32_t next_edges;/* list lengths of the discovered vertices */
+  uint32_t examined;  /* bottom-up candidates examined */
+  uint64_t inspected; /* bottom-up edges inspected */
+  int64_t found;      /* discovered so far, root excluded */
+  int32_t n_total;
+  int32_t num_valid;
+  int32_t action;     /* direction chosen for the next level (0 TD, 1 BU, 2 stop) */
+  int32_t pad;
+} msc_level_rec;
+int msc_ctx_levels(msc_ctx* ctx, msc_level_rec* out, int32_t cap, int32_t* count);
+
+/* ---- the hot path ---------------------------------------------------------------- */
+/* Strongly connected components.
+ * Replaces run_scc(graph&, int*& scc_maps, int max_deg_vert, double avg_degree,
+ * int vert_cutoff) (multistep/scc_main.cpp:210) and Kokkos do_run<ExecSpace>(...)
+ * (kokkos_scc/scc_run.cpp:48-59).  Phases: trim (scc_trim.cpp:89-139) -> FW-BW from the
+ * pivot (scc_fwbw.cpp:469-513) -> colouring until nothing is left (the Kokkos loop,
+ * kokkos_scc/scc_color.cpp:553; the reference's CPU serial tail is not used).
+ *   max_deg_vert: pivot; -1 computes create_csr's rule on device.
+ *   avg_degree:   < 0 computes m/n.
+ *   labels:       n int32 written by the call (caller-owned; host or device per
+ *                 opts.labels_where).  MSC_LABELS_RAW gives the reference's raw labels
+ *                 except for its order-dependent Tarjan tail: trimmed vertex -> itself,
+ *                 pivot's SCC -> pivot, every other SCC -> its maximum vertex id.
+ *   stats:        may be NULL.
+ * Graph arrays may be on host (copied in) or device (used in place). */
+int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
+                const msc_opts* opts, msc_stats* stats);
+
+/* Weakly connected components of a digraph (connected components of its mirror).
+ * Replaces run_cc(graph&, int*& cc_maps, int max_deg_vert, double avg_degree)
+ * (multistep/cc_main.cpp:184) over the mirrored edge list read_edge builds
+ * (cc_main.cpp:102-118); out and in lists are walked together instead of being mirrored.
+ *   max_deg_vert: -1 computes cc_main.cpp:148-160's rule (first max out+in degree).
+ *   avg_degree:   < 0 computes 2m/n (the mirrored graph's average).
+ * MSC_LABELS_RAW: isolated vertex -> itself, pivot's component -> pivot, others -> max id. */
+int msc_cc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
+               const msc_opts* opts, msc_stats* stats);
+
+/* Pivot and average degree on device.  Replaces the loop inside create_csr
+ * (multistep/scc_main.cpp:169-183; kokkos_scc/scc_main.cpp:228-247). */
+int msc_scc_pivot(const msc_csr* g, const msc_opts* opts, int32_t* max_deg_vert,
+                  double* avg_degree);
+
+/* ---- verification ---------------------------------------------------------------- */
+typedef struct msc_verify {
+  int32_t num_components;
+  int32_t nontrivial; /* components with more than one vertex */
+  int32_t max_size;
+  int32_t unassigned; /* labels < 0 */
+} msc_verify;
+
+/* Replaces scc_verify(graph&, int*) (multistep/scc_verify.cpp:47-77) and
+ * scc_verify(int, int*) (kokkos_scc/scc_verify.cpp:47-76): counts only, host labels.
+ * The reference prints "Num SCCs: %d, Nontrivial: %d, Max SCC: %d, Unassigned %d";
+ * msc_verify_format writes that line. */
+int msc_scc_verify(int32_t n, const int32_t* labels, msc_verify* out);
+/* Replaces cc_verify (multistep/cc_verify.cpp:47-74). */
+int msc_cc_verify(int32_t n, const int32_t* labels, msc_verify* out);
+/* kind 0: the SCC line, 1: the CC line ("Num CCs: %d, Max CC: %d, Unassigned %d"). */
+int msc_verify_format(const msc_verify* v, int32_t kind, char* buf, int32_t cap);
+
+/* Relabel every component to its minimum vertex id, host labels (no reference
+ * counterpart: the reference's raw labels differ by phase, so parity is stated on this
+ * canonical form). */
+int msc_canonicalize(int32_t n, int32_t* labels);
+
+/* ---- graph construction (host) --------------------------------------------------- */
+/* read_edge + create_csr (multistep/scc_main.cpp:94-191): "n m" then m lines "u v".
+ * Keeps duplicates and self-loops, adjacency in input order, like the reference. */
+int msc_graph_read_edge_list(const char* path, msc_host_graph** out);
+/* create_csr from arrays (multistep/scc_main.cpp:127-191).  flags: MSC_BUILD_* */
+#define MSC_BUILD_DROP_SELF_LOOPS 1
+#define MSC_BUILD_DEDUPE 2 /* also sorts every adjacency list */
+int msc_graph_from_edges(int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                         int32_t flags, msc_host_graph** out);
+/* output_scc (multistep/scc_main.cpp:193-202): one label per line. */
+int msc_write_labels(const char* path, int32_t n, const int32_t* labels);
+void msc_graph_free(msc_host_graph* g);
+
+/* Synthetic inputs (no reference counterpart; deterministic for a seed, independent of
+ * the thread count).  Self-loops and duplicate edges are removed, lists sorted. */
+/* R-MAT: 2^scale vertices, edge_factor*2^scale draws with quadrant probabilities
+ * a, b, c (d = 1-a-b-c), vertex ids permuted when permute != 0. */
+int msc_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                 uint64_t seed, int32_t permute, msc_host_graph** out);
+/* rows x cols grid: each grid edge kept in one random direction with p_one, in both with
+ * p_both; plus round(arc_frac * n) uniform random arcs. */
+int msc_gen_grid(int32_t rows, int32_t cols, double p_one, double p_both, double arc_frac,
+                 uint64_t seed, msc_host_graph** out);
+/* Planted SCCs: component sizes drawn from a Zipf(s) law (s > 1) truncated at max_size,
+ * each component a directed cycle plus extra_per_vertex random internal arcs, joined by
+ * dag_arcs random arcs that respect a random component order; ids permuted.  The SCC
+ * partition is the planted one; comp_out (may be NULL, n entries) receives it. */
+int msc_gen_planted(int32_t n, double zipf_s, int32_t max_size, double extra_per_vertex,
+                    int64_t dag_arcs, uint64_t seed, msc_host_graph** out, int32_t* comp_out);
+
+/* ---- device graphs --------------------------------------------------------------- */
+/* Copy a host graph into device memory of the context's device; fills *dev (where =
+ * MSC_DEVICE).  Free with msc_csr_device_free. */
+int msc_csr_to_device(msc_ctx* ctx, const msc_csr* host, msc_csr* dev);
+int msc_csr_device_free(msc_ctx* ctx, msc_csr* dev);
+
+/* ---- version / build info -------------------------------------------------------- */
+const char* msc_version(void);
+/* Number of visible gfx950 devices (0 when none). */
+int msc_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSC_H */

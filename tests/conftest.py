@@ -1,0 +1,37 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (run on the MI355X box)")
+    config.addinivalue_line("markers", "slow: large inputs (seconds of CPU work)")
+
+
+@pytest.fixture(scope="session")
+def msc():
+    import connectivity_amd
+
+    return connectivity_amd
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+
+    oracle.lib()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx(msc):
+    if msc.device_count() < 1:
+        pytest.fail("no gfx950 device visible: GPU tests must run on the MI355X box")
+    ctx = msc.Context(0)
+    yield ctx
+    ctx.close()
