@@ -23,7 +23,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 __all__ = [
-    "MscError", "Graph", "DeviceGraph", "Context", "Stats", "LevelRecord",
+    "MscError", "Graph", "DeviceGraph", "DeviceLabels", "Context", "Stats", "LevelRecord",
     "read_edge", "create_csr", "gen_rmat", "gen_grid", "gen_planted",
     "run_scc", "run_cc", "scc_pivot", "scc_verify", "cc_verify", "verify_line",
     "canonicalize", "write_labels", "device_count", "lib_path",
@@ -146,6 +146,11 @@ def _lib():
         "msc_csr_device_free": (C.c_int, [P, C.POINTER(_Csr)]),
         "msc_version": (C.c_char_p, []),
         "msc_device_count": (C.c_int, []),
+        "msc_device_alloc": (C.c_int, [P, C.c_int64, C.POINTER(P)]),
+        "msc_device_free": (C.c_int, [P, P]),
+        "msc_copy_to_host": (C.c_int, [P, P, P, C.c_int64]),
+        "msc_copy_to_device": (C.c_int, [P, P, P, C.c_int64]),
+        "msc_ctx_sync": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -336,6 +341,31 @@ class DeviceGraph:
             pass
 
 
+class DeviceLabels:
+    """n int32 labels kept in device memory (runs write them with labels_where = device)."""
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx, self.n = ctx, n
+        self.ptr = C.c_void_p()
+        _check(_lib().msc_device_alloc(ctx.handle, 4 * max(n, 1), C.byref(self.ptr)), "msc_device_alloc")
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        _check(_lib().msc_copy_to_host(self.ctx.handle, _ptr(out), self.ptr, 4 * self.n), "msc_copy_to_host")
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ctx.handle:
+            _lib().msc_device_free(self.ctx.handle, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 @dataclass
 class LevelRecord:
     phase: int
@@ -398,16 +428,25 @@ def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alph
 def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
          out):
     n = g.n
-    if out is None:
-        out = np.empty(n, dtype=np.int32)
-    elif out.dtype != np.int32 or out.shape != (n,) or not out.flags.c_contiguous:
-        raise ValueError("out must be a contiguous int32 array of length n")
-    o = _opts(policy, labels, timing, record, validate, ctx, alpha=alpha, beta=beta)
+    where = HOST
+    if isinstance(out, DeviceLabels):
+        if out.n != n:
+            raise ValueError("DeviceLabels size differs from the graph")
+        if ctx is None or out.ctx is not ctx:
+            raise ValueError("DeviceLabels must belong to the context used for the run")
+        where, dst = DEVICE, out.ptr.value
+    else:
+        if out is None:
+            out = np.empty(n, dtype=np.int32)
+        elif out.dtype != np.int32 or out.shape != (n,) or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous int32 array of length n")
+        dst = _ptr(out) if n else None
+    o = _opts(policy, labels, timing, record, validate, ctx, labels_where=where, alpha=alpha, beta=beta)
     st = _Stats()
     csr = g.csr()
     piv = -1 if max_deg_vert is None else int(max_deg_vert)
     avg = -1.0 if avg_degree is None else float(avg_degree)
-    rc = getattr(_lib(), fn_name)(C.byref(csr), piv, avg, _ptr(out) if n else None, C.byref(o), C.byref(st))
+    rc = getattr(_lib(), fn_name)(C.byref(csr), piv, avg, dst, C.byref(o), C.byref(st))
     _check(rc, fn_name)
     return out, Stats._from(st)
 

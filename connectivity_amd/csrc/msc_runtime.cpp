@@ -734,6 +734,49 @@ int msc_csr_to_device(msc_ctx* c, const msc_csr* h, msc_csr* d) {
   return MSC_OK;
 }
 
+int msc_device_alloc(msc_ctx* c, int64_t bytes, void** out) {
+  if (!c || !out || bytes < 0) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  *out = nullptr;
+  HIP_TRY(hipMalloc(out, (size_t)(bytes > 0 ? bytes : 1)));
+  return MSC_OK;
+}
+
+int msc_device_free(msc_ctx* c, void* p) {
+  if (!c) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (p) HIP_TRY(hipFree(p));
+  return MSC_OK;
+}
+
+int msc_copy_to_host(msc_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0 || (bytes > 0 && (!dst || !src))) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MSC_OK;
+}
+
+int msc_copy_to_device(msc_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0 || (bytes > 0 && (!dst || !src))) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MSC_OK;
+}
+
+int msc_ctx_sync(msc_ctx* c) {
+  if (!c) return MSC_E_INVALID;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MSC_OK;
+}
+
 int msc_csr_device_free(msc_ctx* c, msc_csr* d) {
   if (!c || !d || d->where != MSC_DEVICE) return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);

@@ -236,6 +236,14 @@ int msc_gen_planted(int32_t n, double zipf_s, int32_t max_size, double extra_per
  * MSC_DEVICE).  Free with msc_csr_device_free. */
 int msc_csr_to_device(msc_ctx* ctx, const msc_csr* host, msc_csr* dev);
 int msc_csr_device_free(msc_ctx* ctx, msc_csr* dev);
+/* Device buffers for labels kept in HBM (opts.labels_where = MSC_DEVICE) and the copies
+ * in and out of them, ordered on the context's stream. */
+int msc_device_alloc(msc_ctx* ctx, int64_t bytes, void** out);
+int msc_device_free(msc_ctx* ctx, void* p);
+int msc_copy_to_host(msc_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int msc_copy_to_device(msc_ctx* ctx, void* dst, const void* src, int64_t bytes);
+/* Waits for all work queued on the context's stream. */
+int msc_ctx_sync(msc_ctx* ctx);
 
 /* ---- version / build info -------------------------------------------------------- */
 const char* msc_version(void);
