@@ -10,17 +10,21 @@
 namespace msc {
 
 constexpr int kBlock = 256;                    // threads per workgroup: 4 wave64s
-constexpr uint32_t kChunk = 2048;              // edges per edge-balanced work item
+constexpr uint32_t kChunk = 1024;              // edges per edge-balanced work item
 constexpr int kPer = (int)(kChunk / kBlock);   // edges per thread per work item
+constexpr uint32_t kStage = 2048;              // claims staged in LDS per workgroup
+constexpr uint32_t kTile = kBlock * 16;        // vertices per ordered-compaction tile
 constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;
 // feature standardisation (mean, std) follows the weights
 constexpr int kMlpBlob = kMlpParams + 2 * kMlpIn;
 
 // What one level kernel does with each edge it walks.
+// Reach ops keep one "claimable" bitmap, cand = allow & ~visited: a top-down edge costs a
+// single probe, and a claim is one atomicAnd that clears the bit.
 enum Op : int {
-  OP_FW = 0,   // forward reach:  allow = valid, seen = fw   (top-down on out-lists)
-  OP_BW = 1,   // backward reach: allow = fw,    seen = bw   (top-down on in-lists)
+  OP_FW = 0,   // forward reach:  allow = valid, top-down on out-lists, bottom-up on in-lists
+  OP_BW = 1,   // backward reach: allow = fw set, top-down on in-lists, bottom-up on out-lists
   OP_CC = 2,   // undirected reach over out+in lists
   OP_PUSH = 3, // max-label push (colour propagation) along down lists
   OP_CBW = 4,  // per-colour backward search: claim valid in-neighbours of equal colour
@@ -69,8 +73,8 @@ struct LevelArgs {
   const int32_t* up_adj;
   const uint32_t* up_off2;
   const int32_t* up_adj2;
-  uint32_t* allow;            // bitmap
-  uint32_t* seen;             // bitmap
+  uint32_t* allow;            // bitmap: vertices the search may visit
+  uint32_t* cand;             // bitmap: allowed and not yet visited (cleared on claim)
   uint32_t* valid;            // bitmap (colour ops)
   uint32_t* inq0;             // colour worklist membership, buffer 0 / 1
   uint32_t* inq1;
@@ -101,6 +105,9 @@ void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_of
                  const uint32_t* in_off, uint32_t* valid, int32_t* labels, int32_t* verts,
                  uint32_t* nverts);
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
+// out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
+void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
+                   int64_t words);
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);
 void launch_commit(const Launch& L, const int32_t* verts, const uint32_t* nverts,
                    const uint32_t* member, uint32_t* valid, int32_t* labels, int32_t label,

@@ -5,13 +5,17 @@
 //    vertex, 2 MB at 16.8M vertices, so random probes are served from L2 / MALL.  Claims
 //    are atomicOr / atomicAnd test-and-set on the containing word, so a frontier never
 //    holds a vertex twice.
-//  * Frontiers are queues built with one 64-bit atomic per wave that reserves both the
-//    queue slots and the slots' edge offsets ((count << 32) | edge sum).  The enqueuing
-//    lane also records, for every kChunk-edge chunk that begins inside its vertex's
-//    list, which queue slot owns that chunk ("starts").  The next top-down level then
-//    gives each workgroup one chunk of exactly kChunk edges: it maps every edge to its
-//    source vertex with an LDS max-scan, and reads the adjacency fully coalesced.  Hubs
-//    (the s24 pivot has ~240k out-edges) are spread over ~120 workgroups this way.
+//  * A frontier is a queue plus, per slot, the exclusive prefix of the slots' list lengths
+//    ("edge offsets").  Claims are staged per workgroup in LDS; when the stage fills (and
+//    at the end of the launch) the workgroup scans the staged lengths and reserves queue
+//    slots and edge offsets together with ONE 64-bit atomic ((count << 32) | edge sum).
+//    A single frontier counter hit once per wave costs milliseconds at s24 (same-address
+//    atomics serialise in L2); once per ~3k claims it is noise.
+//  * While writing a slot, the workgroup also records, for every kChunk-edge chunk that
+//    begins inside that vertex's list, which slot owns the chunk ("starts").  A top-down
+//    level then gives each workgroup whole chunks of exactly kChunk edges: it maps every
+//    edge to its source vertex with an LDS max-scan and reads the adjacency coalesced.
+//    Hubs (the s24 pivot has ~240k out-edges) spread over hundreds of workgroups.
 //  * Bottom-up levels sweep the candidate list one vertex per lane and stop at the first
 //    visited parent.
 //  * Each level is one launch on a fixed grid.  The launch reads its sizes and direction
@@ -30,10 +34,9 @@ namespace {
 // wave64 and bitmap helpers
 
 __device__ __forceinline__ int lane() { return (int)__lane_id(); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
 
-__device__ __forceinline__ unsigned long long lanes_below() {
-  return (1ull << lane()) - 1ull;
-}
+__device__ __forceinline__ unsigned long long lanes_below() { return (1ull << lane()) - 1ull; }
 
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
 #pragma unroll
@@ -50,6 +53,12 @@ __device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
     uint32_t t = __shfl_up(x, o, 64);
     if (lane() >= o) x = max(x, t);
   }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor(x, o, 64));
   return x;
 }
 
@@ -72,12 +81,6 @@ __device__ __forceinline__ unsigned long long wave_max64(unsigned long long x) {
     x = y > x ? y : x;
   }
   return x;
-}
-
-__device__ __forceinline__ unsigned long long bcast64(unsigned long long x, int src) {
-  uint32_t lo = __shfl((uint32_t)x, src, 64);
-  uint32_t hi = __shfl((uint32_t)(x >> 32), src, 64);
-  return ((unsigned long long)hi << 32) | lo;
 }
 
 __device__ __forceinline__ bool bit_test(const uint32_t* b, uint32_t v) {
@@ -104,38 +107,91 @@ __device__ __forceinline__ uint32_t list_len(const uint32_t* off, const uint32_t
   return d;
 }
 
-// Wave-cooperative append of the lanes with `take` to frontier `nx`.  Must be reached by
-// every lane of the wave.  One 64-bit atomic reserves queue slots and edge offsets.
-__device__ __forceinline__ void enqueue(bool take, uint32_t w, uint32_t deg, const Frontier& nx,
-                                        Ctrl* c) {
+// ------------------------------------------------------------------------------------
+// per-workgroup staging of claims
+
+struct Stage {
+  int32_t v[kStage];
+  uint32_t d[kStage];  // list length, then (after the scan) local edge offset
+  uint32_t part[kBlock / 64];
+  uint32_t cnt;
+  uint32_t base_hi, base_lo, total;
+};
+
+// Wave-cooperative: appends the lanes with `take` to the stage (one LDS atomic per wave).
+// The caller guarantees room (stage_flush runs before the stage can overflow).
+__device__ __forceinline__ void stage_push(Stage& S, bool take, uint32_t w, uint32_t deg) {
   unsigned long long mask = __ballot(take);
   if (mask == 0ull) return;
-  uint32_t d = take ? deg : 0u;
-  uint32_t incl = wave_incl_sum(d);
-  uint32_t total = __shfl(incl, 63, 64);
-  uint32_t cnt = (uint32_t)__popcll(mask);
   int leader = __ffsll((long long)mask) - 1;
-  unsigned long long base = 0ull;
-  if (lane() == leader)
-    base = atomicAdd(&c->next_packed, ((unsigned long long)cnt << 32) | (unsigned long long)total);
-  base = bcast64(base, leader);
+  uint32_t at = 0;
+  if (lane() == leader) at = atomicAdd(&S.cnt, (uint32_t)__popcll(mask));
+  at = __shfl(at, leader, 64);
   if (take) {
-    uint32_t pos = (uint32_t)(base >> 32) + (uint32_t)__popcll(mask & lanes_below());
-    uint32_t off = (uint32_t)base + incl - d;
-    nx.q[pos] = (int32_t)w;
-    nx.qo[pos] = off;
-    uint64_t end = (uint64_t)off + d;
-    for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch)
-      nx.starts[ch] = pos;
+    uint32_t slot = at + (uint32_t)__popcll(mask & lanes_below());
+    S.v[slot] = (int32_t)w;
+    S.d[slot] = deg;
   }
 }
 
+// Workgroup-wide (every thread calls it): when more than `keep_below` claims are staged,
+// reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
+// offsets and chunk starts to frontier `nx`.
+__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below) {
+  __syncthreads();
+  const uint32_t n = S.cnt;
+  __syncthreads();  // no wave may push again before every wave has read the count
+  if (n <= keep_below || n == 0u) return;  // uniform
+  constexpr int per = (int)(kStage / kBlock);
+  const int t = threadIdx.x;
+  uint32_t loc[per];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int r = 0; r < per; ++r) {
+    uint32_t i = (uint32_t)t * per + r;
+    loc[r] = sum;
+    sum += i < n ? S.d[i] : 0u;
+  }
+  uint32_t incl = wave_incl_sum(sum);
+  if (lane() == 63) S.part[wave_id()] = incl;
+  __syncthreads();
+  uint32_t carry = 0, total = 0;
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave_id()) carry += S.part[w];
+    total += S.part[w];
+  }
+  const uint32_t excl = incl - sum + carry;
+#pragma unroll
+  for (int r = 0; r < per; ++r) {
+    uint32_t i = (uint32_t)t * per + r;
+    if (i < n) S.d[i] = excl + loc[r];
+  }
+  if (t == 0) {
+    unsigned long long b = atomicAdd(&c->next_packed, ((unsigned long long)n << 32) | (unsigned long long)total);
+    S.base_hi = (uint32_t)(b >> 32);
+    S.base_lo = (uint32_t)b;
+    S.total = total;
+  }
+  __syncthreads();
+  const uint32_t bh = S.base_hi, bl = S.base_lo;
+  for (uint32_t i = (uint32_t)t; i < n; i += kBlock) {
+    uint32_t lo = S.d[i];
+    uint32_t hi = i + 1 < n ? S.d[i + 1] : total;
+    uint32_t pos = bh + i, off = bl + lo;
+    nx.q[pos] = S.v[i];
+    nx.qo[pos] = off;
+    uint64_t end = (uint64_t)off + (hi - lo);
+    for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch) nx.starts[ch] = pos;
+  }
+  __syncthreads();
+  if (t == 0) S.cnt = 0u;
+  __syncthreads();
+}
+
 // Block-wide sum of a per-thread value; result valid in thread 0.
-__device__ __forceinline__ unsigned long long block_sum64(unsigned long long x,
-                                                          unsigned long long* s_red) {
+__device__ __forceinline__ unsigned long long block_sum64(unsigned long long x, unsigned long long* s_red) {
   x = wave_sum64(x);
-  int w = threadIdx.x >> 6;
-  if (lane() == 0) s_red[w] = x;
+  if (lane() == 0) s_red[wave_id()] = x;
   __syncthreads();
   unsigned long long t = 0;
   if (threadIdx.x == 0)
@@ -161,10 +217,11 @@ __device__ __forceinline__ bool last_block(Ctrl* c, uint32_t* s_flag) {
 // ------------------------------------------------------------------------------------
 // learned policy: 12 -> 32 -> 32 -> 3 ReLU MLP, evaluated by one wave
 
-// Standardised features of the level just closed (same order as the training code).
+// Raw features of the level just closed (same order as the training code; the blob
+// carries the standardisation).
 __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges,
                                bool bu, float* f) {
-  float n = (float)a.n_total > 0 ? (float)a.n_total : 1.0f;
+  float n = a.n_total > 0 ? (float)a.n_total : 1.0f;
   float nv = a.nverts ? (float)(*a.nverts) : n;
   float found = (float)c->found;
   f[0] = log1pf((float)next);
@@ -181,7 +238,7 @@ __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next,
   f[11] = (float)c->phase;
 }
 
-// Wave 0 of the closing workgroup evaluates the MLP; returns argmax action in lane 0.
+// Wave 0 of the closing workgroup evaluates the MLP; returns the argmax action in lane 0.
 __device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
   const float* W1 = blob;
   const float* b1 = W1 + kMlpIn * kMlpH;
@@ -192,12 +249,10 @@ __device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
   const float* mean = blob + kMlpParams;
   const float* sd = mean + kMlpIn;
   int l = lane();
-  float h = 0.0f;
   if (l < kMlpH) {
     float acc = b1[l];
     for (int i = 0; i < kMlpIn; ++i) acc += W1[l * kMlpIn + i] * ((feat[i] - mean[i]) / sd[i]);
-    h = fmaxf(acc, 0.0f);
-    s_h[l] = h;
+    s_h[l] = fmaxf(acc, 0.0f);
   }
   __builtin_amdgcn_wave_barrier();
   float h2 = 0.0f;
@@ -224,10 +279,8 @@ __device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
   return best;
 }
 
-// Closes a level (called by the whole last workgroup).  `examined`/`inspected` are the
-// bottom-up counters of this launch, already summed over the grid.
-__device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_t q_edges,
-                            bool bu) {
+// Closes a level (called by the whole last workgroup).
+__device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_t q_edges, bool bu) {
   Ctrl* c = a.ctrl;
   __shared__ float s_feat[kMlpIn];
   __shared__ float s_h[kMlpH];
@@ -235,6 +288,7 @@ __device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_
   unsigned long long p = load_acquire_u64(&c->next_packed);
   uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
   bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
+  bool learned = reach && a.policy == MSC_POLICY_LEARNED && a.mlp;
   if (threadIdx.x == 0) {
     c->found += next;
     c->level += 1;
@@ -251,10 +305,10 @@ __device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_
       }
     }
     s_act = act;
-    if (reach && a.policy == MSC_POLICY_LEARNED && a.mlp) level_features(c, a, next, nedges, bu, s_feat);
+    if (learned) level_features(c, a, next, nedges, bu, s_feat);
   }
   __syncthreads();
-  if (reach && a.policy == MSC_POLICY_LEARNED && a.mlp && threadIdx.x < 64) {
+  if (learned && threadIdx.x < 64) {
     int act = mlp_action(a.mlp, s_feat, s_h);
     if (threadIdx.x == 0) s_act = act;
   }
@@ -307,46 +361,109 @@ __device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_
   }
 }
 
+// Closes a seeding launch: the queue built in buffer 1 becomes the frontier.
+__device__ void close_seed(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    unsigned long long p = load_acquire_u64(&c->next_packed);
+    uint32_t next = (uint32_t)(p >> 32);
+    c->q_size = next;
+    c->q_edges = (uint32_t)p;
+    c->cur = 1u;
+    c->next_packed = 0ull;
+    c->done = next == 0u ? 1u : 0u;
+    __threadfence();
+    c->ticket = 0u;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// ordered compaction of one tile: kTile items, thread t holds items base + r*kBlock + t
+// (r < 16) with flags in bit r of `keep`; survivors are written in item order at a range
+// reserved with one atomic per tile.
+
+struct TileLds {
+  uint32_t pref[16][kBlock / 64];
+  uint32_t base;
+};
+
+__device__ __forceinline__ void tile_reserve(TileLds& T, const unsigned long long* balls, uint32_t* counter) {
+  // balls[r] is this wave's ballot of round r
+  if (lane() == 0)
+    for (int r = 0; r < 16; ++r) T.pref[r][wave_id()] = (uint32_t)__popcll(balls[r]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int r = 0; r < 16; ++r)
+      for (int w = 0; w < kBlock / 64; ++w) {
+        uint32_t k = T.pref[r][w];
+        T.pref[r][w] = run;
+        run += k;
+      }
+    T.base = run ? atomicAdd(counter, run) : 0u;
+  }
+  __syncthreads();
+}
+
 // ------------------------------------------------------------------------------------
 // kernels
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t value, int64_t count) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-       i += (int64_t)gridDim.x * blockDim.x)
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = value;
 }
 
+// Word-wise set algebra on bitmaps: out = a & ~b (b null: out = a), optionally mirrored
+// into out2.
+__global__ void k_andnot(uint32_t* __restrict__ out, uint32_t* __restrict__ out2, const uint32_t* __restrict__ a,
+                         const uint32_t* __restrict__ b, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t x = a[i] & (b ? ~b[i] : 0xffffffffu);
+    out[i] = x;
+    if (out2) out2[i] = x;
+  }
+}
+
+__global__ void k_iota(int32_t* p, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+
 // Degree trim (multistep/scc_trim.cpp:89-139; kokkos_scc/scc_trim.cpp:57-140) and the CC
-// variant (multistep/cc_trim.cpp:47-97).  One wave per 64 consecutive vertices: the ballot
-// is the two bitmap words; survivors are appended in id order within the wave.
+// variant (multistep/cc_trim.cpp:47-97).  Each wave round covers 64 consecutive vertices,
+// whose ballot is two bitmap words; survivors are appended in id order per tile.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __restrict__ oo,
-                                                 const uint32_t* __restrict__ io,
-                                                 uint32_t* __restrict__ valid,
-                                                 int32_t* __restrict__ labels,
-                                                 int32_t* __restrict__ verts,
+                                                 const uint32_t* __restrict__ io, uint32_t* __restrict__ valid,
+                                                 int32_t* __restrict__ labels, int32_t* __restrict__ verts,
                                                  uint32_t* __restrict__ nverts) {
+  __shared__ TileLds T;
   const int64_t nw = ((int64_t)n + 31) / 32;
-  for (int64_t base = ((int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63)); base < n;
-       base += (int64_t)gridDim.x * kBlock) {
-    int64_t v = base + lane();
-    bool keep = false;
-    if (v < n) {
-      uint32_t od = oo[v + 1] - oo[v], id = io[v + 1] - io[v];
-      keep = MIRROR ? (od + id) > 0u : (od > 0u && id > 0u);
-      labels[v] = keep ? -1 : (int32_t)v;
+  for (int64_t base = (int64_t)blockIdx.x * kTile; base < n; base += (int64_t)gridDim.x * kTile) {
+    unsigned long long balls[16];
+    uint32_t keep_bits = 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      int64_t v = base + r * kBlock + threadIdx.x;
+      bool keep = false;
+      if (v < n) {
+        uint32_t od = oo[v + 1] - oo[v], id = io[v + 1] - io[v];
+        keep = MIRROR ? (od + id) > 0u : (od > 0u && id > 0u);
+        labels[v] = keep ? -1 : (int32_t)v;
+      }
+      unsigned long long mask = __ballot(keep);
+      balls[r] = mask;
+      keep_bits |= keep ? (1u << r) : 0u;
+      int64_t word = (base + r * kBlock + (threadIdx.x & ~63)) >> 5;
+      if (lane() == 0 && word < nw) valid[word] = (uint32_t)mask;
+      if (lane() == 32 && word + 1 < nw) valid[word + 1] = (uint32_t)(mask >> 32);
     }
-    unsigned long long mask = __ballot(keep);
-    int64_t word = base >> 5;
-    if (lane() == 0) valid[word] = (uint32_t)mask;
-    if (lane() == 32 && word + 1 < nw) valid[word + 1] = (uint32_t)(mask >> 32);
-    if (mask) {
-      int leader = __ffsll((long long)mask) - 1;
-      uint32_t at = 0;
-      if (lane() == leader) at = atomicAdd(nverts, (uint32_t)__popcll(mask));
-      at = __shfl(at, leader, 64);
-      if (keep) verts[at + __popcll(mask & lanes_below())] = (int32_t)v;
-    }
+    tile_reserve(T, balls, nverts);
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (keep_bits >> r & 1u)
+        verts[T.base + T.pref[r][wave_id()] + __popcll(balls[r] & lanes_below())] =
+            (int32_t)(base + r * kBlock + threadIdx.x);
+    __syncthreads();
   }
 }
 
@@ -370,7 +487,7 @@ __global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
     c->abandoned = 0u;
     c->examined = c->inspected = c->raised = 0ull;
     uint32_t m = 1u << ((uint32_t)root & 31u);
-    a.seen[(uint32_t)root >> 5] |= m;
+    a.cand[(uint32_t)root >> 5] &= ~m;
     a.fr0.q[0] = root;
     a.fr0.qo[0] = 0u;
   }
@@ -397,21 +514,6 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
   }
 }
 
-// Closes a seeding launch: the queue built in buffer 1 becomes the frontier.
-__device__ void close_seed(Ctrl* c) {
-  if (threadIdx.x == 0) {
-    unsigned long long p = load_acquire_u64(&c->next_packed);
-    uint32_t next = (uint32_t)(p >> 32);
-    c->q_size = next;
-    c->q_edges = (uint32_t)p;
-    c->cur = 1u;
-    c->next_packed = 0ull;
-    c->done = next == 0u ? 1u : 0u;
-    __threadfence();
-    c->ticket = 0u;
-  }
-}
-
 // One level of a search or a colour sweep.  Top-down levels: edge-balanced expansion of
 // the current frontier (the "Manhattan collapse" of kokkos_scc/fwbw_manhattan_global.cpp:
 // 162-289, with an LDS max-scan instead of a per-edge binary search).  Bottom-up levels
@@ -420,6 +522,7 @@ template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
   constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
+  __shared__ Stage S;
   __shared__ uint32_t s_own[kChunk];
   __shared__ uint32_t s_base[kChunk + 1];
   __shared__ int32_t s_col[COLOR ? kChunk + 1 : 1];
@@ -440,6 +543,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
   const int t = threadIdx.x;
   unsigned long long my_examined = 0, my_inspected = 0, my_raised = 0;
+  if (t == 0) S.cnt = 0u;
+  __syncthreads();
 
   if (!bu) {
     const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
@@ -486,10 +591,10 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
         uint32_t incl = wave_incl_max(run);
         uint32_t excl = __shfl_up(incl, 1, 64);
         if (lane() == 0) excl = 0u;
-        if (lane() == 63) s_scan[t >> 6] = incl;
+        if (lane() == 63) s_scan[wave_id()] = incl;
         __syncthreads();
         uint32_t carry = 0;
-        for (int w = 0; w < (t >> 6); ++w) carry = max(carry, s_scan[w]);
+        for (int w = 0; w < wave_id(); ++w) carry = max(carry, s_scan[w]);
         excl = max(excl, carry);
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
@@ -498,59 +603,82 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
         }
       }
       __syncthreads();
-#pragma unroll 2
+      // batched per thread so the loads of all kPer edges are in flight together:
+      // adjacency (coalesced), then one state probe per edge, then the claims
+      uint32_t w[kPer], word[kPer];
+      int32_t col[kPer], wcol[kPer];
+      bool ok[kPer], take[kPer];
+#pragma unroll
       for (int r = 0; r < kPer; ++r) {
         uint32_t i = (uint32_t)t + (uint32_t)r * kBlock;
-        bool take = false;
-        uint32_t w = 0, deg = 0;
-        if (i < len) {
+        ok[r] = i < len;
+        w[r] = 0;
+        col[r] = 0;
+        if (ok[r]) {
           uint32_t j = s_own[i];
           uint32_t e = e0 + i;
           if (MIRROR)
-            w = (uint32_t)(e < s_split[j] ? a.down_adj[s_base[j] + e] : a.down_adj2[s_base2[j] + e]);
+            w[r] = (uint32_t)(e < s_split[j] ? a.down_adj[s_base[j] + e] : a.down_adj2[s_base2[j] + e]);
           else
-            w = (uint32_t)a.down_adj[s_base[j] + e];
-          if (REACH) {
-            take = bit_test(a.allow, w) && !bit_test(a.seen, w) && bit_set(a.seen, w);
-          } else if (OP == OP_PUSH) {
-            int32_t col = s_col[j];
-            if (bit_test(a.valid, w) && a.colors[w] < col) {
-              int32_t old = atomicMax(&a.colors[w], col);
-              if (old < col) {
-                ++my_raised;
-                take = bit_set(inq_nxt, w);
-              }
-            }
-          } else {  // OP_CBW
-            int32_t col = s_col[j];
-            if (bit_test(a.valid, w) && a.colors[w] == col && bit_clear(a.valid, w)) {
-              a.labels[w] = col;
-              take = true;
+            w[r] = (uint32_t)a.down_adj[s_base[j] + e];
+          if (COLOR) col[r] = s_col[j];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        word[r] = 0;
+        wcol[r] = 0;
+        if (ok[r]) {
+          word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
+          if (COLOR) wcol[r] = a.colors[w[r]];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
+        take[r] = false;
+        if (REACH) {
+          take[r] = in && bit_clear(a.cand, w[r]);
+        } else if (OP == OP_PUSH) {
+          if (in && wcol[r] < col[r]) {
+            int32_t old = atomicMax(&a.colors[w[r]], col[r]);
+            if (old < col[r]) {
+              ++my_raised;
+              take[r] = bit_set(inq_nxt, w[r]);
             }
           }
-          if (take) deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w);
+        } else {  // OP_CBW
+          if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
+            a.labels[w[r]] = col[r];
+            take[r] = true;
+          }
         }
-        enqueue(take, w, deg, N, c);
       }
-      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        uint32_t deg = take[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
+        stage_push(S, take[r], w[r], deg);
+      }
+      stage_flush(S, N, c, kStage - kChunk);  // also the end-of-chunk barrier
     }
   } else if (REACH) {
     const uint32_t nv = *a.nverts;
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t base = blockIdx.x * kBlock + (t & ~63); base < nv; base += stride) {
-      uint32_t i = base + (uint32_t)lane();
+    for (uint32_t bbase = blockIdx.x * kBlock; bbase < nv; bbase += stride) {
+      uint32_t i = bbase + (uint32_t)t;
       bool cand = false, hit = false;
       uint32_t v = 0;
       if (i < nv) {
         v = (uint32_t)a.verts[i];
-        cand = bit_test(a.allow, v) && !bit_test(a.seen, v);
+        cand = bit_test(a.cand, v);
       }
       if (cand) {
+        // a parent is a visited vertex: allowed and no longer claimable
         ++my_examined;
         for (uint32_t j = a.up_off[v], e = a.up_off[v + 1]; j < e; ++j) {
           ++my_inspected;
           uint32_t u = (uint32_t)a.up_adj[j];
-          if (bit_test(a.seen, u) && bit_test(a.allow, u)) {
+          if (!bit_test(a.cand, u) && bit_test(a.allow, u)) {
             hit = true;
             break;
           }
@@ -559,18 +687,20 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
           for (uint32_t j = a.up_off2[v], e = a.up_off2[v + 1]; j < e; ++j) {
             ++my_inspected;
             uint32_t u = (uint32_t)a.up_adj2[j];
-            if (bit_test(a.seen, u) && bit_test(a.allow, u)) {
+            if (!bit_test(a.cand, u) && bit_test(a.allow, u)) {
               hit = true;
               break;
             }
           }
         }
       }
-      bool take = hit && bit_set(a.seen, v);
+      bool take = hit && bit_clear(a.cand, v);
       uint32_t deg = take ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v) : 0u;
-      enqueue(take, v, deg, N, c);
+      stage_push(S, take, v, deg);
+      stage_flush(S, N, c, kStage - kBlock);
     }
   }
+  stage_flush(S, N, c, 0u);
 
   // grid-wide statistics, one atomic per workgroup
   unsigned long long ex = block_sum64(my_examined, s_red);
@@ -584,60 +714,43 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   if (last_block(c, &s_flag)) close_level(a, OP, q_size, q_edges, bu);
 }
 
-// FW-BW commit (multistep/scc_fwbw.cpp:412-466) and CC-BFS commit (cc_bfs.cpp:217-271):
-// members get `label` and leave the valid set; the others are compacted.
-__global__ void __launch_bounds__(kBlock) k_commit(const int32_t* __restrict__ verts,
-                                                   const uint32_t* __restrict__ nverts,
-                                                   const uint32_t* __restrict__ member,
-                                                   uint32_t* __restrict__ valid,
+// Ordered filter of a vertex list: every listed vertex that is in `member` gets `label`
+// and is dropped (FW-BW commit, multistep/scc_fwbw.cpp:412-466; CC-BFS commit,
+// cc_bfs.cpp:217-271; the caller clears the members' valid bits word by word); with
+// member == null it only keeps the still-valid vertices (multistep/scc_color.cpp:292-337).
+// Survivors keep their list order within a tile.
+__global__ void __launch_bounds__(kBlock) k_filter(const int32_t* __restrict__ verts, const uint32_t* __restrict__ nverts,
+                                                   const uint32_t* __restrict__ member, uint32_t* __restrict__ valid,
                                                    int32_t* __restrict__ labels, int32_t label,
-                                                   int32_t* __restrict__ verts_out,
-                                                   uint32_t* __restrict__ nverts_out) {
+                                                   int32_t* __restrict__ verts_out, uint32_t* __restrict__ nverts_out) {
+  __shared__ TileLds T;
   const uint32_t nv = *nverts;
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
-       base += gridDim.x * kBlock) {
-    uint32_t i = base + (uint32_t)lane();
-    bool keep = false;
-    if (i < nv) {
-      uint32_t v = (uint32_t)verts[i];
-      if (bit_test(member, v)) {
-        labels[v] = label;
-        bit_clear(valid, v);
-      } else {
-        keep = bit_test(valid, v);
+  for (uint32_t base = blockIdx.x * kTile; base < nv; base += gridDim.x * kTile) {
+    unsigned long long balls[16];
+    uint32_t keep_bits = 0;
+    int32_t vs[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      uint32_t i = base + r * kBlock + threadIdx.x;
+      bool keep = false;
+      vs[r] = 0;
+      if (i < nv) {
+        uint32_t v = (uint32_t)verts[i];
+        vs[r] = (int32_t)v;
+        if (member && bit_test(member, v)) {
+          labels[v] = label;
+        } else {
+          keep = bit_test(valid, v);
+        }
       }
+      balls[r] = __ballot(keep);
+      keep_bits |= keep ? (1u << r) : 0u;
     }
-    unsigned long long mask = __ballot(keep);
-    if (mask) {
-      int leader = __ffsll((long long)mask) - 1;
-      uint32_t at = 0;
-      if (lane() == leader) at = atomicAdd(nverts_out, (uint32_t)__popcll(mask));
-      at = __shfl(at, leader, 64);
-      if (keep) verts_out[at + __popcll(mask & lanes_below())] = verts[i];
-    }
-  }
-}
-
-// Keeps the still-valid vertices of a list (multistep/scc_color.cpp:292-337).
-__global__ void __launch_bounds__(kBlock) k_compact(const int32_t* __restrict__ verts,
-                                                    const uint32_t* __restrict__ nverts,
-                                                    const uint32_t* __restrict__ valid,
-                                                    int32_t* __restrict__ verts_out,
-                                                    uint32_t* __restrict__ nverts_out) {
-  const uint32_t nv = *nverts;
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
-       base += gridDim.x * kBlock) {
-    uint32_t i = base + (uint32_t)lane();
-    int32_t v = i < nv ? verts[i] : 0;
-    bool keep = i < nv && bit_test(valid, (uint32_t)v);
-    unsigned long long mask = __ballot(keep);
-    if (mask) {
-      int leader = __ffsll((long long)mask) - 1;
-      uint32_t at = 0;
-      if (lane() == leader) at = atomicAdd(nverts_out, (uint32_t)__popcll(mask));
-      at = __shfl(at, leader, 64);
-      if (keep) verts_out[at + __popcll(mask & lanes_below())] = v;
-    }
+    tile_reserve(T, balls, nverts_out);
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (keep_bits >> r & 1u) verts_out[T.base + T.pref[r][wave_id()] + __popcll(balls[r] & lanes_below())] = vs[r];
+    __syncthreads();
   }
 }
 
@@ -645,11 +758,13 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int32_t* __restrict__ 
 // for every valid vertex, all of them queued.  Needs k_ctrl_reset first.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_seed_colors(LevelArgs a) {
+  __shared__ Stage S;
   __shared__ uint32_t s_flag;
+  if (threadIdx.x == 0) S.cnt = 0u;
+  __syncthreads();
   const uint32_t nv = *a.nverts;
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
-       base += gridDim.x * kBlock) {
-    uint32_t i = base + (uint32_t)lane();
+  for (uint32_t bbase = blockIdx.x * kBlock; bbase < nv; bbase += gridDim.x * kBlock) {
+    uint32_t i = bbase + threadIdx.x;
     bool take = i < nv;
     uint32_t v = 0, deg = 0;
     if (take) {
@@ -657,19 +772,23 @@ __global__ void __launch_bounds__(kBlock) k_seed_colors(LevelArgs a) {
       a.colors[v] = (int32_t)v;
       deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
     }
-    enqueue(take, v, deg, a.fr1, a.ctrl);
+    stage_push(S, take, v, deg);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);
   }
+  stage_flush(S, a.fr1, a.ctrl, 0u);
   if (last_block(a.ctrl, &s_flag)) close_seed(a.ctrl);
 }
 
 // Colour roots (multistep/scc_color.cpp:165-212): colors[v] == v labels v, removes it and
 // queues it for the per-colour backward search.  Needs k_ctrl_reset first.
 __global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
+  __shared__ Stage S;
   __shared__ uint32_t s_flag;
+  if (threadIdx.x == 0) S.cnt = 0u;
+  __syncthreads();
   const uint32_t nv = *a.nverts;
-  for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nv;
-       base += gridDim.x * kBlock) {
-    uint32_t i = base + (uint32_t)lane();
+  for (uint32_t bbase = blockIdx.x * kBlock; bbase < nv; bbase += gridDim.x * kBlock) {
+    uint32_t i = bbase + threadIdx.x;
     bool take = false;
     uint32_t v = 0, deg = 0;
     if (i < nv) {
@@ -681,22 +800,36 @@ __global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
         deg = a.down_off[v + 1] - a.down_off[v];
       }
     }
-    enqueue(take, v, deg, a.fr1, a.ctrl);
+    stage_push(S, take, v, deg);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);
   }
+  stage_flush(S, a.fr1, a.ctrl, 0u);
   if (last_block(a.ctrl, &s_flag)) close_seed(a.ctrl);
 }
 
-// Canonical labels: every component relabelled to its minimum vertex id.
+// Canonical labels: every component relabelled to its minimum vertex id.  mins[] starts
+// as the identity; a raw label is always a member of its component, so only vertices
+// below their label can lower it.  Lanes of a wave sharing a label combine into one
+// atomic (the pivot's SCC holds ~35% of all vertices at s24).
 __global__ void k_canon_min(int32_t n, const int32_t* __restrict__ labels, int32_t* __restrict__ mins) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    int32_t l = labels[v];
-    if (l >= 0) atomicMin(&mins[l], (int32_t)v);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t v = base + threadIdx.x;
+    int32_t l = v < n ? labels[v] : -1;
+    bool active = l >= 0 && (int64_t)l > v;
+    for (;;) {
+      unsigned long long m = __ballot(active);
+      if (m == 0ull) break;
+      int leader = __ffsll((long long)m) - 1;
+      int32_t key = __shfl(l, leader, 64);
+      bool mine = active && l == key;
+      uint32_t x = wave_min(mine ? (uint32_t)v : 0xffffffffu);
+      if (lane() == leader) atomicMin(&mins[key], (int32_t)x);
+      active = active && !mine;
+    }
   }
 }
 __global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int32_t* __restrict__ mins) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     int32_t l = labels[v];
     if (l >= 0) labels[v] = mins[l];
   }
@@ -706,8 +839,7 @@ __global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int
 // first vertex on ties.  Pass 1: max of the product's bit pattern (non-negative doubles
 // order like their bits); pass 2: smallest vertex attaining it.
 __global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t* __restrict__ oo,
-                                                      const uint32_t* __restrict__ io,
-                                                      unsigned long long* work) {
+                                                      const uint32_t* __restrict__ io, unsigned long long* work) {
   __shared__ unsigned long long s_red[kBlock / 64];
   unsigned long long best = 0;
   for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
@@ -716,7 +848,7 @@ __global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t*
     best = b > best ? b : best;
   }
   best = wave_max64(best);
-  if (lane() == 0) s_red[threadIdx.x >> 6] = best;
+  if (lane() == 0) s_red[wave_id()] = best;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
@@ -727,8 +859,7 @@ __global__ void k_pivot_first(int32_t n, const uint32_t* __restrict__ oo, const 
                               unsigned long long* work) {
   const unsigned long long target = work[0];
   if (target == 0ull) return;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     double p = (double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]);
     if ((unsigned long long)__double_as_longlong(p) == target) atomicMin(&work[1], (unsigned long long)v);
   }
@@ -736,8 +867,7 @@ __global__ void k_pivot_first(int32_t n, const uint32_t* __restrict__ oo, const 
 // CC pivot (multistep/cc_main.cpp:148-160): largest out+in degree, first vertex on ties,
 // as one 64-bit max of (degree << 32 | ~v).
 __global__ void __launch_bounds__(kBlock) k_pivot_cc(int32_t n, const uint32_t* __restrict__ oo,
-                                                     const uint32_t* __restrict__ io,
-                                                     unsigned long long* work) {
+                                                     const uint32_t* __restrict__ io, unsigned long long* work) {
   __shared__ unsigned long long s_red[kBlock / 64];
   unsigned long long best = 0;
   for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
@@ -747,7 +877,7 @@ __global__ void __launch_bounds__(kBlock) k_pivot_cc(int32_t n, const uint32_t* 
     best = key > best ? key : best;
   }
   best = wave_max64(best);
-  if (lane() == 0) s_red[threadIdx.x >> 6] = best;
+  if (lane() == 0) s_red[wave_id()] = best;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
@@ -756,8 +886,8 @@ __global__ void __launch_bounds__(kBlock) k_pivot_cc(int32_t n, const uint32_t* 
 }
 
 // Input check: offsets start at 0, never decrease, end at m; ids in [0, n).
-__global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ off,
-                           const int32_t* __restrict__ adj, uint32_t* err) {
+__global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ off, const int32_t* __restrict__ adj,
+                           uint32_t* err) {
   int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
   bool bad = false;
   if (tid == 0) bad = off[0] != 0u || (int64_t)off[n] != m;
@@ -766,8 +896,8 @@ __global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ of
   if (bad) atomicOr(err, 1u);
 }
 
-inline dim3 grid_for(int64_t items, int cap) {
-  int64_t b = (items + kBlock - 1) / kBlock;
+inline dim3 grid_for(int64_t items, int64_t per_block, int cap) {
+  int64_t b = (items + per_block - 1) / per_block;
   if (b < 1) b = 1;
   if (b > cap) b = cap;
   return dim3((unsigned)b);
@@ -780,13 +910,13 @@ inline dim3 grid_for(int64_t items, int cap) {
 
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_fill_u32, grid_for(count, L.grid * 4), dim3(kBlock), 0, L.stream, p, value, count);
+  hipLaunchKernelGGL(k_fill_u32, grid_for(count, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, p, value, count);
 }
 
-void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io,
-                 uint32_t* valid, int32_t* labels, int32_t* verts, uint32_t* nverts) {
+void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io, uint32_t* valid,
+                 int32_t* labels, int32_t* verts, uint32_t* nverts) {
   if (n <= 0) return;
-  dim3 g = grid_for(n, L.grid * 4);
+  dim3 g = grid_for(n, kTile, L.grid * 4);
   if (mirror)
     hipLaunchKernelGGL(k_trim<true>, g, dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, verts, nverts);
   else
@@ -795,6 +925,13 @@ void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, co
 
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
   hipLaunchKernelGGL(k_search_init, dim3(1), dim3(kBlock), 0, L.stream, a, root, phase);
+}
+
+void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
+                   int64_t words) {
+  if (words <= 0) return;
+  hipLaunchKernelGGL(k_andnot, grid_for(words, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, out, out2, a, b,
+                     words);
 }
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
@@ -814,7 +951,7 @@ void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
 
 void launch_commit(const Launch& L, const int32_t* verts, const uint32_t* nverts, const uint32_t* member,
                    uint32_t* valid, int32_t* labels, int32_t label, int32_t* verts_out, uint32_t* nverts_out) {
-  hipLaunchKernelGGL(k_commit, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts, member, valid,
+  hipLaunchKernelGGL(k_filter, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts, member, valid,
                      labels, label, verts_out, nverts_out);
 }
 
@@ -833,33 +970,31 @@ void launch_roots(const Launch& L, const LevelArgs& a) {
 
 void launch_compact(const Launch& L, const int32_t* verts, const uint32_t* nverts, const uint32_t* valid,
                     int32_t* verts_out, uint32_t* nverts_out) {
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts, valid,
-                     verts_out, nverts_out);
+  hipLaunchKernelGGL(k_filter, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts,
+                     (const uint32_t*)nullptr, const_cast<uint32_t*>(valid), (int32_t*)nullptr, 0, verts_out,
+                     nverts_out);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch) {
   if (n <= 0) return;
-  launch_fill_u32(L, (uint32_t*)scratch, 0x7fffffffu, n);
-  dim3 g = grid_for(n, L.grid * 4);
+  dim3 g = grid_for(n, kBlock, L.grid * 4);
+  hipLaunchKernelGGL(k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n);
   hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
   hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
 }
 
-void launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io,
-                      unsigned long long* work) {
-  dim3 g = grid_for(n, L.grid * 4);
+void launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* work) {
+  dim3 g = grid_for(n, kBlock, L.grid * 4);
   hipLaunchKernelGGL(k_pivot_max, g, dim3(kBlock), 0, L.stream, n, oo, io, work);
   hipLaunchKernelGGL(k_pivot_first, g, dim3(kBlock), 0, L.stream, n, oo, io, work);
 }
 
-void launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io,
-                     unsigned long long* work) {
-  hipLaunchKernelGGL(k_pivot_cc, grid_for(n, L.grid * 4), dim3(kBlock), 0, L.stream, n, oo, io, work);
+void launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* work) {
+  hipLaunchKernelGGL(k_pivot_cc, grid_for(n, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, n, oo, io, work);
 }
 
-void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj,
-                     uint32_t* err) {
-  hipLaunchKernelGGL(k_validate, grid_for(m > n ? m : n, L.grid * 4), dim3(kBlock), 0, L.stream, n, m, off,
+void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err) {
+  hipLaunchKernelGGL(k_validate, grid_for(m > n ? m : n, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, n, m, off,
                      adj, err);
 }
 

@@ -41,7 +41,8 @@ struct msc_ctx {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   int64_t cap_n = -1, cap_e = -1;
-  uint32_t *valid = nullptr, *visA = nullptr, *visB = nullptr, *inq0 = nullptr, *inq1 = nullptr;
+  // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
+  uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
   int32_t *labels = nullptr, *colors = nullptr, *verts0 = nullptr, *verts1 = nullptr;
   Frontier fr0{}, fr1{};
   uint32_t* counters = nullptr;  // [0] nverts0, [1] nverts1, [2] validate error
@@ -93,7 +94,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   size_t nw = (size_t)((nn + 31) / 32) + 1;
   size_t nchunks = (size_t)((ee + kChunk - 1) / kChunk) + 2;
   size_t sizes[] = {
-      nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,                 // bitmaps
+      nw * 4, nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,         // bitmaps
       (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,  // labels colors verts0 verts1
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr0
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr1
@@ -116,23 +117,22 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
     q += align256(sizes[i]);
   }
   c->valid = (uint32_t*)parts[0];
-  c->visA = (uint32_t*)parts[1];
-  c->visB = (uint32_t*)parts[2];
-  c->inq0 = (uint32_t*)parts[3];
-  c->inq1 = (uint32_t*)parts[4];
-  c->labels = (int32_t*)parts[5];
-  c->colors = (int32_t*)parts[6];
-  c->verts0 = (int32_t*)parts[7];
-  c->verts1 = (int32_t*)parts[8];
-  c->fr0 = Frontier{(int32_t*)parts[9], (uint32_t*)parts[10], (uint32_t*)parts[11]};
-  c->fr1 = Frontier{(int32_t*)parts[12], (uint32_t*)parts[13], (uint32_t*)parts[14]};
-  c->counters = (uint32_t*)parts[15];
-  c->work = (unsigned long long*)parts[16];
-  c->ctrl = (Ctrl*)parts[17];
-  c->recs = (msc_level_rec*)parts[18];
-  float* old_mlp_host = nullptr;
-  (void)old_mlp_host;
-  c->mlp = (float*)parts[19];
+  c->b1 = (uint32_t*)parts[1];
+  c->b2 = (uint32_t*)parts[2];
+  c->b3 = (uint32_t*)parts[3];
+  c->inq0 = (uint32_t*)parts[4];
+  c->inq1 = (uint32_t*)parts[5];
+  c->labels = (int32_t*)parts[6];
+  c->colors = (int32_t*)parts[7];
+  c->verts0 = (int32_t*)parts[8];
+  c->verts1 = (int32_t*)parts[9];
+  c->fr0 = Frontier{(int32_t*)parts[10], (uint32_t*)parts[11], (uint32_t*)parts[12]};
+  c->fr1 = Frontier{(int32_t*)parts[13], (uint32_t*)parts[14], (uint32_t*)parts[15]};
+  c->counters = (uint32_t*)parts[16];
+  c->work = (unsigned long long*)parts[17];
+  c->ctrl = (Ctrl*)parts[18];
+  c->recs = (msc_level_rec*)parts[19];
+  c->mlp = (float*)parts[20];
   c->has_mlp = false;  // weights must be set again after a regrow
   c->cap_n = nn;
   c->cap_e = ee;
@@ -413,50 +413,58 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   if (pivot >= 0 && nv > 0) {
     const uint32_t* member = nullptr;
     if (!cc) {
-      // FW: allow = valid, seen = visA, top-down on out-lists, bottom-up on in-lists
-      launch_fill_u32(L, c->visA, 0u, nw);
+      // FW: allow = valid, cand = b1, top-down on out-lists, bottom-up on in-lists
+      launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
       LevelArgs fw = a;
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
-      fw.allow = c->valid; fw.seen = c->visA;
+      fw.allow = c->valid; fw.cand = c->b1;
       fw.verts = verts; fw.nverts = nverts;
       launch_search_init(L, fw, pivot, 0);
       if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
-        // BW: allow = visA, seen = visB, top-down on in-lists, bottom-up on out-lists
-        launch_fill_u32(L, c->visB, 0u, nw);
+        // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy), top-down on in-lists,
+        // bottom-up on out-lists
+        launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
         LevelArgs bw = a;
         bw.down_off = g.in_off; bw.down_adj = g.in_adj;
         bw.up_off = g.out_off; bw.up_adj = g.out_adj;
-        bw.allow = c->visA; bw.seen = c->visB;
+        bw.allow = c->b2; bw.cand = c->b3;
         bw.verts = verts; bw.nverts = nverts;
         launch_search_init(L, bw, pivot, 1);
         if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s))) return rc;
         if ((rc = read_ctrl(c))) return rc;
         if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
-        if (!c->ctrl_host->abandoned) member = c->visB;
+        if (!c->ctrl_host->abandoned) {
+          launch_andnot(L, c->b1, nullptr, c->b2, c->b3, nw);  // SCC = fw set & visited by BW
+          member = c->b1;
+        }
       }
     } else {
-      launch_fill_u32(L, c->visA, 0u, nw);
+      launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
       LevelArgs bf = a;
       bf.down_off = g.out_off; bf.down_adj = g.out_adj;
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
-      bf.allow = c->valid; bf.seen = c->visA;
+      bf.allow = c->valid; bf.cand = c->b1;
       bf.verts = verts; bf.nverts = nverts;
       launch_search_init(L, bf, pivot, 2);
       if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
-      if (!c->ctrl_host->abandoned) member = c->visA;
+      if (!c->ctrl_host->abandoned) {
+        launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);  // component = valid & visited
+        member = c->b2;
+      }
     }
     if (member) {
       Timed t(c, MSC_K_COMMIT);
       launch_fill_u32(L, nverts_alt, 0u, 1);
       launch_commit(L, verts, nverts, member, c->valid, labels, pivot, verts_alt, nverts_alt);
+      launch_andnot(L, c->valid, nullptr, c->valid, member, nw);
       committed = true;
     }
   }
