@@ -27,7 +27,8 @@ __all__ = [
     "read_edge", "create_csr", "gen_rmat", "gen_grid", "gen_planted",
     "run_scc", "run_cc", "scc_pivot", "scc_verify", "cc_verify", "verify_line",
     "canonicalize", "write_labels", "device_count", "lib_path",
-    "POLICY_PLAIN", "POLICY_LEARNED", "LABELS_CANONICAL", "LABELS_RAW",
+    "POLICY_PLAIN", "POLICY_LEARNED", "LABELS_CANONICAL", "LABELS_RAW", "schedule_array",
+    "set_policy", "load_policy", "mlp_forward", "mlp_grad", "mlp_train", "POLICY_WEIGHTS",
     "K_TRIM", "K_LEVEL", "K_COMMIT", "K_COLOR_PROP", "K_COLOR_BW", "K_ROOTS", "K_OTHER",
     "KERNEL_NAMES",
 ]
@@ -71,7 +72,8 @@ class _Opts(C.Structure):
     _fields_ = [("alpha", C.c_double), ("beta", C.c_int32), ("trim_level", C.c_int32),
                 ("policy", C.c_int32), ("label_mode", C.c_int32), ("labels_where", C.c_int32),
                 ("device", C.c_int32), ("validate", C.c_int32), ("timing", C.c_int32),
-                ("record", C.c_int32), ("ctx", C.c_void_p)]
+                ("record", C.c_int32), ("ctx", C.c_void_p), ("schedule", C.c_void_p),
+                ("schedule_len", C.c_int32), ("pad_", C.c_int32)]
 
 
 class _KStat(C.Structure):
@@ -97,7 +99,13 @@ class _LevelRec(C.Structure):
                 ("next_size", C.c_int32), ("q_size", C.c_uint32), ("q_edges", C.c_uint32),
                 ("next_edges", C.c_uint32), ("examined", C.c_uint32), ("inspected", C.c_uint64),
                 ("found", C.c_int64), ("n_total", C.c_int32), ("num_valid", C.c_int32),
-                ("action", C.c_int32), ("pad", C.c_int32)]
+                ("action", C.c_int32), ("pad", C.c_int32), ("feat", C.c_float * 12),
+                ("level_us", C.c_float), ("pad2", C.c_int32)]
+
+
+class _TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("weight_decay", C.c_float), ("seed", C.c_uint64)]
 
 
 _lib_handle = None
@@ -151,6 +159,10 @@ def _lib():
         "msc_copy_to_host": (C.c_int, [P, P, P, C.c_int64]),
         "msc_copy_to_device": (C.c_int, [P, P, P, C.c_int64]),
         "msc_ctx_sync": (C.c_int, [P]),
+        "msc_policy_set_weights": (C.c_int, [P, P, C.c_int32]),
+        "msc_mlp_forward": (C.c_int, [P, P, P, C.c_int64, P]),
+        "msc_mlp_grad": (C.c_int, [P, P, P, P, P, C.c_int64, C.POINTER(C.c_float), P]),
+        "msc_mlp_train": (C.c_int, [P, P, P, P, P, C.c_int64, C.POINTER(_TrainOpts), P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -313,7 +325,11 @@ class Context:
             return []
         buf = (_LevelRec * k)()
         _check(_lib().msc_ctx_levels(self.handle, buf, k, C.byref(count)), "msc_ctx_levels")
-        return [LevelRecord(**{f: getattr(r, f) for f, _ in _LevelRec._fields_ if f != "pad"}) for r in buf]
+        out = []
+        for r in buf:
+            vals = {f: getattr(r, f) for f, _ in _LevelRec._fields_ if f not in ("pad", "pad2", "feat")}
+            out.append(LevelRecord(feat=list(r.feat), **vals))
+        return out
 
 
 class DeviceGraph:
@@ -381,6 +397,8 @@ class LevelRecord:
     n_total: int
     num_valid: int
     action: int
+    level_us: float
+    feat: list
 
 
 @dataclass
@@ -409,10 +427,28 @@ class Stats:
         return cls(kernels=ks, **vals)
 
 
+SCHED_LEVELS = 64
+ACT_TD, ACT_BU, ACT_STOP, ACT_POLICY = 0, 1, 2, -1
+
+
+def schedule_array(fw=None, bw=None, cc=None) -> np.ndarray:
+    """Forced-decision table for msc_opts.schedule: per search (FW, BW, CC) a list whose
+    entry L-1 is the action after level L (ACT_TD / ACT_BU / ACT_STOP, ACT_POLICY = free)."""
+    s = np.full(3 * SCHED_LEVELS, -1, dtype=np.int8)
+    for k, seq in enumerate((fw, bw, cc)):
+        if seq is not None:
+            seq = list(seq)[:SCHED_LEVELS]
+            s[k * SCHED_LEVELS:k * SCHED_LEVELS + len(seq)] = seq
+    return s
+
+
 def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alpha=15.0, beta=25,
-          device=0) -> _Opts:
+          device=0, schedule=None) -> _Opts:
     o = _Opts()
     _lib().msc_opts_default(C.byref(o))
+    if schedule is not None:
+        o.schedule = schedule.ctypes.data
+        o.schedule_len = int(schedule.shape[0])
     o.alpha, o.beta = alpha, beta
     o.policy = policy
     o.label_mode = labels
@@ -426,7 +462,7 @@ def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alph
 
 
 def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
-         out):
+         out, schedule):
     n = g.n
     where = HOST
     if isinstance(out, DeviceLabels):
@@ -441,7 +477,10 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
         elif out.dtype != np.int32 or out.shape != (n,) or not out.flags.c_contiguous:
             raise ValueError("out must be a contiguous int32 array of length n")
         dst = _ptr(out) if n else None
-    o = _opts(policy, labels, timing, record, validate, ctx, labels_where=where, alpha=alpha, beta=beta)
+    if schedule is not None:
+        schedule = np.ascontiguousarray(schedule, dtype=np.int8)
+    o = _opts(policy, labels, timing, record, validate, ctx, labels_where=where, alpha=alpha, beta=beta,
+              schedule=schedule)
     st = _Stats()
     csr = g.csr()
     piv = -1 if max_deg_vert is None else int(max_deg_vert)
@@ -454,25 +493,83 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
 def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, *,
             policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
             record: bool = False, validate: bool = False, ctx: Context | None = None,
-            alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False):
+            alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False,
+            schedule: np.ndarray | None = None):
     """SCC labels of ``g`` (a Graph or DeviceGraph) on the GPU.
 
     run_scc (multistep/scc_main.cpp:210).  ``max_deg_vert`` / ``avg_degree`` default to
     create_csr's rule computed on device.  Returns labels, or (labels, Stats) with
-    ``stats=True``."""
+    ``stats=True``.  ``schedule`` (see :func:`schedule_array`) forces per-level decisions."""
     lab, st = _run("msc_scc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
-                   alpha, beta, out)
+                   alpha, beta, out, schedule)
     return (lab, st) if stats else lab
 
 
 def run_cc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, *,
            policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
            record: bool = False, validate: bool = False, ctx: Context | None = None,
-           alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False):
+           alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False,
+           schedule: np.ndarray | None = None):
     """Weakly connected components (run_cc, multistep/cc_main.cpp:184, on the mirror)."""
     lab, st = _run("msc_cc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
-                   alpha, beta, out)
+                   alpha, beta, out, schedule)
     return (lab, st) if stats else lab
+
+
+# ---- learned policy (MLP 12 -> 32 -> 32 -> 3) ------------------------------------------
+POLICY_WEIGHTS = os.path.join(_HERE, "policy_weights.npz")
+
+
+def set_policy(ctx: Context, blob: np.ndarray) -> None:
+    """Loads MLP weights + standardisation (MLP_BLOB float32) into the context."""
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    if b.shape != (MLP_BLOB,):
+        raise ValueError(f"policy blob must hold {MLP_BLOB} floats")
+    _check(_lib().msc_policy_set_weights(ctx.handle, _ptr(b), MLP_BLOB), "msc_policy_set_weights")
+
+
+def load_policy(ctx: Context, path: str = POLICY_WEIGHTS) -> np.ndarray:
+    """Loads the shipped (or given) trained weights into ctx; returns the blob."""
+    with np.load(path, allow_pickle=False) as z:
+        blob = z["blob"].astype(np.float32)
+    set_policy(ctx, blob)
+    return blob
+
+
+def mlp_forward(ctx: Context, blob: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Logits [count, 3] of raw features x [count, 12], computed by the device kernel."""
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    out = np.empty((xx.shape[0], MLP_OUT), dtype=np.float32)
+    _check(_lib().msc_mlp_forward(ctx.handle, _ptr(b), _ptr(xx), xx.shape[0], _ptr(out)), "msc_mlp_forward")
+    return out
+
+
+def mlp_grad(ctx: Context, blob, x, y, w=None) -> tuple[float, np.ndarray]:
+    """(sum_b w_b CE_b, its gradient w.r.t. the MLP_PARAMS weights) from the device kernels."""
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    yy = np.ascontiguousarray(y, dtype=np.int32)
+    ww = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+    grad = np.empty(MLP_PARAMS, dtype=np.float32)
+    loss = C.c_float()
+    _check(_lib().msc_mlp_grad(ctx.handle, _ptr(b), _ptr(xx), _ptr(yy), None if ww is None else _ptr(ww),
+                               xx.shape[0], C.byref(loss), _ptr(grad)), "msc_mlp_grad")
+    return float(loss.value), grad
+
+
+def mlp_train(ctx: Context, blob, x, y, w=None, *, epochs=30, batch=1024, lr=3e-3, beta1=0.9, beta2=0.999,
+              eps=1e-8, weight_decay=0.0, seed=1) -> tuple[np.ndarray, np.ndarray]:
+    """Adam on device; returns (trained blob, per-epoch mean loss)."""
+    b = np.array(blob, dtype=np.float32, copy=True)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+    yy = np.ascontiguousarray(y, dtype=np.int32)
+    ww = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+    curve = np.zeros(epochs, dtype=np.float32)
+    o = _TrainOpts(epochs, batch, lr, beta1, beta2, eps, weight_decay, seed)
+    _check(_lib().msc_mlp_train(ctx.handle, _ptr(b), _ptr(xx), _ptr(yy), None if ww is None else _ptr(ww),
+                                xx.shape[0], C.byref(o), _ptr(curve)), "msc_mlp_train")
+    return b, curve
 
 
 def scc_pivot(g, ctx: Context | None = None) -> tuple[int, float]:

@@ -1,5 +1,5 @@
-// Internal declarations shared by the gfx950 kernels (msc_kernels.hip) and the host
-// runtime (msc_runtime.cpp).  Not part of the C ABI.
+// Internal declarations shared by the gfx950 kernels (msc_kernels.hip, msc_mlp.hip) and
+// the host runtime (msc_runtime.cpp).  Not part of the C ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,15 +13,16 @@ constexpr int kBlock = 256;                    // threads per workgroup: 4 wave6
 constexpr uint32_t kChunk = 1024;              // edges per edge-balanced work item
 constexpr int kPer = (int)(kChunk / kBlock);   // edges per thread per work item
 constexpr uint32_t kStage = 2048;              // claims staged in LDS per workgroup
-constexpr uint32_t kTile = kBlock * 16;        // vertices per ordered-compaction tile
+constexpr int kBW = 32;                        // bitmap words per workgroup iteration
+constexpr int kMaxParts = 8192;                // per-workgroup partial results
 constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;
 // feature standardisation (mean, std) follows the weights
 constexpr int kMlpBlob = kMlpParams + 2 * kMlpIn;
 
-// What one level kernel does with each edge it walks.
-// Reach ops keep one "claimable" bitmap, cand = allow & ~visited: a top-down edge costs a
-// single probe, and a claim is one atomicAnd that clears the bit.
+// What one level kernel does with each edge it walks.  Reach ops keep one "claimable"
+// bitmap, cand = allow & ~visited: a top-down edge costs a single probe, a claim is one
+// atomicAnd that clears the bit, and the set bits of cand are the bottom-up candidates.
 enum Op : int {
   OP_FW = 0,   // forward reach:  allow = valid, top-down on out-lists, bottom-up on in-lists
   OP_BW = 1,   // backward reach: allow = fw set, top-down on in-lists, bottom-up on out-lists
@@ -38,9 +39,9 @@ struct Frontier {
   uint32_t* starts;
 };
 
-// Device-resident loop state.  Every level kernel reads it at entry; the last workgroup
-// to finish a launch (ticket) closes the level and prepares the next one, so the host
-// never needs to look at it between levels.
+// Device-resident loop state.  Level kernels read it at entry; a one-workgroup closing
+// kernel after each level reads the level's counters and prepares the next level, so the
+// host never looks at it between levels.
 struct Ctrl {
   unsigned long long next_packed;  // (vertices enqueued << 32) | their list lengths
   uint32_t q_size;                 // frontier to expand
@@ -50,18 +51,14 @@ struct Ctrl {
   uint32_t switched_back;          // the one-way return to top-down happened
   uint32_t done;
   uint32_t level;                  // levels closed so far
-  uint32_t ticket;
-  int64_t found;                   // vertices discovered, root excluded
   uint32_t nrec;                   // level records written
+  int64_t found;                   // vertices discovered, root excluded
   uint32_t phase;                  // msc_level_rec.phase of this search
   uint32_t abandoned;              // the policy stopped the search; its result is void
-  uint32_t pad[3];
-  // statistics, on their own cache line
-  unsigned long long examined __attribute__((aligned(128)));
-  unsigned long long inspected;
-  unsigned long long raised;
+  unsigned long long reduced[4];   // results of k_reduce
   unsigned long long bytes_op[5];  // algorithmic bytes of closed levels, per Op
   unsigned long long bu_levels;
+  unsigned long long t_start;      // earliest workgroup start of the running launch
 };
 
 struct LevelArgs {
@@ -75,53 +72,64 @@ struct LevelArgs {
   const int32_t* up_adj2;
   uint32_t* allow;            // bitmap: vertices the search may visit
   uint32_t* cand;             // bitmap: allowed and not yet visited (cleared on claim)
-  uint32_t* valid;            // bitmap (colour ops)
+  uint32_t* valid;            // bitmap: vertices without a label yet
   uint32_t* inq0;             // colour worklist membership, buffer 0 / 1
   uint32_t* inq1;
   int32_t* colors;
   int32_t* labels;
-  const int32_t* verts;       // bottom-up candidates
-  const uint32_t* nverts;
   Frontier fr0, fr1;
   Ctrl* ctrl;
+  unsigned long long* part;   // kMaxParts x 4 per-workgroup partials
   msc_level_rec* recs;
   uint32_t rec_cap;
   int32_t n_total;
+  int64_t nwords;             // bitmap words
+  int32_t num_valid;          // valid vertices when the search started
   int32_t beta;
   int32_t policy;
   double avg;
   double alpha;
   const float* mlp;           // kMlpBlob floats (learned policy)
+  const int8_t* sched;        // forced actions, 3 x MSC_SCHED_LEVELS (null: none)
+  float clock_mhz;            // wall_clock64() rate, for per-level durations
 };
 
 struct Launch {
   hipStream_t stream;
-  int grid;  // workgroups for grid-stride kernels
+  int grid;  // workgroups for level kernels
 };
 
+// ---- context accessors for the policy code (msc_runtime.cpp) ----
+int ctx_stream(msc_ctx* ctx, hipStream_t* stream, int* device);
+int ctx_set_policy(msc_ctx* ctx, const float* blob);  // kMlpBlob floats, validated by the caller
+
 // ---- kernel launchers (msc_kernels.hip) ----
+// Launchers that leave per-workgroup partials return how many workgroups wrote them.
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count);
-void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off,
-                 const uint32_t* in_off, uint32_t* valid, int32_t* labels, int32_t* verts,
-                 uint32_t* nverts);
+int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
+                uint32_t* valid, int32_t* labels, unsigned long long* part);
+// out[i] = sums (cols 0..2) and max (col 3) of part[0..nparts)[i]
+void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out);
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
+void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);  // work only
+void launch_close(const Launch& L, int op, const LevelArgs& a);              // one workgroup
+// labels[v] = label for every set bit of member; part col 0 = count
+int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
+                      int32_t label, unsigned long long* part);
+int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
+void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a);
+void launch_roots(const Launch& L, const LevelArgs& a);
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
-void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);
-void launch_commit(const Launch& L, const int32_t* verts, const uint32_t* nverts,
-                   const uint32_t* member, uint32_t* valid, int32_t* labels, int32_t label,
-                   int32_t* verts_out, uint32_t* nverts_out);
-void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a);
-void launch_roots(const Launch& L, const LevelArgs& a);
-void launch_compact(const Launch& L, const int32_t* verts, const uint32_t* nverts,
-                    const uint32_t* valid, int32_t* verts_out, uint32_t* nverts_out);
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch);
-void launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off,
-                      const uint32_t* in_off, unsigned long long* work);
-void launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* out_off,
-                     const uint32_t* in_off, unsigned long long* work);
-void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off,
-                     const int32_t* adj, uint32_t* err);
+// pivot: part col 3 = max key; results via launch_reduce then launch_pivot_first
+int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
+                     unsigned long long* part);
+void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
+                        unsigned long long* red);
+int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
+                    unsigned long long* part);
+void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err);
 
 }  // namespace msc

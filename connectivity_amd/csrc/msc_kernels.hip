@@ -1,27 +1,27 @@
 // gfx950 kernels of the Multistep SCC / WCC path.
 //
 // Design (for a reader with only these sources):
-//  * Vertex state lives in bitmaps (valid / visited / worklist membership): one bit per
+//  * Vertex state lives in bitmaps (valid / claimable / worklist membership): one bit per
 //    vertex, 2 MB at 16.8M vertices, so random probes are served from L2 / MALL.  Claims
 //    are atomicOr / atomicAnd test-and-set on the containing word, so a frontier never
-//    holds a vertex twice.
+//    holds a vertex twice.  Sweeps over "all candidates" walk the bitmap itself: each
+//    workgroup iteration loads kBW words, prefix-sums their popcounts in LDS and hands the
+//    set bits to its threads, so no compacted vertex list has to be maintained.
+//  * Same-address atomics serialise in L2 (~10 ns each when every workgroup hits one
+//    counter at once), and an agent-scope fence on multi-XCD gfx950 costs about as much
+//    again.  So nothing here takes one per workgroup per launch on a shared word:
+//    workgroups write partial results to their own slot of a partials array, and a
+//    one-workgroup kernel reduces them; levels are closed by such a kernel (the launch
+//    boundary provides the visibility a fence would).
 //  * A frontier is a queue plus, per slot, the exclusive prefix of the slots' list lengths
 //    ("edge offsets").  Claims are staged per workgroup in LDS; when the stage fills (and
 //    at the end of the launch) the workgroup scans the staged lengths and reserves queue
 //    slots and edge offsets together with ONE 64-bit atomic ((count << 32) | edge sum).
-//    A single frontier counter hit once per wave costs milliseconds at s24 (same-address
-//    atomics serialise in L2); once per ~3k claims it is noise.
 //  * While writing a slot, the workgroup also records, for every kChunk-edge chunk that
 //    begins inside that vertex's list, which slot owns the chunk ("starts").  A top-down
-//    level then gives each workgroup whole chunks of exactly kChunk edges: it maps every
-//    edge to its source vertex with an LDS max-scan and reads the adjacency coalesced.
-//    Hubs (the s24 pivot has ~240k out-edges) spread over hundreds of workgroups.
-//  * Bottom-up levels sweep the candidate list one vertex per lane and stop at the first
-//    visited parent.
-//  * Each level is one launch on a fixed grid.  The launch reads its sizes and direction
-//    from a device control block; the last workgroup to finish (ticket counter) closes
-//    the level: frontier swap, counters, the direction decision (the reference's
-//    ALPHA/BETA rule or the MLP), a per-level record.  The host only polls `done`.
+//    level gives each workgroup whole chunks of exactly kChunk edges: it maps every edge
+//    to its source vertex with an LDS max-scan and reads the adjacency coalesced.  Hubs
+//    (the s24 pivot has ~240k out-edges) spread over hundreds of workgroups.
 //  * No MFMA: no step is a dense contraction.  Everything is HBM / L2 bound integer work.
 #include <hip/hip_runtime.h>
 
@@ -97,14 +97,69 @@ __device__ __forceinline__ bool bit_clear(uint32_t* b, uint32_t v) {
   return (atomicAnd(&b[v >> 5], ~m) & m) != 0u;
 }
 
-__device__ __forceinline__ unsigned long long load_acquire_u64(unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ uint32_t list_len(const uint32_t* off, const uint32_t* off2, uint32_t v) {
   uint32_t d = off[v + 1] - off[v];
   if (off2) d += off2[v + 1] - off2[v];
   return d;
+}
+
+// Block-wide sum of a per-thread value; result valid in thread 0.
+__device__ __forceinline__ unsigned long long block_sum64(unsigned long long x, unsigned long long* s_red) {
+  x = wave_sum64(x);
+  if (lane() == 0) s_red[wave_id()] = x;
+  __syncthreads();
+  unsigned long long t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kBlock / 64; ++i) t += s_red[i];
+  __syncthreads();
+  return t;
+}
+
+// ------------------------------------------------------------------------------------
+// workgroup-level walk over the set bits of a bitmap
+
+struct BitIter {
+  uint32_t word[kBW];
+  uint32_t pref[kBW + 1];
+};
+
+// Loads words [wb, wb + kBW) and the exclusive prefix of their popcounts (all threads call).
+__device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, int64_t nwords, int64_t wb) {
+  const int t = threadIdx.x;
+  if (t < 64) {
+    uint32_t w = (t < kBW && wb + t < nwords) ? bits[wb + t] : 0u;
+    if (t < kBW) B.word[t] = w;
+    uint32_t incl = wave_incl_sum((uint32_t)__popc(w));
+    if (t < kBW) B.pref[t + 1] = incl;
+    if (t == 0) B.pref[0] = 0u;
+  }
+  __syncthreads();
+  return B.pref[kBW];
+}
+
+// Vertex id of the k-th set bit (k < total) of the loaded words.
+__device__ __forceinline__ uint32_t bits_pick(const BitIter& B, int64_t wb, uint32_t k) {
+  int lo = 0, hi = kBW;  // pref[lo] <= k < pref[hi]
+#pragma unroll
+  for (int s = 0; s < 5; ++s) {
+    int mid = (lo + hi) >> 1;
+    if (B.pref[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  uint32_t w = B.word[lo], j = k - B.pref[lo], pos = 0;
+#pragma unroll
+  for (int width = 16; width >= 1; width >>= 1) {  // j-th set bit of w
+    uint32_t low = w & ((1u << width) - 1u);
+    uint32_t c = (uint32_t)__popc(low);
+    if (j >= c) {
+      j -= c;
+      w >>= width;
+      pos += (uint32_t)width;
+    } else {
+      w = low;
+    }
+  }
+  return (uint32_t)((wb + lo) * 32 + pos);
 }
 
 // ------------------------------------------------------------------------------------
@@ -115,7 +170,7 @@ struct Stage {
   uint32_t d[kStage];  // list length, then (after the scan) local edge offset
   uint32_t part[kBlock / 64];
   uint32_t cnt;
-  uint32_t base_hi, base_lo, total;
+  uint32_t base_hi, base_lo;
 };
 
 // Wave-cooperative: appends the lanes with `take` to the stage (one LDS atomic per wave).
@@ -136,7 +191,7 @@ __device__ __forceinline__ void stage_push(Stage& S, bool take, uint32_t w, uint
 
 // Workgroup-wide (every thread calls it): when more than `keep_below` claims are staged,
 // reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
-// offsets and chunk starts to frontier `nx`.
+// offsets and chunk starts to frontier `nx`.  Always a barrier.
 __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below) {
   __syncthreads();
   const uint32_t n = S.cnt;
@@ -170,7 +225,6 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
     unsigned long long b = atomicAdd(&c->next_packed, ((unsigned long long)n << 32) | (unsigned long long)total);
     S.base_hi = (uint32_t)(b >> 32);
     S.base_lo = (uint32_t)b;
-    S.total = total;
   }
   __syncthreads();
   const uint32_t bh = S.base_hi, bl = S.base_lo;
@@ -188,41 +242,15 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
   __syncthreads();
 }
 
-// Block-wide sum of a per-thread value; result valid in thread 0.
-__device__ __forceinline__ unsigned long long block_sum64(unsigned long long x, unsigned long long* s_red) {
-  x = wave_sum64(x);
-  if (lane() == 0) s_red[wave_id()] = x;
-  __syncthreads();
-  unsigned long long t = 0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < kBlock / 64; ++i) t += s_red[i];
-  __syncthreads();
-  return t;
-}
-
-// Returns true in every thread of the workgroup that finished the launch last.
-__device__ __forceinline__ bool last_block(Ctrl* c, uint32_t* s_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    uint32_t t = atomicAdd(&c->ticket, 1u);
-    *s_flag = (t == gridDim.x - 1) ? 1u : 0u;
-  }
-  __syncthreads();
-  bool last = *s_flag != 0u;
-  if (last) __threadfence();
-  return last;
-}
-
 // ------------------------------------------------------------------------------------
 // learned policy: 12 -> 32 -> 32 -> 3 ReLU MLP, evaluated by one wave
 
 // Raw features of the level just closed (same order as the training code; the blob
 // carries the standardisation).
-__device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges,
-                               bool bu, float* f) {
+__device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges, bool bu,
+                               float* f) {
   float n = a.n_total > 0 ? (float)a.n_total : 1.0f;
-  float nv = a.nverts ? (float)(*a.nverts) : n;
+  float nv = a.num_valid > 0 ? (float)a.num_valid : 1.0f;
   float found = (float)c->found;
   f[0] = log1pf((float)next);
   f[1] = log1pf((float)nedges);
@@ -234,11 +262,11 @@ __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next,
   f[7] = log1pf((float)a.avg);
   f[8] = bu ? 1.0f : 0.0f;
   f[9] = (float)c->switched_back;
-  f[10] = found / fmaxf(nv, 1.0f);
+  f[10] = found / nv;
   f[11] = (float)c->phase;
 }
 
-// Wave 0 of the closing workgroup evaluates the MLP; returns the argmax action in lane 0.
+// Wave 0 evaluates the MLP from weights staged in LDS; returns the argmax action.
 __device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
   const float* W1 = blob;
   const float* b1 = W1 + kMlpIn * kMlpH;
@@ -264,144 +292,21 @@ __device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
   __builtin_amdgcn_wave_barrier();
   if (l < kMlpH) s_h[l] = h2;
   __builtin_amdgcn_wave_barrier();
+  float zo = -3.0e38f;
+  if (l < kMlpOut) {
+    float acc = b3[l];
+    for (int i = 0; i < kMlpH; ++i) acc += W3[l * kMlpH + i] * s_h[i];
+    zo = acc;
+  }
+  float z0 = __shfl(zo, 0, 64), z1 = __shfl(zo, 1, 64), z2 = __shfl(zo, 2, 64);
   int best = 0;
-  if (l == 0) {
-    float bv = -3.0e38f;
-    for (int o = 0; o < kMlpOut; ++o) {
-      float acc = b3[o];
-      for (int i = 0; i < kMlpH; ++i) acc += W3[o * kMlpH + i] * s_h[i];
-      if (acc > bv) {
-        bv = acc;
-        best = o;
-      }
-    }
+  float bv = z0;
+  if (z1 > bv) {
+    bv = z1;
+    best = 1;
   }
+  if (z2 > bv) best = 2;
   return best;
-}
-
-// Closes a level (called by the whole last workgroup).
-__device__ void close_level(const LevelArgs& a, int op, uint32_t q_size, uint32_t q_edges, bool bu) {
-  Ctrl* c = a.ctrl;
-  __shared__ float s_feat[kMlpIn];
-  __shared__ float s_h[kMlpH];
-  __shared__ int s_act;
-  unsigned long long p = load_acquire_u64(&c->next_packed);
-  uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
-  bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
-  bool learned = reach && a.policy == MSC_POLICY_LEARNED && a.mlp;
-  if (threadIdx.x == 0) {
-    c->found += next;
-    c->level += 1;
-    // the reference's switch rule (multistep/scc_fwbw.cpp:168-195, cc_bfs.cpp:167-194)
-    int act = bu ? 1 : 0;
-    if (reach) {
-      if (!c->switched_back && !bu) {
-        double edges_frontier = (double)next * a.avg;
-        double edges_remainder = (double)((int64_t)a.n_total - c->found) * a.avg;
-        if (edges_remainder / a.alpha < edges_frontier) act = 1;
-      } else if (!c->switched_back && (a.n_total / a.beta) > (int32_t)next) {
-        act = 0;
-        c->switched_back = 1;
-      }
-    }
-    s_act = act;
-    if (learned) level_features(c, a, next, nedges, bu, s_feat);
-  }
-  __syncthreads();
-  if (learned && threadIdx.x < 64) {
-    int act = mlp_action(a.mlp, s_feat, s_h);
-    if (threadIdx.x == 0) s_act = act;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int act = s_act;
-    unsigned long long examined = c->examined, inspected = c->inspected, raised = c->raised;
-    // algorithmic bytes of this level (unit costs documented in DESIGN.md)
-    unsigned long long bytes;
-    if (op == OP_PUSH)
-      bytes = 12ull * q_size + 9ull * q_edges + 4ull * raised + 4ull * next;
-    else if (op == OP_CBW)
-      bytes = 12ull * q_size + 9ull * q_edges + 5ull * next;
-    else if (bu)
-      bytes = (a.nverts ? 1ull * (*a.nverts) : 0ull) + 8ull * examined + 5ull * inspected + 5ull * next;
-    else
-      bytes = 12ull * q_size + 5ull * q_edges + 5ull * next;
-    c->bytes_op[op] += bytes;
-    if (bu) c->bu_levels += 1;
-    if (a.recs && c->nrec < a.rec_cap) {
-      msc_level_rec& r = a.recs[c->nrec];
-      r.phase = (int32_t)c->phase;
-      r.level = (int32_t)c->level;
-      r.bottom_up = bu ? 1 : 0;
-      r.next_size = (int32_t)next;
-      r.q_size = q_size;
-      r.q_edges = q_edges;
-      r.next_edges = nedges;
-      r.examined = (uint32_t)examined;
-      r.inspected = inspected;
-      r.found = c->found;
-      r.n_total = a.n_total;
-      r.num_valid = a.nverts ? (int32_t)(*a.nverts) : a.n_total;
-      r.action = act;
-      r.pad = 0;
-    }
-    c->nrec += 1;
-    c->q_size = next;
-    c->q_edges = nedges;
-    c->cur ^= 1u;
-    c->bottom_up = act == 1 ? 1u : 0u;
-    c->next_packed = 0ull;
-    c->examined = 0ull;
-    c->inspected = 0ull;
-    c->raised = 0ull;
-    c->done = (next == 0u || act == 2) ? 1u : 0u;
-    if (act == 2) c->abandoned = 1u;  // the host discards this search's result
-    __threadfence();
-    c->ticket = 0u;
-  }
-}
-
-// Closes a seeding launch: the queue built in buffer 1 becomes the frontier.
-__device__ void close_seed(Ctrl* c) {
-  if (threadIdx.x == 0) {
-    unsigned long long p = load_acquire_u64(&c->next_packed);
-    uint32_t next = (uint32_t)(p >> 32);
-    c->q_size = next;
-    c->q_edges = (uint32_t)p;
-    c->cur = 1u;
-    c->next_packed = 0ull;
-    c->done = next == 0u ? 1u : 0u;
-    __threadfence();
-    c->ticket = 0u;
-  }
-}
-
-// ------------------------------------------------------------------------------------
-// ordered compaction of one tile: kTile items, thread t holds items base + r*kBlock + t
-// (r < 16) with flags in bit r of `keep`; survivors are written in item order at a range
-// reserved with one atomic per tile.
-
-struct TileLds {
-  uint32_t pref[16][kBlock / 64];
-  uint32_t base;
-};
-
-__device__ __forceinline__ void tile_reserve(TileLds& T, const unsigned long long* balls, uint32_t* counter) {
-  // balls[r] is this wave's ballot of round r
-  if (lane() == 0)
-    for (int r = 0; r < 16; ++r) T.pref[r][wave_id()] = (uint32_t)__popcll(balls[r]);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int r = 0; r < 16; ++r)
-      for (int w = 0; w < kBlock / 64; ++w) {
-        uint32_t k = T.pref[r][w];
-        T.pref[r][w] = run;
-        run += k;
-      }
-    T.base = run ? atomicAdd(counter, run) : 0u;
-  }
-  __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------
@@ -410,6 +315,11 @@ __device__ __forceinline__ void tile_reserve(TileLds& T, const unsigned long lon
 __global__ void k_fill_u32(uint32_t* p, uint32_t value, int64_t count) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = value;
+}
+
+__global__ void k_iota(int32_t* p, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
 }
 
 // Word-wise set algebra on bitmaps: out = a & ~b (b null: out = a), optionally mirrored
@@ -423,47 +333,97 @@ __global__ void k_andnot(uint32_t* __restrict__ out, uint32_t* __restrict__ out2
   }
 }
 
-__global__ void k_iota(int32_t* p, int64_t count) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = (int32_t)i;
+// Sums columns 0..2 and takes the max of column 3 over nparts rows of 4 (one workgroup).
+__global__ void __launch_bounds__(kBlock) k_reduce(const unsigned long long* __restrict__ part, int nparts,
+                                                   unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long s0 = 0, s1 = 0, s2 = 0, mx = 0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    s0 += part[4 * i];
+    s1 += part[4 * i + 1];
+    s2 += part[4 * i + 2];
+    mx = part[4 * i + 3] > mx ? part[4 * i + 3] : mx;
+  }
+  s0 = block_sum64(s0, s_red);
+  s1 = block_sum64(s1, s_red);
+  s2 = block_sum64(s2, s_red);
+  mx = wave_max64(mx);
+  __shared__ unsigned long long s_mx[kBlock / 64];
+  if (lane() == 0) s_mx[wave_id()] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kBlock / 64; ++w) mx = s_mx[w] > mx ? s_mx[w] : mx;
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+    out[3] = mx;
+  }
 }
 
 // Degree trim (multistep/scc_trim.cpp:89-139; kokkos_scc/scc_trim.cpp:57-140) and the CC
 // variant (multistep/cc_trim.cpp:47-97).  Each wave round covers 64 consecutive vertices,
-// whose ballot is two bitmap words; survivors are appended in id order per tile.
+// whose ballot is two words of the valid bitmap; part col 0 = survivors of the workgroup.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __restrict__ oo,
                                                  const uint32_t* __restrict__ io, uint32_t* __restrict__ valid,
-                                                 int32_t* __restrict__ labels, int32_t* __restrict__ verts,
-                                                 uint32_t* __restrict__ nverts) {
-  __shared__ TileLds T;
+                                                 int32_t* __restrict__ labels, unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long s_red[kBlock / 64];
   const int64_t nw = ((int64_t)n + 31) / 32;
-  for (int64_t base = (int64_t)blockIdx.x * kTile; base < n; base += (int64_t)gridDim.x * kTile) {
-    unsigned long long balls[16];
-    uint32_t keep_bits = 0;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      int64_t v = base + r * kBlock + threadIdx.x;
-      bool keep = false;
-      if (v < n) {
-        uint32_t od = oo[v + 1] - oo[v], id = io[v + 1] - io[v];
-        keep = MIRROR ? (od + id) > 0u : (od > 0u && id > 0u);
-        labels[v] = keep ? -1 : (int32_t)v;
-      }
-      unsigned long long mask = __ballot(keep);
-      balls[r] = mask;
-      keep_bits |= keep ? (1u << r) : 0u;
-      int64_t word = (base + r * kBlock + (threadIdx.x & ~63)) >> 5;
-      if (lane() == 0 && word < nw) valid[word] = (uint32_t)mask;
-      if (lane() == 32 && word + 1 < nw) valid[word + 1] = (uint32_t)(mask >> 32);
+  unsigned long long kept = 0;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); base < n; base += (int64_t)gridDim.x * kBlock) {
+    int64_t v = base + lane();
+    bool keep = false;
+    if (v < n) {
+      uint32_t od = oo[v + 1] - oo[v], id = io[v + 1] - io[v];
+      keep = MIRROR ? (od + id) > 0u : (od > 0u && id > 0u);
+      labels[v] = keep ? -1 : (int32_t)v;
     }
-    tile_reserve(T, balls, nverts);
-#pragma unroll
-    for (int r = 0; r < 16; ++r)
-      if (keep_bits >> r & 1u)
-        verts[T.base + T.pref[r][wave_id()] + __popcll(balls[r] & lanes_below())] =
-            (int32_t)(base + r * kBlock + threadIdx.x);
-    __syncthreads();
+    unsigned long long mask = __ballot(keep);
+    int64_t word = base >> 5;
+    if (lane() == 0) valid[word] = (uint32_t)mask;
+    if (lane() == 32 && word + 1 < nw) valid[word + 1] = (uint32_t)(mask >> 32);
+    kept += keep ? 1 : 0;
+  }
+  kept = block_sum64(kept, s_red);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = kept;
+    part[4 * blockIdx.x + 1] = part[4 * blockIdx.x + 2] = part[4 * blockIdx.x + 3] = 0;
+  }
+}
+
+// labels[v] = label for each set bit v of `member`; part col 0 = members of the workgroup.
+__global__ void __launch_bounds__(kBlock) k_label_bits(const uint32_t* __restrict__ member, int64_t nwords,
+                                                       int32_t n, int32_t* __restrict__ labels, int32_t label,
+                                                       unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
+    uint32_t w = member[i];
+    cnt += (uint32_t)__popc(w);
+    while (w) {
+      uint32_t b = (uint32_t)__ffs(w) - 1u;
+      int64_t v = i * 32 + b;
+      if (v < n) labels[v] = label;
+      w &= w - 1u;
+    }
+  }
+  cnt = block_sum64(cnt, s_red);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = cnt;
+    part[4 * blockIdx.x + 1] = part[4 * blockIdx.x + 2] = part[4 * blockIdx.x + 3] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_count_bits(const uint32_t* __restrict__ bits, int64_t nwords,
+                                                       unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock)
+    cnt += (uint32_t)__popc(bits[i]);
+  cnt = block_sum64(cnt, s_red);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = cnt;
+    part[4 * blockIdx.x + 1] = part[4 * blockIdx.x + 2] = part[4 * blockIdx.x + 3] = 0;
   }
 }
 
@@ -481,11 +441,10 @@ __global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
-    c->ticket = 0u;
     c->found = 0;
     c->phase = (uint32_t)phase;
     c->abandoned = 0u;
-    c->examined = c->inspected = c->raised = 0ull;
+    c->t_start = ~0ull;
     uint32_t m = 1u << ((uint32_t)root & 31u);
     a.cand[(uint32_t)root >> 5] &= ~m;
     a.fr0.q[0] = root;
@@ -506,18 +465,32 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
-    c->ticket = 0u;
     c->found = 0;
     c->phase = (uint32_t)phase;
     c->abandoned = 0u;
-    c->examined = c->inspected = c->raised = 0ull;
+    c->t_start = ~0ull;
   }
 }
 
-// One level of a search or a colour sweep.  Top-down levels: edge-balanced expansion of
-// the current frontier (the "Manhattan collapse" of kokkos_scc/fwbw_manhattan_global.cpp:
-// 162-289, with an LDS max-scan instead of a per-edge binary search).  Bottom-up levels
-// (reach ops only): one candidate per lane (multistep/scc_fwbw.cpp:118-148).
+// Closes a seeding launch: the queue built in buffer 1 becomes the frontier.
+__global__ void k_close_seed(Ctrl* c) {
+  if (threadIdx.x == 0) {
+    unsigned long long p = c->next_packed;
+    uint32_t next = (uint32_t)(p >> 32);
+    c->q_size = next;
+    c->q_edges = (uint32_t)p;
+    c->cur = 1u;
+    c->next_packed = 0ull;
+    c->done = next == 0u ? 1u : 0u;
+  }
+}
+
+// One level of a search or a colour sweep (work only; k_close ends it).  Top-down levels:
+// edge-balanced expansion of the current frontier (the "Manhattan collapse" of
+// kokkos_scc/fwbw_manhattan_global.cpp:162-289, with an LDS max-scan instead of a per-edge
+// binary search).  Bottom-up levels (reach ops only): every set bit of `cand` is a
+// candidate; the first parent that is allowed and no longer claimable claims it
+// (multistep/scc_fwbw.cpp:118-148).  part row = (examined, inspected, raised, 0).
 template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
@@ -530,7 +503,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   __shared__ uint32_t s_base2[MIRROR ? kChunk + 1 : 1];
   __shared__ uint32_t s_scan[kBlock / 64];
   __shared__ unsigned long long s_red[kBlock / 64];
-  __shared__ uint32_t s_flag;
+  __shared__ BitIter B;
+  __shared__ uint32_t s_claim[kBW];
 
   Ctrl* c = a.ctrl;
   if (c->done) return;  // uniform: the search already ended
@@ -543,7 +517,10 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
   const int t = threadIdx.x;
   unsigned long long my_examined = 0, my_inspected = 0, my_raised = 0;
-  if (t == 0) S.cnt = 0u;
+  if (t == 0) {
+    S.cnt = 0u;
+    if (a.recs) atomicMin(&c->t_start, (unsigned long long)wall_clock64());
+  }
   __syncthreads();
 
   if (!bu) {
@@ -662,149 +639,259 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
       stage_flush(S, N, c, kStage - kChunk);  // also the end-of-chunk barrier
     }
   } else if (REACH) {
-    const uint32_t nv = *a.nverts;
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t bbase = blockIdx.x * kBlock; bbase < nv; bbase += stride) {
-      uint32_t i = bbase + (uint32_t)t;
-      bool cand = false, hit = false;
-      uint32_t v = 0;
-      if (i < nv) {
-        v = (uint32_t)a.verts[i];
-        cand = bit_test(a.cand, v);
+    // A workgroup owns kBW words of `cand` per iteration (up to kR candidates per thread).
+    // Each candidate's first up-list entry is checked with all loads of the thread's
+    // candidates in flight together; most candidates are settled there (on R-MAT the
+    // first in-neighbour is already visited ~90% of the time), the rest scan on.  A
+    // parent is a visited vertex: allowed and no longer claimable.  Claims collect in LDS
+    // word masks and are written back with one plain store per word: no other workgroup
+    // clears bits of these words during a bottom-up level.
+    constexpr int kR = kBW * 32 / kBlock;
+    for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
+      if (t < kBW) s_claim[t] = 0u;
+      const uint32_t total = bits_load(B, a.cand, a.nwords, wb);
+      uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
+      bool has[kR], hit[kR], first[kR];
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        uint32_t k = (uint32_t)(r * kBlock + t);
+        has[r] = k < total;
+        v[r] = has[r] ? bits_pick(B, wb, k) : 0u;
       }
-      if (cand) {
-        // a parent is a visited vertex: allowed and no longer claimable
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        beg[r] = end[r] = beg2[r] = end2[r] = 0u;
+        if (has[r]) {
+          beg[r] = a.up_off[v[r]];
+          end[r] = a.up_off[v[r] + 1];
+          if (MIRROR) {
+            beg2[r] = a.up_off2[v[r]];
+            end2[r] = a.up_off2[v[r] + 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        u[r] = 0u;
+        first[r] = false;
+        if (has[r]) {
+          if (beg[r] < end[r]) {
+            u[r] = (uint32_t)a.up_adj[beg[r]++];
+            first[r] = true;
+          } else if (MIRROR && beg2[r] < end2[r]) {
+            u[r] = (uint32_t)a.up_adj2[beg2[r]++];
+            first[r] = true;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
+        wa[r] = first[r] ? a.allow[u[r] >> 5] : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        hit[r] = false;
+        if (!has[r]) continue;
         ++my_examined;
-        for (uint32_t j = a.up_off[v], e = a.up_off[v + 1]; j < e; ++j) {
+        if (first[r]) {
           ++my_inspected;
-          uint32_t u = (uint32_t)a.up_adj[j];
-          if (!bit_test(a.cand, u) && bit_test(a.allow, u)) {
-            hit = true;
-            break;
-          }
+          uint32_t b = 1u << (u[r] & 31u);
+          hit[r] = !(wc[r] & b) && (wa[r] & b);
         }
-        if (MIRROR && !hit) {
-          for (uint32_t j = a.up_off2[v], e = a.up_off2[v + 1]; j < e; ++j) {
+        for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
+          ++my_inspected;
+          uint32_t x = (uint32_t)a.up_adj[j];
+          hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
+        }
+        if (MIRROR)
+          for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
             ++my_inspected;
-            uint32_t u = (uint32_t)a.up_adj2[j];
-            if (!bit_test(a.cand, u) && bit_test(a.allow, u)) {
-              hit = true;
-              break;
-            }
+            uint32_t x = (uint32_t)a.up_adj2[j];
+            hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
           }
-        }
+        if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
       }
-      bool take = hit && bit_clear(a.cand, v);
-      uint32_t deg = take ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v) : 0u;
-      stage_push(S, take, v, deg);
-      stage_flush(S, N, c, kStage - kBlock);
+#pragma unroll
+      for (int r = 0; r < kR; ++r) {
+        uint32_t deg = hit[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]) : 0u;
+        stage_push(S, hit[r], v[r], deg);
+      }
+      __syncthreads();
+      if (t < kBW && s_claim[t]) a.cand[wb + t] = B.word[t] & ~s_claim[t];
+      stage_flush(S, N, c, kStage - kBW * 32);  // also the barrier before the next words
     }
   }
   stage_flush(S, N, c, 0u);
 
-  // grid-wide statistics, one atomic per workgroup
   unsigned long long ex = block_sum64(my_examined, s_red);
   unsigned long long in = block_sum64(my_inspected, s_red);
   unsigned long long ra = block_sum64(my_raised, s_red);
   if (t == 0) {
-    if (ex) atomicAdd(&c->examined, ex);
-    if (in) atomicAdd(&c->inspected, in);
-    if (ra) atomicAdd(&c->raised, ra);
+    unsigned long long* p = a.part + 4 * blockIdx.x;
+    p[0] = ex;
+    p[1] = in;
+    p[2] = ra;
+    p[3] = 0;
   }
-  if (last_block(c, &s_flag)) close_level(a, OP, q_size, q_edges, bu);
 }
 
-// Ordered filter of a vertex list: every listed vertex that is in `member` gets `label`
-// and is dropped (FW-BW commit, multistep/scc_fwbw.cpp:412-466; CC-BFS commit,
-// cc_bfs.cpp:217-271; the caller clears the members' valid bits word by word); with
-// member == null it only keeps the still-valid vertices (multistep/scc_color.cpp:292-337).
-// Survivors keep their list order within a tile.
-__global__ void __launch_bounds__(kBlock) k_filter(const int32_t* __restrict__ verts, const uint32_t* __restrict__ nverts,
-                                                   const uint32_t* __restrict__ member, uint32_t* __restrict__ valid,
-                                                   int32_t* __restrict__ labels, int32_t label,
-                                                   int32_t* __restrict__ verts_out, uint32_t* __restrict__ nverts_out) {
-  __shared__ TileLds T;
-  const uint32_t nv = *nverts;
-  for (uint32_t base = blockIdx.x * kTile; base < nv; base += gridDim.x * kTile) {
-    unsigned long long balls[16];
-    uint32_t keep_bits = 0;
-    int32_t vs[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      uint32_t i = base + r * kBlock + threadIdx.x;
-      bool keep = false;
-      vs[r] = 0;
-      if (i < nv) {
-        uint32_t v = (uint32_t)verts[i];
-        vs[r] = (int32_t)v;
-        if (member && bit_test(member, v)) {
-          labels[v] = label;
-        } else {
-          keep = bit_test(valid, v);
-        }
-      }
-      balls[r] = __ballot(keep);
-      keep_bits |= keep ? (1u << r) : 0u;
-    }
-    tile_reserve(T, balls, nverts_out);
-#pragma unroll
-    for (int r = 0; r < 16; ++r)
-      if (keep_bits >> r & 1u) verts_out[T.base + T.pref[r][wave_id()] + __popcll(balls[r] & lanes_below())] = vs[r];
-    __syncthreads();
+// Closes the level that just ran (one workgroup): sums the partials, applies the direction
+// rule -- the reference's (multistep/scc_fwbw.cpp:168-195, cc_bfs.cpp:167-194), the MLP, or
+// a forced schedule --, accounts algorithmic bytes, writes a level record and prepares the
+// next level.
+__global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int nparts) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  __shared__ float s_w[kMlpBlob];
+  __shared__ float s_feat[kMlpIn];
+  __shared__ float s_h[kMlpH];
+  __shared__ int s_act;
+  Ctrl* c = a.ctrl;
+  if (c->done) return;
+  unsigned long long ex = 0, in = 0, ra = 0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    ex += a.part[4 * i];
+    in += a.part[4 * i + 1];
+    ra += a.part[4 * i + 2];
   }
+  ex = block_sum64(ex, s_red);
+  in = block_sum64(in, s_red);
+  ra = block_sum64(ra, s_red);
+  const bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
+  const bool learned = reach && a.policy == MSC_POLICY_LEARNED && a.mlp;
+  const unsigned long long p = c->next_packed;
+  const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
+  const uint32_t q_size = c->q_size, q_edges = c->q_edges;
+  const bool bu = reach && c->bottom_up != 0u;
+  if (threadIdx.x == 0) {
+    c->found += next;
+    c->level += 1;
+    int act = bu ? 1 : 0;
+    if (reach) {
+      if (!c->switched_back && !bu) {
+        double edges_frontier = (double)next * a.avg;
+        double edges_remainder = (double)((int64_t)a.n_total - c->found) * a.avg;
+        if (edges_remainder / a.alpha < edges_frontier) act = 1;
+      } else if (!c->switched_back && (a.n_total / a.beta) > (int32_t)next) {
+        act = 0;
+        c->switched_back = 1;
+      }
+    }
+    s_act = act;
+    if (reach && (learned || a.recs)) level_features(c, a, next, nedges, bu, s_feat);
+  }
+  if (learned)
+    for (int i = threadIdx.x; i < kMlpBlob; i += blockDim.x) s_w[i] = a.mlp[i];
+  __syncthreads();
+  if (learned && threadIdx.x < 64) {
+    int act = mlp_action(s_w, s_feat, s_h);
+    if (threadIdx.x == 0) s_act = act;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int act = s_act;
+  if (reach && a.sched && c->level <= (uint32_t)MSC_SCHED_LEVELS) {
+    int f = a.sched[c->phase * MSC_SCHED_LEVELS + c->level - 1];
+    if (f >= 0 && f <= 2) act = f;
+  }
+  // algorithmic bytes of this level (unit costs documented in DESIGN.md)
+  unsigned long long bytes;
+  if (op == OP_PUSH)
+    bytes = 12ull * q_size + 9ull * q_edges + 4ull * ra + 4ull * next;
+  else if (op == OP_CBW)
+    bytes = 12ull * q_size + 9ull * q_edges + 5ull * next;
+  else if (bu)
+    bytes = 4ull * (unsigned long long)a.nwords + 8ull * ex + 5ull * in + 5ull * next;
+  else
+    bytes = 12ull * q_size + 5ull * q_edges + 5ull * next;
+  c->bytes_op[op] += bytes;
+  if (bu) c->bu_levels += 1;
+  if (a.recs && c->nrec < a.rec_cap) {
+    msc_level_rec& r = a.recs[c->nrec];
+    r.phase = (int32_t)c->phase;
+    r.level = (int32_t)c->level;
+    r.bottom_up = bu ? 1 : 0;
+    r.next_size = (int32_t)next;
+    r.q_size = q_size;
+    r.q_edges = q_edges;
+    r.next_edges = nedges;
+    r.examined = (uint32_t)ex;
+    r.inspected = in;
+    r.found = c->found;
+    r.n_total = a.n_total;
+    r.num_valid = a.num_valid;
+    r.action = act;
+    r.pad = 0;
+    for (int i = 0; i < kMlpIn; ++i) r.feat[i] = reach ? s_feat[i] : 0.0f;
+    unsigned long long t0 = c->t_start, t1 = wall_clock64();
+    r.level_us = (t0 != ~0ull && t1 > t0 && a.clock_mhz > 0.0f) ? (float)(t1 - t0) / a.clock_mhz : 0.0f;
+    r.pad2 = 0;
+  }
+  c->t_start = ~0ull;
+  c->nrec += 1;
+  c->q_size = next;
+  c->q_edges = nedges;
+  c->cur ^= 1u;
+  c->bottom_up = act == 1 ? 1u : 0u;
+  c->next_packed = 0ull;
+  c->done = (next == 0u || act == 2) ? 1u : 0u;
+  if (act == 2) c->abandoned = 1u;  // the host discards this search's result
 }
 
 // Colour initialisation (multistep/scc_color.cpp:56-63, cc_color.cpp:62-69): colors[v] = v
-// for every valid vertex, all of them queued.  Needs k_ctrl_reset first.
+// for every valid vertex, all of them queued.  Needs k_ctrl_reset first, k_close_seed after.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_seed_colors(LevelArgs a) {
   __shared__ Stage S;
-  __shared__ uint32_t s_flag;
+  __shared__ BitIter B;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
-  const uint32_t nv = *a.nverts;
-  for (uint32_t bbase = blockIdx.x * kBlock; bbase < nv; bbase += gridDim.x * kBlock) {
-    uint32_t i = bbase + threadIdx.x;
-    bool take = i < nv;
-    uint32_t v = 0, deg = 0;
-    if (take) {
-      v = (uint32_t)a.verts[i];
-      a.colors[v] = (int32_t)v;
-      deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
+  for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
+    const uint32_t total = bits_load(B, a.valid, a.nwords, wb);
+    for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
+      const uint32_t k = k0 + threadIdx.x;
+      bool take = k < total;
+      uint32_t v = 0, deg = 0;
+      if (take) {
+        v = bits_pick(B, wb, k);
+        a.colors[v] = (int32_t)v;
+        deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
+      }
+      stage_push(S, take, v, deg);
     }
-    stage_push(S, take, v, deg);
-    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBW * 32);
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
-  if (last_block(a.ctrl, &s_flag)) close_seed(a.ctrl);
 }
 
 // Colour roots (multistep/scc_color.cpp:165-212): colors[v] == v labels v, removes it and
-// queues it for the per-colour backward search.  Needs k_ctrl_reset first.
+// queues it for the per-colour backward search.  Needs k_ctrl_reset first, k_close_seed
+// after.
 __global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
   __shared__ Stage S;
-  __shared__ uint32_t s_flag;
+  __shared__ BitIter B;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
-  const uint32_t nv = *a.nverts;
-  for (uint32_t bbase = blockIdx.x * kBlock; bbase < nv; bbase += gridDim.x * kBlock) {
-    uint32_t i = bbase + threadIdx.x;
-    bool take = false;
-    uint32_t v = 0, deg = 0;
-    if (i < nv) {
-      v = (uint32_t)a.verts[i];
-      if (a.colors[v] == (int32_t)v) {
-        take = true;
-        a.labels[v] = (int32_t)v;
-        bit_clear(a.valid, v);
-        deg = a.down_off[v + 1] - a.down_off[v];
+  for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
+    const uint32_t total = bits_load(B, a.valid, a.nwords, wb);
+    for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
+      const uint32_t k = k0 + threadIdx.x;
+      bool take = false;
+      uint32_t v = 0, deg = 0;
+      if (k < total) {
+        v = bits_pick(B, wb, k);
+        if (a.colors[v] == (int32_t)v) {
+          take = true;
+          a.labels[v] = (int32_t)v;
+          bit_clear(a.valid, v);
+          deg = a.down_off[v + 1] - a.down_off[v];
+        }
       }
+      stage_push(S, take, v, deg);
     }
-    stage_push(S, take, v, deg);
-    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBW * 32);
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
-  if (last_block(a.ctrl, &s_flag)) close_seed(a.ctrl);
 }
 
 // Canonical labels: every component relabelled to its minimum vertex id.  mins[] starts
@@ -836,10 +923,11 @@ __global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int
 }
 
 // Pivot of create_csr (multistep/scc_main.cpp:169-183): largest (double)out*(double)in,
-// first vertex on ties.  Pass 1: max of the product's bit pattern (non-negative doubles
-// order like their bits); pass 2: smallest vertex attaining it.
+// first vertex on ties.  Pass 1: per-workgroup max of the product's bit pattern
+// (non-negative doubles order like their bits), reduced by k_reduce; pass 2: smallest
+// vertex attaining it.
 __global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t* __restrict__ oo,
-                                                      const uint32_t* __restrict__ io, unsigned long long* work) {
+                                                      const uint32_t* __restrict__ io, unsigned long long* part) {
   __shared__ unsigned long long s_red[kBlock / 64];
   unsigned long long best = 0;
   for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
@@ -852,22 +940,25 @@ __global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t*
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
-    atomicMax(&work[0], best);
+    unsigned long long* p = part + 4 * blockIdx.x;
+    p[0] = p[1] = p[2] = 0;
+    p[3] = best;
   }
 }
+// red[3] holds the max; red[1] receives the first vertex attaining it (~0 when none).
 __global__ void k_pivot_first(int32_t n, const uint32_t* __restrict__ oo, const uint32_t* __restrict__ io,
-                              unsigned long long* work) {
-  const unsigned long long target = work[0];
+                              unsigned long long* red) {
+  const unsigned long long target = red[3];
   if (target == 0ull) return;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     double p = (double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]);
-    if ((unsigned long long)__double_as_longlong(p) == target) atomicMin(&work[1], (unsigned long long)v);
+    if ((unsigned long long)__double_as_longlong(p) == target) atomicMin(&red[1], (unsigned long long)v);
   }
 }
 // CC pivot (multistep/cc_main.cpp:148-160): largest out+in degree, first vertex on ties,
-// as one 64-bit max of (degree << 32 | ~v).
+// as the max of (degree << 32 | ~v).
 __global__ void __launch_bounds__(kBlock) k_pivot_cc(int32_t n, const uint32_t* __restrict__ oo,
-                                                     const uint32_t* __restrict__ io, unsigned long long* work) {
+                                                     const uint32_t* __restrict__ io, unsigned long long* part) {
   __shared__ unsigned long long s_red[kBlock / 64];
   unsigned long long best = 0;
   for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
@@ -881,7 +972,9 @@ __global__ void __launch_bounds__(kBlock) k_pivot_cc(int32_t n, const uint32_t* 
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
-    atomicMax(&work[0], best);
+    unsigned long long* p = part + 4 * blockIdx.x;
+    p[0] = p[1] = p[2] = 0;
+    p[3] = best;
   }
 }
 
@@ -896,11 +989,11 @@ __global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ of
   if (bad) atomicOr(err, 1u);
 }
 
-inline dim3 grid_for(int64_t items, int64_t per_block, int cap) {
+inline int grid_for(int64_t items, int64_t per_block, int cap) {
   int64_t b = (items + per_block - 1) / per_block;
   if (b < 1) b = 1;
   if (b > cap) b = cap;
-  return dim3((unsigned)b);
+  return (int)b;
 }
 
 }  // namespace
@@ -910,28 +1003,26 @@ inline dim3 grid_for(int64_t items, int64_t per_block, int cap) {
 
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_fill_u32, grid_for(count, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, p, value, count);
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, p, value,
+                     count);
 }
 
-void launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io, uint32_t* valid,
-                 int32_t* labels, int32_t* verts, uint32_t* nverts) {
-  if (n <= 0) return;
-  dim3 g = grid_for(n, kTile, L.grid * 4);
+int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io, uint32_t* valid,
+                int32_t* labels, unsigned long long* part) {
+  int g = grid_for(n, kBlock * 8, L.grid * 2);
   if (mirror)
-    hipLaunchKernelGGL(k_trim<true>, g, dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, verts, nverts);
+    hipLaunchKernelGGL(k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, part);
   else
-    hipLaunchKernelGGL(k_trim<false>, g, dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, verts, nverts);
+    hipLaunchKernelGGL(k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, part);
+  return g;
+}
+
+void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, L.stream, part, nparts, out);
 }
 
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
   hipLaunchKernelGGL(k_search_init, dim3(1), dim3(kBlock), 0, L.stream, a, root, phase);
-}
-
-void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
-                   int64_t words) {
-  if (words <= 0) return;
-  hipLaunchKernelGGL(k_andnot, grid_for(words, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, out, out2, a, b,
-                     words);
 }
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
@@ -949,53 +1040,72 @@ void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   }
 }
 
-void launch_commit(const Launch& L, const int32_t* verts, const uint32_t* nverts, const uint32_t* member,
-                   uint32_t* valid, int32_t* labels, int32_t label, int32_t* verts_out, uint32_t* nverts_out) {
-  hipLaunchKernelGGL(k_filter, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts, member, valid,
-                     labels, label, verts_out, nverts_out);
+void launch_close(const Launch& L, int op, const LevelArgs& a) {
+  hipLaunchKernelGGL(k_close, dim3(1), dim3(kBlock), 0, L.stream, a, op, L.grid);
+}
+
+int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
+                      int32_t label, unsigned long long* part) {
+  int g = grid_for(nwords, kBlock * 4, L.grid * 2);
+  hipLaunchKernelGGL(k_label_bits, dim3(g), dim3(kBlock), 0, L.stream, member, nwords, n, labels, label, part);
+  return g;
+}
+
+int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {
+  int g = grid_for(nwords, kBlock * 4, L.grid * 2);
+  hipLaunchKernelGGL(k_count_bits, dim3(g), dim3(kBlock), 0, L.stream, bits, nwords, part);
+  return g;
+}
+
+void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
+                   int64_t words) {
+  if (words <= 0) return;
+  hipLaunchKernelGGL(k_andnot, dim3(grid_for(words, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, out, out2, a,
+                     b, words);
 }
 
 void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a) {
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 3);
-  if (mirror)
-    hipLaunchKernelGGL(k_seed_colors<true>, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, a);
-  else
-    hipLaunchKernelGGL(k_seed_colors<false>, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, a);
+  dim3 g((unsigned)grid_for(a.nwords, kBW, L.grid));
+  if (mirror) hipLaunchKernelGGL(k_seed_colors<true>, g, dim3(kBlock), 0, L.stream, a);
+  else hipLaunchKernelGGL(k_seed_colors<false>, g, dim3(kBlock), 0, L.stream, a);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
 }
 
 void launch_roots(const Launch& L, const LevelArgs& a) {
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 4);
-  hipLaunchKernelGGL(k_roots, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, a);
-}
-
-void launch_compact(const Launch& L, const int32_t* verts, const uint32_t* nverts, const uint32_t* valid,
-                    int32_t* verts_out, uint32_t* nverts_out) {
-  hipLaunchKernelGGL(k_filter, dim3((unsigned)L.grid), dim3(kBlock), 0, L.stream, verts, nverts,
-                     (const uint32_t*)nullptr, const_cast<uint32_t*>(valid), (int32_t*)nullptr, 0, verts_out,
-                     nverts_out);
+  hipLaunchKernelGGL(k_roots, dim3((unsigned)grid_for(a.nwords, kBW, L.grid)), dim3(kBlock), 0, L.stream, a);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch) {
   if (n <= 0) return;
-  dim3 g = grid_for(n, kBlock, L.grid * 4);
+  dim3 g(grid_for(n, kBlock, L.grid * 4));
   hipLaunchKernelGGL(k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n);
   hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
   hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
 }
 
-void launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* work) {
-  dim3 g = grid_for(n, kBlock, L.grid * 4);
-  hipLaunchKernelGGL(k_pivot_max, g, dim3(kBlock), 0, L.stream, n, oo, io, work);
-  hipLaunchKernelGGL(k_pivot_first, g, dim3(kBlock), 0, L.stream, n, oo, io, work);
+int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part) {
+  int g = grid_for(n, kBlock * 8, L.grid * 2);
+  hipLaunchKernelGGL(k_pivot_max, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part);
+  return g;
 }
 
-void launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* work) {
-  hipLaunchKernelGGL(k_pivot_cc, grid_for(n, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, n, oo, io, work);
+void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* red) {
+  hipLaunchKernelGGL(k_pivot_first, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, oo, io,
+                     red);
+}
+
+int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part) {
+  int g = grid_for(n, kBlock * 8, L.grid * 2);
+  hipLaunchKernelGGL(k_pivot_cc, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part);
+  return g;
 }
 
 void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err) {
-  hipLaunchKernelGGL(k_validate, grid_for(m > n ? m : n, kBlock, L.grid * 4), dim3(kBlock), 0, L.stream, n, m, off,
-                     adj, err);
+  hipLaunchKernelGGL(k_validate, dim3(grid_for(m > n ? m : n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, m,
+                     off, adj, err);
 }
 
 }  // namespace msc

@@ -26,7 +26,6 @@ constexpr uint32_t kRecCap = 1u << 16;
 struct Ev {
   hipEvent_t a, b;
   int cls;
-  bool used;
 };
 
 }  // namespace
@@ -43,14 +42,19 @@ struct msc_ctx {
   int64_t cap_n = -1, cap_e = -1;
   // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
   uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
-  int32_t *labels = nullptr, *colors = nullptr, *verts0 = nullptr, *verts1 = nullptr;
+  int32_t *labels = nullptr, *colors = nullptr;
   Frontier fr0{}, fr1{};
-  uint32_t* counters = nullptr;  // [0] nverts0, [1] nverts1, [2] validate error
-  unsigned long long* work = nullptr;
+  unsigned long long* part = nullptr;  // kMaxParts x 4 per-workgroup partials
+  uint32_t* err = nullptr;             // validation flag
   Ctrl* ctrl = nullptr;
   msc_level_rec* recs = nullptr;
+  // persistent across workspace regrowth: policy weights and forced-schedule buffer
+  void* small = nullptr;
   float* mlp = nullptr;
+  int8_t* sched = nullptr;
+  int8_t* sched_host = nullptr;  // pinned staging
   bool has_mlp = false;
+  float clock_mhz = 100.0f;
   // staging for host-resident inputs
   void* in_buf = nullptr;
   size_t in_bytes = 0;
@@ -69,13 +73,13 @@ namespace {
 std::mutex g_default_mu;
 std::vector<msc_ctx*> g_default;  // one default context per device
 
-#define HIP_TRY(expr)                                   \
-  do {                                                  \
-    hipError_t _e = (expr);                             \
-    if (_e != hipSuccess) {                             \
-      (void)hipGetLastError();                          \
+#define HIP_TRY(expr)                                              \
+  do {                                                             \
+    hipError_t _e = (expr);                                        \
+    if (_e != hipSuccess) {                                        \
+      (void)hipGetLastError();                                     \
       return _e == hipErrorOutOfMemory ? MSC_E_OOM : MSC_E_DEVICE; \
-    }                                                   \
+    }                                                              \
   } while (0)
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -91,18 +95,17 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
     (void)hipFree(c->ws);
     c->ws = nullptr;
   }
-  size_t nw = (size_t)((nn + 31) / 32) + 1;
+  size_t nw = (size_t)((nn + 31) / 32) + kBW;  // slack: bitmap walks load whole kBW-word groups
   size_t nchunks = (size_t)((ee + kChunk - 1) / kChunk) + 2;
   size_t sizes[] = {
-      nw * 4, nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,         // bitmaps
-      (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,  // labels colors verts0 verts1
+      nw * 4, nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,           // bitmaps
+      (size_t)nn * 4, (size_t)nn * 4,                           // labels, colors
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr0
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr1
-      64 * 4,                                                  // counters
-      8 * 8,                                                   // work
-      sizeof(Ctrl),                                            // ctrl
-      (size_t)kRecCap * sizeof(msc_level_rec),                 // records
-      (size_t)kMlpBlob * 4,                                    // mlp
+      (size_t)kMaxParts * 4 * 8,                                // partials
+      256,                                                      // err
+      sizeof(Ctrl),                                             // ctrl
+      (size_t)kRecCap * sizeof(msc_level_rec),                  // records
   };
   size_t total = 0;
   for (size_t s : sizes) total += align256(s);
@@ -124,16 +127,12 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->inq1 = (uint32_t*)parts[5];
   c->labels = (int32_t*)parts[6];
   c->colors = (int32_t*)parts[7];
-  c->verts0 = (int32_t*)parts[8];
-  c->verts1 = (int32_t*)parts[9];
-  c->fr0 = Frontier{(int32_t*)parts[10], (uint32_t*)parts[11], (uint32_t*)parts[12]};
-  c->fr1 = Frontier{(int32_t*)parts[13], (uint32_t*)parts[14], (uint32_t*)parts[15]};
-  c->counters = (uint32_t*)parts[16];
-  c->work = (unsigned long long*)parts[17];
-  c->ctrl = (Ctrl*)parts[18];
-  c->recs = (msc_level_rec*)parts[19];
-  c->mlp = (float*)parts[20];
-  c->has_mlp = false;  // weights must be set again after a regrow
+  c->fr0 = Frontier{(int32_t*)parts[8], (uint32_t*)parts[9], (uint32_t*)parts[10]};
+  c->fr1 = Frontier{(int32_t*)parts[11], (uint32_t*)parts[12], (uint32_t*)parts[13]};
+  c->part = (unsigned long long*)parts[14];
+  c->err = (uint32_t*)parts[15];
+  c->ctrl = (Ctrl*)parts[16];
+  c->recs = (msc_level_rec*)parts[17];
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -151,7 +150,6 @@ int ev_begin(msc_ctx* c, int cls) {
   }
   Ev& e = c->events[c->ev_used];
   e.cls = cls;
-  e.used = true;
   (void)hipEventRecord(e.a, c->stream);
   return (int)c->ev_used++;
 }
@@ -179,39 +177,47 @@ struct Timed {
 
 Launch launch_of(const msc_ctx* c) { return Launch{c->stream, c->grid}; }
 
-int read_u32(msc_ctx* c, const uint32_t* dev, uint32_t* out) {
-  HIP_TRY(hipMemcpyAsync(c->pinned_u32, dev, 4, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  *out = *c->pinned_u32;
-  return MSC_OK;
-}
-
 int read_ctrl(msc_ctx* c) {
   HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MSC_OK;
 }
 
-// Runs level launches until the device control block says done.  Every search and every
-// max-label fixpoint settles within n + 1 levels, so more launches than that mean a fault.
-int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, msc_stats* s) {
+// Reduces `nparts` partial rows into ctrl->reduced and reads the result back.
+int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
   Launch L = launch_of(c);
-  int batch = 2;
+  launch_reduce(L, c->part, nparts, c->ctrl->reduced);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(c->ctrl_host->reduced, c->ctrl->reduced, 32, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::memcpy(out, c->ctrl_host->reduced, 32);
+  return MSC_OK;
+}
+
+// Runs level launches (work + closing kernel) until the control block says done.  Every
+// search and every max-label fixpoint settles within n + 1 levels, so more launches than
+// that mean a fault.
+int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, msc_stats* s, int first_batch) {
+  Launch L = launch_of(c);
+  int batch = first_batch;
   const int64_t limit = (int64_t)a.n_total + 64;
   int64_t launched = 0;
   for (;;) {
     if (launched > limit) return MSC_E_DEVICE;
     launched += batch;
     for (int i = 0; i < batch; ++i) {
-      Timed t(c, kcls);
-      launch_level(L, op, mirror, a);
+      {
+        Timed t(c, kcls);
+        launch_level(L, op, mirror, a);
+      }
+      launch_close(L, op, a);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(c->pinned_u32, &c->ctrl->done, 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (s) s->host_syncs += 1;
     if (*c->pinned_u32) break;
-    batch = batch < 16 ? batch * 2 : 16;
+    batch = batch < 32 ? batch * 2 : 32;
   }
   return MSC_OK;
 }
@@ -226,15 +232,34 @@ LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opt
   a.fr0 = c->fr0;
   a.fr1 = c->fr1;
   a.ctrl = c->ctrl;
+  a.part = c->part;
   a.recs = o->record ? c->recs : nullptr;
   a.rec_cap = kRecCap;
   a.n_total = g->n;
+  a.nwords = ((int64_t)g->n + 31) / 32;
   a.beta = o->beta;
   a.alpha = o->alpha;
   a.avg = avg;
   a.policy = o->policy;
   a.mlp = (o->policy == MSC_POLICY_LEARNED && c->has_mlp) ? c->mlp : nullptr;
+  a.sched = o->schedule ? c->sched : nullptr;
+  a.clock_mhz = c->clock_mhz;
   return a;
+}
+
+// Forced schedule: staged through pinned memory, then copied on the context's stream.
+int upload_schedule(msc_ctx* c, const msc_opts* o) {
+  if (!o->schedule) return MSC_OK;
+  const int total = 3 * MSC_SCHED_LEVELS;
+  if (o->schedule_len < 0 || o->schedule_len > total) return MSC_E_INVALID;
+  HIP_TRY(hipStreamSynchronize(c->stream));  // the staging buffer may still feed a copy
+  for (int i = 0; i < total; ++i) {
+    int8_t v = i < o->schedule_len ? o->schedule[i] : (int8_t)-1;
+    if (v < -1 || v > 2) return MSC_E_INVALID;
+    c->sched_host[i] = v;
+  }
+  HIP_TRY(hipMemcpyAsync(c->sched, c->sched_host, (size_t)total, hipMemcpyHostToDevice, c->stream));
+  return MSC_OK;
 }
 
 // Device copies of a host graph, staged in the context.
@@ -296,13 +321,36 @@ int host_graph_checks(const msc_csr* g) {
 
 int device_validate(msc_ctx* c, const msc_csr* d) {
   Launch L = launch_of(c);
-  launch_fill_u32(L, c->counters + 2, 0u, 1);
-  launch_validate(L, d->n, d->m, d->out_off, d->out_adj, c->counters + 2);
-  launch_validate(L, d->n, d->m, d->in_off, d->in_adj, c->counters + 2);
-  uint32_t err = 0;
-  int rc = read_u32(c, c->counters + 2, &err);
-  if (rc) return rc;
-  return err ? MSC_E_INVALID : MSC_OK;
+  launch_fill_u32(L, c->err, 0u, 1);
+  launch_validate(L, d->n, d->m, d->out_off, d->out_adj, c->err);
+  launch_validate(L, d->n, d->m, d->in_off, d->in_adj, c->err);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(c->pinned_u32, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return *c->pinned_u32 ? MSC_E_INVALID : MSC_OK;
+}
+
+// create_csr's pivot rules on device (multistep/scc_main.cpp:169-183, cc_main.cpp:148-160).
+int device_pivot(msc_ctx* c, const msc_csr* d, bool cc, int32_t* pivot) {
+  Launch L = launch_of(c);
+  unsigned long long red[4];
+  if (!cc) {
+    int np = launch_pivot_scc(L, d->n, d->out_off, d->in_off, c->part);
+    launch_reduce(L, c->part, np, c->ctrl->reduced);
+    launch_fill_u32(L, (uint32_t*)&c->ctrl->reduced[1], 0xffffffffu, 2);
+    launch_pivot_first(L, d->n, d->out_off, d->in_off, c->ctrl->reduced);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c->ctrl_host->reduced, c->ctrl->reduced, 32, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(red, c->ctrl_host->reduced, 32);
+    *pivot = (red[3] != 0ull && red[1] != ~0ull) ? (int32_t)red[1] : -1;
+  } else {
+    int np = launch_pivot_cc(L, d->n, d->out_off, d->in_off, c->part);
+    int rc = reduce_read(c, np, red);
+    if (rc) return rc;
+    *pivot = red[3] ? (int32_t)(0xffffffffu - (uint32_t)(red[3] & 0xffffffffull)) : -1;
+  }
+  return MSC_OK;
 }
 
 msc_ctx* default_ctx(int device, int* rc) {
@@ -325,8 +373,8 @@ msc_ctx* default_ctx(int device, int* rc) {
 enum class Kind { SCC, CC };
 
 // The whole hot path for one graph.
-int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t* labels_out,
-             const msc_opts* o, msc_stats* s) {
+int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t* labels_out, const msc_opts* o,
+             msc_stats* s) {
   const bool cc = kind == Kind::CC;
   int rc = check_args(g_in, labels_out, o, cc);
   if (rc) return rc;
@@ -338,6 +386,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     if (!c) return rc;
   }
   std::lock_guard<std::mutex> lk(c->mu);
+  if (o->policy == MSC_POLICY_LEARNED && !c->has_mlp) return MSC_E_INVALID;  // no weights loaded
   HIP_TRY(hipSetDevice(c->device));
   auto t0 = std::chrono::steady_clock::now();
   if (s) std::memset(s, 0, sizeof(*s));
@@ -348,6 +397,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     if (s) s->pivot = -1;
     return MSC_OK;
   }
+  if ((rc = upload_schedule(c, o))) return rc;
   const int64_t edges = cc ? 2 * g_in->m : g_in->m;
   if ((rc = ensure_workspace(c, n, edges))) return rc;
   msc_csr g;
@@ -355,73 +405,49 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   if (o->validate && (rc = device_validate(c, &g))) return rc;
   Launch L = launch_of(c);
   int32_t* labels = o->labels_where == MSC_DEVICE ? labels_out : c->labels;
+  const int64_t nw = ((int64_t)n + 31) / 32;
+  HIP_TRY(hipMemsetAsync(c->ctrl, 0, sizeof(Ctrl), c->stream));
 
-  // pivot / average degree (create_csr rules)
   if (avg < 0.0) avg = cc ? 2.0 * (double)g.m / (double)n : (double)g.m / (double)n;
   if (pivot < 0) {
     Timed t(c, MSC_K_OTHER);
-    HIP_TRY(hipMemsetAsync(c->work, 0, 16, c->stream));
-    launch_fill_u32(L, (uint32_t*)(c->work + 1), 0xffffffffu, 2);
-    if (cc)
-      launch_pivot_cc(L, n, g.out_off, g.in_off, c->work);
-    else
-      launch_pivot_scc(L, n, g.out_off, g.in_off, c->work);
-  }
-  // reset loop state, counters
-  HIP_TRY(hipMemsetAsync(c->ctrl, 0, sizeof(Ctrl), c->stream));
-  launch_fill_u32(L, c->counters, 0u, 2);
-  const int64_t nw = ((int64_t)n + 31) / 32;
-
-  // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97)
-  {
-    Timed t(c, MSC_K_TRIM);
-    launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, labels, c->verts0, c->counters + 0);
-  }
-  HIP_TRY(hipGetLastError());
-  if (pivot < 0) {
-    HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->work, 16, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    unsigned long long w0, w1;
-    std::memcpy(&w0, c->ctrl_host, 8);
-    std::memcpy(&w1, (char*)c->ctrl_host + 8, 8);
-    if (cc)
-      pivot = w0 ? (int32_t)(0xffffffffu - (uint32_t)(w0 & 0xffffffffull)) : -1;
-    else
-      pivot = (w0 && w1 != ~0ull) ? (int32_t)w1 : -1;
+    if ((rc = device_pivot(c, &g, cc, &pivot))) return rc;
   }
   if (s) {
     s->pivot = pivot;
     s->avg_degree = avg;
   }
-  uint32_t nv0 = 0;
-  if ((rc = read_u32(c, c->counters + 0, &nv0))) return rc;
+
+  // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97)
+  unsigned long long red[4];
+  int np;
+  {
+    Timed t(c, MSC_K_TRIM);
+    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, labels, c->part);
+  }
+  if ((rc = reduce_read(c, np, red))) return rc;
+  uint32_t nv = (uint32_t)red[0];
   if (s) {
-    s->num_trim = n - (int32_t)nv0;
-    s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n + 4.0 * nv0;
+    s->num_trim = n - (int32_t)nv;
+    s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n;
   }
 
   LevelArgs a = base_args(c, &g, labels, o, avg);
-  int32_t* verts = c->verts0;
-  int32_t* verts_alt = c->verts1;
-  uint32_t* nverts = c->counters + 0;
-  uint32_t* nverts_alt = c->counters + 1;
-  uint32_t nv = nv0;
+  a.num_valid = (int32_t)nv;
 
   // FW-BW (SCC) or the CC search from the pivot; a search the policy abandons leaves
   // every vertex to colouring
-  bool committed = false;
   if (pivot >= 0 && nv > 0) {
     const uint32_t* member = nullptr;
+    launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
     if (!cc) {
       // FW: allow = valid, cand = b1, top-down on out-lists, bottom-up on in-lists
-      launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
       LevelArgs fw = a;
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
       fw.allow = c->valid; fw.cand = c->b1;
-      fw.verts = verts; fw.nverts = nverts;
       launch_search_init(L, fw, pivot, 0);
-      if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s))) return rc;
+      if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 4))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
@@ -432,9 +458,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         bw.down_off = g.in_off; bw.down_adj = g.in_adj;
         bw.up_off = g.out_off; bw.up_adj = g.out_adj;
         bw.allow = c->b2; bw.cand = c->b3;
-        bw.verts = verts; bw.nverts = nverts;
         launch_search_init(L, bw, pivot, 1);
-        if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s))) return rc;
+        if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 4))) return rc;
         if ((rc = read_ctrl(c))) return rc;
         if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
         if (!c->ctrl_host->abandoned) {
@@ -443,16 +468,14 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         }
       }
     } else {
-      launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
       LevelArgs bf = a;
       bf.down_off = g.out_off; bf.down_adj = g.out_adj;
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
       bf.allow = c->valid; bf.cand = c->b1;
-      bf.verts = verts; bf.nverts = nverts;
       launch_search_init(L, bf, pivot, 2);
-      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s))) return rc;
+      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, 4))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
@@ -461,78 +484,76 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       }
     }
     if (member) {
-      Timed t(c, MSC_K_COMMIT);
-      launch_fill_u32(L, nverts_alt, 0u, 1);
-      launch_commit(L, verts, nverts, member, c->valid, labels, pivot, verts_alt, nverts_alt);
-      launch_andnot(L, c->valid, nullptr, c->valid, member, nw);
-      committed = true;
+      // commit (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271)
+      {
+        Timed t(c, MSC_K_COMMIT);
+        np = launch_label_bits(L, member, nw, n, labels, pivot, c->part);
+        launch_andnot(L, c->valid, nullptr, c->valid, member, nw);
+      }
+      if ((rc = reduce_read(c, np, red))) return rc;
+      uint32_t found = (uint32_t)red[0];
+      if (s) {
+        s->num_fwbw = (int32_t)found;
+        s->k[MSC_K_COMMIT].bytes += 8.0 * nw + 4.0 * found;
+      }
+      nv -= found;
     }
-  }
-  if (committed) {
-    uint32_t left = 0;
-    if ((rc = read_u32(c, nverts_alt, &left))) return rc;
-    if (s) {
-      s->num_fwbw = (int32_t)(nv - left);
-      s->k[MSC_K_COMMIT].bytes += 4.0 * nv + 1.0 * nv + 4.0 * (nv - left) + 4.0 * left;
-    }
-    std::swap(verts, verts_alt);
-    std::swap(nverts, nverts_alt);
-    nv = left;
   }
 
   // colouring
   if (!cc) {
     // rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677)
+    int push_batch = 4, cbw_batch = 4;
     while (nv > 0) {
+      a.num_valid = (int32_t)nv;
       LevelArgs p = a;
       p.down_off = g.out_off; p.down_adj = g.out_adj;
-      p.verts = verts; p.nverts = nverts;
       {
         Timed t(c, MSC_K_OTHER);
         launch_seed_colors(L, false, p);
       }
-      if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s))) return rc;
+      if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s, push_batch))) return rc;
       if ((rc = read_ctrl(c))) return rc;
-      if (s) s->color_iters += (int32_t)c->ctrl_host->level;
+      uint32_t sweeps = c->ctrl_host->level;
+      if (s) s->color_iters += (int32_t)sweeps;
+      push_batch = sweeps > 2 ? (sweeps < 32 ? (int)sweeps : 32) : 2;
       LevelArgs r = a;
       r.down_off = g.in_off; r.down_adj = g.in_adj;
-      r.verts = verts; r.nverts = nverts;
       {
         Timed t(c, MSC_K_ROOTS);
         launch_roots(L, r);
       }
-      if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s))) return rc;
+      if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch))) return rc;
       if ((rc = read_ctrl(c))) return rc;
-      if (s) s->cbw_levels += (int32_t)c->ctrl_host->level;
+      uint32_t cbw = c->ctrl_host->level;
+      if (s) s->cbw_levels += (int32_t)cbw;
+      cbw_batch = cbw > 2 ? (cbw < 32 ? (int)cbw : 32) : 2;
       {
         Timed t(c, MSC_K_ROOTS);
-        launch_fill_u32(L, nverts_alt, 0u, 1);
-        launch_compact(L, verts, nverts, c->valid, verts_alt, nverts_alt);
+        np = launch_count_bits(L, c->valid, nw, c->part);
       }
-      uint32_t left = 0;
-      if ((rc = read_u32(c, nverts_alt, &left))) return rc;
+      if ((rc = reduce_read(c, np, red))) return rc;
+      uint32_t left = (uint32_t)red[0];
       if (s) {
         s->color_rounds += 1;
-        s->k[MSC_K_ROOTS].bytes += 2.0 * (9.0 * nv) + 4.0 * left;
+        s->k[MSC_K_ROOTS].bytes += 2.0 * 4.0 * nw + 4.0 * (nv - left);
         s->host_syncs += 1;
       }
-      std::swap(verts, verts_alt);
-      std::swap(nverts, nverts_alt);
       if (left >= nv) return MSC_E_DEVICE;  // every round removes at least one SCC
       nv = left;
     }
   } else if (nv > 0) {
     // one max-label fixpoint written straight into the labels (multistep/cc_color.cpp:47-175)
+    a.num_valid = (int32_t)nv;
     LevelArgs p = a;
     p.colors = labels;
     p.down_off = g.out_off; p.down_adj = g.out_adj;
     p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
-    p.verts = verts; p.nverts = nverts;
     {
       Timed t(c, MSC_K_OTHER);
       launch_seed_colors(L, true, p);
     }
-    if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s))) return rc;
+    if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s, 4))) return rc;
     if ((rc = read_ctrl(c))) return rc;
     if (s) {
       s->color_iters = (int32_t)c->ctrl_host->level;
@@ -567,6 +588,26 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
 }
 
 }  // namespace
+
+namespace msc {
+
+int ctx_stream(msc_ctx* c, hipStream_t* stream, int* device) {
+  if (!c) return MSC_E_INVALID;
+  *stream = c->stream;
+  *device = c->device;
+  return MSC_OK;
+}
+
+int ctx_set_policy(msc_ctx* c, const float* blob) {
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemcpyAsync(c->mlp, blob, sizeof(float) * kMlpBlob, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->has_mlp = true;
+  return MSC_OK;
+}
+
+}  // namespace msc
 
 extern "C" {
 
@@ -615,6 +656,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
   c->device = device;
   c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   c->grid = c->cus * 4;
+  if (c->grid > kMaxParts) c->grid = kMaxParts;
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {
@@ -625,10 +667,19 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     c->own_stream = true;
   }
   if (hipHostMalloc((void**)&c->ctrl_host, sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->sched_host, 3 * MSC_SCHED_LEVELS, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&c->small, align256((size_t)kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
+    (void)hipGetLastError();
     msc_ctx_destroy(c);
     return MSC_E_OOM;
   }
+  c->mlp = (float*)c->small;
+  c->sched = (int8_t*)c->small + align256((size_t)kMlpBlob * 4);
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    c->clock_mhz = (float)khz / 1000.0f;
+  (void)hipGetLastError();
   *out = c;
   return MSC_OK;
 }
@@ -645,6 +696,8 @@ int msc_ctx_destroy(msc_ctx* c) {
   }
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
   if (c->pinned_u32) (void)hipHostFree(c->pinned_u32);
+  if (c->sched_host) (void)hipHostFree(c->sched_host);
+  if (c->small) (void)hipFree(c->small);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   {
     std::lock_guard<std::mutex> lk(g_default_mu);
@@ -667,8 +720,8 @@ int msc_ctx_levels(msc_ctx* c, msc_level_rec* out, int32_t cap, int32_t* count) 
   return MSC_OK;
 }
 
-int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
-                const msc_opts* opts, msc_stats* stats) {
+int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels, const msc_opts* opts,
+                msc_stats* stats) {
   try {
     return run_path(Kind::SCC, g, max_deg_vert, avg_degree, labels, opts, stats);
   } catch (const std::bad_alloc&) {
@@ -678,8 +731,8 @@ int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32
   }
 }
 
-int msc_cc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
-               const msc_opts* opts, msc_stats* stats) {
+int msc_cc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels, const msc_opts* opts,
+               msc_stats* stats) {
   try {
     return run_path(Kind::CC, g, max_deg_vert, avg_degree, labels, opts, stats);
   } catch (const std::bad_alloc&) {
@@ -704,16 +757,7 @@ int msc_scc_pivot(const msc_csr* g, const msc_opts* o, int32_t* max_deg_vert, do
   if ((rc = ensure_workspace(c, g->n, g->m))) return rc;
   msc_csr d;
   if ((rc = stage_graph(c, g, &d))) return rc;
-  Launch L = launch_of(c);
-  HIP_TRY(hipMemsetAsync(c->work, 0, 16, c->stream));
-  launch_fill_u32(L, (uint32_t*)(c->work + 1), 0xffffffffu, 2);
-  launch_pivot_scc(L, g->n, d.out_off, d.in_off, c->work);
-  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->work, 16, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  unsigned long long w0, w1;
-  std::memcpy(&w0, c->ctrl_host, 8);
-  std::memcpy(&w1, (char*)c->ctrl_host + 8, 8);
-  *max_deg_vert = (w0 && w1 != ~0ull) ? (int32_t)w1 : -1;
+  if ((rc = device_pivot(c, &d, false, max_deg_vert))) return rc;
   *avg_degree = (double)g->m / (double)g->n;
   return MSC_OK;
 }
@@ -724,10 +768,13 @@ int msc_csr_to_device(msc_ctx* c, const msc_csr* h, msc_csr* d) {
   HIP_TRY(hipSetDevice(c->device));
   size_t offb = ((size_t)h->n + 1) * 4, adjb = (size_t)(h->m > 0 ? h->m : 1) * 4;
   void *oo = nullptr, *io = nullptr, *oa = nullptr, *ia = nullptr;
-  if (hipMalloc(&oo, offb) != hipSuccess || hipMalloc(&io, offb) != hipSuccess ||
-      hipMalloc(&oa, adjb) != hipSuccess || hipMalloc(&ia, adjb) != hipSuccess) {
+  if (hipMalloc(&oo, offb) != hipSuccess || hipMalloc(&io, offb) != hipSuccess || hipMalloc(&oa, adjb) != hipSuccess ||
+      hipMalloc(&ia, adjb) != hipSuccess) {
     (void)hipGetLastError();
-    (void)hipFree(oo); (void)hipFree(io); (void)hipFree(oa); (void)hipFree(ia);
+    (void)hipFree(oo);
+    (void)hipFree(io);
+    (void)hipFree(oa);
+    (void)hipFree(ia);
     return MSC_E_OOM;
   }
   HIP_TRY(hipMemcpyAsync(oo, h->out_off, offb, hipMemcpyHostToDevice, c->stream));

@@ -81,7 +81,14 @@ typedef struct msc_opts {
   int32_t timing;       /* nonzero: per-kernel HIP-event timing into msc_stats.k[] */
   int32_t record;       /* nonzero: keep per-level records (msc_ctx_levels) */
   msc_ctx* ctx;         /* NULL: the library's default context for `device` */
+  /* Forced decisions (policy training): schedule[phase * MSC_SCHED_LEVELS + (L - 1)] is the
+   * action taken after level L of search `phase` (0 FW, 1 BW, 2 CC): 0 top-down, 1
+   * bottom-up, 2 abandon the search, -1 the policy's own choice.  NULL: none. */
+  const int8_t* schedule;
+  int32_t schedule_len;
+  int32_t pad_;
 } msc_opts;
+#define MSC_SCHED_LEVELS 64
 
 /* Kernel classes timed when msc_opts.timing is set. */
 enum msc_kernel {
@@ -142,8 +149,43 @@ typedef struct msc_level_rec {
   int32_t num_valid;
   int32_t action;     /* direction chosen for the next level (0 TD, 1 BU, 2 stop) */
   int32_t pad;
+  float feat[12];     /* the learned policy's raw input features at this decision */
+  float level_us;     /* duration of this level's launch (opts.timing), else 0 */
+  int32_t pad2;
 } msc_level_rec;
 int msc_ctx_levels(msc_ctx* ctx, msc_level_rec* out, int32_t cap, int32_t* count);
+
+/* ---- learned direction / phase policy ------------------------------------------- */
+/* No reference counterpart (the reference's switch is the fixed ALPHA/BETA rule).  A
+ * 12 -> 32 -> 32 -> 3 ReLU MLP in FP32 picks, after each BFS level, top-down, bottom-up or
+ * abandoning FW-BW for colouring; it changes only the schedule, never the labels.
+ * Blob layout (MSC_MLP_BLOB floats): W1[32][12] b1[32] W2[32][32] b2[32] W3[3][32] b3[3],
+ * then the input standardisation: mean[12], std[12]. */
+#define MSC_MLP_IN 12
+#define MSC_MLP_HIDDEN 32
+#define MSC_MLP_OUT 3
+#define MSC_MLP_PARAMS 1571
+#define MSC_MLP_BLOB 1595
+int msc_policy_set_weights(msc_ctx* ctx, const float* blob, int32_t count);
+/* Batched inference on device: logits_out[count][3] from raw features x[count][12]. */
+int msc_mlp_forward(msc_ctx* ctx, const float* blob, const float* x, int64_t count, float* logits_out);
+/* Gradient of sum_b w[b] * CE(softmax(f(x_b)), y_b) w.r.t. the MSC_MLP_PARAMS weights, on
+ * device (the training kernels' own forward and backward); loss_out receives the sum. */
+int msc_mlp_grad(msc_ctx* ctx, const float* blob, const float* x, const int32_t* y, const float* w,
+                 int64_t count, float* loss_out, float* grad_out);
+typedef struct msc_train_opts {
+  int32_t epochs;
+  int32_t batch;
+  float lr;
+  float beta1, beta2, eps; /* Adam */
+  float weight_decay;
+  uint64_t seed;           /* mini-batch shuffling */
+} msc_train_opts;
+/* Trains the blob's weights in place on device with Adam on the weighted cross-entropy
+ * (mean over each mini-batch); mean/std in the blob are inputs, not trained.
+ * loss_curve_out (may be NULL) receives the mean training loss of each epoch. */
+int msc_mlp_train(msc_ctx* ctx, float* blob, const float* x, const int32_t* y, const float* w,
+                  int64_t count, const msc_train_opts* opts, float* loss_curve_out);
 
 /* ---- the hot path ---------------------------------------------------------------- */
 /* Strongly connected components.

@@ -1,0 +1,165 @@
+"""Learned direction / phase policy.
+
+CPU: the float64 restatement of the MLP (oracle/mlp_ref.py) against central finite
+differences.  GPU: the device training kernels' loss / gradient and batched logits against
+that restatement; the level closer's in-kernel MLP picks the same actions as the batched
+kernel on the features it records; forced schedules and learned runs leave labels
+unchanged; training on device reduces the loss.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+
+
+def random_blob(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0, scale, mlp_ref.PARAMS)
+    mean = rng.normal(0, 1, mlp_ref.IN)
+    std = rng.uniform(0.5, 2.0, mlp_ref.IN)
+    return np.concatenate([w, mean, std]).astype(np.float32)
+
+
+def random_batch(seed, n):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 2, (n, mlp_ref.IN)).astype(np.float32)
+    y = rng.integers(0, 3, n).astype(np.int32)
+    w = rng.uniform(0.1, 2.0, n).astype(np.float32)
+    return x, y, w
+
+
+def test_restatement_matches_finite_differences():
+    blob = random_blob(1).astype(np.float64)
+    x, y, w = random_batch(2, 7)
+    loss, grad = mlp_ref.loss_and_grad(blob, x, y, w)
+    rng = np.random.default_rng(3)
+    for p in rng.choice(mlp_ref.PARAMS, 60, replace=False):
+        h = 1e-6
+        bp, bm = blob.copy(), blob.copy()
+        bp[p] += h
+        bm[p] -= h
+        fd = (mlp_ref.loss_and_grad(bp, x, y, w)[0] - mlp_ref.loss_and_grad(bm, x, y, w)[0]) / (2 * h)
+        assert abs(fd - grad[p]) <= 1e-5 * max(1.0, abs(fd)), p
+    # a dropped term would show: bias gradients equal the summed deltas
+    assert np.isfinite(loss) and loss > 0
+
+
+def test_feature_and_blob_layout(msc):
+    from connectivity_amd import policy
+
+    assert msc.MLP_PARAMS == mlp_ref.PARAMS == 1571 and msc.MLP_BLOB == mlp_ref.BLOB == 1595
+    assert len(policy.FEATURES) == msc.MLP_IN
+    b = policy.init_blob(np.zeros(12), np.ones(12), seed=0)
+    assert b.shape == (msc.MLP_BLOB,) and np.all(b[-12:] == 1.0)
+
+
+def test_schedule_family_shapes():
+    from connectivity_amd import policy
+
+    c = policy.phase_schedules(6)
+    names = [n for n, _ in c]
+    assert "all_td" in names and "bu1-None" in names and "stop1" in names
+    for _, acts in c:
+        assert all(a in (-1, 0, 1, 2) for a in acts) and len(acts) <= 64
+
+
+@pytest.mark.gpu
+def test_device_gradient_matches_restatement(msc, gpu_ctx):
+    blob = random_blob(4)
+    for n in (1, 63, 64, 65, 500):
+        x, y, w = random_batch(10 + n, n)
+        loss, grad = msc.mlp_grad(gpu_ctx, blob, x, y, w)
+        rl, rg = mlp_ref.loss_and_grad(blob, x, y, w)
+        assert abs(loss - rl) <= 1e-4 * max(1.0, abs(rl))
+        np.testing.assert_allclose(grad, rg, rtol=2e-4, atol=2e-4 * np.abs(rg).max())
+    # unweighted == weights of one
+    x, y, _ = random_batch(99, 100)
+    l1, g1 = msc.mlp_grad(gpu_ctx, blob, x, y)
+    l2, g2 = msc.mlp_grad(gpu_ctx, blob, x, y, np.ones(100, np.float32))
+    assert abs(l1 - l2) < 1e-4 * abs(l1) and np.allclose(g1, g2, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_device_forward_matches_restatement(msc, gpu_ctx):
+    blob = random_blob(5)
+    x, _, _ = random_batch(6, 300)
+    z = msc.mlp_forward(gpu_ctx, blob, x)
+    np.testing.assert_allclose(z, mlp_ref.forward(blob, x)[-1], rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss(msc, gpu_ctx):
+    from connectivity_amd import policy
+
+    rng = np.random.default_rng(7)
+    x = rng.normal(0, 1, (4000, 12)).astype(np.float32)
+    y = (x[:, 0] > 0).astype(np.int32) + (x[:, 1] > 1).astype(np.int32)  # learnable labels 0..2
+    blob0 = policy.init_blob(x.mean(0), x.std(0), seed=1)
+    blob, curve = msc.mlp_train(gpu_ctx, blob0, x, y, epochs=20, batch=256, lr=3e-3)
+    assert curve[-1] < 0.5 * curve[0]
+    acc = np.mean(np.argmax(msc.mlp_forward(gpu_ctx, blob, x), 1) == y)
+    assert acc > 0.9
+    # the trained loss equals the restatement's loss of the returned weights
+    l_dev, _ = msc.mlp_grad(gpu_ctx, blob, x, y)
+    l_ref, _ = mlp_ref.loss_and_grad(blob, x, y)
+    assert abs(l_dev - l_ref) < 1e-3 * l_ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", ["rmat", "grid"])
+def test_forced_schedules_keep_labels(msc, orc, gpu_ctx, graph):
+    g = msc.gen_rmat(14, 16, seed=3) if graph == "rmat" else msc.gen_grid(96, 96, seed=3)
+    print(f"case: forced schedules {graph}", flush=True)
+    want = orc.canonical(orc.scc(g, vert_cutoff=0))
+    want_cc = orc.canonical(orc.cc(g))
+    for fw, bw in [([0] * 64, [0] * 64), ([1] * 64, [1] * 64), ([0, 1, 1, 0], [1, 0, 1]), ([2], None),
+                   (None, [2]), ([0, 0, 2], [1, 2])]:
+        sched = msc.schedule_array(fw=fw, bw=bw)
+        lab, st = msc.run_scc(g, ctx=gpu_ctx, schedule=sched, stats=True)
+        assert np.array_equal(lab, want), (fw, bw)
+    for cc in ([1] * 64, [2], [0, 1]):
+        assert np.array_equal(msc.run_cc(g, ctx=gpu_ctx, schedule=msc.schedule_array(cc=cc)), want_cc)
+
+
+@pytest.mark.gpu
+def test_forced_actions_are_recorded(msc, gpu_ctx):
+    g = msc.gen_rmat(14, 16, seed=3)
+    sched = msc.schedule_array(fw=[1, 1, 0, 0, 0, 0, 0, 0])
+    msc.run_scc(g, ctx=gpu_ctx, schedule=sched, record=True)
+    fw = [r for r in gpu_ctx.levels() if r.phase == 0]
+    assert [r.action for r in fw[:2]] == [1, 1]
+    assert [r.bottom_up for r in fw[1:3]] == [1, 1]
+    assert all(r.level_us > 0 for r in fw)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_learned_runs_keep_labels_and_match_batched_mlp(msc, orc, gpu_ctx, seed):
+    blob = random_blob(seed, scale=0.5)
+    blob[-24:-12] = 0.0   # mean
+    blob[-12:] = 3.0      # std
+    msc.set_policy(gpu_ctx, blob)
+    for g in (msc.gen_rmat(13, 8, seed=seed), msc.gen_grid(64, 64, seed=seed)):
+        want = orc.canonical(orc.scc(g, vert_cutoff=0))
+        lab = msc.run_scc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED, record=True)
+        assert np.array_equal(lab, want)
+        recs = gpu_ctx.levels()
+        feats = np.array([r.feat for r in recs if r.phase in (0, 1)], np.float32)
+        acts = np.array([r.action for r in recs if r.phase in (0, 1)])
+        if len(feats):
+            z = msc.mlp_forward(gpu_ctx, blob, feats)
+            assert np.array_equal(np.argmax(z, 1), acts)
+        assert np.array_equal(msc.run_cc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED), orc.canonical(orc.cc(g)))
+
+
+@pytest.mark.gpu
+def test_shipped_weights_keep_labels(msc, orc, gpu_ctx):
+    import os
+
+    if not os.path.exists(msc.POLICY_WEIGHTS):
+        pytest.skip("no trained weights shipped yet")
+    msc.load_policy(gpu_ctx)
+    for g in (msc.gen_rmat(15, 16, seed=77), msc.gen_grid(128, 128, seed=77), msc.gen_planted(20000, seed=77)[0]):
+        want = orc.canonical(orc.scc(g, vert_cutoff=0))
+        assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED), want)
+        assert np.array_equal(msc.run_cc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED), orc.canonical(orc.cc(g)))
