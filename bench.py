@@ -2,19 +2,20 @@
 
 One step = one msc_scc_run over the whole graph (trim -> FW-BW -> colouring to empty), with
 the CSR / transposed CSR already resident in HBM and the labels left in HBM.  `value` is
-GTEPS = m / t_step (m = directed edges after dedupe); the plain ALPHA/BETA heuristic is the
-timed schedule unless --policy learned.
+GTEPS = m / t_step (m = directed edges after dedupe) with the learned direction policy when
+its weights are shipped (connectivity_amd/policy_weights.npz), else the plain rule; both are
+timed in the same run and reported with their labels checked identical.
 
 The line also carries:
-  roofline      -- dominant kernel: algorithmic bytes / summed HIP-event durations of its
-                   launches in the timed region, against 8 TB/s;
+  roofline      -- dominant kernel of the timed steps: algorithmic bytes / summed HIP-event
+                   durations of its launches, against 8 TB/s; `traffic` comes from the
+                   committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present;
   cpu_baseline  -- the reference's own multistep run_scc (oracle/_ref, compiled from its
                    sources) or, where that was not built, the oracle's restatement, timed on
                    this host's cores on the same graph;
-  wcc_ms        -- msc_cc_run on the same graph (not part of `value`);
-  learned_speedup -- plain ms / learned ms when policy weights are available.
+  wcc_ms        -- msc_cc_run on the same graph (not part of `value`).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 24] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat|grid] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -36,22 +37,40 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def build_graph(args, msc):
+    if args.workload == "grid":
+        g = msc.gen_grid(args.side, args.side, 0.5, 0.1, 0.005, seed=args.seed)
+        desc = (f"synthetic {args.side}x{args.side} grid digraph, edge kept one-way p=0.5 / both p=0.1, "
+                f"+0.5% n random arcs, seed {args.seed}")
+        cfg = {"workload": f"grid{args.side}_scc", "rows": args.side, "cols": args.side}
+    else:
+        g = msc.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+        desc = (f"synthetic directed R-MAT (a,b,c,d)=(.57,.19,.19,.05), self-loops/duplicates removed, "
+                f"ids permuted, seed {args.seed}")
+        cfg = {"workload": f"rmat{args.scale}_scc", "scale": args.scale, "edge_factor": args.edge_factor}
+    return g, desc, cfg
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["rmat", "grid"], default="rmat")
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--policy", choices=["plain", "learned"], default="plain")
+    ap.add_argument("--policy", choices=["auto", "plain", "learned"], default="auto")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-check", action="store_true", help="skip the oracle parity check")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this size first (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is a design "
-                                   "note (DESIGN.md), not a launched path", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build measures one GPU only (every box here has one); how the path "
+                                   "would shard is a design note in DESIGN.md", "n_gpus": args.gpus}))
         return 2
 
     import numpy as np
@@ -62,35 +81,66 @@ def main() -> int:
         log("no gfx950 device visible")
         return 1
     t0 = time.time()
-    g = msc.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
-    log(f"R-MAT s{args.scale} ef{args.edge_factor} seed {args.seed}: n={g.n} m={g.m} "
-        f"pivot={g.max_deg_vert} built in {time.time() - t0:.1f}s")
+    g, desc, cfg = build_graph(args, msc)
+    log(f"{cfg['workload']}: n={g.n} m={g.m} pivot={g.max_deg_vert} built in {time.time() - t0:.1f}s")
     ctx = msc.Context(0)
+    if args.calib_copy_mb:
+        nbytes = args.calib_copy_mb << 20
+        a, b = msc.DeviceLabels(ctx, nbytes // 4), msc.DeviceLabels(ctx, nbytes // 4)
+        msc._lib().msc_copy_device(ctx.handle, b.ptr, a.ptr, nbytes)
+        log(f"calibration copy of {nbytes} bytes done")
+        a.free()
+        b.free()
     dg = msc.DeviceGraph(ctx, g)
     out = msc.DeviceLabels(ctx, g.n)
-    policy = msc.POLICY_LEARNED if args.policy == "learned" else msc.POLICY_PLAIN
+    have_weights = os.path.exists(msc.POLICY_WEIGHTS)
+    if have_weights:
+        msc.load_policy(ctx)
+    policy_name = args.policy if args.policy != "auto" else ("learned" if have_weights else "plain")
+    if policy_name == "learned" and not have_weights:
+        log("no policy weights shipped")
+        return 1
+    policy = msc.POLICY_LEARNED if policy_name == "learned" else msc.POLICY_PLAIN
 
-    def step(pol=policy, timing=True):
+    def step(pol, timing=True):
         _, st = msc.run_scc(dg, g.max_deg_vert, g.avg_degree, ctx=ctx, out=out, policy=pol, timing=timing,
                             stats=True)
         return st
 
-    for _ in range(args.warmup):
-        step()
+    def timed(pol, steps, timing):
+        msc._lib().msc_ctx_sync(ctx.handle)
+        stats = []
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            stats.append(step(pol, timing))
+        msc._lib().msc_ctx_sync(ctx.handle)
+        return (time.perf_counter() - t_start) * 1e3 / steps, stats
+
+    for pol in {policy, msc.POLICY_PLAIN}:
+        for _ in range(args.warmup):
+            step(pol)
     log(f"warmup {args.warmup} done")
-    # timed region: K steps; each call returns stream-synchronised
-    msc._lib().msc_ctx_sync(ctx.handle)
-    walls, stats = [], []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t1 = time.perf_counter()
-        stats.append(step())
-        walls.append((time.perf_counter() - t1) * 1e3)
-    msc._lib().msc_ctx_sync(ctx.handle)
-    total_ms = (time.perf_counter() - t_start) * 1e3
-    ms_per_step = total_ms / args.steps
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (min {min(walls):.3f}, median "
-        f"{statistics.median(walls):.3f})")
+    # the timed region: K steps of the reported policy, per-kernel HIP events inside it
+    ms_per_step, stats = timed(policy, args.steps, True)
+    labels = out.to_host()
+    ms_noevents, _ = timed(policy, args.steps, False)
+    log(f"timed {args.steps} steps ({policy_name}): {ms_per_step:.3f} ms/step with kernel events, "
+        f"{ms_noevents:.3f} without")
+    plain_ms = ms_per_step
+    learned = None
+    if have_weights:
+        other = msc.POLICY_PLAIN if policy == msc.POLICY_LEARNED else msc.POLICY_LEARNED
+        other_ms, _ = timed(other, args.steps, False)
+        other_labels = out.to_host()
+        same = bool(np.array_equal(other_labels, labels))
+        p_ms = other_ms if policy == msc.POLICY_LEARNED else ms_noevents
+        l_ms = ms_noevents if policy == msc.POLICY_LEARNED else other_ms
+        plain_ms = p_ms
+        learned = {"plain_ms": round(p_ms, 4), "learned_ms": round(l_ms, 4), "speedup": round(p_ms / l_ms, 3),
+                   "identical_labels": same}
+        log(f"plain {p_ms:.3f} ms, learned {l_ms:.3f} ms: {p_ms / l_ms:.2f}x, labels identical: {same}")
+        if not same:
+            return 1
 
     # per-kernel-class event time and algorithmic bytes summed over the timed steps
     agg = {}
@@ -104,13 +154,11 @@ def main() -> int:
     d = agg[dom]
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9 if d["ms"] > 0 else 0.0
     s0 = stats[-1]
-    log(f"kernels: " + ", ".join(f"{k}: {v['launches']} launches {v['ms']:.2f} ms {v['bytes'] / 1e9:.3f} GB"
-                                  for k, v in agg.items() if v["launches"]))
-    log(f"phases: trim {s0.num_trim}, fwbw {s0.num_fwbw}, colour sccs {s0.num_color}, rounds "
-        f"{s0.color_rounds}, fw levels {s0.fw_levels}, bw levels {s0.bw_levels}, bu levels {s0.bu_levels}, "
-        f"host syncs {s0.host_syncs}")
+    log("kernels: " + ", ".join(f"{k}: {v['launches']} launches {v['ms']:.2f} ms {v['bytes'] / 1e9:.3f} GB"
+                                for k, v in agg.items() if v["launches"]))
+    log(f"phases: trim {s0.num_trim}, fwbw {s0.num_fwbw}, colour rounds {s0.color_rounds}, fw levels "
+        f"{s0.fw_levels}, bw levels {s0.bw_levels}, bu levels {s0.bu_levels}, host syncs {s0.host_syncs}")
 
-    labels = out.to_host()
     check = "skipped"
     if not args.no_check:
         import oracle
@@ -123,28 +171,18 @@ def main() -> int:
             return 1
     counts = msc.scc_verify(labels)
 
-    # WCC on the same resident graph (reported beside, not in `value`)
     cc_ms = []
     for _ in range(max(2, args.steps // 2)):
         t1 = time.perf_counter()
-        msc.run_cc(dg, ctx=ctx, out=out)
+        msc.run_cc(dg, ctx=ctx, out=out, policy=policy)
         cc_ms.append((time.perf_counter() - t1) * 1e3)
 
-    learned = None
-    if os.path.exists(os.path.join(ROOT, "connectivity_amd", "policy_weights.npz")):
-        try:
-            plain_ms = statistics.median([step(msc.POLICY_PLAIN, False).wall_ms for _ in range(5)])
-            msc.load_policy(ctx)
-            learned_ms = statistics.median([step(msc.POLICY_LEARNED, False).wall_ms for _ in range(5)])
-            learned = {"plain_ms": plain_ms, "learned_ms": learned_ms, "speedup": plain_ms / learned_ms}
-        except Exception as exc:  # reported, never silently replaced
-            learned = {"error": str(exc)}
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(g, args.cpu_runs)
+    cpu = None if args.no_cpu else cpu_baseline(g, args.cpu_runs)
 
     gteps = g.m / (ms_per_step * 1e-3) / 1e9
+    traffic = read_traffic(cfg["workload"], policy_name, dom)
+    cfg.update({"n": g.n, "m": g.m, "pivot": g.max_deg_vert, "policy": policy_name,
+                "phases": "trim -> fw-bw -> colour to empty"})
     line = {
         "metric": METRIC,
         "value": round(gteps, 4),
@@ -157,34 +195,45 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": f"synthetic directed R-MAT (a,b,c,d)=(.57,.19,.19,.05), self-loops/duplicates removed, "
-                f"ids permuted, seed {args.seed}",
-        "config": {"workload": f"rmat{args.scale}_scc", "scale": args.scale, "edge_factor": args.edge_factor,
-                   "n": g.n, "m": g.m, "pivot": g.max_deg_vert, "policy": args.policy,
-                   "phases": "trim -> fw-bw -> colour to empty"},
+        "data": desc,
+        "config": cfg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(d["launches"], 1), 3),
                      "alg_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1), 1)},
         "cpu_baseline": cpu,
+        "ms_per_step_without_kernel_events": round(ms_noevents, 4),
+        "plain_ms": round(plain_ms, 4),
+        "learned": learned,
         "scc_counts": {"num_sccs": counts[0], "nontrivial": counts[1], "max_scc": counts[2]},
         "parity": check,
         "wcc_ms": round(statistics.median(cc_ms), 4),
-        "learned": learned,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def read_traffic(workload: str, policy: str, kernel_class: str):
+    """HBM bytes per launch of the dominant kernel from the committed PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{policy}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get("per_launch_bytes", {}).get(kernel_class)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(g, runs: int):
     """The reference's run_scc on this host's cores (oracle/_ref); the restatement if absent."""
     import oracle
 
-    cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
-        pass
+        cores = os.cpu_count() or 1
     use_ref = oracle.have_ref()
     times = []
     for _ in range(runs):

@@ -103,6 +103,15 @@ class _LevelRec(C.Structure):
                 ("level_us", C.c_float), ("pad2", C.c_int32)]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int32), ("eager_push_edges", C.c_int32),
+                ("eager_cbw_edges", C.c_int32), ("first_batch", C.c_int32), ("max_batch", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+TUNING_FIELDS = ("blocks_per_cu", "eager_push_edges", "eager_cbw_edges", "first_batch", "max_batch")
+
+
 class _TrainOpts(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
                 ("beta2", C.c_float), ("eps", C.c_float), ("weight_decay", C.c_float), ("seed", C.c_uint64)]
@@ -129,6 +138,8 @@ def _lib():
         "msc_ctx_create": (C.c_int, [C.c_int32, P, C.POINTER(P)]),
         "msc_ctx_destroy": (C.c_int, [P]),
         "msc_ctx_levels": (C.c_int, [P, C.POINTER(_LevelRec), C.c_int32, C.POINTER(C.c_int32)]),
+        "msc_ctx_get_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
+        "msc_ctx_set_tuning": (C.c_int, [P, C.POINTER(_Tuning)]),
         "msc_scc_run": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
                                   C.POINTER(_Stats)]),
         "msc_cc_run": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
@@ -158,6 +169,7 @@ def _lib():
         "msc_device_free": (C.c_int, [P, P]),
         "msc_copy_to_host": (C.c_int, [P, P, P, C.c_int64]),
         "msc_copy_to_device": (C.c_int, [P, P, P, C.c_int64]),
+        "msc_copy_device": (C.c_int, [P, P, P, C.c_int64]),
         "msc_ctx_sync": (C.c_int, [P]),
         "msc_policy_set_weights": (C.c_int, [P, P, C.c_int32]),
         "msc_mlp_forward": (C.c_int, [P, P, P, C.c_int64, P]),
@@ -315,6 +327,23 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def tuning(self) -> dict:
+        """Launch parameters of this context (msc_tuning) as a dict."""
+        t = _Tuning()
+        _check(_lib().msc_ctx_get_tuning(self.handle, C.byref(t)), "msc_ctx_get_tuning")
+        return {f: getattr(t, f) for f in TUNING_FIELDS}
+
+    def set_tuning(self, **kw) -> dict:
+        """Changes the named launch parameters; the others keep their values."""
+        cur = self.tuning()
+        bad = set(kw) - set(TUNING_FIELDS)
+        if bad:
+            raise MscError(E_INVALID, f"unknown tuning fields {sorted(bad)}")
+        cur.update(kw)
+        t = _Tuning(**{f: int(cur[f]) for f in TUNING_FIELDS})
+        _check(_lib().msc_ctx_set_tuning(self.handle, C.byref(t)), f"msc_ctx_set_tuning({cur})")
+        return cur
 
     def levels(self) -> list["LevelRecord"]:
         """Per-level records of the last run made with ``record=True``."""

@@ -92,6 +92,7 @@ struct LevelArgs {
   const float* mlp;           // kMlpBlob floats (learned policy)
   const int8_t* sched;        // forced actions, 3 x MSC_SCHED_LEVELS (null: none)
   float clock_mhz;            // wall_clock64() rate, for per-level durations
+  uint32_t eager_edges;       // colour sweeps over at most this many edges walk eagerly
 };
 
 struct Launch {

@@ -243,6 +243,65 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
 }
 
 // ------------------------------------------------------------------------------------
+// eager relaxation for small colour frontiers
+//
+// Max-label pushes and equal-colour claims are monotone, so the fixpoint does not depend
+// on the order of relaxations.  When a colour frontier is small, a lane that raises (or
+// claims) a vertex keeps walking depth-first from it within a fixed edge budget instead
+// of leaving every hop to a later sweep: each sweep on a deep graph (grids, chains of
+// planted SCCs) then covers many hops, and sweeps -- each a launch -- become fewer.
+// Whatever the walk cannot finish (the vertex being walked, the stack) is queued as usual.
+
+constexpr int kEagerStack = 4, kEagerOut = 8, kEagerBudget = 48;
+
+template <int OP, bool MIRROR>
+__device__ int eager_walk(const LevelArgs& a, uint32_t w, int32_t col, uint32_t* out, unsigned long long& raised) {
+  uint32_t stack[kEagerStack];
+  int sp = 0, nout = 0, budget = kEagerBudget;
+  stack[sp++] = w;
+  while (sp > 0) {
+    const uint32_t x = stack[--sp];
+    const uint32_t b1 = a.down_off[x], d1 = a.down_off[x + 1] - b1;
+    uint32_t b2 = 0, d2 = 0;
+    if (MIRROR) {
+      b2 = a.down_off2[x];
+      d2 = a.down_off2[x + 1] - b2;
+    }
+    bool finished = true;
+    for (uint32_t i = 0; i < d1 + d2; ++i) {
+      if (budget == 0 || nout + sp + 2 > kEagerOut) {  // keep room for x and the stack
+        finished = false;
+        break;
+      }
+      --budget;
+      const uint32_t y = (uint32_t)(i < d1 ? a.down_adj[b1 + i] : a.down_adj2[b2 + i - d1]);
+      bool got = false;
+      if (OP == OP_PUSH) {
+        if (bit_test(a.valid, y) && a.colors[y] < col && atomicMax(&a.colors[y], col) < col) {
+          ++raised;
+          got = true;
+        }
+      } else {
+        if (bit_test(a.valid, y) && a.colors[y] == col && bit_clear(a.valid, y)) {
+          a.labels[y] = col;
+          got = true;
+        }
+      }
+      if (got) {
+        if (sp < kEagerStack) stack[sp++] = y;
+        else out[nout++] = y;
+      }
+    }
+    if (!finished) {
+      out[nout++] = x;
+      while (sp > 0) out[nout++] = stack[--sp];
+      break;
+    }
+  }
+  return nout;
+}
+
+// ------------------------------------------------------------------------------------
 // learned policy: 12 -> 32 -> 32 -> 3 ReLU MLP, evaluated by one wave
 
 // Raw features of the level just closed (same order as the training code; the blob
@@ -255,7 +314,7 @@ __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next,
   f[0] = log1pf((float)next);
   f[1] = log1pf((float)nedges);
   f[2] = log1pf(fmaxf(nv - found, 0.0f));
-  f[3] = (float)c->level;
+  f[3] = log1pf((float)c->level);
   f[4] = (float)next / n;
   f[5] = (float)nedges / fmaxf((float)a.avg * n, 1.0f);
   f[6] = next > 0 ? log1pf((float)nedges / (float)next) : 0.0f;
@@ -511,6 +570,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   const uint32_t cur = c->cur;
   const uint32_t q_size = c->q_size, q_edges = c->q_edges;
   const bool bu = REACH && c->bottom_up != 0u;
+  const bool eager = COLOR && q_edges <= a.eager_edges;
   const Frontier F = cur ? a.fr1 : a.fr0;
   const Frontier N = cur ? a.fr0 : a.fr1;
   uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
@@ -610,33 +670,48 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
           if (COLOR) wcol[r] = a.colors[w[r]];
         }
       }
+      bool got[kPer];  // raised (push) / claimed (reach, colour-BW) by this lane
 #pragma unroll
       for (int r = 0; r < kPer; ++r) {
         bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
-        take[r] = false;
+        got[r] = false;
         if (REACH) {
-          take[r] = in && bit_clear(a.cand, w[r]);
+          got[r] = in && bit_clear(a.cand, w[r]);
         } else if (OP == OP_PUSH) {
-          if (in && wcol[r] < col[r]) {
-            int32_t old = atomicMax(&a.colors[w[r]], col[r]);
-            if (old < col[r]) {
-              ++my_raised;
-              take[r] = bit_set(inq_nxt, w[r]);
-            }
+          if (in && wcol[r] < col[r] && atomicMax(&a.colors[w[r]], col[r]) < col[r]) {
+            ++my_raised;
+            got[r] = true;
           }
         } else {  // OP_CBW
           if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
             a.labels[w[r]] = col[r];
-            take[r] = true;
+            got[r] = true;
           }
         }
       }
+      if (!eager) {
 #pragma unroll
-      for (int r = 0; r < kPer; ++r) {
-        uint32_t deg = take[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
-        stage_push(S, take[r], w[r], deg);
+        for (int r = 0; r < kPer; ++r) {
+          take[r] = got[r];
+          if (OP == OP_PUSH && got[r]) take[r] = bit_set(inq_nxt, w[r]);
+          uint32_t deg = take[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
+          stage_push(S, take[r], w[r], deg);
+        }
+        stage_flush(S, N, c, kStage - kChunk);  // also the end-of-chunk barrier
+      } else {
+        for (int r = 0; r < kPer; ++r) {
+          uint32_t out[kEagerOut];
+          int nout = got[r] ? eager_walk<OP, MIRROR>(a, w[r], col[r], out, my_raised) : 0;
+          for (int q = 0; q < kEagerOut; ++q) {
+            bool tk = q < nout;
+            uint32_t x = tk ? out[q] : 0u;
+            if (OP == OP_PUSH && tk) tk = bit_set(inq_nxt, x);
+            uint32_t deg = tk ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, x) : 0u;
+            stage_push(S, tk, x, deg);
+          }
+          stage_flush(S, N, c, kStage - kBlock * kEagerOut);
+        }
       }
-      stage_flush(S, N, c, kStage - kChunk);  // also the end-of-chunk barrier
     }
   } else if (REACH) {
     // A workgroup owns kBW words of `cand` per iteration (up to kR candidates per thread).

@@ -36,6 +36,7 @@ struct msc_ctx {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
+  msc_tuning tune{4, 1 << 16, 1 << 16, 4, 32, 0};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -197,9 +198,14 @@ int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
 // Runs level launches (work + closing kernel) until the control block says done.  Every
 // search and every max-label fixpoint settles within n + 1 levels, so more launches than
 // that mean a fault.
-int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, msc_stats* s, int first_batch) {
+int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls, msc_stats* s, int first_batch) {
   Launch L = launch_of(c);
-  int batch = first_batch;
+  LevelArgs a = a_in;
+  a.eager_edges = op == OP_PUSH ? (uint32_t)c->tune.eager_push_edges
+                  : op == OP_CBW ? (uint32_t)c->tune.eager_cbw_edges
+                                 : 0u;
+  int batch = first_batch > 0 ? first_batch : c->tune.first_batch;
+  const int max_batch = c->tune.max_batch;
   const int64_t limit = (int64_t)a.n_total + 64;
   int64_t launched = 0;
   for (;;) {
@@ -217,7 +223,7 @@ int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, ms
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (s) s->host_syncs += 1;
     if (*c->pinned_u32) break;
-    batch = batch < 32 ? batch * 2 : 32;
+    batch = batch * 2 < max_batch ? batch * 2 : max_batch;
   }
   return MSC_OK;
 }
@@ -447,7 +453,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
       fw.allow = c->valid; fw.cand = c->b1;
       launch_search_init(L, fw, pivot, 0);
-      if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 4))) return rc;
+      if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
@@ -459,7 +465,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         bw.up_off = g.out_off; bw.up_adj = g.out_adj;
         bw.allow = c->b2; bw.cand = c->b3;
         launch_search_init(L, bw, pivot, 1);
-        if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 4))) return rc;
+        if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 0))) return rc;
         if ((rc = read_ctrl(c))) return rc;
         if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
         if (!c->ctrl_host->abandoned) {
@@ -475,7 +481,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
       bf.allow = c->valid; bf.cand = c->b1;
       launch_search_init(L, bf, pivot, 2);
-      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, 4))) return rc;
+      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, 0))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
@@ -503,7 +509,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // colouring
   if (!cc) {
     // rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677)
-    int push_batch = 4, cbw_batch = 4;
+    // the previous round's sweep counts predict this round's: enqueue that many first
+    int push_batch = c->tune.first_batch, cbw_batch = c->tune.first_batch;
     while (nv > 0) {
       a.num_valid = (int32_t)nv;
       LevelArgs p = a;
@@ -516,7 +523,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if ((rc = read_ctrl(c))) return rc;
       uint32_t sweeps = c->ctrl_host->level;
       if (s) s->color_iters += (int32_t)sweeps;
-      push_batch = sweeps > 2 ? (sweeps < 32 ? (int)sweeps : 32) : 2;
+      push_batch = sweeps > 2 ? (sweeps < (uint32_t)c->tune.max_batch ? (int)sweeps : c->tune.max_batch) : 2;
       LevelArgs r = a;
       r.down_off = g.in_off; r.down_adj = g.in_adj;
       {
@@ -527,7 +534,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if ((rc = read_ctrl(c))) return rc;
       uint32_t cbw = c->ctrl_host->level;
       if (s) s->cbw_levels += (int32_t)cbw;
-      cbw_batch = cbw > 2 ? (cbw < 32 ? (int)cbw : 32) : 2;
+      cbw_batch = cbw > 2 ? (cbw < (uint32_t)c->tune.max_batch ? (int)cbw : c->tune.max_batch) : 2;
       {
         Timed t(c, MSC_K_ROOTS);
         np = launch_count_bits(L, c->valid, nw, c->part);
@@ -553,7 +560,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       Timed t(c, MSC_K_OTHER);
       launch_seed_colors(L, true, p);
     }
-    if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s, 4))) return rc;
+    if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s, 0))) return rc;
     if ((rc = read_ctrl(c))) return rc;
     if (s) {
       s->color_iters = (int32_t)c->ctrl_host->level;
@@ -655,7 +662,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
   if (!c) return MSC_E_OOM;
   c->device = device;
   c->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-  c->grid = c->cus * 4;
+  c->grid = c->cus * c->tune.blocks_per_cu;
   if (c->grid > kMaxParts) c->grid = kMaxParts;
   if (stream) {
     c->stream = (hipStream_t)stream;
@@ -705,6 +712,26 @@ int msc_ctx_destroy(msc_ctx* c) {
       if (d == c) d = nullptr;
   }
   delete c;
+  return MSC_OK;
+}
+
+int msc_ctx_get_tuning(msc_ctx* c, msc_tuning* out) {
+  if (!c || !out) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  *out = c->tune;
+  return MSC_OK;
+}
+
+int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
+  if (!c || !t) return MSC_E_INVALID;
+  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->eager_push_edges < 0 || t->eager_cbw_edges < 0 ||
+      t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256)
+    return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->tune = *t;
+  c->tune.pad = 0;
+  c->grid = c->cus * t->blocks_per_cu;
+  if (c->grid > kMaxParts) c->grid = kMaxParts;
   return MSC_OK;
 }
 
@@ -821,6 +848,15 @@ int msc_copy_to_device(msc_ctx* c, void* dst, const void* src, int64_t bytes) {
   std::lock_guard<std::mutex> lk(c->mu);
   HIP_TRY(hipSetDevice(c->device));
   if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MSC_OK;
+}
+
+int msc_copy_device(msc_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || bytes < 0 || (bytes > 0 && (!dst || !src))) return MSC_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  if (bytes) HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MSC_OK;
 }

@@ -187,6 +187,19 @@ typedef struct msc_train_opts {
 int msc_mlp_train(msc_ctx* ctx, float* blob, const float* x, const int32_t* y, const float* w,
                   int64_t count, const msc_train_opts* opts, float* loss_curve_out);
 
+/* Launch and schedule tunables of a context (no reference counterpart; picked by the
+ * autotuner, connectivity_amd/autotune.py).  Defaults are the values the library ships. */
+typedef struct msc_tuning {
+  int32_t blocks_per_cu;    /* level-kernel workgroups per compute unit (1..8) */
+  int32_t eager_push_edges; /* colour push sweeps over fewer frontier edges walk eagerly */
+  int32_t eager_cbw_edges;  /* same for the per-colour backward search (0: never) */
+  int32_t first_batch;      /* levels enqueued before the first host check of a loop */
+  int32_t max_batch;        /* levels enqueued between later host checks */
+  int32_t pad;
+} msc_tuning;
+int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
+int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);
+
 /* ---- the hot path ---------------------------------------------------------------- */
 /* Strongly connected components.
  * Replaces run_scc(graph&, int*& scc_maps, int max_deg_vert, double avg_degree,
@@ -284,6 +297,8 @@ int msc_device_alloc(msc_ctx* ctx, int64_t bytes, void** out);
 int msc_device_free(msc_ctx* ctx, void* p);
 int msc_copy_to_host(msc_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int msc_copy_to_device(msc_ctx* ctx, void* dst, const void* src, int64_t bytes);
+/* Device-to-device copy (also the known-size reference for HBM traffic counters). */
+int msc_copy_device(msc_ctx* ctx, void* dst, const void* src, int64_t bytes);
 /* Waits for all work queued on the context's stream. */
 int msc_ctx_sync(msc_ctx* ctx);
 

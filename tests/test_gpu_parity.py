@@ -135,3 +135,34 @@ def test_stats_and_levels(msc, orc, gpu_ctx):
     assert fw[0].next_size == ofw[0][3]
     assert st.kernels["level"]["launches"] > 0 and st.kernels["level"]["ms"] > 0
     assert st.bytes > 0
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(blocks_per_cu=1, eager_push_edges=0, eager_cbw_edges=0, first_batch=1, max_batch=1),
+    dict(blocks_per_cu=8, eager_push_edges=1 << 20, eager_cbw_edges=1 << 20, first_batch=8, max_batch=64),
+    dict(blocks_per_cu=2, eager_push_edges=1 << 12, eager_cbw_edges=0, first_batch=2, max_batch=16),
+])
+def test_tuning_keeps_labels(msc, orc, gpu_ctx, tuning):
+    """Launch parameters change the schedule only: labels stay the oracle's."""
+    _say(f"case: tuning {tuning}")
+    defaults = gpu_ctx.tuning()
+    try:
+        gpu_ctx.set_tuning(**tuning)
+        assert gpu_ctx.tuning() == tuning
+        for g in (msc.gen_grid(64, 64, seed=3), msc.gen_rmat(12, 8, seed=2),
+                  msc.gen_planted(1 << 13, max_size=512, seed=4)[0]):
+            _check_scc(msc, orc, gpu_ctx, g)
+            _check_cc(msc, orc, gpu_ctx, g)
+    finally:
+        gpu_ctx.set_tuning(**defaults)
+
+
+def test_tuning_rejects_bad_values(msc, gpu_ctx):
+    defaults = gpu_ctx.tuning()
+    for bad in (dict(blocks_per_cu=0), dict(blocks_per_cu=9), dict(eager_push_edges=-1),
+                dict(first_batch=0), dict(first_batch=8, max_batch=4), dict(max_batch=257)):
+        with pytest.raises(msc.MscError):
+            gpu_ctx.set_tuning(**bad)
+        assert gpu_ctx.tuning() == defaults
+    with pytest.raises(msc.MscError):
+        gpu_ctx.set_tuning(no_such_field=1)
