@@ -144,6 +144,13 @@ def _lib():
                                   C.POINTER(_Stats)]),
         "msc_cc_run": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
                                  C.POINTER(_Stats)]),
+        "msc_scc_run_batch": (C.c_int, [C.POINTER(_Csr), C.c_int32, P, P, P, C.POINTER(_Opts),
+                                        C.POINTER(_Stats)]),
+        "msc_cc_run_batch": (C.c_int, [C.POINTER(_Csr), C.c_int32, P, P, P, C.POINTER(_Opts),
+                                       C.POINTER(_Stats)]),
+        "msc_batch_pack": (C.c_int, [C.c_int32, P, C.POINTER(C.POINTER(_HostGraph)), P]),
+        "msc_gen_batch": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_uint64,
+                                    C.POINTER(C.POINTER(_HostGraph)), P, P]),
         "msc_scc_pivot": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.c_int32),
                                     C.POINTER(C.c_double)]),
         "msc_scc_verify": (C.c_int, [C.c_int32, P, C.POINTER(_Verify)]),
@@ -491,7 +498,7 @@ def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alph
 
 
 def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
-         out, schedule):
+         out, schedule, batch=None):
     n = g.n
     where = HOST
     if isinstance(out, DeviceLabels):
@@ -514,7 +521,12 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
     csr = g.csr()
     piv = -1 if max_deg_vert is None else int(max_deg_vert)
     avg = -1.0 if avg_degree is None else float(avg_degree)
-    rc = getattr(_lib(), fn_name)(C.byref(csr), piv, avg, dst, C.byref(o), C.byref(st))
+    if batch is None:
+        rc = getattr(_lib(), fn_name)(C.byref(csr), piv, avg, dst, C.byref(o), C.byref(st))
+    else:
+        base, pivots = batch
+        rc = getattr(_lib(), fn_name)(C.byref(csr), int(base.shape[0] - 1), _ptr(base),
+                                      _ptr(pivots) if pivots is not None else None, dst, C.byref(o), C.byref(st))
     _check(rc, fn_name)
     return out, Stats._from(st)
 
@@ -542,6 +554,65 @@ def run_cc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, 
     """Weakly connected components (run_cc, multistep/cc_main.cpp:184, on the mirror)."""
     lab, st = _run("msc_cc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
                    alpha, beta, out, schedule)
+    return (lab, st) if stats else lab
+
+
+# ---- batches -------------------------------------------------------------------------------
+def pack_graphs(graphs) -> tuple[Graph, np.ndarray]:
+    """Packs graphs into one CSR (msc_batch_pack): returns (union, vert_base[len+1])."""
+    k = len(graphs)
+    arr = (C.POINTER(_HostGraph) * k)()
+    keep = []
+    for i, g in enumerate(graphs):
+        hg = _HostGraph(g.n, g.m, g.out_off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                        g.out_adj.ctypes.data_as(C.POINTER(C.c_int32)),
+                        g.in_off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                        g.in_adj.ctypes.data_as(C.POINTER(C.c_int32)), g.max_deg_vert, g.avg_degree)
+        keep.append(hg)
+        arr[i] = C.pointer(hg)
+    base = np.empty(k + 1, dtype=np.int64)
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_batch_pack(k, arr, C.byref(out), _ptr(base)), "msc_batch_pack")
+    return _wrap(out), base
+
+
+def gen_batch(num_graphs: int = 8192, median_n: float = 4096, sigma: float = 1.4, min_n: int = 256,
+              max_n: int = 500_000, seed: int = 1) -> tuple[Graph, np.ndarray, np.ndarray]:
+    """Synthetic packed batch (msc_gen_batch): (union, vert_base, kinds; 0 R-MAT, 1 planted)."""
+    base = np.empty(num_graphs + 1, dtype=np.int64)
+    kinds = np.empty(num_graphs, dtype=np.int32)
+    out = C.POINTER(_HostGraph)()
+    _check(_lib().msc_gen_batch(num_graphs, median_n, sigma, min_n, max_n, seed, C.byref(out), _ptr(base),
+                                _ptr(kinds)), "msc_gen_batch")
+    return _wrap(out), base, kinds
+
+
+def _batch_args(vert_base, pivots):
+    base = np.ascontiguousarray(vert_base, dtype=np.int64)
+    piv = None if pivots is None else np.ascontiguousarray(pivots, dtype=np.int32)
+    if base.ndim != 1 or base.shape[0] < 2:
+        raise ValueError("vert_base needs num_graphs + 1 entries")
+    if piv is not None and piv.shape != (base.shape[0] - 1,):
+        raise ValueError("pivots needs one entry per graph")
+    return base, piv
+
+
+def run_scc_batch(g, vert_base, pivots=None, *, policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL,
+                  timing: bool = False, record: bool = False, validate: bool = False, ctx: Context | None = None,
+                  alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False):
+    """SCC labels of every graph of a packed batch (msc_scc_run_batch); each graph's slice
+    holds its own local labels.  ``pivots`` (local ids, -1 none) default to create_csr's rule."""
+    lab, st = _run("msc_scc_run_batch", g, None, None, policy, labels, timing, record, validate, ctx, alpha, beta,
+                   out, None, batch=_batch_args(vert_base, pivots))
+    return (lab, st) if stats else lab
+
+
+def run_cc_batch(g, vert_base, pivots=None, *, policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL,
+                 timing: bool = False, record: bool = False, validate: bool = False, ctx: Context | None = None,
+                 alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False):
+    """Weakly connected components of every graph of a packed batch (msc_cc_run_batch)."""
+    lab, st = _run("msc_cc_run_batch", g, None, None, policy, labels, timing, record, validate, ctx, alpha, beta,
+                   out, None, batch=_batch_args(vert_base, pivots))
     return (lab, st) if stats else lab
 
 

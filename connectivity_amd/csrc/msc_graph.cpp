@@ -44,6 +44,7 @@ enum : uint64_t {
   kStreamArcs = 0xffffffff00000002ULL,
   kStreamSizes = 0xffffffff00000003ULL,
   kStreamOrder = 0xffffffff00000004ULL,
+  kStreamBatch = 0xffffffff00000005ULL,
 };
 
 msc_host_graph* alloc_graph(int32_t n, int64_t m) {
@@ -188,6 +189,38 @@ std::vector<int32_t> permutation(int32_t n, uint64_t seed) {
   Rng r(seed, kStreamPerm);
   for (int32_t i = n - 1; i > 0; --i) std::swap(p[i], p[(int32_t)r.below((uint64_t)i + 1)]);
   return p;
+}
+
+// R-MAT draws over [0, n) for any n >= 1: quadrant sampling in the 2^ceil(log2 n) id
+// space, a draw with an endpoint >= n drawn again from the same stream (after 64 tries
+// the endpoints are reduced mod n); ids permuted; self-loops and duplicates removed.
+int rmat_any_n(int32_t n, int64_t draws, double a, double b, double c, uint64_t seed, msc_host_graph** out) {
+  int32_t scale = 0;
+  while ((1LL << scale) < n) ++scale;
+  const uint64_t ta = (uint64_t)(a * 4294967296.0), tab = (uint64_t)((a + b) * 4294967296.0),
+                 tabc = (uint64_t)((a + b + c) * 4294967296.0);
+  std::vector<int32_t> src((size_t)draws), dst((size_t)draws);
+  std::vector<int32_t> perm = permutation(n, seed);
+  for (int64_t e = 0; e < draws; ++e) {
+    Rng r(seed, (uint64_t)e);
+    uint32_t u = 0, v = 0;
+    for (int tries = 0; tries < 64; ++tries) {
+      u = v = 0;
+      uint64_t bits = 0;
+      for (int32_t k = 0; k < scale; ++k) {
+        if ((k & 1) == 0) bits = r.next();
+        uint64_t x = (k & 1) ? (bits >> 32) : (bits & 0xffffffffULL);
+        uint32_t su = x >= tabc || (x >= tab && x < tabc) ? 1u : 0u;
+        uint32_t sv = x >= tabc || (x >= ta && x < tab) ? 1u : 0u;
+        u = (u << 1) | su;
+        v = (v << 1) | sv;
+      }
+      if (u < (uint32_t)n && v < (uint32_t)n) break;
+    }
+    src[e] = perm[u % (uint32_t)n];
+    dst[e] = perm[v % (uint32_t)n];
+  }
+  return build_dedupe(n, draws, src.data(), dst.data(), true, out);
 }
 
 bool read_i64(FILE* f, int64_t* x) {
@@ -476,6 +509,84 @@ int msc_gen_planted(int32_t n, double zipf_s, int32_t max_size, double extra_per
     if (comp_out)
       for (int32_t v = 0; v < n; ++v) comp_out[perm[v]] = comp[v];
     return build_dedupe(n, (int64_t)src.size(), src.data(), dst.data(), true, out);
+  } catch (const std::bad_alloc&) {
+    return MSC_E_OOM;
+  }
+}
+
+int msc_batch_pack(int32_t num_graphs, const msc_host_graph* const* graphs, msc_host_graph** out,
+                   int64_t* vert_base) {
+  if (num_graphs < 1 || !graphs || !out || !vert_base) return MSC_E_INVALID;
+  std::vector<int64_t> ebase((size_t)num_graphs + 1);
+  vert_base[0] = 0;
+  ebase[0] = 0;
+  for (int32_t i = 0; i < num_graphs; ++i) {
+    const msc_host_graph* g = graphs[i];
+    if (!g || g->n < 0 || g->m < 0) return MSC_E_INVALID;
+    vert_base[i + 1] = vert_base[i] + g->n;
+    ebase[i + 1] = ebase[i] + g->m;
+  }
+  const int64_t n = vert_base[num_graphs], m = ebase[num_graphs];
+  if (n > INT32_MAX || m >= (1LL << 32)) return MSC_E_CAPACITY;
+  msc_host_graph* u = alloc_graph((int32_t)n, m);
+  if (!u) return MSC_E_OOM;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t i = 0; i < num_graphs; ++i) {
+    const msc_host_graph* g = graphs[i];
+    const int64_t vb = vert_base[i], eb = ebase[i];
+    for (int32_t v = 0; v < g->n; ++v) {
+      u->out_off[vb + v] = (uint32_t)(eb + g->out_off[v]);
+      u->in_off[vb + v] = (uint32_t)(eb + g->in_off[v]);
+    }
+    for (int64_t e = 0; e < g->m; ++e) {
+      u->out_adj[eb + e] = (int32_t)(vb + g->out_adj[e]);
+      u->in_adj[eb + e] = (int32_t)(vb + g->in_adj[e]);
+    }
+  }
+  u->out_off[n] = u->in_off[n] = (uint32_t)m;
+  pivot_rule(u);
+  *out = u;
+  return MSC_OK;
+}
+
+int msc_gen_batch(int32_t num_graphs, double median_n, double sigma, int32_t min_n, int32_t max_n, uint64_t seed,
+                  msc_host_graph** out, int64_t* vert_base, int32_t* kinds) {
+  if (num_graphs < 1 || !out || !vert_base || !(median_n >= 1.0) || sigma < 0 || min_n < 1 || max_n < min_n)
+    return MSC_E_INVALID;
+  try {
+    // sizes, kinds and edge factors from one stream (Box-Muller normals)
+    std::vector<int32_t> ns(num_graphs), kind(num_graphs), ef(num_graphs);
+    Rng rs(seed, kStreamBatch);
+    for (int32_t i = 0; i < num_graphs; ++i) {
+      double u1 = 1.0 - rs.uniform(), u2 = rs.uniform();
+      double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+      double x = std::round(median_n * std::exp(sigma * z));
+      ns[i] = (int32_t)std::min<double>(std::max<double>(x, (double)min_n), (double)max_n);
+      kind[i] = (int32_t)(rs.next() >> 63);
+      ef[i] = 8 + (int32_t)rs.below(25);
+    }
+    std::vector<msc_host_graph*> gs(num_graphs, nullptr);
+    int err = MSC_OK;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < num_graphs; ++i) {
+      uint64_t gseed = Rng::mix(seed ^ Rng::mix(0x5851f42d4c957f2dULL + (uint64_t)i));
+      int rc;
+      try {
+        rc = kind[i] == 0 ? rmat_any_n(ns[i], (int64_t)ef[i] * ns[i], 0.57, 0.19, 0.19, gseed, &gs[i])
+                          : msc_gen_planted(ns[i], 1.5, ns[i], 1.0, 4LL * ns[i], gseed, &gs[i], nullptr);
+      } catch (const std::bad_alloc&) {
+        rc = MSC_E_OOM;
+      }
+      if (rc) {
+#pragma omp critical
+        err = rc;
+      }
+    }
+    int rc = err;
+    if (!rc) rc = msc_batch_pack(num_graphs, gs.data(), out, vert_base);
+    for (msc_host_graph* g : gs) msc_graph_free(g);
+    if (!rc && kinds) std::copy(kind.begin(), kind.end(), kinds);
+    return rc;
   } catch (const std::bad_alloc&) {
     return MSC_E_OOM;
   }

@@ -132,5 +132,18 @@ void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* out_off, con
 int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
                     unsigned long long* part);
 void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err);
+// ---- batches: graph g owns vertices [base[g], base[g+1]) of a packed union CSR ----
+// piv[g] = per-graph pivot (union id, -1 = none); keys: ng u64 scratch
+void launch_batch_pivots(const Launch& L, bool cc, int32_t n, const uint32_t* oo, const uint32_t* io,
+                         const int64_t* base, uint32_t ng, unsigned long long* keys, int32_t* piv);
+// fresh control block, every pivot claimed and queued (frontier buffer 1)
+void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const int32_t* piv, uint32_t ng,
+                        int32_t phase);
+int launch_label_bits_batch(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
+                            const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part);
+void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng);
+// run only after launch_validate passed (reads adjacency through the offsets)
+void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,
+                           uint32_t ng, uint32_t* err);
 
 }  // namespace msc

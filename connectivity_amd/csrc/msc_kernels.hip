@@ -1064,6 +1064,200 @@ __global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ of
   if (bad) atomicOr(err, 1u);
 }
 
+// ------------------------------------------------------------------------------------
+// batches: many independent graphs packed into one CSR
+//
+// Graph g owns vertices [base[g], base[g+1]) (base[0] = 0, base[ng] = n, non-decreasing)
+// and every edge stays inside its graph.  The whole path runs on the union: trim,
+// colouring and canonicalisation are per-vertex or follow edges, so they never mix
+// graphs; FW-BW and the CC search start from every graph's pivot at once, and because the
+// graphs are disjoint, the union's reach sets are the per-graph reach sets side by side.
+
+// Last g in [lo, hi) with base[g] <= v, given base[lo] <= v < base[hi] (an empty graph
+// shares its base with the next one, and the later graph is the one holding v).
+__device__ __forceinline__ uint32_t graph_of(const int64_t* __restrict__ base, uint32_t lo, uint32_t hi, int64_t v) {
+  while (hi - lo > 1u) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (base[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Graph of vertex wbase + lane (clamped to n - 1) for a wave walking 64 consecutive ids:
+// two searches per wave, and a per-lane one only when the wave straddles graphs.
+__device__ __forceinline__ uint32_t wave_graph_of(const int64_t* __restrict__ base, uint32_t ng, int64_t wbase,
+                                                  int64_t n) {
+  const int64_t last = wbase + 63 < n ? wbase + 63 : n - 1;
+  const uint32_t glo = graph_of(base, 0u, ng, wbase);
+  const uint32_t ghi = graph_of(base, glo, ng, last);
+  int64_t v = wbase + lane();
+  if (v > last) v = last;
+  return glo == ghi ? glo : graph_of(base, glo, ghi + 1u, v);
+}
+
+// Per-graph pivot, pass 1: keys[g] = max over the graph of the product's bit pattern
+// (create_csr's rule, multistep/scc_main.cpp:169-183, applied to each graph).  Lanes of a
+// wave in one graph combine before the atomic.
+__global__ void __launch_bounds__(kBlock) k_bpivot_max(int32_t n, const uint32_t* __restrict__ oo,
+                                                       const uint32_t* __restrict__ io,
+                                                       const int64_t* __restrict__ base, uint32_t ng,
+                                                       unsigned long long* __restrict__ keys) {
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = wb + lane();
+    const uint32_t g = wave_graph_of(base, ng, wb, n);
+    unsigned long long p = 0ull;
+    if (v < n) p = (unsigned long long)__double_as_longlong((double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]));
+    bool active = p != 0ull;
+    for (;;) {
+      unsigned long long m = __ballot(active);
+      if (m == 0ull) break;
+      const int leader = __ffsll((long long)m) - 1;
+      const uint32_t gk = __shfl(g, leader, 64);
+      const bool mine = active && g == gk;
+      const unsigned long long x = wave_max64(mine ? p : 0ull);
+      if (lane() == leader) atomicMax(&keys[gk], x);
+      active = active && !mine;
+    }
+  }
+}
+// Pass 2: piv[g] (initially 0xffffffff, i.e. -1) = first vertex of g attaining keys[g] > 0.
+__global__ void __launch_bounds__(kBlock) k_bpivot_first(int32_t n, const uint32_t* __restrict__ oo,
+                                                         const uint32_t* __restrict__ io,
+                                                         const int64_t* __restrict__ base, uint32_t ng,
+                                                         const unsigned long long* __restrict__ keys,
+                                                         uint32_t* __restrict__ piv) {
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = wb + lane();
+    const uint32_t g = wave_graph_of(base, ng, wb, n);
+    bool hit = false;
+    if (v < n) {
+      unsigned long long p =
+          (unsigned long long)__double_as_longlong((double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]));
+      hit = p != 0ull && p == keys[g];
+    }
+    if (hit) atomicMin(&piv[g], (uint32_t)v);
+  }
+}
+// CC pivot per graph (multistep/cc_main.cpp:148-160): max of (out+in degree << 32 | ~v).
+__global__ void __launch_bounds__(kBlock) k_bpivot_cc(int32_t n, const uint32_t* __restrict__ oo,
+                                                      const uint32_t* __restrict__ io,
+                                                      const int64_t* __restrict__ base, uint32_t ng,
+                                                      unsigned long long* __restrict__ keys) {
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = wb + lane();
+    const uint32_t g = wave_graph_of(base, ng, wb, n);
+    unsigned long long key = 0ull;
+    if (v < n) {
+      uint32_t d = (oo[v + 1] - oo[v]) + (io[v + 1] - io[v]);
+      if (d) key = ((unsigned long long)d << 32) | (unsigned long long)(0xffffffffu - (uint32_t)v);
+    }
+    bool active = key != 0ull;
+    for (;;) {
+      unsigned long long m = __ballot(active);
+      if (m == 0ull) break;
+      const int leader = __ffsll((long long)m) - 1;
+      const uint32_t gk = __shfl(g, leader, 64);
+      const bool mine = active && g == gk;
+      const unsigned long long x = wave_max64(mine ? key : 0ull);
+      if (lane() == leader) atomicMax(&keys[gk], x);
+      active = active && !mine;
+    }
+  }
+}
+__global__ void k_bpivot_cc_final(uint32_t ng, const unsigned long long* __restrict__ keys, int32_t* __restrict__ piv) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+    unsigned long long k = keys[g];
+    piv[g] = k ? (int32_t)(0xffffffffu - (uint32_t)(k & 0xffffffffull)) : -1;
+  }
+}
+
+// Starts a search from every graph's pivot (piv[g] < 0: the graph has none).  Each root
+// is claimed and queued, like the single root of k_search_init.  Needs k_ctrl_reset
+// first, k_close_seed after.
+template <bool MIRROR>
+__global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32_t* __restrict__ piv, uint32_t ng) {
+  __shared__ Stage S;
+  if (threadIdx.x == 0) S.cnt = 0u;
+  __syncthreads();
+  for (uint32_t g0 = blockIdx.x * kBlock; g0 < ng; g0 += gridDim.x * kBlock) {
+    const uint32_t g = g0 + threadIdx.x;
+    const int32_t p = g < ng ? piv[g] : -1;
+    uint32_t deg = 0;
+    if (p >= 0) {
+      atomicAnd(&a.cand[(uint32_t)p >> 5], ~(1u << ((uint32_t)p & 31u)));
+      deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, (uint32_t)p);
+    }
+    stage_push(S, p >= 0, (uint32_t)p, deg);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);
+  }
+  stage_flush(S, a.fr1, a.ctrl, 0u);
+}
+
+// labels[v] = piv[graph of v] for each set bit v of `member`; part col 0 = members.
+__global__ void __launch_bounds__(kBlock) k_label_bits_batch(const uint32_t* __restrict__ member, int64_t nwords,
+                                                             int32_t n, int32_t* __restrict__ labels,
+                                                             const int32_t* __restrict__ piv,
+                                                             const int64_t* __restrict__ base, uint32_t ng,
+                                                             unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
+    uint32_t w = member[i];
+    cnt += (uint32_t)__popc(w);
+    if (w == 0u) continue;
+    int64_t v = i * 32 + (__ffs(w) - 1);
+    if (v >= n) continue;
+    uint32_t g = graph_of(base, 0u, ng, v);
+    while (w) {
+      v = i * 32 + (__ffs(w) - 1);
+      if (v >= n) break;
+      while (base[g + 1] <= v) ++g;
+      labels[v] = piv[g];
+      w &= w - 1u;
+    }
+  }
+  cnt = block_sum64(cnt, s_red);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = cnt;
+    part[4 * blockIdx.x + 1] = part[4 * blockIdx.x + 2] = part[4 * blockIdx.x + 3] = 0;
+  }
+}
+
+// Union ids -> ids local to each graph: labels[v] -= base[graph of v] (labels are vertices
+// of the same graph).
+__global__ void __launch_bounds__(kBlock) k_localise(int32_t n, int32_t* __restrict__ labels,
+                                                     const int64_t* __restrict__ base, uint32_t ng) {
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = wb + lane();
+    const uint32_t g = wave_graph_of(base, ng, wb, n);
+    if (v < n) {
+      int32_t l = labels[v];
+      if (l >= 0) labels[v] = (int32_t)((int64_t)l - base[g]);
+    }
+  }
+}
+
+// Batch input check: every edge of graph g ends inside [base[g], base[g+1]).
+__global__ void __launch_bounds__(kBlock) k_validate_batch(int32_t n, const uint32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ adj,
+                                                           const int64_t* __restrict__ base, uint32_t ng,
+                                                           uint32_t* err) {
+  bool bad = false;
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = wb + lane();
+    const uint32_t g = wave_graph_of(base, ng, wb, n);
+    if (v < n) {
+      const int64_t lo = base[g], hi = base[g + 1];
+      for (uint32_t j = off[v]; j < off[v + 1]; ++j) {
+        int64_t w = adj[j];
+        bad |= w < lo || w >= hi;
+      }
+    }
+  }
+  if (bad) atomicOr(err, 1u);
+}
+
 inline int grid_for(int64_t items, int64_t per_block, int cap) {
   int64_t b = (items + per_block - 1) / per_block;
   if (b < 1) b = 1;
@@ -1176,6 +1370,50 @@ int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32
   int g = grid_for(n, kBlock * 8, L.grid * 2);
   hipLaunchKernelGGL(k_pivot_cc, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part);
   return g;
+}
+
+void launch_batch_pivots(const Launch& L, bool cc, int32_t n, const uint32_t* oo, const uint32_t* io,
+                         const int64_t* base, uint32_t ng, unsigned long long* keys, int32_t* piv) {
+  launch_fill_u32(L, (uint32_t*)keys, 0u, 2 * (int64_t)ng);
+  dim3 g(grid_for(n, kBlock * 8, L.grid * 2)), b(kBlock);
+  if (cc) {
+    hipLaunchKernelGGL(k_bpivot_cc, g, b, 0, L.stream, n, oo, io, base, ng, keys);
+    hipLaunchKernelGGL(k_bpivot_cc_final, dim3(grid_for(ng, kBlock, L.grid)), b, 0, L.stream, ng, keys, piv);
+  } else {
+    launch_fill_u32(L, (uint32_t*)piv, 0xffffffffu, ng);
+    hipLaunchKernelGGL(k_bpivot_max, g, b, 0, L.stream, n, oo, io, base, ng, keys);
+    hipLaunchKernelGGL(k_bpivot_first, g, b, 0, L.stream, n, oo, io, base, ng, keys, (uint32_t*)piv);
+  }
+}
+
+void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const int32_t* piv, uint32_t ng,
+                        int32_t phase) {
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, phase);
+  dim3 g((unsigned)grid_for(ng, kBlock, L.grid));
+  if (mirror) hipLaunchKernelGGL(k_seed_pivots<true>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
+  else hipLaunchKernelGGL(k_seed_pivots<false>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
+}
+
+int launch_label_bits_batch(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
+                            const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part) {
+  int g = grid_for(nwords, kBlock * 4, L.grid * 2);
+  hipLaunchKernelGGL(k_label_bits_batch, dim3(g), dim3(kBlock), 0, L.stream, member, nwords, n, labels, piv, base,
+                     ng, part);
+  return g;
+}
+
+void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_localise, dim3(grid_for(n, kBlock * 4, L.grid * 4)), dim3(kBlock), 0, L.stream, n, labels,
+                     base, ng);
+}
+
+void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,
+                           uint32_t ng, uint32_t* err) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_validate_batch, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, off,
+                     adj, base, ng, err);
 }
 
 void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err) {

@@ -56,6 +56,12 @@ struct msc_ctx {
   int8_t* sched_host = nullptr;  // pinned staging
   bool has_mlp = false;
   float clock_mhz = 100.0f;
+  // batches: vertex bases, per-graph pivots and pivot keys
+  void* bws = nullptr;
+  int64_t bws_graphs = 0;
+  int64_t* bbase = nullptr;
+  int32_t* bpiv = nullptr;
+  unsigned long long* bkeys = nullptr;
   // staging for host-resident inputs
   void* in_buf = nullptr;
   size_t in_bytes = 0;
@@ -359,6 +365,63 @@ int device_pivot(msc_ctx* c, const msc_csr* d, bool cc, int32_t* pivot) {
   return MSC_OK;
 }
 
+// A batch as the host passed it (vert_base / pivots in host memory, pivots local).
+struct BatchIn {
+  int32_t ng;
+  const int64_t* base;
+  const int32_t* pivots;
+};
+
+int check_batch(const BatchIn* b, int32_t n) {
+  if (b->ng < 1 || !b->base) return MSC_E_INVALID;
+  if (b->base[0] != 0 || b->base[b->ng] != (int64_t)n) return MSC_E_INVALID;
+  for (int32_t i = 0; i < b->ng; ++i) {
+    if (b->base[i + 1] < b->base[i]) return MSC_E_INVALID;
+    if (b->pivots && (b->pivots[i] < -1 || (int64_t)b->pivots[i] >= b->base[i + 1] - b->base[i])) return MSC_E_INVALID;
+  }
+  return MSC_OK;
+}
+
+// Device copies of the batch tables; per-graph pivots given by the caller become union ids.
+int stage_batch(msc_ctx* c, const BatchIn* b) {
+  if (b->ng > c->bws_graphs) {
+    if (c->bws) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->bws);
+      c->bws = nullptr;
+      c->bws_graphs = 0;
+    }
+    size_t ng = (size_t)b->ng;
+    size_t sz = align256((ng + 1) * 8) + align256(ng * 4) + align256(ng * 8);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, sz));
+    c->bws = p;
+    c->bws_graphs = b->ng;
+    c->bbase = (int64_t*)p;
+    c->bpiv = (int32_t*)((char*)p + align256((ng + 1) * 8));
+    c->bkeys = (unsigned long long*)((char*)c->bpiv + align256(ng * 4));
+  }
+  HIP_TRY(hipMemcpyAsync(c->bbase, b->base, ((size_t)b->ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  if (b->pivots) {
+    std::vector<int32_t> piv(b->ng);
+    for (int32_t i = 0; i < b->ng; ++i) piv[i] = b->pivots[i] < 0 ? -1 : (int32_t)(b->base[i] + b->pivots[i]);
+    HIP_TRY(hipMemcpyAsync(c->bpiv, piv.data(), (size_t)b->ng * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // piv is a local buffer
+  }
+  return MSC_OK;
+}
+
+int device_validate_batch(msc_ctx* c, const msc_csr* d, uint32_t ng) {
+  Launch L = launch_of(c);
+  launch_fill_u32(L, c->err, 0u, 1);
+  launch_validate_batch(L, d->n, d->out_off, d->out_adj, c->bbase, ng, c->err);
+  launch_validate_batch(L, d->n, d->in_off, d->in_adj, c->bbase, ng, c->err);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(c->pinned_u32, c->err, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return *c->pinned_u32 ? MSC_E_INVALID : MSC_OK;
+}
+
 msc_ctx* default_ctx(int device, int* rc) {
   std::lock_guard<std::mutex> lk(g_default_mu);
   if (device < 0) {
@@ -378,14 +441,15 @@ msc_ctx* default_ctx(int device, int* rc) {
 
 enum class Kind { SCC, CC };
 
-// The whole hot path for one graph.
+// The whole hot path for one graph, or for a batch of graphs packed into one CSR (bt).
 int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t* labels_out, const msc_opts* o,
-             msc_stats* s) {
+             msc_stats* s, const BatchIn* bt = nullptr) {
   const bool cc = kind == Kind::CC;
   int rc = check_args(g_in, labels_out, o, cc);
   if (rc) return rc;
   if ((rc = host_graph_checks(g_in))) return rc;
   if (pivot >= g_in->n || pivot < -1) return MSC_E_INVALID;
+  if (bt && (rc = check_batch(bt, g_in->n))) return rc;
   msc_ctx* c = o->ctx;
   if (!c) {
     c = default_ctx(o->device, &rc);
@@ -409,18 +473,30 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   msc_csr g;
   if ((rc = stage_graph(c, g_in, &g))) return rc;
   if (o->validate && (rc = device_validate(c, &g))) return rc;
+  const uint32_t ng = bt ? (uint32_t)bt->ng : 0u;
+  if (bt) {
+    if ((rc = stage_batch(c, bt))) return rc;
+    if (o->validate && (rc = device_validate_batch(c, &g, ng))) return rc;
+  }
   Launch L = launch_of(c);
   int32_t* labels = o->labels_where == MSC_DEVICE ? labels_out : c->labels;
   const int64_t nw = ((int64_t)n + 31) / 32;
   HIP_TRY(hipMemsetAsync(c->ctrl, 0, sizeof(Ctrl), c->stream));
 
   if (avg < 0.0) avg = cc ? 2.0 * (double)g.m / (double)n : (double)g.m / (double)n;
-  if (pivot < 0) {
+  if (bt) {
+    // per-graph pivots: every graph's search starts together; `pivot` only gates the search
+    if (!bt->pivots) {
+      Timed t(c, MSC_K_OTHER);
+      launch_batch_pivots(L, cc, n, g.out_off, g.in_off, c->bbase, ng, c->bkeys, c->bpiv);
+    }
+    pivot = 0;
+  } else if (pivot < 0) {
     Timed t(c, MSC_K_OTHER);
     if ((rc = device_pivot(c, &g, cc, &pivot))) return rc;
   }
   if (s) {
-    s->pivot = pivot;
+    s->pivot = bt ? -1 : pivot;
     s->avg_degree = avg;
   }
 
@@ -452,7 +528,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
       fw.allow = c->valid; fw.cand = c->b1;
-      launch_search_init(L, fw, pivot, 0);
+      if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
+      else launch_search_init(L, fw, pivot, 0);
       if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
@@ -464,7 +541,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         bw.down_off = g.in_off; bw.down_adj = g.in_adj;
         bw.up_off = g.out_off; bw.up_adj = g.out_adj;
         bw.allow = c->b2; bw.cand = c->b3;
-        launch_search_init(L, bw, pivot, 1);
+        if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
+        else launch_search_init(L, bw, pivot, 1);
         if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 0))) return rc;
         if ((rc = read_ctrl(c))) return rc;
         if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
@@ -480,7 +558,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
       bf.allow = c->valid; bf.cand = c->b1;
-      launch_search_init(L, bf, pivot, 2);
+      if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
+      else launch_search_init(L, bf, pivot, 2);
       if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, 0))) return rc;
       if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
@@ -493,7 +572,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       // commit (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271)
       {
         Timed t(c, MSC_K_COMMIT);
-        np = launch_label_bits(L, member, nw, n, labels, pivot, c->part);
+        np = bt ? launch_label_bits_batch(L, member, nw, n, labels, c->bpiv, c->bbase, ng, c->part)
+                : launch_label_bits(L, member, nw, n, labels, pivot, c->part);
         launch_andnot(L, c->valid, nullptr, c->valid, member, nw);
       }
       if ((rc = reduce_read(c, np, red))) return rc;
@@ -571,6 +651,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   if (o->label_mode == MSC_LABELS_CANONICAL) {
     Timed t(c, MSC_K_OTHER);
     launch_canon(L, n, labels, c->colors);
+  }
+  if (bt) {
+    Timed t(c, MSC_K_OTHER);
+    launch_localise(L, n, labels, c->bbase, ng);
   }
   if (o->labels_where == MSC_HOST)
     HIP_TRY(hipMemcpyAsync(labels_out, labels, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -697,6 +781,7 @@ int msc_ctx_destroy(msc_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->ws) (void)hipFree(c->ws);
   if (c->in_buf) (void)hipFree(c->in_buf);
+  if (c->bws) (void)hipFree(c->bws);
   for (Ev& e : c->events) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
@@ -762,6 +847,30 @@ int msc_cc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_
                msc_stats* stats) {
   try {
     return run_path(Kind::CC, g, max_deg_vert, avg_degree, labels, opts, stats);
+  } catch (const std::bad_alloc&) {
+    return MSC_E_OOM;
+  } catch (...) {
+    return MSC_E_DEVICE;
+  }
+}
+
+int msc_scc_run_batch(const msc_csr* g, int32_t num_graphs, const int64_t* vert_base, const int32_t* pivots,
+                      int32_t* labels, const msc_opts* opts, msc_stats* stats) {
+  try {
+    BatchIn b{num_graphs, vert_base, pivots};
+    return run_path(Kind::SCC, g, -1, -1.0, labels, opts, stats, &b);
+  } catch (const std::bad_alloc&) {
+    return MSC_E_OOM;
+  } catch (...) {
+    return MSC_E_DEVICE;
+  }
+}
+
+int msc_cc_run_batch(const msc_csr* g, int32_t num_graphs, const int64_t* vert_base, const int32_t* pivots,
+                     int32_t* labels, const msc_opts* opts, msc_stats* stats) {
+  try {
+    BatchIn b{num_graphs, vert_base, pivots};
+    return run_path(Kind::CC, g, -1, -1.0, labels, opts, stats, &b);
   } catch (const std::bad_alloc&) {
     return MSC_E_OOM;
   } catch (...) {

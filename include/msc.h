@@ -228,6 +228,26 @@ int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32
 int msc_cc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
                const msc_opts* opts, msc_stats* stats);
 
+/* Batches: many independent graphs decomposed in one pass.  Replaces a loop of
+ * run_scc / run_cc calls (multistep/scc_main.cpp:210, cc_main.cpp:184), one per graph.
+ *   g:           the graphs packed into one CSR: graph i owns the vertices
+ *                [vert_base[i], vert_base[i+1]) and its edges end inside that range
+ *                (msc_batch_pack builds this from separate graphs).
+ *   vert_base:   host, num_graphs + 1 entries, vert_base[0] = 0, non-decreasing,
+ *                vert_base[num_graphs] = g->n (empty graphs allowed).
+ *   pivots:      host, num_graphs pivots in each graph's own ids (-1: that graph has
+ *                none), or NULL to apply create_csr's rule to every graph on device.
+ *   labels:      g->n int32; each graph's slice holds exactly what msc_scc_run returns
+ *                for that graph alone (ids local to the graph, raw or canonical).
+ * All graphs share every launch: trim and colouring are per vertex, and FW-BW starts from
+ * all pivots at once (the graphs are disjoint, so the reach sets do not mix).
+ * stats->pivot is -1; the other counters cover the whole batch. */
+int msc_scc_run_batch(const msc_csr* g, int32_t num_graphs, const int64_t* vert_base, const int32_t* pivots,
+                      int32_t* labels, const msc_opts* opts, msc_stats* stats);
+/* Weakly connected components of every graph of a batch (per graph as msc_cc_run). */
+int msc_cc_run_batch(const msc_csr* g, int32_t num_graphs, const int64_t* vert_base, const int32_t* pivots,
+                     int32_t* labels, const msc_opts* opts, msc_stats* stats);
+
 /* Pivot and average degree on device.  Replaces the loop inside create_csr
  * (multistep/scc_main.cpp:169-183; kokkos_scc/scc_main.cpp:228-247). */
 int msc_scc_pivot(const msc_csr* g, const msc_opts* opts, int32_t* max_deg_vert,
@@ -285,6 +305,20 @@ int msc_gen_grid(int32_t rows, int32_t cols, double p_one, double p_both, double
  * partition is the planted one; comp_out (may be NULL, n entries) receives it. */
 int msc_gen_planted(int32_t n, double zipf_s, int32_t max_size, double extra_per_vertex,
                     int64_t dag_arcs, uint64_t seed, msc_host_graph** out, int32_t* comp_out);
+
+/* Packs graphs into one CSR for msc_*_run_batch: graph i's vertex v becomes
+ * vert_base[i] + v.  vert_base receives num_graphs + 1 entries.  Fails with
+ * MSC_E_CAPACITY when the union reaches 2^31 vertices or 2^32 edges. */
+int msc_batch_pack(int32_t num_graphs, const msc_host_graph* const* graphs, msc_host_graph** out,
+                   int64_t* vert_base);
+/* A synthetic batch, packed: vertex counts log-normal (median median_n, log-space
+ * sigma), clipped to [min_n, max_n]; each graph is, with equal odds, an R-MAT digraph
+ * (a, b, c = .57, .19, .19, edge factor uniform in [8, 32], 2^ceil(log2 n) id space with
+ * draws outside [0, n) drawn again) or planted SCCs (Zipf s = 1.5 sizes up to n, one
+ * extra internal arc per vertex, 4n DAG arcs).  kinds (may be NULL) receives 0 for R-MAT,
+ * 1 for planted, per graph. */
+int msc_gen_batch(int32_t num_graphs, double median_n, double sigma, int32_t min_n, int32_t max_n, uint64_t seed,
+                  msc_host_graph** out, int64_t* vert_base, int32_t* kinds);
 
 /* ---- device graphs --------------------------------------------------------------- */
 /* Copy a host graph into device memory of the context's device; fills *dev (where =
