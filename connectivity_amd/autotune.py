@@ -4,9 +4,9 @@ The level loops have five tunables that no rule of the reference fixes and that 
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
-  eager_push_edges  colour push sweeps whose frontier has at most this many edges walk
-                    depth-first inside one launch instead of one level per launch;
-  eager_cbw_edges   the same for the per-colour backward searches;
+  small_push_edges  colour push sweeps whose frontier has at most this many edges run
+                    inside the one-workgroup closing kernel, many levels per launch;
+  small_cbw_edges   the same for the per-colour backward searches;
   first_batch       levels enqueued before a loop first reads its `done` word back;
   max_batch         levels enqueued between later reads (batches double up to this).
 
@@ -32,8 +32,8 @@ from . import (LABELS_CANONICAL, POLICY_LEARNED, POLICY_PLAIN, Context, DeviceGr
 
 CANDIDATES = {
     "blocks_per_cu": [2, 4, 8],
-    "eager_push_edges": [0, 1 << 12, 1 << 16, 1 << 20],
-    "eager_cbw_edges": [0, 1 << 12, 1 << 16, 1 << 20],
+    "small_push_edges": [0, 1024, 2048, 4096, 8192, 16384],
+    "small_cbw_edges": [0, 1024, 2048, 4096, 8192, 16384],
     "first_batch": [2, 4, 8],
     "max_batch": [16, 32, 64],
 }

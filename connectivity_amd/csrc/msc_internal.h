@@ -59,6 +59,8 @@ struct Ctrl {
   unsigned long long bytes_op[5];  // algorithmic bytes of closed levels, per Op
   unsigned long long bu_levels;
   unsigned long long t_start;      // earliest workgroup start of the running launch
+  uint32_t closes;                 // closing-kernel launches that closed a level
+  uint32_t pad_;
 };
 
 struct LevelArgs {
@@ -92,7 +94,7 @@ struct LevelArgs {
   const float* mlp;           // kMlpBlob floats (learned policy)
   const int8_t* sched;        // forced actions, 3 x MSC_SCHED_LEVELS (null: none)
   float clock_mhz;            // wall_clock64() rate, for per-level durations
-  uint32_t eager_edges;       // colour sweeps over at most this many edges walk eagerly
+  uint32_t small_edges;       // colour levels over at most this many edges run inside the closer
 };
 
 struct Launch {
@@ -113,7 +115,7 @@ int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off
 void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out);
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);  // work only
-void launch_close(const Launch& L, int op, const LevelArgs& a);              // one workgroup
+void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  // one workgroup
 // labels[v] = label for every set bit of member; part col 0 = count
 int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
                       int32_t label, unsigned long long* part);

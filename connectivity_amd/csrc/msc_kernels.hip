@@ -243,77 +243,21 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
 }
 
 // ------------------------------------------------------------------------------------
-// eager relaxation for small colour frontiers
-//
-// Max-label pushes and equal-colour claims are monotone, so the fixpoint does not depend
-// on the order of relaxations.  When a colour frontier is small, a lane that raises (or
-// claims) a vertex keeps walking depth-first from it within a fixed edge budget instead
-// of leaving every hop to a later sweep: each sweep on a deep graph (grids, chains of
-// planted SCCs) then covers many hops, and sweeps -- each a launch -- become fewer.
-// Whatever the walk cannot finish (the vertex being walked, the stack) is queued as usual.
-
-constexpr int kEagerStack = 4, kEagerOut = 8, kEagerBudget = 48;
-
-template <int OP, bool MIRROR>
-__device__ int eager_walk(const LevelArgs& a, uint32_t w, int32_t col, uint32_t* out, unsigned long long& raised) {
-  uint32_t stack[kEagerStack];
-  int sp = 0, nout = 0, budget = kEagerBudget;
-  stack[sp++] = w;
-  while (sp > 0) {
-    const uint32_t x = stack[--sp];
-    const uint32_t b1 = a.down_off[x], d1 = a.down_off[x + 1] - b1;
-    uint32_t b2 = 0, d2 = 0;
-    if (MIRROR) {
-      b2 = a.down_off2[x];
-      d2 = a.down_off2[x + 1] - b2;
-    }
-    bool finished = true;
-    for (uint32_t i = 0; i < d1 + d2; ++i) {
-      if (budget == 0 || nout + sp + 2 > kEagerOut) {  // keep room for x and the stack
-        finished = false;
-        break;
-      }
-      --budget;
-      const uint32_t y = (uint32_t)(i < d1 ? a.down_adj[b1 + i] : a.down_adj2[b2 + i - d1]);
-      bool got = false;
-      if (OP == OP_PUSH) {
-        if (bit_test(a.valid, y) && a.colors[y] < col && atomicMax(&a.colors[y], col) < col) {
-          ++raised;
-          got = true;
-        }
-      } else {
-        if (bit_test(a.valid, y) && a.colors[y] == col && bit_clear(a.valid, y)) {
-          a.labels[y] = col;
-          got = true;
-        }
-      }
-      if (got) {
-        if (sp < kEagerStack) stack[sp++] = y;
-        else out[nout++] = y;
-      }
-    }
-    if (!finished) {
-      out[nout++] = x;
-      while (sp > 0) out[nout++] = stack[--sp];
-      break;
-    }
-  }
-  return nout;
-}
-
-// ------------------------------------------------------------------------------------
 // learned policy: 12 -> 32 -> 32 -> 3 ReLU MLP, evaluated by one wave
 
 // Raw features of the level just closed (same order as the training code; the blob
-// carries the standardisation).
+// carries the standardisation).  Counts enter as logs of their fraction of the graph
+// (vertices of n, edges of avg * n), so a policy trained on small graphs sees the same
+// values on a large graph in the same state.
 __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges, bool bu,
                                float* f) {
   float n = a.n_total > 0 ? (float)a.n_total : 1.0f;
   float nv = a.num_valid > 0 ? (float)a.num_valid : 1.0f;
+  float m = fmaxf((float)a.avg * n, 1.0f);
   float found = (float)c->found;
-  f[0] = log1pf((float)next);
-  f[1] = log1pf((float)nedges);
-  f[2] = log1pf(fmaxf(nv - found, 0.0f));
+  f[0] = logf(fmaxf((float)next, 1.0f) / n);
+  f[1] = logf(fmaxf((float)nedges, 1.0f) / m);
+  f[2] = logf(fmaxf(nv - found, 1.0f) / n);
   f[3] = log1pf((float)c->level);
   f[4] = (float)next / n;
   f[5] = (float)nedges / fmaxf((float)a.avg * n, 1.0f);
@@ -500,6 +444,7 @@ __global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
+    c->closes = 0u;
     c->found = 0;
     c->phase = (uint32_t)phase;
     c->abandoned = 0u;
@@ -524,6 +469,7 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
+    c->closes = 0u;
     c->found = 0;
     c->phase = (uint32_t)phase;
     c->abandoned = 0u;
@@ -544,23 +490,152 @@ __global__ void k_close_seed(Ctrl* c) {
   }
 }
 
-// One level of a search or a colour sweep (work only; k_close ends it).  Top-down levels:
-// edge-balanced expansion of the current frontier (the "Manhattan collapse" of
-// kokkos_scc/fwbw_manhattan_global.cpp:162-289, with an LDS max-scan instead of a per-edge
-// binary search).  Bottom-up levels (reach ops only): every set bit of `cand` is a
-// candidate; the first parent that is allowed and no longer claimable claims it
+// LDS of one top-down chunk.
+template <int OP, bool MIRROR>
+struct ChunkLds {
+  static constexpr bool kColor = OP == OP_PUSH || OP == OP_CBW;
+  uint32_t own[kChunk];
+  uint32_t base[kChunk + 1];
+  int32_t col[kColor ? kChunk + 1 : 1];
+  uint32_t split[MIRROR ? kChunk + 1 : 1];
+  uint32_t base2[MIRROR ? kChunk + 1 : 1];
+  uint32_t scan[kBlock / 64];
+};
+
+// Expands chunk `ch` (kChunk edges of the frontier's edge space) with the whole
+// workgroup: the slots overlapping the chunk scatter their start positions into LDS, a
+// block max-scan turns them into a per-edge owner table (the "Manhattan collapse" of
+// kokkos_scc/fwbw_manhattan_global.cpp:162-289 without its per-edge binary search), the
+// adjacency is read coalesced, and claims / raises are staged for frontier N.  Ends with
+// a workgroup barrier.  Every queued vertex has a non-empty list, so at most len + 1
+// slots overlap a chunk.
+template <int OP, bool MIRROR>
+__device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, const Frontier& N, uint32_t* inq_cur,
+                                         uint32_t* inq_nxt, uint32_t ch, uint32_t nchunks, uint32_t q_size,
+                                         uint32_t q_edges, Stage& S, ChunkLds<OP, MIRROR>& L,
+                                         unsigned long long& my_raised) {
+  constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
+  constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
+  const int t = threadIdx.x;
+  const uint32_t e0 = ch * kChunk;
+  const uint32_t len = min(kChunk, q_edges - e0);
+  const uint32_t k0 = F.starts[ch];
+  const uint32_t kend = (ch + 1 < nchunks) ? F.starts[ch + 1] : q_size - 1u;
+  // the clamp only keeps LDS writes in bounds
+  const uint32_t nk = min(kend - k0 + 1u, kChunk + 1u);
+  for (uint32_t i = t; i < len; i += kBlock) L.own[i] = 0u;
+  __syncthreads();
+  for (uint32_t j = t; j < nk; j += kBlock) {
+    uint32_t k = k0 + j;
+    uint32_t qo = F.qo[k];
+    uint32_t v = (uint32_t)F.q[k];
+    uint32_t s = qo > e0 ? qo - e0 : 0u;
+    if (s < len) {
+      atomicMax(&L.own[s], j);
+      if (OP == OP_PUSH) bit_clear(inq_cur, v);
+    }
+    uint32_t b1 = a.down_off[v];
+    L.base[j] = b1 - qo;
+    if (MIRROR) {
+      uint32_t split = qo + (a.down_off[v + 1] - b1);
+      L.split[j] = split;
+      L.base2[j] = a.down_off2[v] - split;
+    }
+    if (COLOR) L.col[j] = a.colors[v];
+  }
+  __syncthreads();
+  // inclusive max-scan of own[0..len): thread t owns entries [t*kPer, t*kPer+kPer)
+  {
+    uint32_t vals[kPer];
+    uint32_t run = 0;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      uint32_t idx = (uint32_t)t * kPer + r;
+      uint32_t x = idx < len ? L.own[idx] : 0u;
+      run = max(run, x);
+      vals[r] = run;
+    }
+    uint32_t incl = wave_incl_max(run);
+    uint32_t excl = __shfl_up(incl, 1, 64);
+    if (lane() == 0) excl = 0u;
+    if (lane() == 63) L.scan[wave_id()] = incl;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (int w = 0; w < wave_id(); ++w) carry = max(carry, L.scan[w]);
+    excl = max(excl, carry);
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      uint32_t idx = (uint32_t)t * kPer + r;
+      if (idx < len) L.own[idx] = max(vals[r], excl);
+    }
+  }
+  __syncthreads();
+  // batched per thread so the loads of all kPer edges are in flight together:
+  // adjacency (coalesced), then one state probe per edge, then the claims
+  uint32_t w[kPer], word[kPer];
+  int32_t col[kPer], wcol[kPer];
+  bool ok[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    uint32_t i = (uint32_t)t + (uint32_t)r * kBlock;
+    ok[r] = i < len;
+    w[r] = 0;
+    col[r] = 0;
+    if (ok[r]) {
+      uint32_t j = L.own[i];
+      uint32_t e = e0 + i;
+      if (MIRROR)
+        w[r] = (uint32_t)(e < L.split[j] ? a.down_adj[L.base[j] + e] : a.down_adj2[L.base2[j] + e]);
+      else
+        w[r] = (uint32_t)a.down_adj[L.base[j] + e];
+      if (COLOR) col[r] = L.col[j];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    word[r] = 0;
+    wcol[r] = 0;
+    if (ok[r]) {
+      word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
+      if (COLOR) wcol[r] = a.colors[w[r]];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
+    bool got = false;  // claimed (reach, colour-BW) / raised (push) by this lane
+    if (REACH) {
+      got = in && bit_clear(a.cand, w[r]);
+    } else if (OP == OP_PUSH) {
+      if (in && wcol[r] < col[r] && atomicMax(&a.colors[w[r]], col[r]) < col[r]) {
+        ++my_raised;
+        got = true;
+      }
+    } else {  // OP_CBW
+      if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
+        a.labels[w[r]] = col[r];
+        got = true;
+      }
+    }
+    bool take = got;
+    if (OP == OP_PUSH && got) take = bit_set(inq_nxt, w[r]);
+    uint32_t deg = take ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
+    stage_push(S, take, w[r], deg);
+  }
+  stage_flush(S, N, a.ctrl, kStage - kChunk);  // also the end-of-chunk barrier
+}
+
+// One level of a search or a colour sweep (work only; a closing kernel ends it).
+// Top-down levels: edge-balanced expansion of the current frontier, one chunk per
+// workgroup at a time; workgroups without a chunk exit at once (the closer sums only the
+// rows of the ones that ran).  Bottom-up levels (reach ops only): every set bit of `cand`
+// is a candidate; the first parent that is allowed and no longer claimable claims it
 // (multistep/scc_fwbw.cpp:118-148).  part row = (examined, inspected, raised, 0).
 template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
-  constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
   __shared__ Stage S;
-  __shared__ uint32_t s_own[kChunk];
-  __shared__ uint32_t s_base[kChunk + 1];
-  __shared__ int32_t s_col[COLOR ? kChunk + 1 : 1];
-  __shared__ uint32_t s_split[MIRROR ? kChunk + 1 : 1];
-  __shared__ uint32_t s_base2[MIRROR ? kChunk + 1 : 1];
-  __shared__ uint32_t s_scan[kBlock / 64];
+  __shared__ ChunkLds<OP, MIRROR> L;
   __shared__ unsigned long long s_red[kBlock / 64];
   __shared__ BitIter B;
   __shared__ uint32_t s_claim[kBW];
@@ -570,7 +645,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   const uint32_t cur = c->cur;
   const uint32_t q_size = c->q_size, q_edges = c->q_edges;
   const bool bu = REACH && c->bottom_up != 0u;
-  const bool eager = COLOR && q_edges <= a.eager_edges;
+  const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
+  if (!bu && blockIdx.x >= nchunks) return;  // uniform: no chunk for this workgroup
   const Frontier F = cur ? a.fr1 : a.fr0;
   const Frontier N = cur ? a.fr0 : a.fr1;
   uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
@@ -584,135 +660,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   __syncthreads();
 
   if (!bu) {
-    const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-      const uint32_t e0 = ch * kChunk;
-      const uint32_t len = min(kChunk, q_edges - e0);
-      const uint32_t k0 = F.starts[ch];
-      const uint32_t kend = (ch + 1 < nchunks) ? F.starts[ch + 1] : q_size - 1u;
-      // every queued vertex has a non-empty list, so at most len + 1 slots overlap the
-      // chunk; the clamp only keeps LDS writes in bounds
-      const uint32_t nk = min(kend - k0 + 1u, kChunk + 1u);
-      for (uint32_t i = t; i < len; i += kBlock) s_own[i] = 0u;
-      __syncthreads();
-      for (uint32_t j = t; j < nk; j += kBlock) {
-        uint32_t k = k0 + j;
-        uint32_t qo = F.qo[k];
-        uint32_t v = (uint32_t)F.q[k];
-        uint32_t s = qo > e0 ? qo - e0 : 0u;
-        if (s < len) {
-          atomicMax(&s_own[s], j);
-          if (OP == OP_PUSH) bit_clear(inq_cur, v);
-        }
-        uint32_t b1 = a.down_off[v];
-        s_base[j] = b1 - qo;
-        if (MIRROR) {
-          uint32_t split = qo + (a.down_off[v + 1] - b1);
-          s_split[j] = split;
-          s_base2[j] = a.down_off2[v] - split;
-        }
-        if (COLOR) s_col[j] = a.colors[v];
-      }
-      __syncthreads();
-      // inclusive max-scan of s_own[0..len): thread t owns entries [t*kPer, t*kPer+kPer)
-      {
-        uint32_t vals[kPer];
-        uint32_t run = 0;
-#pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-          uint32_t idx = (uint32_t)t * kPer + r;
-          uint32_t x = idx < len ? s_own[idx] : 0u;
-          run = max(run, x);
-          vals[r] = run;
-        }
-        uint32_t incl = wave_incl_max(run);
-        uint32_t excl = __shfl_up(incl, 1, 64);
-        if (lane() == 0) excl = 0u;
-        if (lane() == 63) s_scan[wave_id()] = incl;
-        __syncthreads();
-        uint32_t carry = 0;
-        for (int w = 0; w < wave_id(); ++w) carry = max(carry, s_scan[w]);
-        excl = max(excl, carry);
-#pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-          uint32_t idx = (uint32_t)t * kPer + r;
-          if (idx < len) s_own[idx] = max(vals[r], excl);
-        }
-      }
-      __syncthreads();
-      // batched per thread so the loads of all kPer edges are in flight together:
-      // adjacency (coalesced), then one state probe per edge, then the claims
-      uint32_t w[kPer], word[kPer];
-      int32_t col[kPer], wcol[kPer];
-      bool ok[kPer], take[kPer];
-#pragma unroll
-      for (int r = 0; r < kPer; ++r) {
-        uint32_t i = (uint32_t)t + (uint32_t)r * kBlock;
-        ok[r] = i < len;
-        w[r] = 0;
-        col[r] = 0;
-        if (ok[r]) {
-          uint32_t j = s_own[i];
-          uint32_t e = e0 + i;
-          if (MIRROR)
-            w[r] = (uint32_t)(e < s_split[j] ? a.down_adj[s_base[j] + e] : a.down_adj2[s_base2[j] + e]);
-          else
-            w[r] = (uint32_t)a.down_adj[s_base[j] + e];
-          if (COLOR) col[r] = s_col[j];
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < kPer; ++r) {
-        word[r] = 0;
-        wcol[r] = 0;
-        if (ok[r]) {
-          word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
-          if (COLOR) wcol[r] = a.colors[w[r]];
-        }
-      }
-      bool got[kPer];  // raised (push) / claimed (reach, colour-BW) by this lane
-#pragma unroll
-      for (int r = 0; r < kPer; ++r) {
-        bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
-        got[r] = false;
-        if (REACH) {
-          got[r] = in && bit_clear(a.cand, w[r]);
-        } else if (OP == OP_PUSH) {
-          if (in && wcol[r] < col[r] && atomicMax(&a.colors[w[r]], col[r]) < col[r]) {
-            ++my_raised;
-            got[r] = true;
-          }
-        } else {  // OP_CBW
-          if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
-            a.labels[w[r]] = col[r];
-            got[r] = true;
-          }
-        }
-      }
-      if (!eager) {
-#pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-          take[r] = got[r];
-          if (OP == OP_PUSH && got[r]) take[r] = bit_set(inq_nxt, w[r]);
-          uint32_t deg = take[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
-          stage_push(S, take[r], w[r], deg);
-        }
-        stage_flush(S, N, c, kStage - kChunk);  // also the end-of-chunk barrier
-      } else {
-        for (int r = 0; r < kPer; ++r) {
-          uint32_t out[kEagerOut];
-          int nout = got[r] ? eager_walk<OP, MIRROR>(a, w[r], col[r], out, my_raised) : 0;
-          for (int q = 0; q < kEagerOut; ++q) {
-            bool tk = q < nout;
-            uint32_t x = tk ? out[q] : 0u;
-            if (OP == OP_PUSH && tk) tk = bit_set(inq_nxt, x);
-            uint32_t deg = tk ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, x) : 0u;
-            stage_push(S, tk, x, deg);
-          }
-          stage_flush(S, N, c, kStage - kBlock * kEagerOut);
-        }
-      }
-    }
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
+      td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
   } else if (REACH) {
     // A workgroup owns kBW words of `cand` per iteration (up to kR candidates per thread).
     // Each candidate's first up-list entry is checked with all loads of the thread's
@@ -811,64 +760,20 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   }
 }
 
-// Closes the level that just ran (one workgroup): sums the partials, applies the direction
-// rule -- the reference's (multistep/scc_fwbw.cpp:168-195, cc_bfs.cpp:167-194), the MLP, or
-// a forced schedule --, accounts algorithmic bytes, writes a level record and prepares the
-// next level.
-__global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int nparts) {
-  __shared__ unsigned long long s_red[kBlock / 64];
-  __shared__ float s_w[kMlpBlob];
-  __shared__ float s_feat[kMlpIn];
-  __shared__ float s_h[kMlpH];
-  __shared__ int s_act;
-  Ctrl* c = a.ctrl;
-  if (c->done) return;
-  unsigned long long ex = 0, in = 0, ra = 0;
-  for (int i = threadIdx.x; i < nparts; i += kBlock) {
-    ex += a.part[4 * i];
-    in += a.part[4 * i + 1];
-    ra += a.part[4 * i + 2];
-  }
-  ex = block_sum64(ex, s_red);
-  in = block_sum64(in, s_red);
-  ra = block_sum64(ra, s_red);
-  const bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
-  const bool learned = reach && a.policy == MSC_POLICY_LEARNED && a.mlp;
-  const unsigned long long p = c->next_packed;
-  const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
-  const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-  const bool bu = reach && c->bottom_up != 0u;
-  if (threadIdx.x == 0) {
-    c->found += next;
-    c->level += 1;
-    int act = bu ? 1 : 0;
-    if (reach) {
-      if (!c->switched_back && !bu) {
-        double edges_frontier = (double)next * a.avg;
-        double edges_remainder = (double)((int64_t)a.n_total - c->found) * a.avg;
-        if (edges_remainder / a.alpha < edges_frontier) act = 1;
-      } else if (!c->switched_back && (a.n_total / a.beta) > (int32_t)next) {
-        act = 0;
-        c->switched_back = 1;
-      }
-    }
-    s_act = act;
-    if (reach && (learned || a.recs)) level_features(c, a, next, nedges, bu, s_feat);
-  }
-  if (learned)
-    for (int i = threadIdx.x; i < kMlpBlob; i += blockDim.x) s_w[i] = a.mlp[i];
-  __syncthreads();
-  if (learned && threadIdx.x < 64) {
-    int act = mlp_action(s_w, s_feat, s_h);
-    if (threadIdx.x == 0) s_act = act;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  int act = s_act;
-  if (reach && a.sched && c->level <= (uint32_t)MSC_SCHED_LEVELS) {
-    int f = a.sched[c->phase * MSC_SCHED_LEVELS + c->level - 1];
-    if (f >= 0 && f <= 2) act = f;
-  }
+// Rows of the partials array written by the level that just ran: every workgroup of a
+// bottom-up level, only those that had a chunk in a top-down level.
+__device__ __forceinline__ int active_rows(int nparts, bool bu, uint32_t q_edges) {
+  if (bu) return nparts;
+  uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
+  return (int)min((uint32_t)nparts, nchunks);
+}
+
+// Thread 0 only: books the level that just closed (next frontier from next_packed,
+// algorithmic bytes, level record) and turns the control block to the next level with
+// direction `act` (0 top-down, 1 bottom-up, 2 abandon the search).
+__device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int act, uint32_t q_size,
+                             uint32_t q_edges, unsigned long long ex, unsigned long long in, unsigned long long ra,
+                             uint32_t next, uint32_t nedges, const float* feat) {
   // algorithmic bytes of this level (unit costs documented in DESIGN.md)
   unsigned long long bytes;
   if (op == OP_PUSH)
@@ -897,7 +802,7 @@ __global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int npart
     r.num_valid = a.num_valid;
     r.action = act;
     r.pad = 0;
-    for (int i = 0; i < kMlpIn; ++i) r.feat[i] = reach ? s_feat[i] : 0.0f;
+    for (int i = 0; i < kMlpIn; ++i) r.feat[i] = feat ? feat[i] : 0.0f;
     unsigned long long t0 = c->t_start, t1 = wall_clock64();
     r.level_us = (t0 != ~0ull && t1 > t0 && a.clock_mhz > 0.0f) ? (float)(t1 - t0) / a.clock_mhz : 0.0f;
     r.pad2 = 0;
@@ -911,6 +816,134 @@ __global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int npart
   c->next_packed = 0ull;
   c->done = (next == 0u || act == 2) ? 1u : 0u;
   if (act == 2) c->abandoned = 1u;  // the host discards this search's result
+}
+
+// Closes a search level (one workgroup): sums the partials, applies the direction rule --
+// the reference's (multistep/scc_fwbw.cpp:168-195, cc_bfs.cpp:167-194), the MLP, or a
+// forced schedule -- and prepares the next level.
+__global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int nparts) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  __shared__ float s_w[kMlpBlob];
+  __shared__ float s_feat[kMlpIn];
+  __shared__ float s_h[kMlpH];
+  __shared__ int s_act;
+  Ctrl* c = a.ctrl;
+  if (c->done) return;
+  const uint32_t q_size = c->q_size, q_edges = c->q_edges;
+  const bool bu = c->bottom_up != 0u;
+  const int rows = active_rows(nparts, bu, q_edges);
+  unsigned long long ex = 0, in = 0, ra = 0;
+  for (int i = threadIdx.x; i < rows; i += kBlock) {
+    ex += a.part[4 * i];
+    in += a.part[4 * i + 1];
+    ra += a.part[4 * i + 2];
+  }
+  ex = block_sum64(ex, s_red);
+  in = block_sum64(in, s_red);
+  ra = block_sum64(ra, s_red);
+  const bool learned = a.policy == MSC_POLICY_LEARNED && a.mlp;
+  const unsigned long long p = c->next_packed;
+  const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
+  if (threadIdx.x == 0) {
+    c->found += next;
+    c->level += 1;
+    c->closes += 1;
+    int act = bu ? 1 : 0;
+    if (!c->switched_back && !bu) {
+      double edges_frontier = (double)next * a.avg;
+      double edges_remainder = (double)((int64_t)a.n_total - c->found) * a.avg;
+      if (edges_remainder / a.alpha < edges_frontier) act = 1;
+    } else if (!c->switched_back && (a.n_total / a.beta) > (int32_t)next) {
+      act = 0;
+      c->switched_back = 1;
+    }
+    s_act = act;
+    level_features(c, a, next, nedges, bu, s_feat);
+  }
+  if (learned)
+    for (int i = threadIdx.x; i < kMlpBlob; i += blockDim.x) s_w[i] = a.mlp[i];
+  __syncthreads();
+  if (learned && threadIdx.x < 64) {
+    int act = mlp_action(s_w, s_feat, s_h);
+    if (threadIdx.x == 0) s_act = act;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int act = s_act;
+  if (a.sched && c->level <= (uint32_t)MSC_SCHED_LEVELS) {
+    int f = a.sched[c->phase * MSC_SCHED_LEVELS + c->level - 1];
+    if (f >= 0 && f <= 2) act = f;
+  }
+  finish_level(c, a, op, bu, act, q_size, q_edges, ex, in, ra, next, nedges, s_feat);
+}
+
+// Closes a colour sweep (push or per-colour backward level), then keeps going in this one
+// workgroup while the frontier has at most a.small_edges edges: each such level is the
+// same top-down chunk expansion as in k_level.  All of a level's claims, raises and queue
+// writes come from this workgroup (one CU, one L1), so a workgroup barrier is all the next
+// level needs to see them; no agent-scope fence (an L2 write-back on multi-XCD gfx950) is
+// taken.  Small frontiers thus cost no launch and no grid-wide dispatch; the loop hands
+// back to the grid-wide kernel when the frontier grows past the threshold, and stops
+// after kSmallLevels levels so a launch stays bounded.
+constexpr int kSmallLevels = 4096;
+
+template <int OP, bool MIRROR>
+__global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts) {
+  __shared__ Stage S;
+  __shared__ ChunkLds<OP, MIRROR> L;
+  __shared__ unsigned long long s_red[kBlock / 64];
+  __shared__ uint32_t s_ctl[4];  // done, cur, q_size, q_edges
+  Ctrl* c = a.ctrl;
+  if (c->done) return;
+  const int t = threadIdx.x;
+  {
+    const uint32_t q_size = c->q_size, q_edges = c->q_edges;
+    const int rows = active_rows(nparts, false, q_edges);
+    unsigned long long ra = 0;
+    for (int i = t; i < rows; i += kBlock) ra += a.part[4 * i + 2];
+    ra = block_sum64(ra, s_red);
+    if (t == 0) {
+      const unsigned long long p = c->next_packed;
+      const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
+      c->found += next;
+      c->level += 1;
+      c->closes += 1;
+      finish_level(c, a, OP, false, 0, q_size, q_edges, 0ull, 0ull, ra, next, nedges, nullptr);
+      s_ctl[0] = c->done;
+      s_ctl[1] = c->cur;
+      s_ctl[2] = c->q_size;
+      s_ctl[3] = c->q_edges;
+      S.cnt = 0u;
+    }
+    __syncthreads();
+  }
+  for (int it = 0; it < kSmallLevels; ++it) {
+    const uint32_t done = s_ctl[0], cur = s_ctl[1], q_size = s_ctl[2], q_edges = s_ctl[3];
+    if (done || q_edges > a.small_edges) break;  // uniform
+    if (t == 0 && a.recs) c->t_start = wall_clock64();
+    const Frontier F = cur ? a.fr1 : a.fr0;
+    const Frontier N = cur ? a.fr0 : a.fr1;
+    uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
+    uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
+    const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
+    unsigned long long raised = 0;
+    for (uint32_t ch = 0; ch < nchunks; ++ch)
+      td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, raised);
+    stage_flush(S, N, c, 0u);
+    raised = block_sum64(raised, s_red);  // ends with a workgroup barrier
+    if (t == 0) {
+      const unsigned long long p = c->next_packed;
+      const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
+      c->found += next;
+      c->level += 1;
+      finish_level(c, a, OP, false, 0, q_size, q_edges, 0ull, 0ull, raised, next, nedges, nullptr);
+      s_ctl[0] = c->done;
+      s_ctl[1] = c->cur;
+      s_ctl[2] = c->q_size;
+      s_ctl[3] = c->q_edges;
+    }
+    __syncthreads();
+  }
 }
 
 // Colour initialisation (multistep/scc_color.cpp:56-63, cc_color.cpp:62-69): colors[v] = v
@@ -1188,7 +1221,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32
       atomicAnd(&a.cand[(uint32_t)p >> 5], ~(1u << ((uint32_t)p & 31u)));
       deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, (uint32_t)p);
     }
-    stage_push(S, p >= 0, (uint32_t)p, deg);
+    stage_push(S, deg > 0u, (uint32_t)p, deg);  // a root without a list reaches nothing more
     stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
@@ -1309,8 +1342,16 @@ void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   }
 }
 
-void launch_close(const Launch& L, int op, const LevelArgs& a) {
-  hipLaunchKernelGGL(k_close, dim3(1), dim3(kBlock), 0, L.stream, a, op, L.grid);
+void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
+  dim3 g(1), b(kBlock);
+  switch (op) {
+    case OP_PUSH:
+      if (mirror) hipLaunchKernelGGL((k_close_color<OP_PUSH, true>), g, b, 0, L.stream, a, L.grid);
+      else hipLaunchKernelGGL((k_close_color<OP_PUSH, false>), g, b, 0, L.stream, a, L.grid);
+      break;
+    case OP_CBW: hipLaunchKernelGGL((k_close_color<OP_CBW, false>), g, b, 0, L.stream, a, L.grid); break;
+    default: hipLaunchKernelGGL(k_close, g, b, 0, L.stream, a, op, L.grid); break;
+  }
 }
 
 int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,

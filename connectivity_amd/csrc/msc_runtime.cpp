@@ -36,7 +36,7 @@ struct msc_ctx {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{4, 1 << 16, 1 << 16, 4, 32, 0};
+  msc_tuning tune{4, 4096, 4096, 2, 32, 0};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -207,8 +207,8 @@ int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
 int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls, msc_stats* s, int first_batch) {
   Launch L = launch_of(c);
   LevelArgs a = a_in;
-  a.eager_edges = op == OP_PUSH ? (uint32_t)c->tune.eager_push_edges
-                  : op == OP_CBW ? (uint32_t)c->tune.eager_cbw_edges
+  a.small_edges = op == OP_PUSH ? (uint32_t)c->tune.small_push_edges
+                  : op == OP_CBW ? (uint32_t)c->tune.small_cbw_edges
                                  : 0u;
   int batch = first_batch > 0 ? first_batch : c->tune.first_batch;
   const int max_batch = c->tune.max_batch;
@@ -222,7 +222,7 @@ int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls,
         Timed t(c, kcls);
         launch_level(L, op, mirror, a);
       }
-      launch_close(L, op, a);
+      launch_close(L, op, mirror, a);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(c->pinned_u32, &c->ctrl->done, 4, hipMemcpyDeviceToHost, c->stream));
@@ -601,9 +601,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       }
       if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s, push_batch))) return rc;
       if ((rc = read_ctrl(c))) return rc;
-      uint32_t sweeps = c->ctrl_host->level;
+      uint32_t sweeps = c->ctrl_host->level, closes = c->ctrl_host->closes;
       if (s) s->color_iters += (int32_t)sweeps;
-      push_batch = sweeps > 2 ? (sweeps < (uint32_t)c->tune.max_batch ? (int)sweeps : c->tune.max_batch) : 2;
+      push_batch = closes > 2 ? (closes < (uint32_t)c->tune.max_batch ? (int)closes : c->tune.max_batch) : 2;
       LevelArgs r = a;
       r.down_off = g.in_off; r.down_adj = g.in_adj;
       {
@@ -612,9 +612,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       }
       if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch))) return rc;
       if ((rc = read_ctrl(c))) return rc;
-      uint32_t cbw = c->ctrl_host->level;
+      uint32_t cbw = c->ctrl_host->level, cbw_closes = c->ctrl_host->closes;
       if (s) s->cbw_levels += (int32_t)cbw;
-      cbw_batch = cbw > 2 ? (cbw < (uint32_t)c->tune.max_batch ? (int)cbw : c->tune.max_batch) : 2;
+      cbw_batch = cbw_closes > 2 ? (cbw_closes < (uint32_t)c->tune.max_batch ? (int)cbw_closes : c->tune.max_batch)
+                                 : 2;
       {
         Timed t(c, MSC_K_ROOTS);
         np = launch_count_bits(L, c->valid, nw, c->part);
@@ -809,7 +810,7 @@ int msc_ctx_get_tuning(msc_ctx* c, msc_tuning* out) {
 
 int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (!c || !t) return MSC_E_INVALID;
-  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->eager_push_edges < 0 || t->eager_cbw_edges < 0 ||
+  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);

@@ -35,7 +35,7 @@ from . import (ACT_STOP, DeviceGraph, DeviceLabels, LABELS_CANONICAL, MLP_BLOB, 
                MLP_PARAMS, POLICY_LEARNED, POLICY_PLAIN, POLICY_WEIGHTS, SCHED_LEVELS, Context, gen_grid,
                gen_planted, gen_rmat, mlp_train, run_cc, run_scc, schedule_array, set_policy)
 
-FEATURES = ["log1p(next)", "log1p(next_edges)", "log1p(unvisited)", "log1p(level)", "next/n", "next_edges/(avg*n)",
+FEATURES = ["log(next/n)", "log(next_edges/(avg*n))", "log(unvisited/n)", "log1p(level)", "next/n", "next_edges/(avg*n)",
             "log1p(next_edges/next)", "log1p(avg_degree)", "bottom_up", "switched_back", "found/num_valid",
             "phase"]
 PHASES = {0: "fw", 1: "bw", 2: "cc"}

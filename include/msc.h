@@ -191,8 +191,9 @@ int msc_mlp_train(msc_ctx* ctx, float* blob, const float* x, const int32_t* y, c
  * autotuner, connectivity_amd/autotune.py).  Defaults are the values the library ships. */
 typedef struct msc_tuning {
   int32_t blocks_per_cu;    /* level-kernel workgroups per compute unit (1..8) */
-  int32_t eager_push_edges; /* colour push sweeps over fewer frontier edges walk eagerly */
-  int32_t eager_cbw_edges;  /* same for the per-colour backward search (0: never) */
+  int32_t small_push_edges; /* colour push sweeps over at most this many frontier edges run
+                               inside the one-workgroup closing kernel, several per launch */
+  int32_t small_cbw_edges;  /* same for the per-colour backward search (0: only empty ones) */
   int32_t first_batch;      /* levels enqueued before the first host check of a loop */
   int32_t max_batch;        /* levels enqueued between later host checks */
   int32_t pad;

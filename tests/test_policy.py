@@ -163,3 +163,31 @@ def test_shipped_weights_keep_labels(msc, orc, gpu_ctx):
         want = orc.canonical(orc.scc(g, vert_cutoff=0))
         assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED), want)
         assert np.array_equal(msc.run_cc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED), orc.canonical(orc.cc(g)))
+
+
+@pytest.mark.gpu
+def test_recorded_features_follow_their_definition(msc, gpu_ctx):
+    """Features the level closer computes, recomputed on the host from the same record.
+    Counts enter as log fractions of the graph, so they do not grow with the scale."""
+    for g in (msc.gen_rmat(14, 16, seed=5), msc.gen_rmat(17, 16, seed=5)):
+        msc.run_scc(g, g.max_deg_vert, g.avg_degree, ctx=gpu_ctx, record=True)
+        recs = [r for r in gpu_ctx.levels() if r.phase in (0, 1)]
+        assert recs
+        n, m = np.float32(g.n), np.float32(g.avg_degree) * np.float32(g.n)
+        for r in recs:
+            f = np.array(r.feat, np.float32)
+            want = {
+                0: np.log(max(r.next_size, 1) / n),
+                1: np.log(max(r.next_edges, 1) / m),
+                2: np.log(max(r.num_valid - r.found, 1) / n),
+                3: np.log1p(r.level),
+                4: r.next_size / n,
+                5: r.next_edges / m,
+                6: np.log1p(r.next_edges / r.next_size) if r.next_size else 0.0,
+                7: np.log1p(g.avg_degree),
+                8: float(r.bottom_up),
+                10: r.found / r.num_valid,
+                11: float(r.phase),
+            }
+            for i, w in want.items():
+                assert abs(f[i] - w) <= 1e-4 * max(1.0, abs(w)), (r.phase, r.level, i, f[i], w)
