@@ -61,6 +61,11 @@ struct Ctrl {
   unsigned long long t_start;      // earliest workgroup start of the running launch
   uint32_t closes;                 // closing-kernel launches that closed a level
   uint32_t pad_;
+  // counts the host reads with the next control-block copy instead of a round trip each
+  unsigned long long red_trim[4];    // col 0: trim survivors
+  unsigned long long red_commit[4];  // col 0: vertices committed by FW-BW / the CC search
+  unsigned long long red_left[4];    // col 0: valid vertices after the last colour round
+  unsigned long long red_min[4];     // col 3: 0xffffffff - smallest member of the committed set
 };
 
 struct LevelArgs {
@@ -86,7 +91,7 @@ struct LevelArgs {
   uint32_t rec_cap;
   int32_t n_total;
   int64_t nwords;             // bitmap words
-  int32_t num_valid;          // valid vertices when the search started
+  const unsigned long long* nvalid;  // device: valid vertices when the search started (null: 0)
   int32_t beta;
   int32_t policy;
   double avg;
@@ -95,6 +100,7 @@ struct LevelArgs {
   const int8_t* sched;        // forced actions, 3 x MSC_SCHED_LEVELS (null: none)
   float clock_mhz;            // wall_clock64() rate, for per-level durations
   uint32_t small_edges;       // colour levels over at most this many edges run inside the closer
+  int32_t* cmin;              // canonical labels in flight: min member per colour root (null: off)
 };
 
 struct Launch {
@@ -116,9 +122,15 @@ void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, 
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);  // work only
 void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  // one workgroup
-// labels[v] = label for every set bit of member; part col 0 = count
+// labels[v] = label for every set bit of member; part col 0 = count.  With min_red, the
+// label is instead the smallest member found by launch_min_bit (0xffffffff - min_red[3]).
 int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
-                      int32_t label, unsigned long long* part);
+                      int32_t label, unsigned long long* part, const unsigned long long* min_red = nullptr);
+// part col 3 = 0xffffffff - smallest set bit in each workgroup's words (0: none)
+int launch_min_bit(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
+// labels[v] = cmin[labels[v]] for every set bit v of `coloured`
+void launch_canon_colour(const Launch& L, const uint32_t* coloured, int64_t nwords, int32_t n, int32_t* labels,
+                         const int32_t* cmin);
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
 void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a);
 void launch_roots(const Launch& L, const LevelArgs& a);

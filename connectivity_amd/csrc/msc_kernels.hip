@@ -252,7 +252,8 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
 __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges, bool bu,
                                float* f) {
   float n = a.n_total > 0 ? (float)a.n_total : 1.0f;
-  float nv = a.num_valid > 0 ? (float)a.num_valid : 1.0f;
+  const unsigned long long nvl = a.nvalid ? *a.nvalid : 0ull;
+  float nv = nvl > 0ull ? (float)nvl : 1.0f;
   float m = fmaxf((float)a.avg * n, 1.0f);
   float found = (float)c->found;
   f[0] = logf(fmaxf((float)next, 1.0f) / n);
@@ -395,10 +396,13 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
 }
 
 // labels[v] = label for each set bit v of `member`; part col 0 = members of the workgroup.
+// min_red (optional) replaces the label by the smallest member (canonical labels).
 __global__ void __launch_bounds__(kBlock) k_label_bits(const uint32_t* __restrict__ member, int64_t nwords,
                                                        int32_t n, int32_t* __restrict__ labels, int32_t label,
-                                                       unsigned long long* __restrict__ part) {
+                                                       unsigned long long* __restrict__ part,
+                                                       const unsigned long long* __restrict__ min_red) {
   __shared__ unsigned long long s_red[kBlock / 64];
+  if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
   unsigned long long cnt = 0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
     uint32_t w = member[i];
@@ -414,6 +418,45 @@ __global__ void __launch_bounds__(kBlock) k_label_bits(const uint32_t* __restric
   if (threadIdx.x == 0) {
     part[4 * blockIdx.x] = cnt;
     part[4 * blockIdx.x + 1] = part[4 * blockIdx.x + 2] = part[4 * blockIdx.x + 3] = 0;
+  }
+}
+
+// part col 3 = 0xffffffff - smallest set bit of the workgroup's words (0 when none), so
+// that k_reduce's max yields the smallest set bit overall.
+__global__ void __launch_bounds__(kBlock) k_min_bit(const uint32_t* __restrict__ bits, int64_t nwords,
+                                                    unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long s_red[kBlock / 64];
+  unsigned long long best = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
+    uint32_t w = bits[i];
+    if (w) {
+      unsigned long long key = 0xffffffffull - (unsigned long long)(i * 32 + (__ffs(w) - 1));
+      best = key > best ? key : best;
+    }
+  }
+  best = wave_max64(best);
+  if (lane() == 0) s_red[wave_id()] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kBlock / 64; ++i) best = s_red[i] > best ? s_red[i] : best;
+    unsigned long long* p = part + 4 * blockIdx.x;
+    p[0] = p[1] = p[2] = 0;
+    p[3] = best;
+  }
+}
+
+// Canonical labels of the colouring phase: every vertex coloured gets the smallest member
+// of its colour root's SCC, tracked in cmin by the per-colour backward search.
+__global__ void __launch_bounds__(kBlock) k_canon_colour(const uint32_t* __restrict__ coloured, int64_t nwords,
+                                                         int32_t n, int32_t* __restrict__ labels,
+                                                         const int32_t* __restrict__ cmin) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
+    uint32_t w = coloured[i];
+    while (w) {
+      int64_t v = i * 32 + (__ffs(w) - 1);
+      w &= w - 1u;
+      if (v < n) labels[v] = cmin[labels[v]];
+    }
   }
 }
 
@@ -615,6 +658,7 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
       if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
         a.labels[w[r]] = col[r];
         got = true;
+        if (a.cmin && (int32_t)w[r] < a.cmin[col[r]]) atomicMin(&a.cmin[col[r]], (int32_t)w[r]);
       }
     }
     bool take = got;
@@ -799,7 +843,7 @@ __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int a
     r.inspected = in;
     r.found = c->found;
     r.n_total = a.n_total;
-    r.num_valid = a.num_valid;
+    r.num_valid = a.nvalid ? (int32_t)*a.nvalid : 0;
     r.action = act;
     r.pad = 0;
     for (int i = 0; i < kMlpIn; ++i) r.feat[i] = feat ? feat[i] : 0.0f;
@@ -991,6 +1035,7 @@ __global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
         if (a.colors[v] == (int32_t)v) {
           take = true;
           a.labels[v] = (int32_t)v;
+          if (a.cmin) a.cmin[v] = (int32_t)v;
           bit_clear(a.valid, v);
           deg = a.down_off[v + 1] - a.down_off[v];
         }
@@ -1355,10 +1400,24 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
 }
 
 int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
-                      int32_t label, unsigned long long* part) {
+                      int32_t label, unsigned long long* part, const unsigned long long* min_red) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_label_bits, dim3(g), dim3(kBlock), 0, L.stream, member, nwords, n, labels, label, part);
+  hipLaunchKernelGGL(k_label_bits, dim3(g), dim3(kBlock), 0, L.stream, member, nwords, n, labels, label, part,
+                     min_red);
   return g;
+}
+
+int launch_min_bit(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {
+  int g = grid_for(nwords, kBlock * 4, L.grid * 2);
+  hipLaunchKernelGGL(k_min_bit, dim3(g), dim3(kBlock), 0, L.stream, bits, nwords, part);
+  return g;
+}
+
+void launch_canon_colour(const Launch& L, const uint32_t* coloured, int64_t nwords, int32_t n, int32_t* labels,
+                         const int32_t* cmin) {
+  if (nwords <= 0) return;
+  hipLaunchKernelGGL(k_canon_colour, dim3(grid_for(nwords, kBlock * 4, L.grid * 2)), dim3(kBlock), 0, L.stream,
+                     coloured, nwords, n, labels, cmin);
 }
 
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {

@@ -43,7 +43,7 @@ struct msc_ctx {
   int64_t cap_n = -1, cap_e = -1;
   // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
   uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
-  int32_t *labels = nullptr, *colors = nullptr;
+  int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
   Frontier fr0{}, fr1{};
   unsigned long long* part = nullptr;  // kMaxParts x 4 per-workgroup partials
   uint32_t* err = nullptr;             // validation flag
@@ -106,7 +106,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   size_t nchunks = (size_t)((ee + kChunk - 1) / kChunk) + 2;
   size_t sizes[] = {
       nw * 4, nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,           // bitmaps
-      (size_t)nn * 4, (size_t)nn * 4,                           // labels, colors
+      (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // labels, colors, cmin
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr0
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr1
       (size_t)kMaxParts * 4 * 8,                                // partials
@@ -134,12 +134,13 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->inq1 = (uint32_t*)parts[5];
   c->labels = (int32_t*)parts[6];
   c->colors = (int32_t*)parts[7];
-  c->fr0 = Frontier{(int32_t*)parts[8], (uint32_t*)parts[9], (uint32_t*)parts[10]};
-  c->fr1 = Frontier{(int32_t*)parts[11], (uint32_t*)parts[12], (uint32_t*)parts[13]};
-  c->part = (unsigned long long*)parts[14];
-  c->err = (uint32_t*)parts[15];
-  c->ctrl = (Ctrl*)parts[16];
-  c->recs = (msc_level_rec*)parts[17];
+  c->cmin = (int32_t*)parts[8];
+  c->fr0 = Frontier{(int32_t*)parts[9], (uint32_t*)parts[10], (uint32_t*)parts[11]};
+  c->fr1 = Frontier{(int32_t*)parts[12], (uint32_t*)parts[13], (uint32_t*)parts[14]};
+  c->part = (unsigned long long*)parts[15];
+  c->err = (uint32_t*)parts[16];
+  c->ctrl = (Ctrl*)parts[17];
+  c->recs = (msc_level_rec*)parts[18];
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -201,10 +202,12 @@ int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
   return MSC_OK;
 }
 
-// Runs level launches (work + closing kernel) until the control block says done.  Every
+// Runs level launches (work + closing kernel) until the control block says done; the host
+// copy of the control block (ctrl_host) is current when it returns.  Every
 // search and every max-label fixpoint settles within n + 1 levels, so more launches than
 // that mean a fault.
-int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls, msc_stats* s, int first_batch) {
+int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls, msc_stats* s, int first_batch,
+               bool count_left = false) {
   Launch L = launch_of(c);
   LevelArgs a = a_in;
   a.small_edges = op == OP_PUSH ? (uint32_t)c->tune.small_push_edges
@@ -224,11 +227,16 @@ int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls,
       }
       launch_close(L, op, mirror, a);
     }
+    if (count_left) {
+      // valid vertices left: read back with this copy, so a colour round needs no extra trip
+      int np = launch_count_bits(L, c->valid, a.nwords, c->part);
+      launch_reduce(L, c->part, np, c->ctrl->red_left);
+    }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c->pinned_u32, &c->ctrl->done, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (s) s->host_syncs += 1;
-    if (*c->pinned_u32) break;
+    if (c->ctrl_host->done) break;
     batch = batch * 2 < max_batch ? batch * 2 : max_batch;
   }
   return MSC_OK;
@@ -500,26 +508,26 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     s->avg_degree = avg;
   }
 
-  // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97)
-  unsigned long long red[4];
+  // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97); the survivor count stays on
+  // device (the searches read it there) and reaches the host with a later copy
   int np;
   {
     Timed t(c, MSC_K_TRIM);
     np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, labels, c->part);
+    launch_reduce(L, c->part, np, c->ctrl->red_trim);
   }
-  if ((rc = reduce_read(c, np, red))) return rc;
-  uint32_t nv = (uint32_t)red[0];
-  if (s) {
-    s->num_trim = n - (int32_t)nv;
-    s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n;
-  }
+  if (s) s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n;
 
   LevelArgs a = base_args(c, &g, labels, o, avg);
-  a.num_valid = (int32_t)nv;
+  a.nvalid = &c->ctrl->red_trim[0];
+  // canonical SCC labels of one graph are written as the phases go: the pivot's SCC takes
+  // its smallest member, colour SCCs track theirs in cmin; the batch and WCC paths
+  // relabel at the end instead
+  const bool fast_canon = !cc && !bt && o->label_mode == MSC_LABELS_CANONICAL;
 
   // FW-BW (SCC) or the CC search from the pivot; a search the policy abandons leaves
   // every vertex to colouring
-  if (pivot >= 0 && nv > 0) {
+  if (pivot >= 0) {
     const uint32_t* member = nullptr;
     launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
     if (!cc) {
@@ -531,7 +539,6 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
       else launch_search_init(L, fw, pivot, 0);
       if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
-      if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
         // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy), top-down on in-lists,
@@ -544,7 +551,6 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
         else launch_search_init(L, bw, pivot, 1);
         if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 0))) return rc;
-        if ((rc = read_ctrl(c))) return rc;
         if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
         if (!c->ctrl_host->abandoned) {
           launch_andnot(L, c->b1, nullptr, c->b2, c->b3, nw);  // SCC = fw set & visited by BW
@@ -561,7 +567,6 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
       else launch_search_init(L, bf, pivot, 2);
       if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, 0))) return rc;
-      if ((rc = read_ctrl(c))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
       if (!c->ctrl_host->abandoned) {
         launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);  // component = valid & visited
@@ -572,68 +577,69 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       // commit (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271)
       {
         Timed t(c, MSC_K_COMMIT);
+        if (fast_canon) launch_reduce(L, c->part, launch_min_bit(L, member, nw, c->part), c->ctrl->red_min);
         np = bt ? launch_label_bits_batch(L, member, nw, n, labels, c->bpiv, c->bbase, ng, c->part)
-                : launch_label_bits(L, member, nw, n, labels, pivot, c->part);
+                : launch_label_bits(L, member, nw, n, labels, pivot, c->part, fast_canon ? c->ctrl->red_min : nullptr);
         launch_andnot(L, c->valid, nullptr, c->valid, member, nw);
+        launch_reduce(L, c->part, np, c->ctrl->red_commit);
       }
-      if ((rc = reduce_read(c, np, red))) return rc;
-      uint32_t found = (uint32_t)red[0];
-      if (s) {
-        s->num_fwbw = (int32_t)found;
-        s->k[MSC_K_COMMIT].bytes += 8.0 * nw + 4.0 * found;
-      }
-      nv -= found;
     }
   }
 
   // colouring
   if (!cc) {
-    // rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677)
-    // the previous round's sweep counts predict this round's: enqueue that many first
+    // rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677); each round's count
+    // of remaining vertices comes back with the control block after its last colour-BW
+    // batch.  The previous round's closer launches predict this round's: enqueue that many.
     int push_batch = c->tune.first_batch, cbw_batch = c->tune.first_batch;
-    while (nv > 0) {
-      a.num_valid = (int32_t)nv;
+    uint64_t before = 0;
+    if (fast_canon) {
+      launch_andnot(L, c->b2, nullptr, c->valid, nullptr, nw);  // the vertices colouring will label
+      a.cmin = c->cmin;
+    }
+    for (int round = 0;; ++round) {
       LevelArgs p = a;
+      p.nvalid = nullptr;
       p.down_off = g.out_off; p.down_adj = g.out_adj;
       {
         Timed t(c, MSC_K_OTHER);
         launch_seed_colors(L, false, p);
       }
       if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s, push_batch))) return rc;
-      if ((rc = read_ctrl(c))) return rc;
       uint32_t sweeps = c->ctrl_host->level, closes = c->ctrl_host->closes;
       if (s) s->color_iters += (int32_t)sweeps;
       push_batch = closes > 2 ? (closes < (uint32_t)c->tune.max_batch ? (int)closes : c->tune.max_batch) : 2;
       LevelArgs r = a;
+      r.nvalid = nullptr;
       r.down_off = g.in_off; r.down_adj = g.in_adj;
       {
         Timed t(c, MSC_K_ROOTS);
         launch_roots(L, r);
       }
-      if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch))) return rc;
-      if ((rc = read_ctrl(c))) return rc;
-      uint32_t cbw = c->ctrl_host->level, cbw_closes = c->ctrl_host->closes;
+      if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch, true))) return rc;
+      const Ctrl* h = c->ctrl_host;
+      uint32_t cbw = h->level, cbw_closes = h->closes;
       if (s) s->cbw_levels += (int32_t)cbw;
       cbw_batch = cbw_closes > 2 ? (cbw_closes < (uint32_t)c->tune.max_batch ? (int)cbw_closes : c->tune.max_batch)
                                  : 2;
-      {
-        Timed t(c, MSC_K_ROOTS);
-        np = launch_count_bits(L, c->valid, nw, c->part);
-      }
-      if ((rc = reduce_read(c, np, red))) return rc;
-      uint32_t left = (uint32_t)red[0];
+      if (round == 0) before = h->red_trim[0] - h->red_commit[0];
+      const uint64_t left = h->red_left[0];
       if (s) {
         s->color_rounds += 1;
-        s->k[MSC_K_ROOTS].bytes += 2.0 * 4.0 * nw + 4.0 * (nv - left);
-        s->host_syncs += 1;
+        s->k[MSC_K_ROOTS].bytes += 2.0 * 4.0 * nw + 4.0 * (double)(before - left);
       }
-      if (left >= nv) return MSC_E_DEVICE;  // every round removes at least one SCC
-      nv = left;
+      if (left == 0) break;
+      if (left >= before) return MSC_E_DEVICE;  // every round removes at least one SCC
+      before = left;
     }
-  } else if (nv > 0) {
+    if (fast_canon) {
+      Timed t(c, MSC_K_OTHER);
+      launch_canon_colour(L, c->b2, nw, n, labels, c->cmin);
+    }
+  } else {
     // one max-label fixpoint written straight into the labels (multistep/cc_color.cpp:47-175)
-    a.num_valid = (int32_t)nv;
     LevelArgs p = a;
+    p.nvalid = nullptr;
     p.colors = labels;
     p.down_off = g.out_off; p.down_adj = g.out_adj;
     p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
@@ -642,14 +648,13 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       launch_seed_colors(L, true, p);
     }
     if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s, 0))) return rc;
-    if ((rc = read_ctrl(c))) return rc;
     if (s) {
       s->color_iters = (int32_t)c->ctrl_host->level;
       s->color_rounds = 1;
     }
   }
 
-  if (o->label_mode == MSC_LABELS_CANONICAL) {
+  if (o->label_mode == MSC_LABELS_CANONICAL && !fast_canon) {
     Timed t(c, MSC_K_OTHER);
     launch_canon(L, n, labels, c->colors);
   }
@@ -665,6 +670,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   auto t1 = std::chrono::steady_clock::now();
   if (s) {
     s->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    s->num_trim = n - (int32_t)c->ctrl_host->red_trim[0];
+    s->num_fwbw = (int32_t)c->ctrl_host->red_commit[0];
+    if (c->ctrl_host->red_commit[0]) s->k[MSC_K_COMMIT].bytes += 8.0 * nw + 4.0 * (double)c->ctrl_host->red_commit[0];
     s->bu_levels = (int32_t)c->ctrl_host->bu_levels;
     for (int op = 0; op < 5; ++op) {
       int cls = op == OP_PUSH ? MSC_K_COLOR_PROP : op == OP_CBW ? MSC_K_COLOR_BW : MSC_K_LEVEL;
