@@ -1,21 +1,27 @@
 """Benchmark: Multistep SCC on a directed R-MAT scale-24 graph on one MI355X.
 
-One step = one msc_scc_run over the whole graph (trim -> FW-BW -> colouring to empty), with
-the CSR / transposed CSR already resident in HBM and the labels left in HBM.  `value` is
-GTEPS = m / t_step (m = directed edges after dedupe) with the learned direction policy when
-its weights are shipped (connectivity_amd/policy_weights.npz), else the plain rule; both are
+One step = one msc_scc_run over the whole graph (trim -> FW-BW -> colouring to empty,
+canonical labels), with the CSR / transposed CSR already resident in HBM and the labels
+left in HBM.  `value` is GTEPS = m / t_step (m = directed edges after dedupe) over K timed
+steps bracketed by stream synchronisation, with the learned direction policy when its
+weights are shipped (connectivity_amd/policy_weights.npz), else the plain rule; both are
 timed in the same run and reported with their labels checked identical.
 
 The line also carries:
-  roofline      -- dominant kernel of the timed steps: algorithmic bytes / summed HIP-event
-                   durations of its launches, against 8 TB/s; `traffic` comes from the
-                   committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present;
+  roofline      -- dominant kernel class: algorithmic bytes / summed HIP-event durations of
+                   its launches (events recorded on the library's stream around every
+                   launch, in a second pass of the same K steps, so that `value` carries no
+                   event overhead), against 8 TB/s; `traffic` comes from the committed
+                   rocprofv3 FETCH_SIZE / WRITE_SIZE summary when present;
   cpu_baseline  -- the reference's own multistep run_scc (oracle/_ref, compiled from its
                    sources) or, where that was not built, the oracle's restatement, timed on
                    this host's cores on the same graph;
   wcc_ms        -- msc_cc_run on the same graph (not part of `value`).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat|grid] [--no-cpu]
+Workloads: rmat (scale 24, the metric's), grid (4096 x 4096 grid digraph), batch (8192
+packed graphs, msc_scc_run_batch).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat|grid|batch] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -38,6 +44,13 @@ def log(msg: str) -> None:
 
 
 def build_graph(args, msc):
+    if args.workload == "batch":
+        g, base, kinds = msc.gen_batch(args.graphs, seed=args.seed)
+        desc = (f"synthetic batch of {args.graphs} digraphs, n log-normal (median 4096, sigma 1.4, clipped to "
+                f"[256, 500000]), half R-MAT (edge factor 8-32), half planted Zipf(1.5) SCCs joined by 4n DAG "
+                f"arcs, packed into one CSR, seed {args.seed}")
+        cfg = {"workload": f"batch{args.graphs}_scc", "graphs": args.graphs, "rmat_graphs": int((kinds == 0).sum())}
+        return g, desc, cfg, base
     if args.workload == "grid":
         g = msc.gen_grid(args.side, args.side, 0.5, 0.1, 0.005, seed=args.seed)
         desc = (f"synthetic {args.side}x{args.side} grid digraph, edge kept one-way p=0.5 / both p=0.1, "
@@ -48,7 +61,7 @@ def build_graph(args, msc):
         desc = (f"synthetic directed R-MAT (a,b,c,d)=(.57,.19,.19,.05), self-loops/duplicates removed, "
                 f"ids permuted, seed {args.seed}")
         cfg = {"workload": f"rmat{args.scale}_scc", "scale": args.scale, "edge_factor": args.edge_factor}
-    return g, desc, cfg
+    return g, desc, cfg, None
 
 
 def main() -> int:
@@ -56,7 +69,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["rmat", "grid"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "grid", "batch"], default="rmat")
+    ap.add_argument("--graphs", type=int, default=8192, help="graphs in the batch workload")
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--side", type=int, default=4096)
@@ -65,6 +79,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-check", action="store_true", help="skip the oracle parity check")
+    ap.add_argument("--check-graphs", type=int, default=64, help="batch graphs checked against the oracle")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="run the reported policy only (no plain/learned comparison; for profiling)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU work budget of the batch baseline")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (PMC traffic calibration)")
     args = ap.parse_args()
@@ -81,7 +99,7 @@ def main() -> int:
         log("no gfx950 device visible")
         return 1
     t0 = time.time()
-    g, desc, cfg = build_graph(args, msc)
+    g, desc, cfg, base = build_graph(args, msc)
     log(f"{cfg['workload']}: n={g.n} m={g.m} pivot={g.max_deg_vert} built in {time.time() - t0:.1f}s")
     ctx = msc.Context(0)
     if args.calib_copy_mb:
@@ -103,8 +121,11 @@ def main() -> int:
     policy = msc.POLICY_LEARNED if policy_name == "learned" else msc.POLICY_PLAIN
 
     def step(pol, timing=True):
-        _, st = msc.run_scc(dg, g.max_deg_vert, g.avg_degree, ctx=ctx, out=out, policy=pol, timing=timing,
-                            stats=True)
+        if base is not None:
+            _, st = msc.run_scc_batch(dg, base, ctx=ctx, out=out, policy=pol, timing=timing, stats=True)
+        else:
+            _, st = msc.run_scc(dg, g.max_deg_vert, g.avg_degree, ctx=ctx, out=out, policy=pol, timing=timing,
+                                stats=True)
         return st
 
     def timed(pol, steps, timing):
@@ -116,19 +137,20 @@ def main() -> int:
         msc._lib().msc_ctx_sync(ctx.handle)
         return (time.perf_counter() - t_start) * 1e3 / steps, stats
 
-    for pol in {policy, msc.POLICY_PLAIN}:
+    for pol in ({policy} if args.no_compare else {policy, msc.POLICY_PLAIN}):
         for _ in range(args.warmup):
             step(pol)
     log(f"warmup {args.warmup} done")
-    # the timed region: K steps of the reported policy, per-kernel HIP events inside it
-    ms_per_step, stats = timed(policy, args.steps, True)
+    # the timed region: K steps of the reported policy; then the same K steps again with HIP
+    # events around every launch, for the per-kernel durations of the roofline
+    ms_per_step, _ = timed(policy, args.steps, False)
     labels = out.to_host()
-    ms_noevents, _ = timed(policy, args.steps, False)
-    log(f"timed {args.steps} steps ({policy_name}): {ms_per_step:.3f} ms/step with kernel events, "
-        f"{ms_noevents:.3f} without")
+    ms_events, stats = timed(policy, args.steps, True)
+    ms_noevents = ms_per_step
+    log(f"timed {args.steps} steps ({policy_name}): {ms_per_step:.3f} ms/step; {ms_events:.3f} with kernel events")
     plain_ms = ms_per_step
     learned = None
-    if have_weights:
+    if have_weights and not args.no_compare:
         other = msc.POLICY_PLAIN if policy == msc.POLICY_LEARNED else msc.POLICY_LEARNED
         other_ms, _ = timed(other, args.steps, False)
         other_labels = out.to_host()
@@ -164,25 +186,33 @@ def main() -> int:
         import oracle
 
         t1 = time.time()
-        want = oracle.canonical(oracle.scc(g, g.max_deg_vert, g.avg_degree, vert_cutoff=0))
-        check = "identical" if np.array_equal(labels, want) else "MISMATCH"
+        if base is not None:
+            check = check_batch(msc, oracle, g, base, labels, args.check_graphs)
+        else:
+            want = oracle.canonical(oracle.scc(g, g.max_deg_vert, g.avg_degree, vert_cutoff=0))
+            check = "identical" if np.array_equal(labels, want) else "MISMATCH"
         log(f"parity vs oracle (canonical labels): {check} ({time.time() - t1:.1f}s)")
-        if check != "identical":
+        if not check.startswith("identical"):
             return 1
-    counts = msc.scc_verify(labels)
+    counts = msc.scc_verify(labels) if base is None else None
 
     cc_ms = []
     for _ in range(max(2, args.steps // 2)):
         t1 = time.perf_counter()
-        msc.run_cc(dg, ctx=ctx, out=out, policy=policy)
+        if base is not None:
+            msc.run_cc_batch(dg, base, ctx=ctx, out=out, policy=policy)
+        else:
+            msc.run_cc(dg, ctx=ctx, out=out, policy=policy)
         cc_ms.append((time.perf_counter() - t1) * 1e3)
 
-    cpu = None if args.no_cpu else cpu_baseline(g, args.cpu_runs)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(g, base, args.cpu_seconds) if base is not None else cpu_baseline(g, args.cpu_runs)
 
     gteps = g.m / (ms_per_step * 1e-3) / 1e9
     traffic = read_traffic(cfg["workload"], policy_name, dom)
-    cfg.update({"n": g.n, "m": g.m, "pivot": g.max_deg_vert, "policy": policy_name,
-                "phases": "trim -> fw-bw -> colour to empty"})
+    cfg.update({"n": g.n, "m": g.m, "pivot": g.max_deg_vert if base is None else "per graph",
+                "policy": policy_name, "phases": "trim -> fw-bw -> colour to empty"})
     line = {
         "metric": METRIC,
         "value": round(gteps, 4),
@@ -202,15 +232,94 @@ def main() -> int:
                      "launches": d["launches"], "avg_launch_us": round(d["ms"] * 1e3 / max(d["launches"], 1), 3),
                      "alg_bytes_per_launch": round(d["bytes"] / max(d["launches"], 1), 1)},
         "cpu_baseline": cpu,
-        "ms_per_step_without_kernel_events": round(ms_noevents, 4),
+        "ms_per_step_with_kernel_events": round(ms_events, 4),
         "plain_ms": round(plain_ms, 4),
         "learned": learned,
-        "scc_counts": {"num_sccs": counts[0], "nontrivial": counts[1], "max_scc": counts[2]},
+        "scc_counts": ({"num_sccs": counts[0], "nontrivial": counts[1], "max_scc": counts[2]}
+                       if counts else None),
         "parity": check,
         "wcc_ms": round(statistics.median(cc_ms), 4),
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def unpack(msc, u, base, i):
+    """Graph i of a packed batch, in its own ids."""
+    import numpy as np
+
+    b0, b1 = int(base[i]), int(base[i + 1])
+    e0, e1 = int(u.out_off[b0]), int(u.out_off[b1])
+    f0, f1 = int(u.in_off[b0]), int(u.in_off[b1])
+    return msc.Graph(n=b1 - b0, out_off=(u.out_off[b0:b1 + 1] - u.out_off[b0]).astype(np.uint32),
+                     out_adj=(u.out_adj[e0:e1] - b0).astype(np.int32),
+                     in_off=(u.in_off[b0:b1 + 1] - u.in_off[b0]).astype(np.uint32),
+                     in_adj=(u.in_adj[f0:f1] - b0).astype(np.int32))
+
+
+def check_batch(msc, oracle, u, base, labels, count):
+    """Every graph's slice against the oracle run on that graph alone, for `count` graphs
+    spread over the batch (the smallest and the largest graph always among them)."""
+    import numpy as np
+
+    k = len(base) - 1
+    sizes = np.diff(base)
+    pick = set(np.linspace(0, k - 1, min(count, k)).astype(int).tolist())
+    pick |= {int(np.argmin(sizes)), int(np.argmax(sizes))}
+    for i in sorted(pick):
+        g = unpack(msc, u, base, i)
+        want = oracle.canonical(oracle.scc(g, vert_cutoff=0))
+        if not np.array_equal(labels[int(base[i]):int(base[i + 1])], want):
+            log(f"graph {i} differs from the oracle")
+            return "MISMATCH"
+    return f"identical ({len(pick)} of {k} graphs checked)"
+
+
+def cpu_baseline_batch(u, base, seconds: float):
+    """The reference's run_scc over a bounded sample of the batch (graphs in batch order
+    until about `seconds` of wall time), one single-threaded call per graph with the graphs
+    spread over this host's cores; reported as edges per second of wall time.  Single
+    threads per graph: with several threads the reference's racy enqueue (check-then-set of
+    its visited flags) can queue a vertex twice and overrun queues sized num_valid on
+    graphs whose pivot reaches nearly every vertex (it aborts on some planted graphs)."""
+    import concurrent.futures as cf
+
+    import oracle
+
+    import connectivity_amd as msc
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, 16)
+    use_ref = oracle.have_ref()
+
+    def one(i):
+        oracle.set_threads(1)
+        g = unpack(msc, u, base, i)
+        piv, avg = oracle.pivot_scc(g)
+        if piv < 0:  # the reference's run_scc needs a pivot; such graphs stay out of the sample
+            return 0, 0.0
+        t = oracle.ref_time_scc(g, piv, avg) if use_ref else oracle.scc_time(g, piv, avg)
+        return g.m, t
+
+    edges, graphs, i, k = 0, 0, 0, len(base) - 1
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        while i < k and time.perf_counter() - t0 < seconds:
+            chunk = list(range(i, min(k, i + 4 * cores)))
+            for m, _ in ex.map(one, chunk):
+                edges += m
+                graphs += 1 if m else 0
+            i += len(chunk)
+    wall = time.perf_counter() - t0
+    log(f"cpu baseline ({'reference' if use_ref else 'port'}, {cores} threads x 1): {graphs} graphs, "
+        f"{edges} edges in {wall:.2f} s")
+    return {"value": round(edges / wall / 1e9, 5), "unit": "GTEPS", "cores": cores,
+            "kind": "reference" if use_ref else "port", "ms": round(wall * 1e3, 2),
+            "sample": f"run_scc (multistep, vert_cutoff 10000) on the first {i} graphs of the batch ({graphs} with "
+                      f"a pivot), one single-threaded call per graph, {cores} graphs at a time; edges / wall time"}
 
 
 def read_traffic(workload: str, policy: str, kernel_class: str):

@@ -13,7 +13,7 @@ constexpr int kBlock = 256;                    // threads per workgroup: 4 wave6
 constexpr uint32_t kChunk = 1024;              // edges per edge-balanced work item
 constexpr int kPer = (int)(kChunk / kBlock);   // edges per thread per work item
 constexpr uint32_t kStage = 2048;              // claims staged in LDS per workgroup
-constexpr int kBW = 32;                        // bitmap words per workgroup iteration
+constexpr int kBW = kBlock;                    // bitmap words per workgroup iteration (one per thread)
 constexpr int kMaxParts = 8192;                // per-workgroup partial results
 constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;

@@ -121,18 +121,23 @@ __device__ __forceinline__ unsigned long long block_sum64(unsigned long long x, 
 struct BitIter {
   uint32_t word[kBW];
   uint32_t pref[kBW + 1];
+  uint32_t wsum[kBlock / 64];
 };
 
-// Loads words [wb, wb + kBW) and the exclusive prefix of their popcounts (all threads call).
+// Loads words [wb, wb + kBW) (one per thread) and the exclusive prefix of their popcounts;
+// returns the set bits in the group.  All threads call; ends with a barrier.
 __device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, int64_t nwords, int64_t wb) {
+  static_assert(kBW == kBlock, "one word per thread");
   const int t = threadIdx.x;
-  if (t < 64) {
-    uint32_t w = (t < kBW && wb + t < nwords) ? bits[wb + t] : 0u;
-    if (t < kBW) B.word[t] = w;
-    uint32_t incl = wave_incl_sum((uint32_t)__popc(w));
-    if (t < kBW) B.pref[t + 1] = incl;
-    if (t == 0) B.pref[0] = 0u;
-  }
+  const uint32_t w = wb + t < nwords ? bits[wb + t] : 0u;
+  B.word[t] = w;
+  const uint32_t incl = wave_incl_sum((uint32_t)__popc(w));
+  if (lane() == 63) B.wsum[wave_id()] = incl;
+  __syncthreads();
+  uint32_t carry = 0;
+  for (int i = 0; i < wave_id(); ++i) carry += B.wsum[i];
+  B.pref[t + 1] = incl + carry;
+  if (t == 0) B.pref[0] = 0u;
   __syncthreads();
   return B.pref[kBW];
 }
@@ -141,7 +146,7 @@ __device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, 
 __device__ __forceinline__ uint32_t bits_pick(const BitIter& B, int64_t wb, uint32_t k) {
   int lo = 0, hi = kBW;  // pref[lo] <= k < pref[hi]
 #pragma unroll
-  for (int s = 0; s < 5; ++s) {
+  for (int s = 0; s < 8; ++s) {
     int mid = (lo + hi) >> 1;
     if (B.pref[mid] <= k) lo = mid;
     else hi = mid;
@@ -707,87 +712,90 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
     for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
       td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
   } else if (REACH) {
-    // A workgroup owns kBW words of `cand` per iteration (up to kR candidates per thread).
-    // Each candidate's first up-list entry is checked with all loads of the thread's
-    // candidates in flight together; most candidates are settled there (on R-MAT the
-    // first in-neighbour is already visited ~90% of the time), the rest scan on.  A
-    // parent is a visited vertex: allowed and no longer claimable.  Claims collect in LDS
-    // word masks and are written back with one plain store per word: no other workgroup
-    // clears bits of these words during a bottom-up level.
-    constexpr int kR = kBW * 32 / kBlock;
+    // A workgroup owns kBW words of `cand` per iteration and takes their candidates in
+    // rounds of kR per thread.  Each candidate's first up-list entry is checked with all
+    // loads of the thread's candidates in flight together; most candidates are settled
+    // there (on R-MAT the first in-neighbour is already visited ~90% of the time), the
+    // rest scan on.  A parent is a visited vertex: allowed and no longer claimable.
+    // Claims collect in LDS word masks and are written back with one plain store per
+    // word: no other workgroup clears bits of these words during a bottom-up level.
+    constexpr int kR = 4;
     for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
-      if (t < kBW) s_claim[t] = 0u;
-      const uint32_t total = bits_load(B, a.cand, a.nwords, wb);
-      uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
-      bool has[kR], hit[kR], first[kR];
+      s_claim[t] = 0u;
+      const uint32_t total = bits_load(B, a.cand, a.nwords, wb);  // barrier: s_claim cleared
+      for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
+        uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
+        bool has[kR], hit[kR], first[kR];
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        uint32_t k = (uint32_t)(r * kBlock + t);
-        has[r] = k < total;
-        v[r] = has[r] ? bits_pick(B, wb, k) : 0u;
-      }
+        for (int r = 0; r < kR; ++r) {
+          uint32_t k = k0 + (uint32_t)(r * kBlock + t);
+          has[r] = k < total;
+          v[r] = has[r] ? bits_pick(B, wb, k) : 0u;
+        }
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        beg[r] = end[r] = beg2[r] = end2[r] = 0u;
-        if (has[r]) {
-          beg[r] = a.up_off[v[r]];
-          end[r] = a.up_off[v[r] + 1];
-          if (MIRROR) {
-            beg2[r] = a.up_off2[v[r]];
-            end2[r] = a.up_off2[v[r] + 1];
+        for (int r = 0; r < kR; ++r) {
+          beg[r] = end[r] = beg2[r] = end2[r] = 0u;
+          if (has[r]) {
+            beg[r] = a.up_off[v[r]];
+            end[r] = a.up_off[v[r] + 1];
+            if (MIRROR) {
+              beg2[r] = a.up_off2[v[r]];
+              end2[r] = a.up_off2[v[r] + 1];
+            }
           }
         }
-      }
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        u[r] = 0u;
-        first[r] = false;
-        if (has[r]) {
-          if (beg[r] < end[r]) {
-            u[r] = (uint32_t)a.up_adj[beg[r]++];
-            first[r] = true;
-          } else if (MIRROR && beg2[r] < end2[r]) {
-            u[r] = (uint32_t)a.up_adj2[beg2[r]++];
-            first[r] = true;
+        for (int r = 0; r < kR; ++r) {
+          u[r] = 0u;
+          first[r] = false;
+          if (has[r]) {
+            if (beg[r] < end[r]) {
+              u[r] = (uint32_t)a.up_adj[beg[r]++];
+              first[r] = true;
+            } else if (MIRROR && beg2[r] < end2[r]) {
+              u[r] = (uint32_t)a.up_adj2[beg2[r]++];
+              first[r] = true;
+            }
           }
         }
-      }
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
-        wa[r] = first[r] ? a.allow[u[r] >> 5] : 0u;
-      }
+        for (int r = 0; r < kR; ++r) {
+          wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
+          wa[r] = first[r] ? a.allow[u[r] >> 5] : 0u;
+        }
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        hit[r] = false;
-        if (!has[r]) continue;
-        ++my_examined;
-        if (first[r]) {
-          ++my_inspected;
-          uint32_t b = 1u << (u[r] & 31u);
-          hit[r] = !(wc[r] & b) && (wa[r] & b);
-        }
-        for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
-          ++my_inspected;
-          uint32_t x = (uint32_t)a.up_adj[j];
-          hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
-        }
-        if (MIRROR)
-          for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
+        for (int r = 0; r < kR; ++r) {
+          hit[r] = false;
+          if (!has[r]) continue;
+          ++my_examined;
+          if (first[r]) {
             ++my_inspected;
-            uint32_t x = (uint32_t)a.up_adj2[j];
+            uint32_t b = 1u << (u[r] & 31u);
+            hit[r] = !(wc[r] & b) && (wa[r] & b);
+          }
+          for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
+            ++my_inspected;
+            uint32_t x = (uint32_t)a.up_adj[j];
             hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
           }
-        if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
-      }
+          if (MIRROR)
+            for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
+              ++my_inspected;
+              uint32_t x = (uint32_t)a.up_adj2[j];
+              hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
+            }
+          if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
+        }
 #pragma unroll
-      for (int r = 0; r < kR; ++r) {
-        uint32_t deg = hit[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]) : 0u;
-        stage_push(S, hit[r], v[r], deg);
+        for (int r = 0; r < kR; ++r) {
+          uint32_t deg = hit[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]) : 0u;
+          stage_push(S, hit[r], v[r], deg);
+        }
+        stage_flush(S, N, c, kStage - kBlock * kR);  // also the barrier before the next round
       }
-      __syncthreads();
-      if (t < kBW && s_claim[t]) a.cand[wb + t] = B.word[t] & ~s_claim[t];
-      stage_flush(S, N, c, kStage - kBW * 32);  // also the barrier before the next words
+      // after the last round's barrier every claim of this group is in s_claim; each thread
+      // reads only its own word and mask, so the next group's loads need no barrier first
+      if (s_claim[t]) a.cand[wb + t] = B.word[t] & ~s_claim[t];
     }
   }
   stage_flush(S, N, c, 0u);
@@ -1010,8 +1018,9 @@ __global__ void __launch_bounds__(kBlock) k_seed_colors(LevelArgs a) {
         deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
       }
       stage_push(S, take, v, deg);
+      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
     }
-    stage_flush(S, a.fr1, a.ctrl, kStage - kBW * 32);
+    __syncthreads();  // B is reloaded next
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }
@@ -1041,8 +1050,9 @@ __global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
         }
       }
       stage_push(S, take, v, deg);
+      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
     }
-    stage_flush(S, a.fr1, a.ctrl, kStage - kBW * 32);
+    __syncthreads();  // B is reloaded next
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }

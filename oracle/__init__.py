@@ -63,6 +63,11 @@ def lib():
     return _lib
 
 
+def set_threads(k: int) -> None:
+    """OpenMP threads of the calling thread's next parallel regions (omp_set_num_threads)."""
+    C.CDLL("libgomp.so.1").omp_set_num_threads(int(k))
+
+
 def have_ref() -> bool:
     return os.path.exists(REF_SCC_SO) and os.path.exists(REF_CC_SO)
 
