@@ -479,13 +479,14 @@ def schedule_array(fw=None, bw=None, cc=None) -> np.ndarray:
 
 
 def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alpha=15.0, beta=25,
-          device=0, schedule=None) -> _Opts:
+          device=0, schedule=None, trim_level=1) -> _Opts:
     o = _Opts()
     _lib().msc_opts_default(C.byref(o))
     if schedule is not None:
         o.schedule = schedule.ctypes.data
         o.schedule_len = int(schedule.shape[0])
     o.alpha, o.beta = alpha, beta
+    o.trim_level = trim_level
     o.policy = policy
     o.label_mode = labels
     o.labels_where = labels_where
@@ -498,7 +499,7 @@ def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alph
 
 
 def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
-         out, schedule, batch=None):
+         out, schedule, batch=None, trim_level=1):
     n = g.n
     where = HOST
     if isinstance(out, DeviceLabels):
@@ -516,7 +517,7 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
     if schedule is not None:
         schedule = np.ascontiguousarray(schedule, dtype=np.int8)
     o = _opts(policy, labels, timing, record, validate, ctx, labels_where=where, alpha=alpha, beta=beta,
-              schedule=schedule)
+              schedule=schedule, trim_level=trim_level)
     st = _Stats()
     csr = g.csr()
     piv = -1 if max_deg_vert is None else int(max_deg_vert)
@@ -535,14 +536,15 @@ def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None,
             policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
             record: bool = False, validate: bool = False, ctx: Context | None = None,
             alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False,
-            schedule: np.ndarray | None = None):
+            schedule: np.ndarray | None = None, trim_level: int = 1):
     """SCC labels of ``g`` (a Graph or DeviceGraph) on the GPU.
 
     run_scc (multistep/scc_main.cpp:210).  ``max_deg_vert`` / ``avg_degree`` default to
     create_csr's rule computed on device.  Returns labels, or (labels, Stats) with
-    ``stats=True``.  ``schedule`` (see :func:`schedule_array`) forces per-level decisions."""
+    ``stats=True``.  ``schedule`` (see :func:`schedule_array`) forces per-level decisions.
+    ``trim_level`` 2 is the reference's complete trim (TRIM_LEVEL 2, scc_trim.cpp:141-297)."""
     lab, st = _run("msc_scc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
-                   alpha, beta, out, schedule)
+                   alpha, beta, out, schedule, trim_level=trim_level)
     return (lab, st) if stats else lab
 
 
@@ -599,11 +601,11 @@ def _batch_args(vert_base, pivots):
 
 def run_scc_batch(g, vert_base, pivots=None, *, policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL,
                   timing: bool = False, record: bool = False, validate: bool = False, ctx: Context | None = None,
-                  alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False):
+                  alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False, trim_level: int = 1):
     """SCC labels of every graph of a packed batch (msc_scc_run_batch); each graph's slice
     holds its own local labels.  ``pivots`` (local ids, -1 none) default to create_csr's rule."""
     lab, st = _run("msc_scc_run_batch", g, None, None, policy, labels, timing, record, validate, ctx, alpha, beta,
-                   out, None, batch=_batch_args(vert_base, pivots))
+                   out, None, batch=_batch_args(vert_base, pivots), trim_level=trim_level)
     return (lab, st) if stats else lab
 
 

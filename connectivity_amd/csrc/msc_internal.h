@@ -29,6 +29,7 @@ enum Op : int {
   OP_CC = 2,   // undirected reach over out+in lists
   OP_PUSH = 3, // max-label push (colour propagation) along down lists
   OP_CBW = 4,  // per-colour backward search: claim valid in-neighbours of equal colour
+  OP_TRIM2 = 5,  // one round of the complete trim (down = out-lists, up = in-lists)
 };
 
 // Double-buffered frontier: queue of vertices, exclusive prefix of their list lengths,
@@ -56,7 +57,7 @@ struct Ctrl {
   uint32_t phase;                  // msc_level_rec.phase of this search
   uint32_t abandoned;              // the policy stopped the search; its result is void
   unsigned long long reduced[4];   // results of k_reduce
-  unsigned long long bytes_op[5];  // algorithmic bytes of closed levels, per Op
+  unsigned long long bytes_op[6];  // algorithmic bytes of closed levels, per Op
   unsigned long long bu_levels;
   unsigned long long t_start;      // earliest workgroup start of the running launch
   uint32_t closes;                 // closing-kernel launches that closed a level
@@ -120,6 +121,8 @@ int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off
 // out[i] = sums (cols 0..2) and max (col 3) of part[0..nparts)[i]
 void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out);
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
+// complete trim: every vertex valid and unlabelled, control block reset (round 1 examines all)
+void launch_trim2_init(const Launch& L, const LevelArgs& a);
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);  // work only
 void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  // one workgroup
 // labels[v] = label for every set bit of member; part col 0 = count.  With min_red, the

@@ -1153,6 +1153,129 @@ __global__ void k_validate(int32_t n, int64_t m, const uint32_t* __restrict__ of
 }
 
 // ------------------------------------------------------------------------------------
+// complete trim (multistep/scc_trim.cpp:141-297, TRIM_LEVEL 2)
+
+// All vertices valid (bits past n clear), labels -1, fresh control block.
+__global__ void k_trim2_init(LevelArgs a) {
+  const int64_t nw = a.nwords, n = a.n_total;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = i * 32;
+    a.valid[i] = lo + 32 <= n ? 0xffffffffu : (lo < n ? (1u << (uint32_t)(n - lo)) - 1u : 0u);
+    a.inq0[i] = 0u;
+    a.inq1[i] = 0u;
+  }
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+    a.labels[v] = -1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Ctrl* c = a.ctrl;
+    c->next_packed = 0ull;
+    c->q_size = 0u;
+    c->q_edges = 0u;
+    c->cur = 0u;
+    c->bottom_up = 0u;
+    c->switched_back = 0u;
+    c->done = 0u;
+    c->level = 0u;
+    c->closes = 0u;
+    c->found = 0;
+    c->phase = 5u;
+    c->abandoned = 0u;
+    c->t_start = ~0ull;
+  }
+}
+
+// One round: a vertex with no valid out-neighbour or no valid in-neighbour other than
+// itself is labelled with its own id and leaves the valid set; one trimmed for lack of
+// out-neighbours re-queues its in-neighbours (they lost an out-neighbour), one trimmed
+// for lack of in-neighbours its out-neighbours (scc_trim.cpp:203-256).  Round 1 examines
+// every vertex, later rounds the queue.  Removals only shrink neighbour sets, and whoever
+// loses a needed neighbour is re-queued by the vertex that leaves, so the rounds end at the
+// largest set in which every vertex keeps both kinds of neighbour, whatever the order.
+// Re-queues are deduplicated by the worklist bitmaps and reserved with one atomic per wave.
+__global__ void __launch_bounds__(kBlock) k_trim2(LevelArgs a) {
+  Ctrl* c = a.ctrl;
+  if (c->done) return;
+  const bool first = c->level == 0u;
+  const uint32_t cur = c->cur;
+  const int64_t items = first ? (int64_t)a.n_total : (int64_t)c->q_size;
+  const Frontier F = cur ? a.fr1 : a.fr0;
+  const Frontier N = cur ? a.fr0 : a.fr1;
+  uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
+  uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < items;
+       wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = wb + lane();
+    uint32_t v = 0;
+    bool act = false;
+    if (i < items) {
+      v = first ? (uint32_t)i : (uint32_t)F.q[i];
+      if (!first) bit_clear(inq_cur, v);
+      act = bit_test(a.valid, v);
+    }
+    bool has_out = false, has_in = false;
+    if (act) {
+      for (uint32_t j = a.down_off[v], e = a.down_off[v + 1]; j < e && !has_out; ++j) {
+        uint32_t w = (uint32_t)a.down_adj[j];
+        has_out = w != v && bit_test(a.valid, w);
+      }
+      for (uint32_t j = a.up_off[v], e = a.up_off[v + 1]; j < e && !has_in; ++j) {
+        uint32_t w = (uint32_t)a.up_adj[j];
+        has_in = w != v && bit_test(a.valid, w);
+      }
+    }
+    const bool trim = act && !(has_out && has_in);
+    if (trim) {
+      a.labels[v] = (int32_t)v;
+      bit_clear(a.valid, v);
+    }
+    uint32_t rb = 0, re = 0;
+    const int32_t* adj = a.down_adj;
+    if (trim && has_out != has_in) {
+      if (!has_out) {
+        rb = a.up_off[v];
+        re = a.up_off[v + 1];
+        adj = a.up_adj;
+      } else {
+        rb = a.down_off[v];
+        re = a.down_off[v + 1];
+      }
+    }
+    for (;;) {  // wave-uniform walk over the lists being re-queued
+      const bool more = rb < re;
+      if (__ballot(more) == 0ull) break;
+      bool push = false;
+      uint32_t w = 0;
+      if (more) {
+        w = (uint32_t)adj[rb++];
+        push = w != v && bit_test(a.valid, w) && bit_set(inq_nxt, w);
+      }
+      const unsigned long long m = __ballot(push);
+      if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane() == leader)
+          base = (uint32_t)(atomicAdd(&c->next_packed, (unsigned long long)__popcll(m) << 32) >> 32);
+        base = __shfl(base, leader, 64);
+        if (push) N.q[base + (uint32_t)__popcll(m & lanes_below())] = (int32_t)w;
+      }
+    }
+  }
+}
+
+// Closes a trim round: the re-queued vertices become the next round's worklist.
+__global__ void k_trim2_close(Ctrl* c) {
+  if (threadIdx.x != 0 || c->done) return;
+  const uint32_t next = (uint32_t)(c->next_packed >> 32);
+  c->level += 1;
+  c->closes += 1;
+  c->q_size = next;
+  c->q_edges = 0u;
+  c->cur ^= 1u;
+  c->next_packed = 0ull;
+  c->done = next == 0u ? 1u : 0u;
+}
+
+// ------------------------------------------------------------------------------------
 // batches: many independent graphs packed into one CSR
 //
 // Graph g owns vertices [base[g], base[g+1]) (base[0] = 0, base[ng] = n, non-decreasing)
@@ -1378,6 +1501,11 @@ void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, 
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, L.stream, part, nparts, out);
 }
 
+void launch_trim2_init(const Launch& L, const LevelArgs& a) {
+  hipLaunchKernelGGL(k_trim2_init, dim3(grid_for(a.n_total > a.nwords ? a.n_total : a.nwords, kBlock, L.grid * 4)),
+                     dim3(kBlock), 0, L.stream, a);
+}
+
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
   hipLaunchKernelGGL(k_search_init, dim3(1), dim3(kBlock), 0, L.stream, a, root, phase);
 }
@@ -1393,6 +1521,7 @@ void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
       else hipLaunchKernelGGL((k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
       break;
     case OP_CBW: hipLaunchKernelGGL((k_level<OP_CBW, false>), g, b, 0, L.stream, a); break;
+    case OP_TRIM2: hipLaunchKernelGGL(k_trim2, g, b, 0, L.stream, a); break;
     default: break;
   }
 }
@@ -1405,6 +1534,7 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
       else hipLaunchKernelGGL((k_close_color<OP_PUSH, false>), g, b, 0, L.stream, a, L.grid);
       break;
     case OP_CBW: hipLaunchKernelGGL((k_close_color<OP_CBW, false>), g, b, 0, L.stream, a, L.grid); break;
+    case OP_TRIM2: hipLaunchKernelGGL(k_trim2_close, g, dim3(64), 0, L.stream, a.ctrl); break;
     default: hipLaunchKernelGGL(k_close, g, b, 0, L.stream, a, op, L.grid); break;
   }
 }

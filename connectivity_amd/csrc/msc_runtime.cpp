@@ -322,7 +322,7 @@ int check_args(const msc_csr* g, const int32_t* labels, const msc_opts* o, bool 
   if (g->n > 0 && (!g->out_off || !g->in_off)) return MSC_E_INVALID;
   if (g->m > 0 && (!g->out_adj || !g->in_adj)) return MSC_E_INVALID;
   if (g->m >= (cc ? (1LL << 31) : (1LL << 32))) return MSC_E_CAPACITY;
-  if (o->trim_level != 1) return MSC_E_INVALID;
+  if (o->trim_level != 1 && (o->trim_level != 2 || cc)) return MSC_E_INVALID;  // complete trim: SCC only
   if (!(o->alpha > 0.0) || o->beta <= 0) return MSC_E_INVALID;
   if (o->policy != MSC_POLICY_PLAIN && o->policy != MSC_POLICY_LEARNED) return MSC_E_INVALID;
   if (o->label_mode != MSC_LABELS_CANONICAL && o->label_mode != MSC_LABELS_RAW) return MSC_E_INVALID;
@@ -511,7 +511,20 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97); the survivor count stays on
   // device (the searches read it there) and reaches the host with a later copy
   int np;
-  {
+  if (o->trim_level == 2) {
+    // complete trim (multistep/scc_trim.cpp:141-297): worklist rounds until nothing leaves;
+    // the pivot stays the caller's choice even when it is trimmed, as in the reference
+    LevelArgs t = base_args(c, &g, labels, o, avg);
+    t.down_off = g.out_off; t.down_adj = g.out_adj;
+    t.up_off = g.in_off; t.up_adj = g.in_adj;
+    {
+      Timed tt(c, MSC_K_TRIM);
+      launch_trim2_init(L, t);
+    }
+    if ((rc = run_levels(c, OP_TRIM2, false, t, MSC_K_TRIM, s, 0))) return rc;
+    np = launch_count_bits(L, c->valid, nw, c->part);
+    launch_reduce(L, c->part, np, c->ctrl->red_trim);
+  } else {
     Timed t(c, MSC_K_TRIM);
     np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, labels, c->part);
     launch_reduce(L, c->part, np, c->ctrl->red_trim);

@@ -72,7 +72,8 @@ typedef struct msc_ctx msc_ctx; /* per-device, per-stream workspace; see msc_ctx
 typedef struct msc_opts {
   double alpha;         /* ALPHA = 15.0 (multistep/scc_main.cpp:64) */
   int32_t beta;         /* BETA  = 25   (multistep/scc_main.cpp:65) */
-  int32_t trim_level;   /* TRIM_LEVEL = 1 (multistep/scc_main.cpp:61); only 1 is supported */
+  int32_t trim_level;   /* TRIM_LEVEL = 1 (multistep/scc_main.cpp:61): scc_trim; 2: the complete
+                           trim scc_trim_complete (scc_trim.cpp:141-297), SCC only */
   int32_t policy;       /* enum msc_policy */
   int32_t label_mode;   /* enum msc_label_mode */
   int32_t labels_where; /* MSC_HOST or MSC_DEVICE: where the `labels` output lives */

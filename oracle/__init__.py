@@ -51,6 +51,9 @@ def lib():
         L.orc_pivot_cc.argtypes = [C.c_int32, P, P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.orc_scc_run.argtypes = [C.c_int32, P, P, P, P, C.c_int32, C.c_double, C.c_int32, C.c_double,
                                   C.c_int32, P, C.POINTER(_Trace)]
+        L.orc_scc_run_trim.argtypes = [C.c_int32, P, P, P, P, C.c_int32, C.c_double, C.c_int32, C.c_double,
+                                       C.c_int32, C.c_int32, P, C.POINTER(_Trace)]
+        L.orc_trim_complete.argtypes = [C.c_int32, P, P, P, P, P]
         L.orc_cc_run.argtypes = [C.c_int32, P, P, P, P, C.c_int32, C.c_double, C.c_double, C.c_int32, P,
                                  C.POINTER(_Trace)]
         L.orc_canonicalize.argtypes = [C.c_int32, P]
@@ -82,6 +85,7 @@ def ref_scc():
         L.ref_scc_time.restype = C.c_double
         L.ref_create_csr.argtypes = [C.c_int, C.c_uint, P, P, P, P, P, P, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]
+        L.ref_trim_complete.argtypes = [C.c_int, C.c_uint, P, P, P, P, P]
         _ref_scc = L
     return _ref_scc
 
@@ -121,7 +125,16 @@ def pivot_cc(g):
     return piv.value, avg.value
 
 
-def scc(g, pivot=None, avg=None, vert_cutoff=10000, alpha=15.0, beta=25, trace_cap=0):
+def trim_complete(g):
+    """The complete trim (TRIM_LEVEL 2) alone: v for trimmed vertices, -1 for survivors."""
+    oo, oa, io, ia = _arrays(g)
+    labels = np.empty(g.n, dtype=np.int32)
+    if lib().orc_trim_complete(g.n, _p(oo), _p(oa), _p(io), _p(ia), _p(labels)) < 0:
+        raise ValueError("orc_trim_complete failed")
+    return labels
+
+
+def scc(g, pivot=None, avg=None, vert_cutoff=10000, alpha=15.0, beta=25, trace_cap=0, trim_level=1):
     """Raw labels of the restated run_scc; with trace_cap > 0 also (levels, summary)."""
     oo, oa, io, ia = _arrays(g)
     if pivot is None or avg is None:
@@ -133,8 +146,8 @@ def scc(g, pivot=None, avg=None, vert_cutoff=10000, alpha=15.0, beta=25, trace_c
     buf = (_Level * trace_cap)() if trace_cap else None
     tr.levels = buf if trace_cap else None
     tr.cap = trace_cap
-    rc = lib().orc_scc_run(g.n, _p(oo), _p(oa), _p(io), _p(ia), int(pivot), float(avg), int(vert_cutoff),
-                           float(alpha), int(beta), _p(labels), C.byref(tr))
+    rc = lib().orc_scc_run_trim(g.n, _p(oo), _p(oa), _p(io), _p(ia), int(pivot), float(avg), int(vert_cutoff),
+                                float(alpha), int(beta), int(trim_level), _p(labels), C.byref(tr))
     if rc != 0:
         raise ValueError(f"orc_scc_run failed ({rc})")
     if trace_cap:
@@ -213,6 +226,14 @@ def ref_time_scc(g, pivot, avg, vert_cutoff=10000):
     oo, oa, io, ia = _arrays(g)
     return ref_scc().ref_scc_time(g.n, g.m, _p(oo), _p(oa), _p(io), _p(ia), int(pivot), float(avg),
                                   int(vert_cutoff), None)
+
+
+def ref_trim_complete(g):
+    """The reference's scc_trim_complete on a fresh valid set (v trimmed, -1 survivor)."""
+    oo, oa, io, ia = _arrays(g)
+    labels = np.empty(g.n, dtype=np.int32)
+    ref_scc().ref_trim_complete(g.n, g.m, _p(oo), _p(oa), _p(io), _p(ia), _p(labels))
+    return labels
 
 
 def ref_create_csr(n, srcs, dsts):

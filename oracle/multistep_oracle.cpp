@@ -231,6 +231,47 @@ int propagate_max(const Lists& lists, const u8* valid, const std::vector<int32_t
 // multistep/scc_serial.cpp:160-172 (run_valid) and :216-258 (scc_serial_valid).  Visits
 // vertices and out-edges in the same order as the recursive version, so each component is
 // labelled with the same DFS root.  Returns the number of components.
+// Complete trim (multistep/scc_trim.cpp:141-297, TRIM_LEVEL 2): a vertex goes when it has no
+// valid out-neighbour other than itself or no valid in-neighbour other than itself, until
+// none goes.  Restated as peeling with counters: cnt_out[v] / cnt_in[v] count v's list
+// entries that are valid and not v (duplicates counted each time); a vertex whose count
+// reaches zero is trimmed and decrements the counters of its valid neighbours.  The
+// surviving set is the largest one in which every vertex keeps a valid out- and in-
+// neighbour besides itself, so it does not depend on the order of removals (the
+// reference's thread-order races only change the order).  Sequential on purpose.
+int64_t trim_complete(int32_t n, const uint32_t* oo, const int32_t* oa, const uint32_t* io, const int32_t* ia,
+                      u8* valid, int32_t* labels) {
+  std::vector<int64_t> cnt_out(n, 0), cnt_in(n, 0);
+  for (int32_t v = 0; v < n; ++v) {
+    valid[v] = 1;
+    for (uint32_t j = oo[v]; j < oo[v + 1]; ++j) cnt_out[v] += oa[j] != v;
+    for (uint32_t j = io[v]; j < io[v + 1]; ++j) cnt_in[v] += ia[j] != v;
+  }
+  std::vector<int32_t> stack;
+  for (int32_t v = 0; v < n; ++v)
+    if (cnt_out[v] == 0 || cnt_in[v] == 0) stack.push_back(v);
+  int64_t trimmed = 0;
+  while (!stack.empty()) {
+    int32_t v = stack.back();
+    stack.pop_back();
+    if (!valid[v]) continue;
+    valid[v] = 0;
+    ++trimmed;
+    // v no longer counts as an in-neighbour of its out-neighbours, nor as an
+    // out-neighbour of its in-neighbours
+    for (uint32_t j = oo[v]; j < oo[v + 1]; ++j) {
+      int32_t w = oa[j];
+      if (w != v && valid[w] && --cnt_in[w] == 0) stack.push_back(w);
+    }
+    for (uint32_t j = io[v]; j < io[v + 1]; ++j) {
+      int32_t w = ia[j];
+      if (w != v && valid[w] && --cnt_out[w] == 0) stack.push_back(w);
+    }
+  }
+  for (int32_t v = 0; v < n; ++v) labels[v] = valid[v] ? -1 : v;
+  return trimmed;
+}
+
 int tarjan_valid(int32_t n, const uint32_t* oo, const int32_t* oa, const u8* valid,
                  const int32_t* order, int64_t count, int32_t* labels) {
   std::vector<int32_t> index(n, -1), low(n, -1), stack;
@@ -333,7 +374,20 @@ int orc_pivot_cc(int32_t n, const uint32_t* oo, const uint32_t* io, int32_t* piv
 int orc_scc_run(int32_t n, const uint32_t* oo, const int32_t* oa, const uint32_t* io,
                 const int32_t* ia, int32_t pivot, double avg, int32_t vert_cutoff,
                 double alpha, int32_t beta, int32_t* labels, orc_trace* tr) {
-  if (n < 0 || (pivot >= n)) return -1;
+  return orc_scc_run_trim(n, oo, oa, io, ia, pivot, avg, vert_cutoff, alpha, beta, 1, labels, tr);
+}
+
+int orc_trim_complete(int32_t n, const uint32_t* oo, const int32_t* oa, const uint32_t* io,
+                      const int32_t* ia, int32_t* labels) {
+  if (n < 0) return -1;
+  std::vector<u8> valid(n > 0 ? n : 1);
+  return (int)trim_complete(n, oo, oa, io, ia, valid.data(), labels);
+}
+
+int orc_scc_run_trim(int32_t n, const uint32_t* oo, const int32_t* oa, const uint32_t* io,
+                     const int32_t* ia, int32_t pivot, double avg, int32_t vert_cutoff,
+                     double alpha, int32_t beta, int32_t trim_level, int32_t* labels, orc_trace* tr) {
+  if (n < 0 || (pivot >= n) || trim_level < 1 || trim_level > 2) return -1;
   if (tr) {
     tr->count = 0;
     tr->num_trim = tr->num_fwbw = tr->num_color = tr->num_serial = 0;
@@ -342,13 +396,21 @@ int orc_scc_run(int32_t n, const uint32_t* oo, const int32_t* oa, const uint32_t
   if (n == 0) return 0;
   std::vector<u8> valid(n, 1);
 
-  // Trim (multistep/scc_trim.cpp:89-139): out- or in-degree 0 in the full graph.
-  auto keeps = [&](int32_t v) { return (oo[v + 1] > oo[v]) && (io[v + 1] > io[v]); };
+  // Trim (multistep/scc_trim.cpp:89-139): out- or in-degree 0 in the full graph; or the
+  // complete trim of TRIM_LEVEL 2 (scc_main.cpp:253-258), after which the valid list is
+  // rebuilt in id order.  The pivot stays whatever the caller chose, trimmed or not.
+  auto keeps = [&](int32_t v) {
+    return trim_level == 2 ? valid[v] != 0 : (oo[v + 1] > oo[v]) && (io[v + 1] > io[v]);
+  };
+  if (trim_level == 2) {
+    trim_complete(n, oo, oa, io, ia, valid.data(), labels);
+  } else {
 #pragma omp parallel for schedule(static)
-  for (int32_t v = 0; v < n; ++v) {
-    bool k = keeps(v);
-    valid[v] = k ? 1 : 0;
-    labels[v] = k ? -1 : v;
+    for (int32_t v = 0; v < n; ++v) {
+      bool k = keeps(v);
+      valid[v] = k ? 1 : 0;
+      labels[v] = k ? -1 : v;
+    }
   }
   std::vector<int32_t> all(n);
   for (int32_t v = 0; v < n; ++v) all[v] = v;

@@ -60,6 +60,17 @@ int orc_scc_run(int32_t n, const uint32_t* out_off, const int32_t* out_adj,
                 orc_trace* trace);
 
 /* run_cc (multistep/cc_main.cpp:184-245) on the digraph's mirror (out + in lists). */
+/* Same with the trim level of multistep/scc_main.cpp:241-258: 1 (scc_trim) or 2 (the
+ * complete trim scc_trim_complete, scc_trim.cpp:141-297). */
+int orc_scc_run_trim(int32_t n, const uint32_t* out_off, const int32_t* out_adj,
+                     const uint32_t* in_off, const int32_t* in_adj, int32_t pivot, double avg,
+                     int32_t vert_cutoff, double alpha, int32_t beta, int32_t trim_level,
+                     int32_t* labels, orc_trace* trace);
+/* The complete trim alone: labels[v] = v for trimmed vertices, -1 for survivors; returns
+ * the number trimmed. */
+int orc_trim_complete(int32_t n, const uint32_t* out_off, const int32_t* out_adj,
+                      const uint32_t* in_off, const int32_t* in_adj, int32_t* labels);
+
 int orc_cc_run(int32_t n, const uint32_t* out_off, const int32_t* out_adj,
                const uint32_t* in_off, const int32_t* in_adj, int32_t pivot, double avg_degree,
                double alpha, int32_t beta, int32_t* labels, orc_trace* trace);

@@ -59,6 +59,25 @@ REF_API double ref_scc_time(int n, unsigned m, const unsigned* out_off, const in
   return t1 - t0;
 }
 
+// scc_trim_complete(...): multistep/scc_trim.cpp:141-297 (the TRIM_LEVEL 2 trim) on a fresh
+// valid set, as run_scc sets it up (scc_main.cpp:225-238); labels_out[v] = v for trimmed
+// vertices and -1 for survivors.
+REF_API int ref_trim_complete(int n, unsigned m, const unsigned* out_off, const int* out_adj,
+                              const unsigned* in_off, const int* in_adj, int* labels_out)
+{
+  graph g = {n, m, const_cast<int*>(out_adj), const_cast<int*>(in_adj),
+             const_cast<unsigned*>(out_off), const_cast<unsigned*>(in_off)};
+  int* maps = new int[n > 0 ? n : 1];
+  bool* valid = new bool[n > 0 ? n : 1];
+  int* valid_verts = new int[n > 0 ? n : 1];
+  int num_valid = n;
+  for (int i = 0; i < n; ++i) { maps[i] = -1; valid[i] = true; valid_verts[i] = i; }
+  int trimmed = scc_trim_complete(g, valid, valid_verts, num_valid, maps);
+  for (int i = 0; i < n; ++i) labels_out[i] = maps[i];
+  delete[] maps; delete[] valid; delete[] valid_verts;
+  return trimmed;
+}
+
 // create_csr(...): multistep/scc_main.cpp:127-191 (CSR + transposed CSR, pivot, avg degree)
 REF_API int ref_create_csr(int n, unsigned m, const int* srcs, const int* dsts,
                            unsigned* out_off, int* out_adj, unsigned* in_off, int* in_adj,

@@ -63,8 +63,10 @@ def test_argument_errors_are_status_codes(msc):
     assert L.msc_scc_run(ctypes.byref(csr), 7, -1.0, lab.ctypes.data, ctypes.byref(o), None) == msc.E_INVALID
     bad = msc._Opts()
     L.msc_opts_default(ctypes.byref(bad))
-    bad.trim_level = 2
+    bad.trim_level = 3  # 1 (scc_trim) and 2 (scc_trim_complete) exist; CC takes only 1
     assert L.msc_scc_run(ctypes.byref(csr), -1, -1.0, lab.ctypes.data, ctypes.byref(bad), None) == msc.E_INVALID
+    bad.trim_level = 2
+    assert L.msc_cc_run(ctypes.byref(csr), -1, -1.0, lab.ctypes.data, ctypes.byref(bad), None) == msc.E_INVALID
     bad.trim_level, bad.beta = 1, 0
     assert L.msc_scc_run(ctypes.byref(csr), -1, -1.0, lab.ctypes.data, ctypes.byref(bad), None) == msc.E_INVALID
     # malformed offsets are refused before any device work

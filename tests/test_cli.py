@@ -61,12 +61,22 @@ def test_tools_match_reference_lines(msc, orc, tmp_path, tool, line):
             verify = "Num CCs: %d, Max CC: %d, Unassigned %d" % (c[0], c[2], c[3])
         assert np.array_equal(labels, want)
         assert out[1] == verify
+        # The reference executables are compared on their own fixture only: on other inputs
+        # their output depends on the process's heap (scc_fwbw_bw reads fw[] entries that
+        # scc_fwbw_fw never initialised, scc_fwbw.cpp:59-61, :269; on R-MAT s12 the `scc`
+        # executable was seen to report 2037 SCCs where there are 2040), and the cc one can
+        # overrun a list (cc_bfs.cpp:231 walks g.n entries of a num_valid list).  The partition
+        # of the generated graph is checked against scipy instead.
         ref = os.path.join(REF, tool)
-        # the reference cc can overrun a list on some inputs (cc_bfs.cpp:231 walks g.n entries
-        # of a num_valid list), so its binary is compared on the reference's own fixture only
-        if os.path.exists(ref) and (tool == "scc" or graph.endswith("test.graph")):
+        if os.path.exists(ref) and graph.endswith("test.graph"):
             rc, rout = run([ref, graph])
             assert rc == 0 and rout[1] == out[1]
+        if not graph.endswith("test.graph"):
+            import scipy.sparse as sp
+            from scipy.sparse.csgraph import connected_components
+            a = sp.csr_matrix((np.ones(h.m, np.int8), h.out_adj, h.out_off.astype(np.int64)), shape=(h.n, h.n))
+            _, lab = connected_components(a, directed=True, connection="strong" if tool == "scc" else "weak")
+            assert np.array_equal(orc.canonical(labels.astype(np.int32)), orc.canonical(lab.astype(np.int32)))
 
 
 @pytest.mark.gpu

@@ -166,3 +166,44 @@ def test_tuning_rejects_bad_values(msc, gpu_ctx):
         assert gpu_ctx.tuning() == defaults
     with pytest.raises(msc.MscError):
         gpu_ctx.set_tuning(no_such_field=1)
+
+
+@pytest.mark.parametrize("kind", ["random", "rmat", "grid", "planted", "chains"])
+def test_complete_trim(msc, orc, gpu_ctx, kind):
+    """TRIM_LEVEL 2 (scc_trim.cpp:141-297): raw labels equal the oracle's run with the same
+    trim level, the pivot kept even when the complete trim removes it."""
+    _say(f"case: complete trim {kind}")
+    rng = np.random.default_rng(11)
+    if kind == "random":
+        gs = [msc.create_csr(2000, rng.integers(0, 2000, 2600), rng.integers(0, 2000, 2600)) for _ in range(3)]
+    elif kind == "rmat":
+        gs = [msc.gen_rmat(s, 8, seed=s) for s in (10, 14)]
+    elif kind == "grid":
+        gs = [msc.gen_grid(64, 64, seed=2)]
+    elif kind == "planted":
+        gs = [msc.gen_planted(20000, max_size=400, seed=5)[0]]
+    else:  # long chains into a cycle: many trim rounds
+        n = 3000
+        src = list(range(n - 1)) + [n - 1, 1000]
+        dst = list(range(1, n)) + [1000, 999]
+        gs = [msc.create_csr(n, src, dst)]
+    for g in gs:
+        piv, avg = orc.pivot_scc(g)
+        want = orc.scc(g, piv, avg, vert_cutoff=0, trim_level=2)
+        got, st = msc.run_scc(g, piv, avg, labels=msc.LABELS_RAW, ctx=gpu_ctx, trim_level=2, stats=True)
+        assert np.array_equal(got, want)
+        assert st.num_trim == int((orc.trim_complete(g) >= 0).sum())
+        assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx, trim_level=2), orc.canonical(want))
+    # a batch with the complete trim: every slice as the graph alone
+    u, base = msc.pack_graphs(gs)
+    raw = msc.run_scc_batch(u, base, labels=msc.LABELS_RAW, ctx=gpu_ctx, trim_level=2)
+    for i, g in enumerate(gs):
+        assert np.array_equal(raw[int(base[i]):int(base[i + 1])], orc.scc(g, vert_cutoff=0, trim_level=2))
+
+
+def test_complete_trim_rejected_for_wcc(msc, gpu_ctx):
+    g = msc.gen_rmat(8, 4, seed=1)
+    msc.run_cc(g, ctx=gpu_ctx)  # the reference's CC has only the simple trim (cc_trim.cpp:47-97)
+    with pytest.raises(msc.MscError):
+        msc._run("msc_cc_run", g, None, None, msc.POLICY_PLAIN, msc.LABELS_CANONICAL, False, False, False,
+                 gpu_ctx, 15.0, 25, None, None, trim_level=2)

@@ -291,3 +291,69 @@ def test_switch_rule_and_topdown_layers(msc, orc):
                     bu = True
             elif not switched and g.n // 25 > nxt:
                 bu, switched = False, True
+
+
+# ---- complete trim (TRIM_LEVEL 2) -------------------------------------------------------------
+def test_trim_complete_closed_forms(msc, orc):
+    n = 200
+    chain = msc.create_csr(n, np.arange(n - 1), np.arange(1, n))          # a path: all trimmed
+    assert np.array_equal(orc.trim_complete(chain), np.arange(n))
+    cycle = msc.create_csr(n, np.arange(n), (np.arange(n) + 1) % n)        # a cycle: none
+    assert np.all(orc.trim_complete(cycle) == -1)
+    # a cycle 0..9 with a tail 9 -> 10 -> ... -> 19 and a spur 20 -> 0: only the cycle stays
+    src = list(range(10)) + list(range(9, 19)) + [20]
+    dst = [(i + 1) % 10 for i in range(10)] + list(range(10, 20)) + [0]
+    g = msc.create_csr(21, src, dst)
+    want = np.array([-1] * 10 + list(range(10, 21)), dtype=np.int32)
+    assert np.array_equal(orc.trim_complete(g), want)
+    # self-loops do not keep a vertex (scc_trim.cpp:183, :192 skip out == vert)
+    g = msc.create_csr(3, [0, 1, 1, 2], [0, 1, 2, 1])
+    assert list(orc.trim_complete(g)) == [0, -1, -1]
+    # duplicated arcs count once per entry but change nothing
+    g = msc.create_csr(3, [0, 0, 1, 2, 2], [1, 1, 2, 0, 0])
+    assert np.all(orc.trim_complete(g) == -1)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_trim_complete_is_the_largest_closed_set(msc, orc, seed):
+    """Brute-force fixpoint: repeat 'drop every vertex lacking a valid in- or out-neighbour
+    other than itself' until nothing changes; the peeling restatement must agree."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(5, 60))
+    m = int(n * rng.uniform(0.8, 2.0))
+    s, d = rng.integers(0, n, m), rng.integers(0, n, m)
+    g = msc.create_csr(n, s, d)
+    valid = np.ones(n, bool)
+    while True:
+        keep = np.array([valid[v] and any(valid[w] and w != v for w in d[s == v])
+                         and any(valid[u] and u != v for u in s[d == v]) for v in range(n)])
+        if np.array_equal(keep, valid):
+            break
+        valid = keep
+    assert np.array_equal(orc.trim_complete(g), np.where(valid, -1, np.arange(n)).astype(np.int32))
+
+
+@pytest.mark.parametrize("kind", ["random", "rmat", "planted"])
+def test_trim_complete_equals_reference(msc, orc, kind):
+    if not orc.have_ref():
+        pytest.skip("reference build (oracle/_ref) absent")
+    rng = np.random.default_rng(7)
+    if kind == "random":
+        gs = [msc.create_csr(3000, rng.integers(0, 3000, 4500), rng.integers(0, 3000, 4500)) for _ in range(4)]
+    elif kind == "rmat":
+        gs = [msc.gen_rmat(s, 8, seed=s) for s in (10, 14, 16)]
+    else:
+        gs = [msc.gen_planted(20000, max_size=500, seed=3)[0]]
+    for g in gs:
+        assert np.array_equal(orc.trim_complete(g), orc.ref_trim_complete(g))
+
+
+def test_trim_level_two_keeps_the_partition(msc, orc):
+    for g in (msc.gen_rmat(13, 8, seed=4), msc.gen_grid(40, 40, seed=4), msc.gen_planted(5000, seed=4)[0]):
+        a = orc.scc(g, vert_cutoff=0, trim_level=2)
+        b = orc.scc(g, vert_cutoff=0, trim_level=1)
+        assert np.array_equal(orc.canonical(a), orc.canonical(b))
+        assert np.array_equal(orc.canonical(a), orc.canonical(orc.tarjan(g)))
+        # every vertex the complete trim removes keeps its own id as raw label
+        t = orc.trim_complete(g)
+        assert np.array_equal(a[t >= 0], t[t >= 0])
