@@ -1,6 +1,6 @@
 """Autotuner for a context's launch parameters (msc_tuning).
 
-The level loops have five tunables that no rule of the reference fixes and that trade
+The level loops have six tunables that no rule of the reference fixes and that trade
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
@@ -8,7 +8,8 @@ against each other differently on shallow power-law graphs and on deep meshes:
                     inside the one-workgroup closing kernel, many levels per launch;
   small_cbw_edges   the same for the per-colour backward searches;
   first_batch       levels enqueued before a loop first reads its `done` word back;
-  max_batch         levels enqueued between later reads (batches double up to this).
+  max_batch         levels enqueued between later reads (batches double up to this);
+  concurrent_fwbw   backward search beside the forward one on a second stream (1) or after it.
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -36,6 +37,7 @@ CANDIDATES = {
     "small_cbw_edges": [0, 1024, 2048, 4096, 8192, 16384],
     "first_batch": [2, 4, 8],
     "max_batch": [16, 32, 64],
+    "concurrent_fwbw": [0, 1],
 }
 
 

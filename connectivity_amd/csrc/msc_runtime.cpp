@@ -26,6 +26,7 @@ constexpr uint32_t kRecCap = 1u << 16;
 struct Ev {
   hipEvent_t a, b;
   int cls;
+  hipStream_t st;
 };
 
 }  // namespace
@@ -34,9 +35,11 @@ struct msc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  hipStream_t stream2 = nullptr;  // the backward search runs here beside the forward one
+  hipEvent_t ev_fork = nullptr;   // stream2 waits for the set-up on `stream`
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{4, 4096, 4096, 2, 32, 0};
+  msc_tuning tune{4, 4096, 4096, 2, 32, 1};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -44,10 +47,12 @@ struct msc_ctx {
   // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
   uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
   int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
-  Frontier fr0{}, fr1{};
-  unsigned long long* part = nullptr;  // kMaxParts x 4 per-workgroup partials
-  uint32_t* err = nullptr;             // validation flag
+  Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the concurrent backward search's
+  unsigned long long* part = nullptr;   // kMaxParts x 4 per-workgroup partials
+  unsigned long long* part2 = nullptr;  // the same for the concurrent backward search
+  uint32_t* err = nullptr;              // validation flag
   Ctrl* ctrl = nullptr;
+  Ctrl* ctrl2 = nullptr;                // control block of the concurrent backward search
   msc_level_rec* recs = nullptr;
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
@@ -67,11 +72,12 @@ struct msc_ctx {
   size_t in_bytes = 0;
   // host side
   Ctrl* ctrl_host = nullptr;  // pinned
+  Ctrl* ctrl_host2 = nullptr;  // pinned
   uint32_t* pinned_u32 = nullptr;
   std::vector<Ev> events;
   size_t ev_used = 0;
   bool timing = false;
-  uint32_t last_nrec = 0;
+  uint32_t last_nrec = 0, last_nrec2 = 0;
   std::mutex mu;
 };
 
@@ -112,7 +118,11 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)kMaxParts * 4 * 8,                                // partials
       256,                                                      // err
       sizeof(Ctrl),                                             // ctrl
-      (size_t)kRecCap * sizeof(msc_level_rec),                  // records
+      (size_t)kRecCap * sizeof(msc_level_rec),                  // records (two halves)
+      (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr2
+      (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr3
+      (size_t)kMaxParts * 4 * 8,                                // partials of stream2
+      sizeof(Ctrl),                                             // ctrl2
   };
   size_t total = 0;
   for (size_t s : sizes) total += align256(s);
@@ -141,6 +151,10 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->err = (uint32_t*)parts[16];
   c->ctrl = (Ctrl*)parts[17];
   c->recs = (msc_level_rec*)parts[18];
+  c->fr2 = Frontier{(int32_t*)parts[19], (uint32_t*)parts[20], (uint32_t*)parts[21]};
+  c->fr3 = Frontier{(int32_t*)parts[22], (uint32_t*)parts[23], (uint32_t*)parts[24]};
+  c->part2 = (unsigned long long*)parts[25];
+  c->ctrl2 = (Ctrl*)parts[26];
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -149,7 +163,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
 
 // ---- timing -----------------------------------------------------------------------
 
-int ev_begin(msc_ctx* c, int cls) {
+int ev_begin(msc_ctx* c, int cls, hipStream_t st) {
   if (!c->timing) return -1;
   if (c->ev_used == c->events.size()) {
     Ev e{};
@@ -158,12 +172,13 @@ int ev_begin(msc_ctx* c, int cls) {
   }
   Ev& e = c->events[c->ev_used];
   e.cls = cls;
-  (void)hipEventRecord(e.a, c->stream);
+  e.st = st;
+  (void)hipEventRecord(e.a, st);
   return (int)c->ev_used++;
 }
 void ev_end(msc_ctx* c, int idx) {
   if (idx < 0) return;
-  (void)hipEventRecord(c->events[idx].b, c->stream);
+  (void)hipEventRecord(c->events[idx].b, c->events[idx].st);
 }
 void ev_collect(msc_ctx* c, msc_stats* s) {
   for (size_t i = 0; i < c->ev_used; ++i) {
@@ -179,7 +194,7 @@ void ev_collect(msc_ctx* c, msc_stats* s) {
 struct Timed {
   msc_ctx* c;
   int idx;
-  Timed(msc_ctx* ctx, int cls) : c(ctx), idx(ev_begin(ctx, cls)) {}
+  Timed(msc_ctx* ctx, int cls, hipStream_t st = nullptr) : c(ctx), idx(ev_begin(ctx, cls, st ? st : ctx->stream)) {}
   ~Timed() { ev_end(c, idx); }
 };
 
@@ -202,44 +217,77 @@ int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
   return MSC_OK;
 }
 
-// Runs level launches (work + closing kernel) until the control block says done; the host
-// copy of the control block (ctrl_host) is current when it returns.  Every
-// search and every max-label fixpoint settles within n + 1 levels, so more launches than
-// that mean a fault.
-int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a_in, int kcls, msc_stats* s, int first_batch,
-               bool count_left = false) {
-  Launch L = launch_of(c);
-  LevelArgs a = a_in;
-  a.small_edges = op == OP_PUSH ? (uint32_t)c->tune.small_push_edges
-                  : op == OP_CBW ? (uint32_t)c->tune.small_cbw_edges
-                                 : 0u;
+// One level loop: its op, arguments (which name its control block and buffers), stream and
+// the pinned host copy of its control block.
+struct Search {
+  int op;
+  bool mirror;
+  LevelArgs a;
+  hipStream_t stream;
+  Ctrl* host;
+  int kcls;
+  bool count_left;  // after each batch, count the valid vertices into ctrl->red_left
+};
+
+// Runs level launches (work + closing kernel) of one or two searches until their control
+// blocks say done; each search's host copy is current when this returns.  Two searches run
+// on their own streams, batch for batch, so their levels overlap on the device and share
+// the host round trips.  Every search and max-label fixpoint settles within n + 1 levels,
+// so more launches than that mean a fault.
+int run_searches(msc_ctx* c, Search* ss, int k, msc_stats* s, int first_batch) {
+  bool done[2] = {false, false};
+  for (int j = 0; j < k; ++j) {
+    LevelArgs& a = ss[j].a;
+    a.small_edges = ss[j].op == OP_PUSH ? (uint32_t)c->tune.small_push_edges
+                    : ss[j].op == OP_CBW ? (uint32_t)c->tune.small_cbw_edges
+                                         : 0u;
+  }
   int batch = first_batch > 0 ? first_batch : c->tune.first_batch;
   const int max_batch = c->tune.max_batch;
-  const int64_t limit = (int64_t)a.n_total + 64;
+  const int64_t limit = (int64_t)ss[0].a.n_total + 64;
   int64_t launched = 0;
   for (;;) {
     if (launched > limit) return MSC_E_DEVICE;
     launched += batch;
-    for (int i = 0; i < batch; ++i) {
-      {
-        Timed t(c, kcls);
-        launch_level(L, op, mirror, a);
+    for (int i = 0; i < batch; ++i)
+      for (int j = 0; j < k; ++j) {
+        if (done[j]) continue;
+        Launch L{ss[j].stream, c->grid};
+        {
+          Timed t(c, ss[j].kcls, ss[j].stream);
+          launch_level(L, ss[j].op, ss[j].mirror, ss[j].a);
+        }
+        launch_close(L, ss[j].op, ss[j].mirror, ss[j].a);
       }
-      launch_close(L, op, mirror, a);
+    for (int j = 0; j < k; ++j) {
+      if (done[j]) continue;
+      Launch L{ss[j].stream, c->grid};
+      if (ss[j].count_left) {
+        // valid vertices left: read back with this copy, so a colour round needs no extra trip
+        int np = launch_count_bits(L, c->valid, ss[j].a.nwords, ss[j].a.part);
+        launch_reduce(L, ss[j].a.part, np, ss[j].a.ctrl->red_left);
+      }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(ss[j].host, ss[j].a.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ss[j].stream));
     }
-    if (count_left) {
-      // valid vertices left: read back with this copy, so a colour round needs no extra trip
-      int np = launch_count_bits(L, c->valid, a.nwords, c->part);
-      launch_reduce(L, c->part, np, c->ctrl->red_left);
+    bool all = true;
+    for (int j = 0; j < k; ++j) {
+      if (done[j]) continue;
+      HIP_TRY(hipStreamSynchronize(ss[j].stream));
+      done[j] = ss[j].host->done != 0u;
+      all = all && done[j];
     }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
     if (s) s->host_syncs += 1;
-    if (c->ctrl_host->done) break;
+    if (all) break;
     batch = batch * 2 < max_batch ? batch * 2 : max_batch;
   }
   return MSC_OK;
+}
+
+int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, msc_stats* s, int first_batch,
+               bool count_left = false) {
+  Search one{op, mirror, a, c->stream, c->ctrl_host, kcls, count_left};
+  return run_searches(c, &one, 1, s, first_batch);
 }
 
 LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opts* o, double avg) {
@@ -254,7 +302,7 @@ LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opt
   a.ctrl = c->ctrl;
   a.part = c->part;
   a.recs = o->record ? c->recs : nullptr;
-  a.rec_cap = kRecCap;
+  a.rec_cap = kRecCap / 2;  // the second half is the concurrent backward search's
   a.n_total = g->n;
   a.nwords = ((int64_t)g->n + 31) / 32;
   a.beta = o->beta;
@@ -490,6 +538,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   int32_t* labels = o->labels_where == MSC_DEVICE ? labels_out : c->labels;
   const int64_t nw = ((int64_t)n + 31) / 32;
   HIP_TRY(hipMemsetAsync(c->ctrl, 0, sizeof(Ctrl), c->stream));
+  HIP_TRY(hipMemsetAsync(c->ctrl2, 0, sizeof(Ctrl), c->stream));
 
   if (avg < 0.0) avg = cc ? 2.0 * (double)g.m / (double)n : (double)g.m / (double)n;
   if (bt) {
@@ -549,25 +598,54 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
       fw.allow = c->valid; fw.cand = c->b1;
+      // BW: top-down on in-lists, bottom-up on out-lists.  Run concurrently (the default) it
+      // searches the valid set on its own stream and buffers, and the SCC is FW set & BW set:
+      // a path from v to the pivot that starts in the FW set stays in it, so this equals the
+      // reference's BW restricted to the FW set (scc_fwbw.cpp:230-409).  Run after FW it is
+      // that restricted search.
+      LevelArgs bw = a;
+      bw.down_off = g.in_off; bw.down_adj = g.in_adj;
+      bw.up_off = g.out_off; bw.up_adj = g.out_adj;
       if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
       else launch_search_init(L, fw, pivot, 0);
-      if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
-      if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
-      if (!c->ctrl_host->abandoned) {
-        // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy), top-down on in-lists,
-        // bottom-up on out-lists
-        launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
-        LevelArgs bw = a;
-        bw.down_off = g.in_off; bw.down_adj = g.in_adj;
-        bw.up_off = g.out_off; bw.up_adj = g.out_adj;
-        bw.allow = c->b2; bw.cand = c->b3;
-        if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
-        else launch_search_init(L, bw, pivot, 1);
-        if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 0))) return rc;
-        if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
+      if (c->tune.concurrent_fwbw) {
+        bw.allow = c->valid; bw.cand = c->b3;
+        bw.ctrl = c->ctrl2; bw.part = c->part2;
+        bw.fr0 = c->fr2; bw.fr1 = c->fr3;
+        bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
+        launch_andnot(L, c->b3, nullptr, c->valid, nullptr, nw);  // cand = allow
+        HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+        Launch L2{c->stream2, c->grid};
+        if (bt) launch_seed_pivots(L2, false, bw, c->bpiv, ng, 1);
+        else launch_search_init(L2, bw, pivot, 1);
+        Search both[2] = {{OP_FW, false, fw, c->stream, c->ctrl_host, MSC_K_LEVEL, false},
+                          {OP_BW, false, bw, c->stream2, c->ctrl_host2, MSC_K_LEVEL, false}};
+        if ((rc = run_searches(c, both, 2, s, 0))) return rc;
+        if (s) {
+          s->fw_levels = (int32_t)c->ctrl_host->level;
+          s->bw_levels = (int32_t)c->ctrl_host2->level;
+        }
+        if (!c->ctrl_host->abandoned && !c->ctrl_host2->abandoned) {
+          launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);  // FW set
+          launch_andnot(L, c->b2, nullptr, c->b2, c->b3, nw);     // & BW set
+          member = c->b2;
+        }
+      } else {
+        if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
+        if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
         if (!c->ctrl_host->abandoned) {
-          launch_andnot(L, c->b1, nullptr, c->b2, c->b3, nw);  // SCC = fw set & visited by BW
-          member = c->b1;
+          // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy)
+          launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
+          bw.allow = c->b2; bw.cand = c->b3;
+          if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
+          else launch_search_init(L, bw, pivot, 1);
+          if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 0))) return rc;
+          if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
+          if (!c->ctrl_host->abandoned) {
+            launch_andnot(L, c->b1, nullptr, c->b2, c->b3, nw);  // SCC = fw set & visited by BW
+            member = c->b1;
+          }
         }
       }
     } else {
@@ -679,17 +757,21 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     HIP_TRY(hipMemcpyAsync(labels_out, labels, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipGetLastError());
   if ((rc = read_ctrl(c))) return rc;
-  c->last_nrec = c->ctrl_host->nrec < kRecCap ? c->ctrl_host->nrec : kRecCap;
+  c->last_nrec = c->ctrl_host->nrec < kRecCap / 2 ? c->ctrl_host->nrec : kRecCap / 2;
+  HIP_TRY(hipMemcpyAsync(c->ctrl_host2, c->ctrl2, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->last_nrec2 = c->ctrl_host2->nrec < kRecCap / 2 ? c->ctrl_host2->nrec : kRecCap / 2;
   auto t1 = std::chrono::steady_clock::now();
   if (s) {
     s->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     s->num_trim = n - (int32_t)c->ctrl_host->red_trim[0];
     s->num_fwbw = (int32_t)c->ctrl_host->red_commit[0];
     if (c->ctrl_host->red_commit[0]) s->k[MSC_K_COMMIT].bytes += 8.0 * nw + 4.0 * (double)c->ctrl_host->red_commit[0];
-    s->bu_levels = (int32_t)c->ctrl_host->bu_levels;
+    const bool two = !cc && c->tune.concurrent_fwbw;  // ctrl_host2 is this run's backward search
+    s->bu_levels = (int32_t)(c->ctrl_host->bu_levels + (two ? c->ctrl_host2->bu_levels : 0));
     for (int op = 0; op < 5; ++op) {
       int cls = op == OP_PUSH ? MSC_K_COLOR_PROP : op == OP_CBW ? MSC_K_COLOR_BW : MSC_K_LEVEL;
-      s->k[cls].bytes += (double)c->ctrl_host->bytes_op[op];
+      s->k[cls].bytes += (double)c->ctrl_host->bytes_op[op] + (two ? (double)c->ctrl_host2->bytes_op[op] : 0.0);
     }
     ev_collect(c, s);
     s->bytes = 0;
@@ -779,7 +861,15 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     }
     c->own_stream = true;
   }
+  // the second stream keeps the default priority, like the first
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    msc_ctx_destroy(c);
+    return MSC_E_DEVICE;
+  }
   if (hipHostMalloc((void**)&c->ctrl_host, sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->ctrl_host2, sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->sched_host, 3 * MSC_SCHED_LEVELS, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&c->small, align256((size_t)kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
@@ -801,6 +891,7 @@ int msc_ctx_destroy(msc_ctx* c) {
   if (!c) return MSC_E_INVALID;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   if (c->ws) (void)hipFree(c->ws);
   if (c->in_buf) (void)hipFree(c->in_buf);
   if (c->bws) (void)hipFree(c->bws);
@@ -809,6 +900,9 @@ int msc_ctx_destroy(msc_ctx* c) {
     (void)hipEventDestroy(e.b);
   }
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
+  if (c->ctrl_host2) (void)hipHostFree(c->ctrl_host2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->pinned_u32) (void)hipHostFree(c->pinned_u32);
   if (c->sched_host) (void)hipHostFree(c->sched_host);
   if (c->small) (void)hipFree(c->small);
@@ -832,11 +926,11 @@ int msc_ctx_get_tuning(msc_ctx* c, msc_tuning* out) {
 int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (!c || !t) return MSC_E_INVALID;
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
-      t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256)
+      t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
+      t->concurrent_fwbw > 1)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;
-  c->tune.pad = 0;
   c->grid = c->cus * t->blocks_per_cu;
   if (c->grid > kMaxParts) c->grid = kMaxParts;
   return MSC_OK;
@@ -845,11 +939,18 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
 int msc_ctx_levels(msc_ctx* c, msc_level_rec* out, int32_t cap, int32_t* count) {
   if (!c || !count || cap < 0 || (cap > 0 && !out)) return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
-  uint32_t k = c->last_nrec < (uint32_t)cap ? c->last_nrec : (uint32_t)cap;
-  *count = (int32_t)c->last_nrec;
-  if (k == 0 || !c->recs) return MSC_OK;
+  // records of the main control block, then those of the concurrent backward search
+  const uint32_t total = c->last_nrec + c->last_nrec2;
+  *count = (int32_t)total;
+  uint32_t k1 = c->last_nrec < (uint32_t)cap ? c->last_nrec : (uint32_t)cap;
+  uint32_t k2 = c->last_nrec2 < (uint32_t)cap - k1 ? c->last_nrec2 : (uint32_t)cap - k1;
+  if ((k1 == 0 && k2 == 0) || !c->recs) return MSC_OK;
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipMemcpyAsync(out, c->recs, (size_t)k * sizeof(msc_level_rec), hipMemcpyDeviceToHost, c->stream));
+  if (k1)
+    HIP_TRY(hipMemcpyAsync(out, c->recs, (size_t)k1 * sizeof(msc_level_rec), hipMemcpyDeviceToHost, c->stream));
+  if (k2)
+    HIP_TRY(hipMemcpyAsync(out + k1, c->recs + kRecCap / 2, (size_t)k2 * sizeof(msc_level_rec),
+                           hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MSC_OK;
 }
