@@ -125,10 +125,11 @@ void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32
 void launch_trim2_init(const Launch& L, const LevelArgs& a);
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);  // work only
 void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  // one workgroup
-// labels[v] = label for every set bit of member; part col 0 = count.  With min_red, the
-// label is instead the smallest member found by launch_min_bit (0xffffffff - min_red[3]).
-int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
-                      int32_t label, unsigned long long* part, const unsigned long long* min_red = nullptr);
+// labels[v] = label and valid &= ~member for every set bit of member; part col 0 = count.
+// With min_red, the label is the smallest member found by launch_min_bit
+// (0xffffffff - min_red[3]).
+int launch_commit(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
+                  const unsigned long long* min_red, unsigned long long* part);
 // part col 3 = 0xffffffff - smallest set bit in each workgroup's words (0: none)
 int launch_min_bit(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
 // labels[v] = cmin[labels[v]] for every set bit v of `coloured`
@@ -156,8 +157,9 @@ void launch_batch_pivots(const Launch& L, bool cc, int32_t n, const uint32_t* oo
 // fresh control block, every pivot claimed and queued (frontier buffer 1)
 void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const int32_t* piv, uint32_t ng,
                         int32_t phase);
-int launch_label_bits_batch(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
-                            const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part);
+// the same with labels[v] = piv[graph of v]
+int launch_commit_batch(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels,
+                        const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part);
 void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng);
 // run only after launch_validate passed (reads adjacency through the offsets)
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,

@@ -400,32 +400,6 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
   }
 }
 
-// labels[v] = label for each set bit v of `member`; part col 0 = members of the workgroup.
-// min_red (optional) replaces the label by the smallest member (canonical labels).
-__global__ void __launch_bounds__(kBlock) k_label_bits(const uint32_t* __restrict__ member, int64_t nwords,
-                                                       int32_t n, int32_t* __restrict__ labels, int32_t label,
-                                                       unsigned long long* __restrict__ part,
-                                                       const unsigned long long* __restrict__ min_red) {
-  __shared__ unsigned long long s_red[kBlock / 64];
-  if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
-  unsigned long long cnt = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
-    uint32_t w = member[i];
-    cnt += (uint32_t)__popc(w);
-    while (w) {
-      uint32_t b = (uint32_t)__ffs(w) - 1u;
-      int64_t v = i * 32 + b;
-      if (v < n) labels[v] = label;
-      w &= w - 1u;
-    }
-  }
-  cnt = block_sum64(cnt, s_red);
-  if (threadIdx.x == 0) {
-    part[4 * blockIdx.x] = cnt;
-    part[4 * blockIdx.x + 1] = part[4 * blockIdx.x + 2] = part[4 * blockIdx.x + 3] = 0;
-  }
-}
-
 // part col 3 = 0xffffffff - smallest set bit of the workgroup's words (0 when none), so
 // that k_reduce's max yields the smallest set bit overall.
 __global__ void __launch_bounds__(kBlock) k_min_bit(const uint32_t* __restrict__ bits, int64_t nwords,
@@ -1405,27 +1379,34 @@ __global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }
 
-// labels[v] = piv[graph of v] for each set bit v of `member`; part col 0 = members.
-__global__ void __launch_bounds__(kBlock) k_label_bits_batch(const uint32_t* __restrict__ member, int64_t nwords,
-                                                             int32_t n, int32_t* __restrict__ labels,
-                                                             const int32_t* __restrict__ piv,
-                                                             const int64_t* __restrict__ base, uint32_t ng,
-                                                             unsigned long long* __restrict__ part) {
+// Commit of the pivot's component (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271): every
+// member gets the label and leaves the valid set; part col 0 = members of the workgroup.
+// One vertex per lane so the label stores of a wave are one contiguous 256-byte run; the
+// first lane of each word clears the word's members from `valid` (no other writer).  The
+// label is `label`, or the smallest member when min_red is given (0xffffffff - min_red[3],
+// canonical labels), or in a batch the pivot of the vertex's own graph.
+template <bool BATCH>
+__global__ void __launch_bounds__(kBlock) k_commit(const uint32_t* __restrict__ member, uint32_t* __restrict__ valid,
+                                                   int32_t n, int32_t* __restrict__ labels, int32_t label,
+                                                   const unsigned long long* __restrict__ min_red,
+                                                   const int32_t* __restrict__ piv, const int64_t* __restrict__ base,
+                                                   uint32_t ng, unsigned long long* __restrict__ part) {
   __shared__ unsigned long long s_red[kBlock / 64];
+  if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
   unsigned long long cnt = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
-    uint32_t w = member[i];
-    cnt += (uint32_t)__popc(w);
-    if (w == 0u) continue;
-    int64_t v = i * 32 + (__ffs(w) - 1);
-    if (v >= n) continue;
-    uint32_t g = graph_of(base, 0u, ng, v);
-    while (w) {
-      v = i * 32 + (__ffs(w) - 1);
-      if (v >= n) break;
-      while (base[g + 1] <= v) ++g;
-      labels[v] = piv[g];
-      w &= w - 1u;
+  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
+    const int64_t v = wb + lane();
+    const uint32_t w = member[v >> 5];  // bitmaps carry slack words past n
+    const bool in = v < n && ((w >> (v & 31)) & 1u);
+    if (BATCH) {
+      const uint32_t g = wave_graph_of(base, ng, wb, n);
+      if (in) labels[v] = piv[g];
+    } else if (in) {
+      labels[v] = label;
+    }
+    if ((lane() & 31) == 0 && v < n && w) {
+      valid[v >> 5] &= ~w;
+      cnt += (uint32_t)__popc(w);
     }
   }
   cnt = block_sum64(cnt, s_red);
@@ -1539,11 +1520,19 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   }
 }
 
-int launch_label_bits(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
-                      int32_t label, unsigned long long* part, const unsigned long long* min_red) {
-  int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_label_bits, dim3(g), dim3(kBlock), 0, L.stream, member, nwords, n, labels, label, part,
-                     min_red);
+int launch_commit(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
+                  const unsigned long long* min_red, unsigned long long* part) {
+  int g = grid_for(n, kBlock * 8, L.grid * 2);
+  hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member, valid, n, labels, label, min_red,
+                     nullptr, nullptr, 0u, part);
+  return g;
+}
+
+int launch_commit_batch(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels,
+                        const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part) {
+  int g = grid_for(n, kBlock * 8, L.grid * 2);
+  hipLaunchKernelGGL(k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member, valid, n, labels, 0, nullptr, piv,
+                     base, ng, part);
   return g;
 }
 
@@ -1633,14 +1622,6 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
   if (mirror) hipLaunchKernelGGL(k_seed_pivots<true>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
   else hipLaunchKernelGGL(k_seed_pivots<false>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
   hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
-}
-
-int launch_label_bits_batch(const Launch& L, const uint32_t* member, int64_t nwords, int32_t n, int32_t* labels,
-                            const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part) {
-  int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_label_bits_batch, dim3(g), dim3(kBlock), 0, L.stream, member, nwords, n, labels, piv, base,
-                     ng, part);
-  return g;
 }
 
 void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng) {

@@ -669,9 +669,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       {
         Timed t(c, MSC_K_COMMIT);
         if (fast_canon) launch_reduce(L, c->part, launch_min_bit(L, member, nw, c->part), c->ctrl->red_min);
-        np = bt ? launch_label_bits_batch(L, member, nw, n, labels, c->bpiv, c->bbase, ng, c->part)
-                : launch_label_bits(L, member, nw, n, labels, pivot, c->part, fast_canon ? c->ctrl->red_min : nullptr);
-        launch_andnot(L, c->valid, nullptr, c->valid, member, nw);
+        np = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part)
+                : launch_commit(L, member, c->valid, n, labels, pivot, fast_canon ? c->ctrl->red_min : nullptr,
+                                c->part);
         launch_reduce(L, c->part, np, c->ctrl->red_commit);
       }
     }
