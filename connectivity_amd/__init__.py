@@ -170,6 +170,11 @@ def _lib():
         "msc_gen_planted": (C.c_int, [C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_int64,
                                       C.c_uint64, C.POINTER(C.POINTER(_HostGraph)), P]),
         "msc_csr_to_device": (C.c_int, [P, C.POINTER(_Csr), C.POINTER(_Csr)]),
+        "msc_graph_ingest_text": (C.c_int, [P, C.c_char_p, C.c_int64, C.POINTER(_Csr), C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_double)]),
+        "msc_graph_read_edge_list_device": (C.c_int, [P, C.c_char_p, C.POINTER(_Csr), C.POINTER(C.c_int32),
+                                                      C.POINTER(C.c_double)]),
+        "msc_format_edge_list": (C.c_int, [C.POINTER(_HostGraph), P, C.c_int64, C.POINTER(C.c_int64)]),
         "msc_csr_device_free": (C.c_int, [P, C.POINTER(_Csr)]),
         "msc_version": (C.c_char_p, []),
         "msc_device_count": (C.c_int, []),
@@ -370,18 +375,34 @@ class Context:
 
 
 class DeviceGraph:
-    """A Graph copied into device memory (msc_csr_to_device); runs use it in place."""
+    """A Graph copied into device memory (msc_csr_to_device); runs use it in place.  Also
+    built on the device from edge-list text (:func:`ingest_text`, :func:`read_edge_device`)."""
 
-    def __init__(self, ctx: Context, g: Graph):
+    def __init__(self, ctx: Context, g: Graph | None):
         self.ctx = ctx
+        self._csr = _Csr()
+        if g is None:
+            return
         self.n, self.m = g.n, g.m
         self.max_deg_vert, self.avg_degree = g.max_deg_vert, g.avg_degree
-        self._csr = _Csr()
         host = g.csr()
         _check(_lib().msc_csr_to_device(ctx.handle, C.byref(host), C.byref(self._csr)), "msc_csr_to_device")
 
     def csr(self) -> _Csr:
         return self._csr
+
+    def to_host(self) -> Graph:
+        """Copies the four CSR arrays back (for checks)."""
+        c = self._csr
+        arrs = []
+        for ptr, count, dt in ((c.out_off, c.n + 1, np.uint32), (c.out_adj, c.m, np.int32),
+                               (c.in_off, c.n + 1, np.uint32), (c.in_adj, c.m, np.int32)):
+            a = np.empty(count, dtype=dt)
+            if count:
+                _check(_lib().msc_copy_to_host(self.ctx.handle, _ptr(a), ptr, a.nbytes), "msc_copy_to_host")
+            arrs.append(a)
+        return Graph(n=c.n, out_off=arrs[0], out_adj=arrs[1], in_off=arrs[2], in_adj=arrs[3],
+                     max_deg_vert=self.max_deg_vert, avg_degree=self.avg_degree)
 
     def free(self) -> None:
         if self._csr.where == DEVICE and self.ctx.handle:
@@ -558,6 +579,41 @@ def run_cc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, 
     lab, st = _run("msc_cc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
                    alpha, beta, out, schedule)
     return (lab, st) if stats else lab
+
+
+# ---- edge-list text on the device -----------------------------------------------------------
+def format_edge_list(g: Graph) -> bytes:
+    """The reference's edge-list text of g ("n m" then "u v" per edge, CSR order)."""
+    hg = _HostGraph(g.n, g.m, g.out_off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                    g.out_adj.ctypes.data_as(C.POINTER(C.c_int32)), g.in_off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                    g.in_adj.ctypes.data_as(C.POINTER(C.c_int32)), g.max_deg_vert, g.avg_degree)
+    size = C.c_int64()
+    _check(_lib().msc_format_edge_list(C.byref(hg), None, 0, C.byref(size)), "msc_format_edge_list")
+    buf = C.create_string_buffer(size.value + 16)
+    _check(_lib().msc_format_edge_list(C.byref(hg), buf, size.value + 16, C.byref(size)), "msc_format_edge_list")
+    return buf.raw[:size.value]
+
+
+def _ingested(ctx: Context, call, what: str) -> DeviceGraph:
+    dg = DeviceGraph(ctx, None)
+    piv, avg = C.c_int32(), C.c_double()
+    _check(call(C.byref(dg._csr), C.byref(piv), C.byref(avg)), what)
+    dg.n, dg.m = dg._csr.n, dg._csr.m
+    dg.max_deg_vert, dg.avg_degree = piv.value, avg.value
+    return dg
+
+
+def ingest_text(ctx: Context, text: bytes) -> DeviceGraph:
+    """read_edge + create_csr on the device (msc_graph_ingest_text)."""
+    return _ingested(ctx, lambda d, p, a: _lib().msc_graph_ingest_text(ctx.handle, text, len(text), d, p, a),
+                     "msc_graph_ingest_text")
+
+
+def read_edge_device(ctx: Context, path: str) -> DeviceGraph:
+    """read_edge + create_csr on the device from a file (msc_graph_read_edge_list_device)."""
+    return _ingested(ctx, lambda d, p, a: _lib().msc_graph_read_edge_list_device(ctx.handle, os.fsencode(path), d,
+                                                                                  p, a),
+                     f"msc_graph_read_edge_list_device({path})")
 
 
 # ---- batches -------------------------------------------------------------------------------

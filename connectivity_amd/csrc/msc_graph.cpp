@@ -592,6 +592,49 @@ int msc_gen_batch(int32_t num_graphs, double median_n, double sigma, int32_t min
   }
 }
 
+int msc_format_edge_list(const msc_host_graph* g, char* out, int64_t cap, int64_t* bytes) {
+  if (!g || !bytes || g->n < 0 || g->m < 0) return MSC_E_INVALID;
+  auto digits = [](uint64_t x) {
+    int d = 1;
+    while (x >= 10) {
+      x /= 10;
+      ++d;
+    }
+    return d;
+  };
+  char head[64];
+  const int hl = std::snprintf(head, sizeof(head), "%d %lld\n", g->n, (long long)g->m);
+  // bytes of each vertex's lines, then their prefix: every vertex writes its own slice
+  std::vector<int64_t> at((size_t)g->n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int32_t v = 0; v < g->n; ++v) {
+    int64_t b = 0;
+    const int dv = digits((uint64_t)v);
+    for (uint32_t j = g->out_off[v]; j < g->out_off[v + 1]; ++j) b += dv + 1 + digits((uint64_t)g->out_adj[j]) + 1;
+    at[(size_t)v + 1] = b;
+  }
+  at[0] = hl;
+  for (int32_t v = 0; v < g->n; ++v) at[(size_t)v + 1] += at[v];
+  *bytes = at[g->n];
+  if (!out) return MSC_OK;
+  if (cap < at[g->n]) return MSC_E_CAPACITY;
+  std::memcpy(out, head, (size_t)hl);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int32_t v = 0; v < g->n; ++v) {
+    char* p = out + at[v];
+    char num[16];
+    const int dv = std::snprintf(num, sizeof(num), "%d", v);
+    for (uint32_t j = g->out_off[v]; j < g->out_off[v + 1]; ++j) {
+      std::memcpy(p, num, (size_t)dv);
+      p += dv;
+      *p++ = ' ';
+      p += std::snprintf(p, 16, "%d", g->out_adj[j]);  // writes its own NUL, overwritten next
+      *p++ = '\n';
+    }
+  }
+  return MSC_OK;
+}
+
 int msc_scc_verify(int32_t n, const int32_t* labels, msc_verify* out) {
   if (n < 0 || !out || (n > 0 && !labels)) return MSC_E_INVALID;
   try {

@@ -150,6 +150,11 @@ void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* out_off, con
 int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
                     unsigned long long* part);
 void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err);
+// ---- device ingest (msc_ingest.hip) ----
+// read_edge + create_csr on the device from the edge-list text in host memory; fills *out
+// with device arrays (hipMalloc) owned by the caller.
+int ingest_text(hipStream_t stream, const char* text, int64_t bytes, msc_csr* out);
+
 // ---- batches: graph g owns vertices [base[g], base[g+1]) of a packed union CSR ----
 // piv[g] = per-graph pivot (union id, -1 = none); keys: ng u64 scratch
 void launch_batch_pivots(const Launch& L, bool cc, int32_t n, const uint32_t* oo, const uint32_t* io,

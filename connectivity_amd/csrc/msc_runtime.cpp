@@ -1021,6 +1021,60 @@ int msc_scc_pivot(const msc_csr* g, const msc_opts* o, int32_t* max_deg_vert, do
   return MSC_OK;
 }
 
+int msc_graph_ingest_text(msc_ctx* c, const char* text, int64_t bytes, msc_csr* dev, int32_t* max_deg_vert,
+                          double* avg_degree) {
+  if (!c || !text || bytes <= 0 || !dev) return MSC_E_INVALID;
+  try {
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIP_TRY(hipSetDevice(c->device));
+    int rc = ingest_text(c->stream, text, bytes, dev);
+    if (rc) return rc;
+    int32_t piv = -1;
+    if (max_deg_vert && dev->n > 0) {
+      // create_csr's pivot rule (multistep/scc_main.cpp:169-183) on the new device CSR
+      if ((rc = ensure_workspace(c, dev->n, dev->m)) || (rc = device_pivot(c, dev, false, &piv))) {
+        (void)hipFree((void*)dev->out_off);
+        (void)hipFree((void*)dev->in_off);
+        (void)hipFree((void*)dev->out_adj);
+        (void)hipFree((void*)dev->in_adj);
+        std::memset(dev, 0, sizeof(*dev));
+        return rc;
+      }
+    }
+    if (max_deg_vert) *max_deg_vert = piv;
+    if (avg_degree) *avg_degree = dev->n > 0 ? (double)dev->m / (double)dev->n : 0.0;
+    return MSC_OK;
+  } catch (const std::bad_alloc&) {
+    return MSC_E_OOM;
+  }
+}
+
+int msc_graph_read_edge_list_device(msc_ctx* c, const char* path, msc_csr* dev, int32_t* max_deg_vert,
+                                    double* avg_degree) {
+  if (!c || !path || !dev) return MSC_E_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return MSC_E_IO;
+  std::fseek(f, 0, SEEK_END);
+  long long bytes = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (bytes <= 0) {
+    std::fclose(f);
+    return MSC_E_IO;
+  }
+  char* buf = nullptr;  // pinned, so the upload runs at full link speed
+  if (hipHostMalloc((void**)&buf, (size_t)bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    std::fclose(f);
+    return MSC_E_OOM;
+  }
+  size_t got = std::fread(buf, 1, (size_t)bytes, f);
+  std::fclose(f);
+  int rc = got == (size_t)bytes ? msc_graph_ingest_text(c, buf, (int64_t)bytes, dev, max_deg_vert, avg_degree)
+                                : MSC_E_IO;
+  (void)hipHostFree(buf);
+  return rc;
+}
+
 int msc_csr_to_device(msc_ctx* c, const msc_csr* h, msc_csr* d) {
   if (!c || !h || !d || h->n < 0 || h->m < 0 || h->where != MSC_HOST) return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);

@@ -324,6 +324,23 @@ int msc_batch_pack(int32_t num_graphs, const msc_host_graph* const* graphs, msc_
 int msc_gen_batch(int32_t num_graphs, double median_n, double sigma, int32_t min_n, int32_t max_n, uint64_t seed,
                   msc_host_graph** out, int64_t* vert_base, int32_t* kinds);
 
+/* Device-side read_edge + create_csr (multistep/scc_main.cpp:94-191): `text` (host memory,
+ * `bytes` long) is the edge-list file ("n m" line, then m lines "u v"), parsed on the device
+ * with read_edge's rules (getline(' ') + atoi for the source, getline + atoi for the
+ * destination); the CSR and transposed CSR are built in device memory in create_csr's order
+ * (adjacency in input order, duplicates and self-loops kept), and the pivot / average degree
+ * follow create_csr's rule (either pointer may be NULL).  *dev receives device arrays; free
+ * them with msc_csr_device_free.  Malformed text, ids outside [0, n) or fewer than m lines:
+ * MSC_E_INVALID. */
+int msc_graph_ingest_text(msc_ctx* ctx, const char* text, int64_t bytes, msc_csr* dev, int32_t* max_deg_vert,
+                          double* avg_degree);
+/* The same from a file (read into pinned host memory, then ingested). */
+int msc_graph_read_edge_list_device(msc_ctx* ctx, const char* path, msc_csr* dev, int32_t* max_deg_vert,
+                                    double* avg_degree);
+/* The reference's edge-list text of a graph ("n m\n" then "u v\n" per edge in CSR order),
+ * formatted in parallel.  out == NULL: *bytes receives the size needed. */
+int msc_format_edge_list(const msc_host_graph* g, char* out, int64_t cap, int64_t* bytes);
+
 /* ---- device graphs --------------------------------------------------------------- */
 /* Copy a host graph into device memory of the context's device; fills *dev (where =
  * MSC_DEVICE).  Free with msc_csr_device_free. */
