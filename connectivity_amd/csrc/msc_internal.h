@@ -102,6 +102,13 @@ struct LevelArgs {
   float clock_mhz;            // wall_clock64() rate, for per-level durations
   uint32_t small_edges;       // colour levels over at most this many edges run inside the closer
   int32_t* cmin;              // canonical labels in flight: min member per colour root (null: off)
+  int32_t op;                 // Op of this search or sweep (books its algorithmic bytes)
+};
+
+// The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or
+// two (the forward and backward searches of one pivot, side by side).
+struct ReachArgs {
+  LevelArgs s[2];
 };
 
 struct Launch {
@@ -123,8 +130,12 @@ void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, 
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
 // complete trim: every vertex valid and unlabelled, control block reset (round 1 examines all)
 void launch_trim2_init(const Launch& L, const LevelArgs& a);
+// colour sweeps (OP_PUSH, OP_CBW) and complete-trim rounds (OP_TRIM2)
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a);  // work only
 void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  // one workgroup
+// reach searches (OP_FW, OP_BW, OP_CC): p.s[0 .. ns) advance by one level together
+void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns);  // work only
+void launch_close_reach(const Launch& L, const ReachArgs& p, int ns);        // one workgroup
 // labels[v] = label and valid &= ~member for every set bit of member; part col 0 = count.
 // With min_red, the label is the smallest member found by launch_min_bit
 // (0xffffffff - min_red[3]).

@@ -275,49 +275,6 @@ __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next,
   f[11] = (float)c->phase;
 }
 
-// Wave 0 evaluates the MLP from weights staged in LDS; returns the argmax action.
-__device__ int mlp_action(const float* blob, const float* feat, float* s_h) {
-  const float* W1 = blob;
-  const float* b1 = W1 + kMlpIn * kMlpH;
-  const float* W2 = b1 + kMlpH;
-  const float* b2 = W2 + kMlpH * kMlpH;
-  const float* W3 = b2 + kMlpH;
-  const float* b3 = W3 + kMlpH * kMlpOut;
-  const float* mean = blob + kMlpParams;
-  const float* sd = mean + kMlpIn;
-  int l = lane();
-  if (l < kMlpH) {
-    float acc = b1[l];
-    for (int i = 0; i < kMlpIn; ++i) acc += W1[l * kMlpIn + i] * ((feat[i] - mean[i]) / sd[i]);
-    s_h[l] = fmaxf(acc, 0.0f);
-  }
-  __builtin_amdgcn_wave_barrier();
-  float h2 = 0.0f;
-  if (l < kMlpH) {
-    float acc = b2[l];
-    for (int i = 0; i < kMlpH; ++i) acc += W2[l * kMlpH + i] * s_h[i];
-    h2 = fmaxf(acc, 0.0f);
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (l < kMlpH) s_h[l] = h2;
-  __builtin_amdgcn_wave_barrier();
-  float zo = -3.0e38f;
-  if (l < kMlpOut) {
-    float acc = b3[l];
-    for (int i = 0; i < kMlpH; ++i) acc += W3[l * kMlpH + i] * s_h[i];
-    zo = acc;
-  }
-  float z0 = __shfl(zo, 0, 64), z1 = __shfl(zo, 1, 64), z2 = __shfl(zo, 2, 64);
-  int best = 0;
-  float bv = z0;
-  if (z1 > bv) {
-    bv = z1;
-    best = 1;
-  }
-  if (z2 > bv) best = 2;
-  return best;
-}
-
 // ------------------------------------------------------------------------------------
 // kernels
 
@@ -648,148 +605,227 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
   stage_flush(S, N, a.ctrl, kStage - kChunk);  // also the end-of-chunk barrier
 }
 
-// One level of a search or a colour sweep (work only; a closing kernel ends it).
-// Top-down levels: edge-balanced expansion of the current frontier, one chunk per
-// workgroup at a time; workgroups without a chunk exit at once (the closer sums only the
-// rows of the ones that ran).  Bottom-up levels (reach ops only): every set bit of `cand`
-// is a candidate; the first parent that is allowed and no longer claimable claims it
-// (multistep/scc_fwbw.cpp:118-148).  part row = (examined, inspected, raised, 0).
+// One bottom-up word group of a reach search (multistep/scc_fwbw.cpp:118-148): every set bit
+// of cand words [wb, wb + kBW) is a candidate, claimed when some vertex of its up-lists is
+// visited (allowed and no longer claimable, the reference's `fw[in] && valid[in]`).  The
+// workgroup takes the candidates in rounds of kR per thread; each candidate's first up-list
+// entry is checked with all loads of the thread's candidates in flight together, and most
+// candidates are settled there (on R-MAT the first in-neighbour is already visited ~90% of
+// the time); the rest scan on.  Claims collect in LDS word masks and are written back with
+// one plain store per word: no other workgroup clears bits of these words during a
+// bottom-up level.  Every thread calls; claims are staged for frontier N.
+template <bool MIRROR>
+__device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, int64_t wb, Stage& S, BitIter& B,
+                                         uint32_t* s_claim, unsigned long long& my_examined,
+                                         unsigned long long& my_inspected) {
+  constexpr int kR = 4;
+  const int t = threadIdx.x;
+  s_claim[t] = 0u;
+  const uint32_t total = bits_load(B, a.cand, a.nwords, wb);  // barrier: s_claim cleared
+  for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
+    uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
+    bool has[kR], hit[kR], first[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      uint32_t k = k0 + (uint32_t)(r * kBlock + t);
+      has[r] = k < total;
+      v[r] = has[r] ? bits_pick(B, wb, k) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      beg[r] = end[r] = beg2[r] = end2[r] = 0u;
+      if (has[r]) {
+        beg[r] = a.up_off[v[r]];
+        end[r] = a.up_off[v[r] + 1];
+        if (MIRROR) {
+          beg2[r] = a.up_off2[v[r]];
+          end2[r] = a.up_off2[v[r] + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      u[r] = 0u;
+      first[r] = false;
+      if (has[r]) {
+        if (beg[r] < end[r]) {
+          u[r] = (uint32_t)a.up_adj[beg[r]++];
+          first[r] = true;
+        } else if (MIRROR && beg2[r] < end2[r]) {
+          u[r] = (uint32_t)a.up_adj2[beg2[r]++];
+          first[r] = true;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
+      wa[r] = first[r] ? a.allow[u[r] >> 5] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      hit[r] = false;
+      if (!has[r]) continue;
+      ++my_examined;
+      if (first[r]) {
+        ++my_inspected;
+        uint32_t b = 1u << (u[r] & 31u);
+        hit[r] = !(wc[r] & b) && (wa[r] & b);
+      }
+      for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
+        ++my_inspected;
+        uint32_t x = (uint32_t)a.up_adj[j];
+        hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
+      }
+      if (MIRROR)
+        for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
+          ++my_inspected;
+          uint32_t x = (uint32_t)a.up_adj2[j];
+          hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
+        }
+      if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      uint32_t deg = hit[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]) : 0u;
+      stage_push(S, hit[r], v[r], deg);
+    }
+    stage_flush(S, N, a.ctrl, kStage - kBlock * kR);  // also the barrier before the next round
+  }
+  // after the last round's barrier every claim of this group is in s_claim; each thread
+  // reads only its own word and mask, so the next group's loads need no barrier first
+  if (s_claim[t]) a.cand[wb + t] = B.word[t] & ~s_claim[t];
+}
+
+// One colour sweep level (push or per-colour backward; work only, k_close_color ends it):
+// edge-balanced expansion of the current frontier, one chunk per workgroup at a time;
+// workgroups without a chunk exit at once (the closer sums only the rows of the ones that
+// ran).  part row = (0, 0, raised, 0).
 template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
-  constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
+  static_assert(OP == OP_PUSH || OP == OP_CBW, "colour ops only; searches run in k_reach");
   __shared__ Stage S;
   __shared__ ChunkLds<OP, MIRROR> L;
   __shared__ unsigned long long s_red[kBlock / 64];
-  __shared__ BitIter B;
-  __shared__ uint32_t s_claim[kBW];
 
   Ctrl* c = a.ctrl;
-  if (c->done) return;  // uniform: the search already ended
+  if (c->done) return;  // uniform: the sweep already ended
   const uint32_t cur = c->cur;
   const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-  const bool bu = REACH && c->bottom_up != 0u;
   const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
-  if (!bu && blockIdx.x >= nchunks) return;  // uniform: no chunk for this workgroup
+  if (blockIdx.x >= nchunks) return;  // uniform: no chunk for this workgroup
   const Frontier F = cur ? a.fr1 : a.fr0;
   const Frontier N = cur ? a.fr0 : a.fr1;
   uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
   uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
   const int t = threadIdx.x;
-  unsigned long long my_examined = 0, my_inspected = 0, my_raised = 0;
+  unsigned long long my_raised = 0;
   if (t == 0) {
     S.cnt = 0u;
     if (a.recs) atomicMin(&c->t_start, (unsigned long long)wall_clock64());
   }
   __syncthreads();
-
-  if (!bu) {
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
-      td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
-  } else if (REACH) {
-    // A workgroup owns kBW words of `cand` per iteration and takes their candidates in
-    // rounds of kR per thread.  Each candidate's first up-list entry is checked with all
-    // loads of the thread's candidates in flight together; most candidates are settled
-    // there (on R-MAT the first in-neighbour is already visited ~90% of the time), the
-    // rest scan on.  A parent is a visited vertex: allowed and no longer claimable.
-    // Claims collect in LDS word masks and are written back with one plain store per
-    // word: no other workgroup clears bits of these words during a bottom-up level.
-    constexpr int kR = 4;
-    for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
-      s_claim[t] = 0u;
-      const uint32_t total = bits_load(B, a.cand, a.nwords, wb);  // barrier: s_claim cleared
-      for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
-        uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
-        bool has[kR], hit[kR], first[kR];
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          uint32_t k = k0 + (uint32_t)(r * kBlock + t);
-          has[r] = k < total;
-          v[r] = has[r] ? bits_pick(B, wb, k) : 0u;
-        }
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          beg[r] = end[r] = beg2[r] = end2[r] = 0u;
-          if (has[r]) {
-            beg[r] = a.up_off[v[r]];
-            end[r] = a.up_off[v[r] + 1];
-            if (MIRROR) {
-              beg2[r] = a.up_off2[v[r]];
-              end2[r] = a.up_off2[v[r] + 1];
-            }
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          u[r] = 0u;
-          first[r] = false;
-          if (has[r]) {
-            if (beg[r] < end[r]) {
-              u[r] = (uint32_t)a.up_adj[beg[r]++];
-              first[r] = true;
-            } else if (MIRROR && beg2[r] < end2[r]) {
-              u[r] = (uint32_t)a.up_adj2[beg2[r]++];
-              first[r] = true;
-            }
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
-          wa[r] = first[r] ? a.allow[u[r] >> 5] : 0u;
-        }
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          hit[r] = false;
-          if (!has[r]) continue;
-          ++my_examined;
-          if (first[r]) {
-            ++my_inspected;
-            uint32_t b = 1u << (u[r] & 31u);
-            hit[r] = !(wc[r] & b) && (wa[r] & b);
-          }
-          for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
-            ++my_inspected;
-            uint32_t x = (uint32_t)a.up_adj[j];
-            hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
-          }
-          if (MIRROR)
-            for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
-              ++my_inspected;
-              uint32_t x = (uint32_t)a.up_adj2[j];
-              hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
-            }
-          if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
-        }
-#pragma unroll
-        for (int r = 0; r < kR; ++r) {
-          uint32_t deg = hit[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]) : 0u;
-          stage_push(S, hit[r], v[r], deg);
-        }
-        stage_flush(S, N, c, kStage - kBlock * kR);  // also the barrier before the next round
-      }
-      // after the last round's barrier every claim of this group is in s_claim; each thread
-      // reads only its own word and mask, so the next group's loads need no barrier first
-      if (s_claim[t]) a.cand[wb + t] = B.word[t] & ~s_claim[t];
-    }
-  }
+  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
+    td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
   stage_flush(S, N, c, 0u);
-
-  unsigned long long ex = block_sum64(my_examined, s_red);
-  unsigned long long in = block_sum64(my_inspected, s_red);
   unsigned long long ra = block_sum64(my_raised, s_red);
   if (t == 0) {
     unsigned long long* p = a.part + 4 * blockIdx.x;
-    p[0] = ex;
-    p[1] = in;
+    p[0] = 0;
+    p[1] = 0;
     p[2] = ra;
     p[3] = 0;
   }
 }
 
-// Rows of the partials array written by the level that just ran: every workgroup of a
-// bottom-up level, only those that had a chunk in a top-down level.
-__device__ __forceinline__ int active_rows(int nparts, bool bu, uint32_t q_edges) {
-  if (bu) return nparts;
+// Work units of a reach search's coming level: bitmap word groups (bottom-up) or
+// kChunk-edge chunks of the frontier (top-down); none once the search is done.
+__device__ __forceinline__ uint32_t reach_units(const Ctrl* c, int64_t nwords) {
+  if (c->done) return 0u;
+  if (c->bottom_up) return (uint32_t)((nwords + kBW - 1) / kBW);
+  return (c->q_edges + kChunk - 1) / kChunk;
+}
+
+// The units [first, units) of one reach search, stepping by the grid, then a flush of the
+// claims staged for it.  Every thread calls (uniform arguments).
+template <bool MIRROR>
+__device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first, uint32_t units, Stage& S,
+                                             ChunkLds<OP_FW, MIRROR>& L, BitIter& B, uint32_t* s_claim,
+                                             unsigned long long& ex, unsigned long long& in) {
+  Ctrl* c = a.ctrl;
+  const uint32_t cur = c->cur, q_size = c->q_size, q_edges = c->q_edges;
+  const bool bu = c->bottom_up != 0u;
+  const Frontier F = cur ? a.fr1 : a.fr0;
+  const Frontier N = cur ? a.fr0 : a.fr1;
+  unsigned long long unused = 0ull;
+  if (bu) {
+    for (uint32_t lu = first; lu < units; lu += gridDim.x)
+      bu_group<MIRROR>(a, N, (int64_t)lu * kBW, S, B, s_claim, ex, in);
+  } else {
+    for (uint32_t lu = first; lu < units; lu += gridDim.x)
+      td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
+  }
+  stage_flush(S, N, c, 0u);
+}
+
+// One level of NS reach searches (FW / BW / CC) in a single launch.  The searches' work
+// units are laid end to end (search 0's, then search 1's) and the grid strides over the
+// union, so a forward and a backward search of one pivot share every launch: half the
+// launches and closers of running them one after the other, and no closer waits for a
+// compute unit behind the other search's level.  Each search keeps its own direction,
+// control block, frontiers and bitmaps; a workgroup flushes the claims staged for one
+// search before it moves on to the other's units.  Workgroups without a unit exit at once;
+// every other one writes one partial row per search, (examined, inspected, 0, 0).
+template <int NS, bool MIRROR>
+__global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
+  __shared__ Stage S;
+  __shared__ ChunkLds<OP_FW, MIRROR> L;
+  __shared__ unsigned long long s_red[kBlock / 64];
+  __shared__ BitIter B;
+  __shared__ uint32_t s_claim[kBW];
+
+  const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
+  const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
+  const uint32_t b = blockIdx.x, G = gridDim.x;
+  if (b >= u0 + u1) return;  // uniform
+  const int t = threadIdx.x;
+  if (t == 0) {
+    S.cnt = 0u;
+    if (u0 && p.s[0].recs) atomicMin(&p.s[0].ctrl->t_start, (unsigned long long)wall_clock64());
+    if (NS == 2 && u1 && p.s[1].recs) atomicMin(&p.s[1].ctrl->t_start, (unsigned long long)wall_clock64());
+  }
+  __syncthreads();
+  unsigned long long ex0 = 0ull, in0 = 0ull, ex1 = 0ull, in1 = 0ull;
+  if (b < u0) reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, ex0, in0);
+  if (NS == 2) {
+    // this workgroup's first unit of the union at or past u0, in search 1's numbering
+    const uint32_t first = b >= u0 ? b - u0 : b + (u0 - b + G - 1) / G * G - u0;
+    if (first < u1) reach_search<MIRROR>(p.s[1], first, u1, S, L, B, s_claim, ex1, in1);
+  }
+  unsigned long long e = block_sum64(ex0, s_red), i = block_sum64(in0, s_red);
+  if (t == 0) {
+    unsigned long long* r = p.s[0].part + 4 * b;
+    r[0] = e;
+    r[1] = i;
+    r[2] = 0;
+    r[3] = 0;
+  }
+  if (NS == 2) {
+    e = block_sum64(ex1, s_red);
+    i = block_sum64(in1, s_red);
+    if (t == 0) {
+      unsigned long long* r = p.s[1].part + 4 * b;
+      r[0] = e;
+      r[1] = i;
+      r[2] = 0;
+      r[3] = 0;
+    }
+  }
+}
+
+// Rows of the partials array written by the colour level that just ran: those of the
+// workgroups that had a chunk.
+__device__ __forceinline__ int active_rows(int nparts, uint32_t q_edges) {
   uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
   return (int)min((uint32_t)nparts, nchunks);
 }
@@ -844,37 +880,101 @@ __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int a
   if (act == 2) c->abandoned = 1u;  // the host discards this search's result
 }
 
-// Closes a search level (one workgroup): sums the partials, applies the direction rule --
-// the reference's (multistep/scc_fwbw.cpp:168-195, cc_bfs.cpp:167-194), the MLP, or a
-// forced schedule -- and prepares the next level.
-__global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int nparts) {
-  __shared__ unsigned long long s_red[kBlock / 64];
-  __shared__ float s_w[kMlpBlob];
-  __shared__ float s_feat[kMlpIn];
-  __shared__ float s_h[kMlpH];
-  __shared__ int s_act;
-  Ctrl* c = a.ctrl;
-  if (c->done) return;
-  const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-  const bool bu = c->bottom_up != 0u;
-  const int rows = active_rows(nparts, bu, q_edges);
-  unsigned long long ex = 0, in = 0, ra = 0;
-  for (int i = threadIdx.x; i < rows; i += kBlock) {
-    ex += a.part[4 * i];
-    in += a.part[4 * i + 1];
-    ra += a.part[4 * i + 2];
+// One lane's share of the policy MLP, held in registers: lane l < kMlpH owns hidden unit l
+// of both layers, lane l < kMlpOut output l, lane l < kMlpIn the standardisation of
+// feature l.
+struct MlpLane {
+  float w1[kMlpIn], w2[kMlpH], w3[kMlpH];
+  float b1, b2, b3, mean, sd;
+};
+
+__device__ __forceinline__ void mlp_lane_load(const float* blob, MlpLane& m) {
+  const float* W1 = blob;
+  const float* B1 = W1 + kMlpIn * kMlpH;
+  const float* W2 = B1 + kMlpH;
+  const float* B2 = W2 + kMlpH * kMlpH;
+  const float* W3 = B2 + kMlpH;
+  const float* B3 = W3 + kMlpH * kMlpOut;
+  const float* mean = blob + kMlpParams;
+  const float* sd = mean + kMlpIn;
+  const int l = lane();
+  const int h = l < kMlpH ? l : 0, o = l < kMlpOut ? l : 0, f = l < kMlpIn ? l : 0;
+#pragma unroll
+  for (int i = 0; i < kMlpIn; ++i) m.w1[i] = W1[h * kMlpIn + i];
+#pragma unroll
+  for (int i = 0; i < kMlpH; ++i) m.w2[i] = W2[h * kMlpH + i];
+#pragma unroll
+  for (int i = 0; i < kMlpH; ++i) m.w3[i] = W3[o * kMlpH + i];
+  m.b1 = B1[h];
+  m.b2 = B2[h];
+  m.b3 = B3[o];
+  m.mean = mean[f];
+  m.sd = sd[f];
+}
+
+// The MLP's action for the raw features in s_feat (whole wave; same arithmetic, term
+// order and argmax as the training code's forward pass).  s_z / s_h: wave-private LDS.
+__device__ __forceinline__ int mlp_eval(const MlpLane& m, const float* s_feat, float* s_z, float* s_h) {
+  const int l = lane();
+  if (l < kMlpIn) s_z[l] = (s_feat[l] - m.mean) / m.sd;
+  __builtin_amdgcn_wave_barrier();
+  float h1 = 0.0f;
+  if (l < kMlpH) {
+    float acc = m.b1;
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) acc += m.w1[i] * s_z[i];
+    h1 = fmaxf(acc, 0.0f);
+    s_h[l] = h1;
   }
-  ex = block_sum64(ex, s_red);
-  in = block_sum64(in, s_red);
-  ra = block_sum64(ra, s_red);
-  const bool learned = a.policy == MSC_POLICY_LEARNED && a.mlp;
-  const unsigned long long p = c->next_packed;
-  const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
-  if (threadIdx.x == 0) {
+  __builtin_amdgcn_wave_barrier();
+  float h2 = 0.0f;
+  if (l < kMlpH) {
+    float acc = m.b2;
+#pragma unroll
+    for (int i = 0; i < kMlpH; ++i) acc += m.w2[i] * s_h[i];
+    h2 = fmaxf(acc, 0.0f);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (l < kMlpH) s_h[l] = h2;
+  __builtin_amdgcn_wave_barrier();
+  float zo = -3.0e38f;
+  if (l < kMlpOut) {
+    float acc = m.b3;
+#pragma unroll
+    for (int i = 0; i < kMlpH; ++i) acc += m.w3[i] * s_h[i];
+    zo = acc;
+  }
+  float z0 = __shfl(zo, 0, 64), z1 = __shfl(zo, 1, 64), z2 = __shfl(zo, 2, 64);
+  int best = 0;
+  float bv = z0;
+  if (z1 > bv) {
+    bv = z1;
+    best = 1;
+  }
+  if (z2 > bv) best = 2;
+  return best;
+}
+
+// Closes one reach search's level; one whole wave calls it.  Lane 0 books the level and
+// applies the direction rule -- the reference's (multistep/scc_fwbw.cpp:168-195,
+// cc_bfs.cpp:167-194), the MLP (when `learned`), or a forced schedule.
+__device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned long long in, bool learned,
+                             const MlpLane& m, float* s_feat, float* s_z, float* s_h) {
+  Ctrl* c = a.ctrl;
+  int act = 0;
+  uint32_t q_size = 0, q_edges = 0, next = 0, nedges = 0;
+  bool bu = false;
+  if (lane() == 0) {
+    q_size = c->q_size;
+    q_edges = c->q_edges;
+    bu = c->bottom_up != 0u;
+    const unsigned long long np = c->next_packed;
+    next = (uint32_t)(np >> 32);
+    nedges = (uint32_t)np;
     c->found += next;
     c->level += 1;
     c->closes += 1;
-    int act = bu ? 1 : 0;
+    act = bu ? 1 : 0;
     if (!c->switched_back && !bu) {
       double edges_frontier = (double)next * a.avg;
       double edges_remainder = (double)((int64_t)a.n_total - c->found) * a.avg;
@@ -883,24 +983,73 @@ __global__ void __launch_bounds__(kBlock) k_close(LevelArgs a, int op, int npart
       act = 0;
       c->switched_back = 1;
     }
-    s_act = act;
     level_features(c, a, next, nedges, bu, s_feat);
   }
-  if (learned)
-    for (int i = threadIdx.x; i < kMlpBlob; i += blockDim.x) s_w[i] = a.mlp[i];
-  __syncthreads();
-  if (learned && threadIdx.x < 64) {
-    int act = mlp_action(s_w, s_feat, s_h);
-    if (threadIdx.x == 0) s_act = act;
+  __builtin_amdgcn_wave_barrier();
+  if (learned) {
+    int mact = mlp_eval(m, s_feat, s_z, s_h);
+    if (lane() == 0) act = mact;
   }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  int act = s_act;
+  if (lane() != 0) return;
   if (a.sched && c->level <= (uint32_t)MSC_SCHED_LEVELS) {
     int f = a.sched[c->phase * MSC_SCHED_LEVELS + c->level - 1];
     if (f >= 0 && f <= 2) act = f;
   }
-  finish_level(c, a, op, bu, act, q_size, q_edges, ex, in, ra, next, nedges, s_feat);
+  finish_level(c, a, a.op, bu, act, q_size, q_edges, ex, in, 0ull, next, nedges, s_feat);
+}
+
+// Closes a level of NS reach searches (one workgroup): all threads sum the partial rows,
+// then wave s closes search s, the searches in parallel.  The level kernel wrote rows
+// [0, min(nparts, units)) of every search's partials, where units counts both searches'
+// work units of that level (unchanged until this closer).  The MLP weights are loaded into
+// registers first, so their latency overlaps the partial sums.
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts) {
+  __shared__ unsigned long long s_sum[2][2][kBlock / 64];
+  __shared__ float s_feat[2][kMlpIn];
+  __shared__ float s_z[2][kMlpIn];
+  __shared__ float s_h[2][kMlpH];
+  const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
+  const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
+  if (u0 + u1 == 0u) return;  // uniform: every search is done
+  const int w = wave_id();
+  const bool learned0 = u0 && p.s[0].policy == MSC_POLICY_LEARNED && p.s[0].mlp;
+  const bool learned1 = NS == 2 && u1 && p.s[1].policy == MSC_POLICY_LEARNED && p.s[1].mlp;
+  MlpLane m;
+  if (w == 0 && learned0) mlp_lane_load(p.s[0].mlp, m);
+  if (NS == 2 && w == 1 && learned1) mlp_lane_load(p.s[1].mlp, m);
+  const int rows = (int)min((uint32_t)nparts, u0 + u1);
+  unsigned long long e0 = 0, i0 = 0, e1 = 0, i1 = 0;
+  for (int r = threadIdx.x; r < rows; r += kBlock) {
+    e0 += p.s[0].part[4 * r];
+    i0 += p.s[0].part[4 * r + 1];
+    if (NS == 2) {
+      e1 += p.s[1].part[4 * r];
+      i1 += p.s[1].part[4 * r + 1];
+    }
+  }
+  e0 = wave_sum64(e0);
+  i0 = wave_sum64(i0);
+  if (NS == 2) {
+    e1 = wave_sum64(e1);
+    i1 = wave_sum64(i1);
+  }
+  if (lane() == 0) {
+    s_sum[0][0][w] = e0;
+    s_sum[0][1][w] = i0;
+    s_sum[1][0][w] = e1;
+    s_sum[1][1][w] = i1;
+  }
+  __syncthreads();  // also: every thread has read the control blocks
+  if (w < NS && (w == 0 ? u0 : u1)) {
+    unsigned long long ex = 0, in = 0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      ex += s_sum[w][0][k];
+      in += s_sum[w][1][k];
+    }
+    if (w == 0) close_search(p.s[0], ex, in, learned0, m, s_feat[0], s_z[0], s_h[0]);
+    else close_search(p.s[1], ex, in, learned1, m, s_feat[1], s_z[1], s_h[1]);
+  }
 }
 
 // Closes a colour sweep (push or per-colour backward level), then keeps going in this one
@@ -924,7 +1073,7 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
   const int t = threadIdx.x;
   {
     const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-    const int rows = active_rows(nparts, false, q_edges);
+    const int rows = active_rows(nparts, q_edges);
     unsigned long long ra = 0;
     for (int i = t; i < rows; i += kBlock) ra += a.part[4 * i + 2];
     ra = block_sum64(ra, s_red);
@@ -1494,9 +1643,6 @@ void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   dim3 g((unsigned)L.grid), b(kBlock);
   switch (op) {
-    case OP_FW: hipLaunchKernelGGL((k_level<OP_FW, false>), g, b, 0, L.stream, a); break;
-    case OP_BW: hipLaunchKernelGGL((k_level<OP_BW, false>), g, b, 0, L.stream, a); break;
-    case OP_CC: hipLaunchKernelGGL((k_level<OP_CC, true>), g, b, 0, L.stream, a); break;
     case OP_PUSH:
       if (mirror) hipLaunchKernelGGL((k_level<OP_PUSH, true>), g, b, 0, L.stream, a);
       else hipLaunchKernelGGL((k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
@@ -1516,8 +1662,20 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
       break;
     case OP_CBW: hipLaunchKernelGGL((k_close_color<OP_CBW, false>), g, b, 0, L.stream, a, L.grid); break;
     case OP_TRIM2: hipLaunchKernelGGL(k_trim2_close, g, dim3(64), 0, L.stream, a.ctrl); break;
-    default: hipLaunchKernelGGL(k_close, g, b, 0, L.stream, a, op, L.grid); break;
+    default: break;
   }
+}
+
+void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns) {
+  dim3 g((unsigned)L.grid), b(kBlock);
+  if (ns == 2) hipLaunchKernelGGL((k_reach<2, false>), g, b, 0, L.stream, p);
+  else if (mirror) hipLaunchKernelGGL((k_reach<1, true>), g, b, 0, L.stream, p);
+  else hipLaunchKernelGGL((k_reach<1, false>), g, b, 0, L.stream, p);
+}
+
+void launch_close_reach(const Launch& L, const ReachArgs& p, int ns) {
+  if (ns == 2) hipLaunchKernelGGL(k_close_reach<2>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
+  else hipLaunchKernelGGL(k_close_reach<1>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
 }
 
 int launch_commit(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,

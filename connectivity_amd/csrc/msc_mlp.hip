@@ -1,7 +1,7 @@
 // FP32 MLP of the learned direction / phase policy: 12 -> 32 -> 32 -> 3, ReLU, softmax
 // cross-entropy.  Training (forward, backward, Adam) and batched inference kernels, and the
 // C ABI entry points that drive them.  The same weights are evaluated inside the level
-// closer (msc_kernels.hip: mlp_action) with identical arithmetic order per output.
+// closer (msc_kernels.hip: mlp_eval) with identical arithmetic order per output.
 //
 // Layout of a blob (kMlpBlob floats): W1[32][12] b1[32] W2[32][32] b2[32] W3[3][32] b3[3]
 // mean[12] std[12].  Inputs are standardised as (x - mean) / std before layer 1.

@@ -35,8 +35,6 @@ struct msc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  hipStream_t stream2 = nullptr;  // the backward search runs here beside the forward one
-  hipEvent_t ev_fork = nullptr;   // stream2 waits for the set-up on `stream`
   int grid = 1024;
   int cus = 256;
   msc_tuning tune{4, 4096, 4096, 2, 32, 1};
@@ -49,10 +47,10 @@ struct msc_ctx {
   int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
   Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the concurrent backward search's
   unsigned long long* part = nullptr;   // kMaxParts x 4 per-workgroup partials
-  unsigned long long* part2 = nullptr;  // the same for the concurrent backward search
+  unsigned long long* part2 = nullptr;  // the same for the backward search beside the forward one
   uint32_t* err = nullptr;              // validation flag
   Ctrl* ctrl = nullptr;
-  Ctrl* ctrl2 = nullptr;                // control block of the concurrent backward search
+  Ctrl* ctrl2 = nullptr;                // that backward search's control block: ctrl + 1
   msc_level_rec* recs = nullptr;
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
@@ -71,8 +69,8 @@ struct msc_ctx {
   void* in_buf = nullptr;
   size_t in_bytes = 0;
   // host side
-  Ctrl* ctrl_host = nullptr;  // pinned
-  Ctrl* ctrl_host2 = nullptr;  // pinned
+  Ctrl* ctrl_host = nullptr;   // pinned copies of ctrl and ctrl2 (adjacent, like on device)
+  Ctrl* ctrl_host2 = nullptr;  // ctrl_host + 1
   uint32_t* pinned_u32 = nullptr;
   std::vector<Ev> events;
   size_t ev_used = 0;
@@ -117,12 +115,11 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr1
       (size_t)kMaxParts * 4 * 8,                                // partials
       256,                                                      // err
-      sizeof(Ctrl),                                             // ctrl
+      2 * sizeof(Ctrl),                                         // ctrl, ctrl2
       (size_t)kRecCap * sizeof(msc_level_rec),                  // records (two halves)
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr2
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr3
-      (size_t)kMaxParts * 4 * 8,                                // partials of stream2
-      sizeof(Ctrl),                                             // ctrl2
+      (size_t)kMaxParts * 4 * 8,                                // partials of the backward search
   };
   size_t total = 0;
   for (size_t s : sizes) total += align256(s);
@@ -154,7 +151,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->fr2 = Frontier{(int32_t*)parts[19], (uint32_t*)parts[20], (uint32_t*)parts[21]};
   c->fr3 = Frontier{(int32_t*)parts[22], (uint32_t*)parts[23], (uint32_t*)parts[24]};
   c->part2 = (unsigned long long*)parts[25];
-  c->ctrl2 = (Ctrl*)parts[26];
+  c->ctrl2 = c->ctrl + 1;
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -200,8 +197,9 @@ struct Timed {
 
 Launch launch_of(const msc_ctx* c) { return Launch{c->stream, c->grid}; }
 
+// Both control blocks (ctrl, ctrl2) back to their pinned host copies, one copy.
 int read_ctrl(msc_ctx* c) {
-  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, 2 * sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MSC_OK;
 }
@@ -217,77 +215,62 @@ int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
   return MSC_OK;
 }
 
-// One level loop: its op, arguments (which name its control block and buffers), stream and
-// the pinned host copy of its control block.
-struct Search {
-  int op;
-  bool mirror;
-  LevelArgs a;
-  hipStream_t stream;
-  Ctrl* host;
-  int kcls;
-  bool count_left;  // after each batch, count the valid vertices into ctrl->red_left
-};
-
-// Runs level launches (work + closing kernel) of one or two searches until their control
-// blocks say done; each search's host copy is current when this returns.  Two searches run
-// on their own streams, batch for batch, so their levels overlap on the device and share
-// the host round trips.  Every search and max-label fixpoint settles within n + 1 levels,
-// so more launches than that mean a fault.
-int run_searches(msc_ctx* c, Search* ss, int k, msc_stats* s, int first_batch) {
-  bool done[2] = {false, false};
+// Level loop of one colour sweep or complete trim, or of one or two reach searches that
+// advance together, one k_reach launch per level (the forward and backward searches of a
+// pivot).  Level + closer launches are enqueued in growing batches; after each batch the
+// control blocks come back with one copy (a pair's blocks are adjacent: as[1].ctrl ==
+// as[0].ctrl + 1) and one synchronisation, and the host copies are current when this
+// returns.  Every search and max-label fixpoint settles within n + 1 levels, so more
+// launches than that mean a fault.
+int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, msc_stats* s, int first_batch,
+             bool count_left = false) {
+  const bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
+  if (k < 1 || k > 2 || (k == 2 && (!reach || as[1].ctrl != as[0].ctrl + 1))) return MSC_E_INVALID;
+  ReachArgs p{};
   for (int j = 0; j < k; ++j) {
-    LevelArgs& a = ss[j].a;
-    a.small_edges = ss[j].op == OP_PUSH ? (uint32_t)c->tune.small_push_edges
-                    : ss[j].op == OP_CBW ? (uint32_t)c->tune.small_cbw_edges
-                                         : 0u;
+    as[j].small_edges = op == OP_PUSH ? (uint32_t)c->tune.small_push_edges
+                        : op == OP_CBW ? (uint32_t)c->tune.small_cbw_edges
+                                       : 0u;
+    p.s[j] = as[j];
   }
+  Ctrl* host = c->ctrl_host;
   int batch = first_batch > 0 ? first_batch : c->tune.first_batch;
   const int max_batch = c->tune.max_batch;
-  const int64_t limit = (int64_t)ss[0].a.n_total + 64;
+  const int64_t limit = (int64_t)as[0].n_total + 64;
   int64_t launched = 0;
+  Launch L = launch_of(c);
   for (;;) {
     if (launched > limit) return MSC_E_DEVICE;
     launched += batch;
-    for (int i = 0; i < batch; ++i)
-      for (int j = 0; j < k; ++j) {
-        if (done[j]) continue;
-        Launch L{ss[j].stream, c->grid};
-        {
-          Timed t(c, ss[j].kcls, ss[j].stream);
-          launch_level(L, ss[j].op, ss[j].mirror, ss[j].a);
-        }
-        launch_close(L, ss[j].op, ss[j].mirror, ss[j].a);
+    for (int i = 0; i < batch; ++i) {
+      {
+        Timed t(c, kcls);
+        if (reach) launch_reach(L, mirror, p, k);
+        else launch_level(L, op, mirror, as[0]);
       }
-    for (int j = 0; j < k; ++j) {
-      if (done[j]) continue;
-      Launch L{ss[j].stream, c->grid};
-      if (ss[j].count_left) {
-        // valid vertices left: read back with this copy, so a colour round needs no extra trip
-        int np = launch_count_bits(L, c->valid, ss[j].a.nwords, ss[j].a.part);
-        launch_reduce(L, ss[j].a.part, np, ss[j].a.ctrl->red_left);
-      }
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(ss[j].host, ss[j].a.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ss[j].stream));
+      if (reach) launch_close_reach(L, p, k);
+      else launch_close(L, op, mirror, as[0]);
     }
-    bool all = true;
-    for (int j = 0; j < k; ++j) {
-      if (done[j]) continue;
-      HIP_TRY(hipStreamSynchronize(ss[j].stream));
-      done[j] = ss[j].host->done != 0u;
-      all = all && done[j];
+    if (count_left) {
+      // valid vertices left: read back with this copy, so a colour round needs no extra trip
+      int np = launch_count_bits(L, c->valid, as[0].nwords, as[0].part);
+      launch_reduce(L, as[0].part, np, as[0].ctrl->red_left);
     }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host, as[0].ctrl, (size_t)k * sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     if (s) s->host_syncs += 1;
+    bool all = true;
+    for (int j = 0; j < k; ++j) all = all && host[j].done != 0u;
     if (all) break;
     batch = batch * 2 < max_batch ? batch * 2 : max_batch;
   }
   return MSC_OK;
 }
 
-int run_levels(msc_ctx* c, int op, bool mirror, const LevelArgs& a, int kcls, msc_stats* s, int first_batch,
+int run_levels(msc_ctx* c, int op, bool mirror, LevelArgs a, int kcls, msc_stats* s, int first_batch,
                bool count_left = false) {
-  Search one{op, mirror, a, c->stream, c->ctrl_host, kcls, count_left};
-  return run_searches(c, &one, 1, s, first_batch);
+  return run_loop(c, op, mirror, &a, 1, kcls, s, first_batch, count_left);
 }
 
 LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opts* o, double avg) {
@@ -564,6 +547,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     // complete trim (multistep/scc_trim.cpp:141-297): worklist rounds until nothing leaves;
     // the pivot stays the caller's choice even when it is trimmed, as in the reference
     LevelArgs t = base_args(c, &g, labels, o, avg);
+    t.op = OP_TRIM2;
     t.down_off = g.out_off; t.down_adj = g.out_adj;
     t.up_off = g.in_off; t.up_adj = g.in_adj;
     {
@@ -591,47 +575,51 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // every vertex to colouring
   if (pivot >= 0) {
     const uint32_t* member = nullptr;
-    launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
     if (!cc) {
       // FW: allow = valid, cand = b1, top-down on out-lists, bottom-up on in-lists
       LevelArgs fw = a;
+      fw.op = OP_FW;
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
       fw.allow = c->valid; fw.cand = c->b1;
-      // BW: top-down on in-lists, bottom-up on out-lists.  Run concurrently (the default) it
-      // searches the valid set on its own stream and buffers, and the SCC is FW set & BW set:
-      // a path from v to the pivot that starts in the FW set stays in it, so this equals the
-      // reference's BW restricted to the FW set (scc_fwbw.cpp:230-409).  Run after FW it is
-      // that restricted search.
+      // BW: top-down on in-lists, bottom-up on out-lists.  Searched together with FW (the
+      // default) it searches the valid set with its own control block, frontiers and
+      // bitmap, both advancing one level per k_reach launch, and the SCC is FW set & BW
+      // set: a path from v to the pivot that starts in the FW set stays in it, so this
+      // equals the reference's BW restricted to the FW set (scc_fwbw.cpp:230-409).  Run
+      // after FW it is that restricted search.
       LevelArgs bw = a;
+      bw.op = OP_BW;
       bw.down_off = g.in_off; bw.down_adj = g.in_adj;
       bw.up_off = g.out_off; bw.up_adj = g.out_adj;
-      if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
-      else launch_search_init(L, fw, pivot, 0);
       if (c->tune.concurrent_fwbw) {
         bw.allow = c->valid; bw.cand = c->b3;
         bw.ctrl = c->ctrl2; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
-        launch_andnot(L, c->b3, nullptr, c->valid, nullptr, nw);  // cand = allow
-        HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
-        HIP_TRY(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-        Launch L2{c->stream2, c->grid};
-        if (bt) launch_seed_pivots(L2, false, bw, c->bpiv, ng, 1);
-        else launch_search_init(L2, bw, pivot, 1);
-        Search both[2] = {{OP_FW, false, fw, c->stream, c->ctrl_host, MSC_K_LEVEL, false},
-                          {OP_BW, false, bw, c->stream2, c->ctrl_host2, MSC_K_LEVEL, false}};
-        if ((rc = run_searches(c, both, 2, s, 0))) return rc;
-        if (s) {
-          s->fw_levels = (int32_t)c->ctrl_host->level;
-          s->bw_levels = (int32_t)c->ctrl_host2->level;
+        launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);  // cand = allow, for both
+        if (bt) {
+          launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
+          launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
+        } else {
+          launch_search_init(L, fw, pivot, 0);
+          launch_search_init(L, bw, pivot, 1);
         }
-        if (!c->ctrl_host->abandoned && !c->ctrl_host2->abandoned) {
+        LevelArgs both[2] = {fw, bw};
+        if ((rc = run_loop(c, OP_FW, false, both, 2, MSC_K_LEVEL, s, 0))) return rc;
+        if (s) {
+          s->fw_levels = (int32_t)c->ctrl_host[0].level;
+          s->bw_levels = (int32_t)c->ctrl_host[1].level;
+        }
+        if (!c->ctrl_host[0].abandoned && !c->ctrl_host[1].abandoned) {
           launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);  // FW set
           launch_andnot(L, c->b2, nullptr, c->b2, c->b3, nw);     // & BW set
           member = c->b2;
         }
       } else {
+        launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
+        if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
+        else launch_search_init(L, fw, pivot, 0);
         if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
         if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
         if (!c->ctrl_host->abandoned) {
@@ -650,6 +638,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       }
     } else {
       LevelArgs bf = a;
+      bf.op = OP_CC;
+      launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
       bf.down_off = g.out_off; bf.down_adj = g.out_adj;
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
@@ -690,6 +680,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     }
     for (int round = 0;; ++round) {
       LevelArgs p = a;
+      p.op = OP_PUSH;
       p.nvalid = nullptr;
       p.down_off = g.out_off; p.down_adj = g.out_adj;
       {
@@ -701,6 +692,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if (s) s->color_iters += (int32_t)sweeps;
       push_batch = closes > 2 ? (closes < (uint32_t)c->tune.max_batch ? (int)closes : c->tune.max_batch) : 2;
       LevelArgs r = a;
+      r.op = OP_CBW;
       r.nvalid = nullptr;
       r.down_off = g.in_off; r.down_adj = g.in_adj;
       {
@@ -730,6 +722,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   } else {
     // one max-label fixpoint written straight into the labels (multistep/cc_color.cpp:47-175)
     LevelArgs p = a;
+    p.op = OP_PUSH;
     p.nvalid = nullptr;
     p.colors = labels;
     p.down_off = g.out_off; p.down_adj = g.out_adj;
@@ -758,8 +751,6 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   HIP_TRY(hipGetLastError());
   if ((rc = read_ctrl(c))) return rc;
   c->last_nrec = c->ctrl_host->nrec < kRecCap / 2 ? c->ctrl_host->nrec : kRecCap / 2;
-  HIP_TRY(hipMemcpyAsync(c->ctrl_host2, c->ctrl2, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
   c->last_nrec2 = c->ctrl_host2->nrec < kRecCap / 2 ? c->ctrl_host2->nrec : kRecCap / 2;
   auto t1 = std::chrono::steady_clock::now();
   if (s) {
@@ -861,15 +852,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     }
     c->own_stream = true;
   }
-  // the second stream keeps the default priority, like the first
-  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess) {
-    (void)hipGetLastError();
-    msc_ctx_destroy(c);
-    return MSC_E_DEVICE;
-  }
-  if (hipHostMalloc((void**)&c->ctrl_host, sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->ctrl_host2, sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
+  if (hipHostMalloc((void**)&c->ctrl_host, 2 * sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->sched_host, 3 * MSC_SCHED_LEVELS, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&c->small, align256((size_t)kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
@@ -877,6 +860,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     msc_ctx_destroy(c);
     return MSC_E_OOM;
   }
+  c->ctrl_host2 = c->ctrl_host + 1;
   c->mlp = (float*)c->small;
   c->sched = (int8_t*)c->small + align256((size_t)kMlpBlob * 4);
   int khz = 0;
@@ -891,7 +875,6 @@ int msc_ctx_destroy(msc_ctx* c) {
   if (!c) return MSC_E_INVALID;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   if (c->ws) (void)hipFree(c->ws);
   if (c->in_buf) (void)hipFree(c->in_buf);
   if (c->bws) (void)hipFree(c->bws);
@@ -900,9 +883,6 @@ int msc_ctx_destroy(msc_ctx* c) {
     (void)hipEventDestroy(e.b);
   }
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
-  if (c->ctrl_host2) (void)hipHostFree(c->ctrl_host2);
-  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->pinned_u32) (void)hipHostFree(c->pinned_u32);
   if (c->sched_host) (void)hipHostFree(c->sched_host);
   if (c->small) (void)hipFree(c->small);
