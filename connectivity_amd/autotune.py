@@ -1,6 +1,6 @@
 """Autotuner for a context's launch parameters (msc_tuning).
 
-The level loops have six tunables that no rule of the reference fixes and that trade
+The level loops have seven tunables that no rule of the reference fixes and that trade
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
@@ -9,7 +9,8 @@ against each other differently on shallow power-law graphs and on deep meshes:
   small_cbw_edges   the same for the per-colour backward searches;
   first_batch       levels enqueued before a loop first reads its `done` word back;
   max_batch         levels enqueued between later reads (batches double up to this);
-  concurrent_fwbw   backward search beside the forward one on a second stream (1) or after it.
+  concurrent_fwbw   backward search beside the forward one in the same launches (1) or after it;
+  search_batch      levels of a FW-BW / CC search enqueued before its first host check.
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -32,12 +33,13 @@ from . import (LABELS_CANONICAL, POLICY_LEARNED, POLICY_PLAIN, Context, DeviceGr
                gen_grid, gen_planted, gen_rmat, load_policy, run_cc, run_scc)
 
 CANDIDATES = {
-    "blocks_per_cu": [2, 4, 8],
+    "blocks_per_cu": [2, 4, 5, 6, 8],
     "small_push_edges": [0, 1024, 2048, 4096, 8192, 16384],
     "small_cbw_edges": [0, 1024, 2048, 4096, 8192, 16384],
     "first_batch": [2, 4, 8],
     "max_batch": [16, 32, 64],
     "concurrent_fwbw": [0, 1],
+    "search_batch": [2, 4, 6, 8],
 }
 
 

@@ -61,7 +61,7 @@ struct Ctrl {
   unsigned long long bu_levels;
   unsigned long long t_start;      // earliest workgroup start of the running launch
   uint32_t closes;                 // closing-kernel launches that closed a level
-  uint32_t pad_;
+  uint32_t list_len;               // vertex list written by the last seeding launch (kept by resets)
   // counts the host reads with the next control-block copy instead of a round trip each
   unsigned long long red_trim[4];    // col 0: trim survivors
   unsigned long long red_commit[4];  // col 0: vertices committed by FW-BW / the CC search
@@ -123,11 +123,14 @@ int ctx_set_policy(msc_ctx* ctx, const float* blob);  // kMlpBlob floats, valida
 // ---- kernel launchers (msc_kernels.hip) ----
 // Launchers that leave per-workgroup partials return how many workgroups wrote them.
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count);
+// valid bitmap (and copies in cand1 / cand2, when given), trimmed labels; part col 0 = survivors
 int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
-                uint32_t* valid, int32_t* labels, unsigned long long* part);
+                uint32_t* valid, uint32_t* cand1, uint32_t* cand2, int32_t* labels, unsigned long long* part);
 // out[i] = sums (cols 0..2) and max (col 3) of part[0..nparts)[i]
 void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out);
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
+// both searches of a pivot at once: a0 gets phase 0 (FW), a1 phase 1 (BW)
+void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root);
 // complete trim: every vertex valid and unlabelled, control block reset (round 1 examines all)
 void launch_trim2_init(const Launch& L, const LevelArgs& a);
 // colour sweeps (OP_PUSH, OP_CBW) and complete-trim rounds (OP_TRIM2)
@@ -136,19 +139,38 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  //
 // reach searches (OP_FW, OP_BW, OP_CC): p.s[0 .. ns) advance by one level together
 void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns);  // work only
 void launch_close_reach(const Launch& L, const ReachArgs& p, int ns);        // one workgroup
-// labels[v] = label and valid &= ~member for every set bit of member; part col 0 = count.
-// With min_red, the label is the smallest member found by launch_min_bit
-// (0xffffffff - min_red[3]).
-int launch_commit(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
-                  const unsigned long long* min_red, unsigned long long* part);
-// part col 3 = 0xffffffff - smallest set bit in each workgroup's words (0: none)
-int launch_min_bit(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
-// labels[v] = cmin[labels[v]] for every set bit v of `coloured`
-void launch_canon_colour(const Launch& L, const uint32_t* coloured, int64_t nwords, int32_t n, int32_t* labels,
-                         const int32_t* cmin);
+// Members of the pivot's component: an explicit bitmap, or valid & ~ca & ~cb, the valid
+// vertices both searches visited (ca, cb: their claimable bitmaps; cb may be null).  All
+// null: no member.
+struct MemberSrc {
+  const uint32_t* member;
+  const uint32_t* ca;
+  const uint32_t* cb;
+};
+// labels[v] = label and valid &= ~member for every member; part col 0 = count.  With
+// min_red, the label is the smallest member found by launch_min_member
+// (0xffffffff - min_red[3]).  With `seed` (the colouring's arguments), every vertex still
+// valid afterwards is seeded for colouring round 0 (colors[v] = v, queued in frontier 1 of
+// seed->ctrl, written to `list`): run launch_seed_begin on seed->ctrl first and
+// launch_seed_end after.
+int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
+                  const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
+                  int32_t* list);
+// part col 3 = 0xffffffff - smallest member in each workgroup's words (0: none)
+int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
+                      unsigned long long* part);
+// labels[v] = cmin[labels[v]] for the `count` vertices of `list`
+void launch_canon_list(const Launch& L, const int32_t* list, uint32_t count, int32_t* labels, const int32_t* cmin);
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
-void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a);
-void launch_roots(const Launch& L, const LevelArgs& a);
+// control block reset before / frontier hand-over after a seeding kernel
+void launch_seed_begin(const Launch& L, Ctrl* c, int32_t phase);
+void launch_seed_end(const Launch& L, Ctrl* c);
+// colouring round > 0: reset, seed from the previous round's list `in` (length
+// a.ctrl->list_len) into `out`, hand-over; bound >= that length sizes the grid
+void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound);
+// colour roots of this round's list (length gate->list_len) into a.ctrl's frontier, once
+// the push fixpoint of `gate` is done (else nothing); reset and hand-over included
+void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list, const Ctrl* gate, int64_t bound);
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
@@ -173,9 +195,10 @@ void launch_batch_pivots(const Launch& L, bool cc, int32_t n, const uint32_t* oo
 // fresh control block, every pivot claimed and queued (frontier buffer 1)
 void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const int32_t* piv, uint32_t ng,
                         int32_t phase);
-// the same with labels[v] = piv[graph of v]
-int launch_commit_batch(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels,
-                        const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part);
+// launch_commit with labels[v] = piv[graph of v]
+int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
+                        const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
+                        const LevelArgs* seed, int32_t* list);
 void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng);
 // run only after launch_validate passed (reads adjacency through the offsets)
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,

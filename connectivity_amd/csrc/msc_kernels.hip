@@ -196,8 +196,9 @@ __device__ __forceinline__ void stage_push(Stage& S, bool take, uint32_t w, uint
 
 // Workgroup-wide (every thread calls it): when more than `keep_below` claims are staged,
 // reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
-// offsets and chunk starts to frontier `nx`.  Always a barrier.
-__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below) {
+// offsets and chunk starts to frontier `nx` (and the ids to `list` at the same slots, when
+// given).  Always a barrier.
+__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr) {
   __syncthreads();
   const uint32_t n = S.cnt;
   __syncthreads();  // no wave may push again before every wave has read the count
@@ -239,6 +240,7 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
     uint32_t pos = bh + i, off = bl + lo;
     nx.q[pos] = S.v[i];
     nx.qo[pos] = off;
+    if (list) list[pos] = S.v[i];
     uint64_t end = (uint64_t)off + (hi - lo);
     for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch) nx.starts[ch] = pos;
   }
@@ -328,10 +330,12 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const unsigned long long* __r
 
 // Degree trim (multistep/scc_trim.cpp:89-139; kokkos_scc/scc_trim.cpp:57-140) and the CC
 // variant (multistep/cc_trim.cpp:47-97).  Each wave round covers 64 consecutive vertices,
-// whose ballot is two words of the valid bitmap; part col 0 = survivors of the workgroup.
+// whose ballot is two words of the valid bitmap (also written to the searches' claimable
+// bitmaps cand1 / cand2 when given); part col 0 = survivors of the workgroup.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __restrict__ oo,
                                                  const uint32_t* __restrict__ io, uint32_t* __restrict__ valid,
+                                                 uint32_t* __restrict__ cand1, uint32_t* __restrict__ cand2,
                                                  int32_t* __restrict__ labels, unsigned long long* __restrict__ part) {
   __shared__ unsigned long long s_red[kBlock / 64];
   const int64_t nw = ((int64_t)n + 31) / 32;
@@ -346,8 +350,13 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
     }
     unsigned long long mask = __ballot(keep);
     int64_t word = base >> 5;
-    if (lane() == 0) valid[word] = (uint32_t)mask;
-    if (lane() == 32 && word + 1 < nw) valid[word + 1] = (uint32_t)(mask >> 32);
+    if ((lane() & 31) == 0 && word + (lane() >> 5) < nw) {
+      const int64_t wi = word + (lane() >> 5);
+      const uint32_t bits = (uint32_t)(mask >> (lane() & 32));
+      valid[wi] = bits;
+      if (cand1) cand1[wi] = bits;
+      if (cand2) cand2[wi] = bits;
+    }
     kept += keep ? 1 : 0;
   }
   kept = block_sum64(kept, s_red);
@@ -357,14 +366,20 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
   }
 }
 
-// part col 3 = 0xffffffff - smallest set bit of the workgroup's words (0 when none), so
-// that k_reduce's max yields the smallest set bit overall.
-__global__ void __launch_bounds__(kBlock) k_min_bit(const uint32_t* __restrict__ bits, int64_t nwords,
-                                                    unsigned long long* __restrict__ part) {
+__device__ __forceinline__ uint32_t member_word(const MemberSrc& m, uint32_t valid_word, int64_t i) {
+  if (m.member) return m.member[i];
+  if (m.ca) return valid_word & ~m.ca[i] & (m.cb ? ~m.cb[i] : 0xffffffffu);
+  return 0u;
+}
+
+// part col 3 = 0xffffffff - smallest member in the workgroup's words (0 when none), so
+// that k_reduce's max yields the smallest member overall.
+__global__ void __launch_bounds__(kBlock) k_min_member(MemberSrc m, const uint32_t* __restrict__ valid,
+                                                       int64_t nwords, unsigned long long* __restrict__ part) {
   __shared__ unsigned long long s_red[kBlock / 64];
   unsigned long long best = 0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
-    uint32_t w = bits[i];
+    uint32_t w = member_word(m, valid[i], i);
     if (w) {
       unsigned long long key = 0xffffffffull - (unsigned long long)(i * 32 + (__ffs(w) - 1));
       best = key > best ? key : best;
@@ -381,18 +396,14 @@ __global__ void __launch_bounds__(kBlock) k_min_bit(const uint32_t* __restrict__
   }
 }
 
-// Canonical labels of the colouring phase: every vertex coloured gets the smallest member
-// of its colour root's SCC, tracked in cmin by the per-colour backward search.
-__global__ void __launch_bounds__(kBlock) k_canon_colour(const uint32_t* __restrict__ coloured, int64_t nwords,
-                                                         int32_t n, int32_t* __restrict__ labels,
-                                                         const int32_t* __restrict__ cmin) {
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
-    uint32_t w = coloured[i];
-    while (w) {
-      int64_t v = i * 32 + (__ffs(w) - 1);
-      w &= w - 1u;
-      if (v < n) labels[v] = cmin[labels[v]];
-    }
+// Canonical labels of the colouring phase: every vertex coloured (the list of colouring
+// round 0) gets the smallest member of its colour root's SCC, tracked in cmin by the
+// per-colour backward search.
+__global__ void __launch_bounds__(kBlock) k_canon_list(const int32_t* __restrict__ list, uint32_t count,
+                                                       int32_t* __restrict__ labels, const int32_t* __restrict__ cmin) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+    const int32_t v = list[i];
+    labels[v] = cmin[labels[v]];
   }
 }
 
@@ -409,12 +420,16 @@ __global__ void __launch_bounds__(kBlock) k_count_bits(const uint32_t* __restric
   }
 }
 
-// Starts a search from `root`: fresh control block, root claimed and queued alone.
-// Follows scc_fwbw.cpp:72-73 (queue = {root}, fw[root] = true, set even if root is invalid).
-__global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
+// Starts searches p.s[0 .. ns) from `root` (wave s starts search s, phase phase0 + s):
+// fresh control block, root claimed and queued alone.  Follows scc_fwbw.cpp:72-73
+// (queue = {root}, fw[root] = true, set even if root is invalid).
+__global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t phase0) {
+  const int s = wave_id();
+  if (s >= ns || s >= 2) return;
+  const LevelArgs& a = p.s[s];
   Ctrl* c = a.ctrl;
   uint32_t deg = list_len(a.down_off, a.down_off2, (uint32_t)root);
-  if (threadIdx.x == 0) {
+  if (lane() == 0) {
     c->next_packed = 0ull;
     c->q_size = 1u;
     c->q_edges = deg;
@@ -425,7 +440,7 @@ __global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
     c->level = 0u;
     c->closes = 0u;
     c->found = 0;
-    c->phase = (uint32_t)phase;
+    c->phase = (uint32_t)(phase0 + s);
     c->abandoned = 0u;
     c->t_start = ~0ull;
     uint32_t m = 1u << ((uint32_t)root & 31u);
@@ -434,7 +449,7 @@ __global__ void k_search_init(LevelArgs a, int32_t root, int32_t phase) {
     a.fr0.qo[0] = 0u;
   }
   uint32_t nch = (deg + kChunk - 1) / kChunk;
-  for (uint32_t ch = threadIdx.x; ch < nch; ch += blockDim.x) a.fr0.starts[ch] = 0u;
+  for (uint32_t ch = lane(); ch < nch; ch += 64) a.fr0.starts[ch] = 0u;
 }
 
 // Resets the control block before a seeding kernel fills frontier buffer 1.
@@ -456,11 +471,13 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
   }
 }
 
-// Closes a seeding launch: the queue built in buffer 1 becomes the frontier.
+// Closes a seeding launch: the queue built in buffer 1 becomes the frontier (and the
+// length of the vertex list the seed wrote beside it, if any).
 __global__ void k_close_seed(Ctrl* c) {
   if (threadIdx.x == 0) {
     unsigned long long p = c->next_packed;
     uint32_t next = (uint32_t)(p >> 32);
+    c->list_len = next;
     c->q_size = next;
     c->q_edges = (uint32_t)p;
     c->cur = 1u;
@@ -1121,61 +1138,62 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
   }
 }
 
-// Colour initialisation (multistep/scc_color.cpp:56-63, cc_color.cpp:62-69): colors[v] = v
-// for every valid vertex, all of them queued.  Needs k_ctrl_reset first, k_close_seed after.
-template <bool MIRROR>
-__global__ void __launch_bounds__(kBlock) k_seed_colors(LevelArgs a) {
+// Colour initialisation of a later colouring round (multistep/scc_color.cpp:56-63): the
+// previous round's vertex list (length a.ctrl->list_len, which k_ctrl_reset keeps) filtered
+// to the vertices still valid; each gets colors[v] = v and is queued, and the queue is also
+// written to `out` as this round's list.  Round 0 is seeded by k_commit.  Needs
+// k_ctrl_reset first, k_close_seed after.
+__global__ void __launch_bounds__(kBlock) k_seed_list(LevelArgs a, const int32_t* __restrict__ in,
+                                                      int32_t* __restrict__ out) {
   __shared__ Stage S;
-  __shared__ BitIter B;
+  const uint32_t in_n = a.ctrl->list_len;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
-  for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
-    const uint32_t total = bits_load(B, a.valid, a.nwords, wb);
-    for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
-      const uint32_t k = k0 + threadIdx.x;
-      bool take = k < total;
-      uint32_t v = 0, deg = 0;
-      if (take) {
-        v = bits_pick(B, wb, k);
+  for (uint32_t i0 = blockIdx.x * kBlock; i0 < in_n; i0 += gridDim.x * kBlock) {
+    const uint32_t i = i0 + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0, deg = 0;
+    if (i < in_n) {
+      v = (uint32_t)in[i];
+      if (bit_test(a.valid, v)) {
+        take = true;
         a.colors[v] = (int32_t)v;
-        deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
+        deg = list_len(a.down_off, a.down_off2, v);
       }
-      stage_push(S, take, v, deg);
-      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
     }
-    __syncthreads();  // B is reloaded next
+    stage_push(S, take, v, deg);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, out);  // also the barrier before the next round
   }
-  stage_flush(S, a.fr1, a.ctrl, 0u);
+  stage_flush(S, a.fr1, a.ctrl, 0u, out);
 }
 
-// Colour roots (multistep/scc_color.cpp:165-212): colors[v] == v labels v, removes it and
-// queues it for the per-colour backward search.  Needs k_ctrl_reset first, k_close_seed
-// after.
-__global__ void __launch_bounds__(kBlock) k_roots(LevelArgs a) {
+// Colour roots (multistep/scc_color.cpp:165-212) among this round's vertex list (length
+// gate->list_len): colors[v] == v labels v, removes it and queues it for the per-colour
+// backward search of control block a.ctrl.  Runs only once the push fixpoint of control
+// block `gate` has settled (gate->done); otherwise it does nothing and the host repeats it
+// after finishing the push levels.  Needs k_ctrl_reset first, k_close_seed after.
+__global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_t* __restrict__ list,
+                                                       const Ctrl* __restrict__ gate) {
   __shared__ Stage S;
-  __shared__ BitIter B;
+  if (!gate->done) return;  // uniform
+  const uint32_t nl = gate->list_len;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
-  for (int64_t wb = (int64_t)blockIdx.x * kBW; wb < a.nwords; wb += (int64_t)gridDim.x * kBW) {
-    const uint32_t total = bits_load(B, a.valid, a.nwords, wb);
-    for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
-      const uint32_t k = k0 + threadIdx.x;
-      bool take = false;
-      uint32_t v = 0, deg = 0;
-      if (k < total) {
-        v = bits_pick(B, wb, k);
-        if (a.colors[v] == (int32_t)v) {
-          take = true;
-          a.labels[v] = (int32_t)v;
-          if (a.cmin) a.cmin[v] = (int32_t)v;
-          bit_clear(a.valid, v);
-          deg = a.down_off[v + 1] - a.down_off[v];
-        }
+  for (uint32_t i0 = blockIdx.x * kBlock; i0 < nl; i0 += gridDim.x * kBlock) {
+    const uint32_t i = i0 + threadIdx.x;
+    bool take = false;
+    uint32_t v = 0, deg = 0;
+    if (i < nl) {
+      v = (uint32_t)list[i];
+      if (a.colors[v] == (int32_t)v && bit_clear(a.valid, v)) {
+        take = true;
+        a.labels[v] = (int32_t)v;
+        if (a.cmin) a.cmin[v] = (int32_t)v;
+        deg = a.down_off[v + 1] - a.down_off[v];
       }
-      stage_push(S, take, v, deg);
-      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
     }
-    __syncthreads();  // B is reloaded next
+    stage_push(S, take, v, deg);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }
@@ -1528,36 +1546,68 @@ __global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }
 
-// Commit of the pivot's component (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271): every
-// member gets the label and leaves the valid set; part col 0 = members of the workgroup.
-// One vertex per lane so the label stores of a wave are one contiguous 256-byte run; the
-// first lane of each word clears the word's members from `valid` (no other writer).  The
-// label is `label`, or the smallest member when min_red is given (0xffffffff - min_red[3],
-// canonical labels), or in a batch the pivot of the vertex's own graph.
-template <bool BATCH>
-__global__ void __launch_bounds__(kBlock) k_commit(const uint32_t* __restrict__ member, uint32_t* __restrict__ valid,
-                                                   int32_t n, int32_t* __restrict__ labels, int32_t label,
+// Commit of the pivot's component (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271)
+// fused with the seed of the colouring that follows (multistep/scc_color.cpp:56-63,
+// cc_color.cpp:62-69): every member gets the label and leaves the valid set; with SEED,
+// every vertex still valid afterwards gets colors[v] = v and is queued in frontier 1 of
+// a.ctrl, and the queue is also written to `list` (the vertices colouring round 0 works
+// on).  part col 0 = members of the workgroup.  One vertex per thread, so the label stores
+// of a wave are one contiguous 256-byte run; the first lane of each word writes the word's
+// new valid bits (no other writer).  The label is `label`, or the smallest member when
+// min_red is given (0xffffffff - min_red[3], canonical labels), or in a batch the pivot of
+// the vertex's own graph.
+template <bool BATCH, bool SEED>
+__global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __restrict__ valid, int32_t n,
+                                                   int32_t* __restrict__ labels, int32_t label,
                                                    const unsigned long long* __restrict__ min_red,
                                                    const int32_t* __restrict__ piv, const int64_t* __restrict__ base,
-                                                   uint32_t ng, unsigned long long* __restrict__ part) {
+                                                   uint32_t ng, unsigned long long* __restrict__ part, LevelArgs a,
+                                                   int32_t* __restrict__ list) {
   __shared__ unsigned long long s_red[kBlock / 64];
+  __shared__ Stage S;
   if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
+  if (SEED) {
+    if (threadIdx.x == 0) S.cnt = 0u;
+    __syncthreads();
+  }
   unsigned long long cnt = 0;
-  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
-    const int64_t v = wb + lane();
-    const uint32_t w = member[v >> 5];  // bitmaps carry slack words past n
-    const bool in = v < n && ((w >> (v & 31)) & 1u);
+  // the stage holds kStage claims, at most kBlock per iteration: flush every kEvery
+  // iterations, so the waves run unsynchronised in between
+  constexpr int kEvery = (int)(kStage / kBlock) - 1;
+  int since = 0;
+  for (int64_t wg = (int64_t)blockIdx.x * kBlock; wg < n; wg += (int64_t)gridDim.x * kBlock) {  // uniform
+    const int64_t wb = wg + (threadIdx.x & ~63);
+    const int64_t v = wg + threadIdx.x;
+    const int64_t wi = v >> 5;  // bitmaps carry slack words past n
+    const uint32_t vw = valid[wi];
+    const uint32_t w = member_word(m, vw, wi);
+    const uint32_t bit = 1u << (v & 31);
+    const bool in = v < n && (w & bit);
     if (BATCH) {
-      const uint32_t g = wave_graph_of(base, ng, wb, n);
+      const uint32_t g = wave_graph_of(base, ng, wb < n ? wb : n - 1, n);
       if (in) labels[v] = piv[g];
     } else if (in) {
       labels[v] = label;
     }
     if ((lane() & 31) == 0 && v < n && w) {
-      valid[v >> 5] &= ~w;
+      valid[wi] = vw & ~w;
       cnt += (uint32_t)__popc(w);
     }
+    if (SEED) {
+      const bool take = v < n && (vw & ~w & bit);
+      uint32_t deg = 0;
+      if (take) {
+        a.colors[v] = (int32_t)v;
+        deg = list_len(a.down_off, a.down_off2, (uint32_t)v);
+      }
+      stage_push(S, take, (uint32_t)v, deg);
+      if (++since == kEvery) {
+        since = 0;
+        stage_flush(S, a.fr1, a.ctrl, kStage - kEvery * kBlock, list);
+      }
+    }
   }
+  if (SEED) stage_flush(S, a.fr1, a.ctrl, 0u, list);
   cnt = block_sum64(cnt, s_red);
   if (threadIdx.x == 0) {
     part[4 * blockIdx.x] = cnt;
@@ -1618,12 +1668,14 @@ void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count
 }
 
 int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io, uint32_t* valid,
-                int32_t* labels, unsigned long long* part) {
+                uint32_t* cand1, uint32_t* cand2, int32_t* labels, unsigned long long* part) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
   if (mirror)
-    hipLaunchKernelGGL(k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, part);
+    hipLaunchKernelGGL(k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, labels,
+                       part);
   else
-    hipLaunchKernelGGL(k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, labels, part);
+    hipLaunchKernelGGL(k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, labels,
+                       part);
   return g;
 }
 
@@ -1637,7 +1689,16 @@ void launch_trim2_init(const Launch& L, const LevelArgs& a) {
 }
 
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
-  hipLaunchKernelGGL(k_search_init, dim3(1), dim3(kBlock), 0, L.stream, a, root, phase);
+  ReachArgs p{};
+  p.s[0] = a;
+  hipLaunchKernelGGL(k_search_init, dim3(1), dim3(64), 0, L.stream, p, 1, root, phase);
+}
+
+void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root) {
+  ReachArgs p{};
+  p.s[0] = a0;
+  p.s[1] = a1;
+  hipLaunchKernelGGL(k_search_init, dim3(1), dim3(128), 0, L.stream, p, 2, root, 0);
 }
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
@@ -1678,33 +1739,45 @@ void launch_close_reach(const Launch& L, const ReachArgs& p, int ns) {
   else hipLaunchKernelGGL(k_close_reach<1>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
 }
 
-int launch_commit(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
-                  const unsigned long long* min_red, unsigned long long* part) {
+int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
+                  const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
+                  int32_t* list) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
-  hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member, valid, n, labels, label, min_red,
-                     nullptr, nullptr, 0u, part);
+  LevelArgs a = seed ? *seed : LevelArgs{};
+  if (seed)
+    hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, label,
+                       min_red, nullptr, nullptr, 0u, part, a, list);
+  else
+    hipLaunchKernelGGL((k_commit<false, false>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, label,
+                       min_red, nullptr, nullptr, 0u, part, a, list);
   return g;
 }
 
-int launch_commit_batch(const Launch& L, const uint32_t* member, uint32_t* valid, int32_t n, int32_t* labels,
-                        const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part) {
+int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
+                        const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
+                        const LevelArgs* seed, int32_t* list) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
-  hipLaunchKernelGGL(k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member, valid, n, labels, 0, nullptr, piv,
-                     base, ng, part);
+  LevelArgs a = seed ? *seed : LevelArgs{};
+  if (seed)
+    hipLaunchKernelGGL((k_commit<true, true>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, 0, nullptr,
+                       piv, base, ng, part, a, list);
+  else
+    hipLaunchKernelGGL((k_commit<true, false>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, 0, nullptr,
+                       piv, base, ng, part, a, list);
   return g;
 }
 
-int launch_min_bit(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {
+int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
+                      unsigned long long* part) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_min_bit, dim3(g), dim3(kBlock), 0, L.stream, bits, nwords, part);
+  hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, m, valid, nwords, part);
   return g;
 }
 
-void launch_canon_colour(const Launch& L, const uint32_t* coloured, int64_t nwords, int32_t n, int32_t* labels,
-                         const int32_t* cmin) {
-  if (nwords <= 0) return;
-  hipLaunchKernelGGL(k_canon_colour, dim3(grid_for(nwords, kBlock * 4, L.grid * 2)), dim3(kBlock), 0, L.stream,
-                     coloured, nwords, n, labels, cmin);
+void launch_canon_list(const Launch& L, const int32_t* list, uint32_t count, int32_t* labels, const int32_t* cmin) {
+  if (count == 0u) return;
+  hipLaunchKernelGGL(k_canon_list, dim3(grid_for(count, kBlock, L.grid * 8)), dim3(kBlock), 0, L.stream, list,
+                     count, labels, cmin);
 }
 
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {
@@ -1720,17 +1793,23 @@ void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_
                      b, words);
 }
 
-void launch_seed_colors(const Launch& L, bool mirror, const LevelArgs& a) {
+void launch_seed_begin(const Launch& L, Ctrl* c, int32_t phase) {
+  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, c, phase);
+}
+
+void launch_seed_end(const Launch& L, Ctrl* c) {
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, c);
+}
+
+void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound) {
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 3);
-  dim3 g((unsigned)grid_for(a.nwords, kBW, L.grid));
-  if (mirror) hipLaunchKernelGGL(k_seed_colors<true>, g, dim3(kBlock), 0, L.stream, a);
-  else hipLaunchKernelGGL(k_seed_colors<false>, g, dim3(kBlock), 0, L.stream, a);
+  hipLaunchKernelGGL(k_seed_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, in, out);
   hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
 }
 
-void launch_roots(const Launch& L, const LevelArgs& a) {
+void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list, const Ctrl* gate, int64_t bound) {
   hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 4);
-  hipLaunchKernelGGL(k_roots, dim3((unsigned)grid_for(a.nwords, kBW, L.grid)), dim3(kBlock), 0, L.stream, a);
+  hipLaunchKernelGGL(k_roots_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, list, gate);
   hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
 }
 

@@ -37,7 +37,7 @@ struct msc_ctx {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{4, 4096, 4096, 2, 32, 1};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -45,7 +45,8 @@ struct msc_ctx {
   // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
   uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
   int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
-  Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the concurrent backward search's
+  Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the backward search's (beside the forward one)
+  int32_t* lists[3] = {nullptr, nullptr, nullptr};  // vertex lists of the colouring rounds
   unsigned long long* part = nullptr;   // kMaxParts x 4 per-workgroup partials
   unsigned long long* part2 = nullptr;  // the same for the backward search beside the forward one
   uint32_t* err = nullptr;              // validation flag
@@ -120,6 +121,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr2
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr3
       (size_t)kMaxParts * 4 * 8,                                // partials of the backward search
+      (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // colouring vertex lists
   };
   size_t total = 0;
   for (size_t s : sizes) total += align256(s);
@@ -152,6 +154,9 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->fr3 = Frontier{(int32_t*)parts[22], (uint32_t*)parts[23], (uint32_t*)parts[24]};
   c->part2 = (unsigned long long*)parts[25];
   c->ctrl2 = c->ctrl + 1;
+  c->lists[0] = (int32_t*)parts[26];
+  c->lists[1] = (int32_t*)parts[27];
+  c->lists[2] = (int32_t*)parts[28];
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -215,15 +220,12 @@ int reduce_read(msc_ctx* c, int nparts, unsigned long long out[4]) {
   return MSC_OK;
 }
 
-// Level loop of one colour sweep or complete trim, or of one or two reach searches that
-// advance together, one k_reach launch per level (the forward and backward searches of a
-// pivot).  Level + closer launches are enqueued in growing batches; after each batch the
-// control blocks come back with one copy (a pair's blocks are adjacent: as[1].ctrl ==
-// as[0].ctrl + 1) and one synchronisation, and the host copies are current when this
-// returns.  Every search and max-label fixpoint settles within n + 1 levels, so more
-// launches than that mean a fault.
-int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, msc_stats* s, int first_batch,
-             bool count_left = false) {
+// Enqueues `count` levels (work launch + closing kernel each) of one colour sweep or
+// complete trim, or of k (1 or 2) reach searches that advance together, one k_reach launch
+// per level (the forward and backward searches of a pivot; their blocks are adjacent:
+// as[1].ctrl == as[0].ctrl + 1).  Levels past the end of a loop find `done` set and return
+// at once.
+int enqueue_levels(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, int count) {
   const bool reach = op == OP_FW || op == OP_BW || op == OP_CC;
   if (k < 1 || k > 2 || (k == 2 && (!reach || as[1].ctrl != as[0].ctrl + 1))) return MSC_E_INVALID;
   ReachArgs p{};
@@ -233,30 +235,38 @@ int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, ms
                                        : 0u;
     p.s[j] = as[j];
   }
-  Ctrl* host = c->ctrl_host;
+  Launch L = launch_of(c);
+  for (int i = 0; i < count; ++i) {
+    {
+      Timed t(c, kcls);
+      if (reach) launch_reach(L, mirror, p, k);
+      else launch_level(L, op, mirror, as[0]);
+    }
+    if (reach) launch_close_reach(L, p, k);
+    else launch_close(L, op, mirror, as[0]);
+  }
+  HIP_TRY(hipGetLastError());
+  return MSC_OK;
+}
+
+// Level loop: enqueues levels in growing batches until the control blocks say done.  After
+// each batch the blocks come back with one copy and one synchronisation (into the host
+// copy of the same block: ctrl -> ctrl_host[0], ctrl2 -> ctrl_host[1]), and they are
+// current when this returns.  Every search and max-label fixpoint settles within n + 1
+// levels, so more launches than that mean a fault.
+int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, msc_stats* s, int first_batch) {
+  if (as[0].ctrl != c->ctrl && as[0].ctrl != c->ctrl2) return MSC_E_INVALID;
+  Ctrl* host = c->ctrl_host + (as[0].ctrl - c->ctrl);
+  if (host + k > c->ctrl_host + 2) return MSC_E_INVALID;
   int batch = first_batch > 0 ? first_batch : c->tune.first_batch;
   const int max_batch = c->tune.max_batch;
   const int64_t limit = (int64_t)as[0].n_total + 64;
   int64_t launched = 0;
-  Launch L = launch_of(c);
+  int rc;
   for (;;) {
     if (launched > limit) return MSC_E_DEVICE;
     launched += batch;
-    for (int i = 0; i < batch; ++i) {
-      {
-        Timed t(c, kcls);
-        if (reach) launch_reach(L, mirror, p, k);
-        else launch_level(L, op, mirror, as[0]);
-      }
-      if (reach) launch_close_reach(L, p, k);
-      else launch_close(L, op, mirror, as[0]);
-    }
-    if (count_left) {
-      // valid vertices left: read back with this copy, so a colour round needs no extra trip
-      int np = launch_count_bits(L, c->valid, as[0].nwords, as[0].part);
-      launch_reduce(L, as[0].part, np, as[0].ctrl->red_left);
-    }
-    HIP_TRY(hipGetLastError());
+    if ((rc = enqueue_levels(c, op, mirror, as, k, kcls, batch))) return rc;
     HIP_TRY(hipMemcpyAsync(host, as[0].ctrl, (size_t)k * sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (s) s->host_syncs += 1;
@@ -268,9 +278,8 @@ int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, ms
   return MSC_OK;
 }
 
-int run_levels(msc_ctx* c, int op, bool mirror, LevelArgs a, int kcls, msc_stats* s, int first_batch,
-               bool count_left = false) {
-  return run_loop(c, op, mirror, &a, 1, kcls, s, first_batch, count_left);
+int run_levels(msc_ctx* c, int op, bool mirror, LevelArgs a, int kcls, msc_stats* s, int first_batch) {
+  return run_loop(c, op, mirror, &a, 1, kcls, s, first_batch);
 }
 
 LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opts* o, double avg) {
@@ -559,7 +568,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   } else {
     Timed t(c, MSC_K_TRIM);
-    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, labels, c->part);
+    // the searches' claimable bitmaps start as the valid set: written by the trim as well
+    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, labels, c->part);
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   }
   if (s) s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n;
@@ -573,8 +583,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
 
   // FW-BW (SCC) or the CC search from the pivot; a search the policy abandons leaves
   // every vertex to colouring
+  MemberSrc member{};  // none unless a search finished
+  const bool cand_ready = o->trim_level != 2;  // the degree trim wrote b1 (and b3) = valid
   if (pivot >= 0) {
-    const uint32_t* member = nullptr;
     if (!cc) {
       // FW: allow = valid, cand = b1, top-down on out-lists, bottom-up on in-lists
       LevelArgs fw = a;
@@ -597,30 +608,26 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         bw.ctrl = c->ctrl2; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
-        launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);  // cand = allow, for both
+        if (!cand_ready) launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);  // cand = allow, for both
         if (bt) {
           launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
           launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
         } else {
-          launch_search_init(L, fw, pivot, 0);
-          launch_search_init(L, bw, pivot, 1);
+          launch_search_init_pair(L, fw, bw, pivot);
         }
         LevelArgs both[2] = {fw, bw};
-        if ((rc = run_loop(c, OP_FW, false, both, 2, MSC_K_LEVEL, s, 0))) return rc;
+        if ((rc = run_loop(c, OP_FW, false, both, 2, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
         if (s) {
           s->fw_levels = (int32_t)c->ctrl_host[0].level;
           s->bw_levels = (int32_t)c->ctrl_host[1].level;
         }
-        if (!c->ctrl_host[0].abandoned && !c->ctrl_host[1].abandoned) {
-          launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);  // FW set
-          launch_andnot(L, c->b2, nullptr, c->b2, c->b3, nw);     // & BW set
-          member = c->b2;
-        }
+        // SCC = FW set & BW set = valid & ~b1 & ~b3
+        if (!c->ctrl_host[0].abandoned && !c->ctrl_host[1].abandoned) member = MemberSrc{nullptr, c->b1, c->b3};
       } else {
-        launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
+        if (!cand_ready) launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
         if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
         else launch_search_init(L, fw, pivot, 0);
-        if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, 0))) return rc;
+        if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
         if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
         if (!c->ctrl_host->abandoned) {
           // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy)
@@ -628,18 +635,16 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
           bw.allow = c->b2; bw.cand = c->b3;
           if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
           else launch_search_init(L, bw, pivot, 1);
-          if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, 0))) return rc;
+          if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
           if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
-          if (!c->ctrl_host->abandoned) {
-            launch_andnot(L, c->b1, nullptr, c->b2, c->b3, nw);  // SCC = fw set & visited by BW
-            member = c->b1;
-          }
+          // SCC = fw set & visited by BW = (valid & ~b1) & ~b3, as b3 started as the fw set
+          if (!c->ctrl_host->abandoned) member = MemberSrc{nullptr, c->b1, c->b3};
         }
       }
     } else {
       LevelArgs bf = a;
       bf.op = OP_CC;
-      launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
+      if (!cand_ready) launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
       bf.down_off = g.out_off; bf.down_adj = g.out_adj;
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
@@ -647,90 +652,112 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bf.allow = c->valid; bf.cand = c->b1;
       if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
       else launch_search_init(L, bf, pivot, 2);
-      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, 0))) return rc;
+      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
-      if (!c->ctrl_host->abandoned) {
-        launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);  // component = valid & visited
-        member = c->b2;
-      }
-    }
-    if (member) {
-      // commit (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271)
-      {
-        Timed t(c, MSC_K_COMMIT);
-        if (fast_canon) launch_reduce(L, c->part, launch_min_bit(L, member, nw, c->part), c->ctrl->red_min);
-        np = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part)
-                : launch_commit(L, member, c->valid, n, labels, pivot, fast_canon ? c->ctrl->red_min : nullptr,
-                                c->part);
-        launch_reduce(L, c->part, np, c->ctrl->red_commit);
-      }
+      if (!c->ctrl_host->abandoned) member = MemberSrc{nullptr, c->b1, nullptr};  // valid & visited
     }
   }
 
+  // commit of the pivot's component fused with the seed of colouring round 0: one pass
+  // over the vertices labels the members, drops them from `valid`, and queues every vertex
+  // left (colors[v] = v) in the colouring's first frontier and in list 0 (multistep/
+  // scc_fwbw.cpp:412-466 then scc_color.cpp:56-63; cc_bfs.cpp:217-271 then
+  // cc_color.cpp:62-69)
+  LevelArgs p = a;
+  p.op = OP_PUSH;
+  p.nvalid = nullptr;
+  p.down_off = g.out_off; p.down_adj = g.out_adj;
+  if (cc) {
+    p.colors = labels;  // the CC fixpoint writes straight into the labels
+    p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
+  }
+  if (fast_canon) p.cmin = c->cmin;
+  const bool has_member = member.member || member.ca;
+  launch_seed_begin(L, c->ctrl, 3);
+  {
+    Timed t(c, MSC_K_COMMIT);
+    const bool want_min = fast_canon && has_member;
+    if (want_min) launch_reduce(L, c->part, launch_min_member(L, member, c->valid, nw, c->part), c->ctrl->red_min);
+    np = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p, c->lists[0])
+            : launch_commit(L, member, c->valid, n, labels, pivot, want_min ? c->ctrl->red_min : nullptr, c->part, &p,
+                            c->lists[0]);
+    launch_reduce(L, c->part, np, c->ctrl->red_commit);
+  }
+  launch_seed_end(L, c->ctrl);
+
   // colouring
   if (!cc) {
-    // rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677); each round's count
-    // of remaining vertices comes back with the control block after its last colour-BW
-    // batch.  The previous round's closer launches predict this round's: enqueue that many.
+    // Rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677).  A round enqueues
+    // its push levels, the colour roots (which run only once the push fixpoint is done) and
+    // the colour-BW levels, then reads both control blocks back once (push: ctrl, colour-BW:
+    // ctrl2); only a round that needed more levels than enqueued pays more round trips.
+    // The previous round's closer launches predict this round's level counts.  Round r works
+    // on list r: list 0 is kept for the canonical pass, later rounds alternate 1 and 2.
+    LevelArgs r = a;
+    r.op = OP_CBW;
+    r.nvalid = nullptr;
+    r.down_off = g.in_off; r.down_adj = g.in_adj;
+    r.ctrl = c->ctrl2;
+    r.part = c->part2;
+    r.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
+    if (fast_canon) r.cmin = c->cmin;
     int push_batch = c->tune.first_batch, cbw_batch = c->tune.first_batch;
-    uint64_t before = 0;
-    if (fast_canon) {
-      launch_andnot(L, c->b2, nullptr, c->valid, nullptr, nw);  // the vertices colouring will label
-      a.cmin = c->cmin;
-    }
+    uint64_t before = (uint64_t)n;  // bound of this round's list: the vertices left by the last round
+    uint32_t list0 = 0, listed_prev = 0;
+    int cur = 0;
+    const Ctrl* h = c->ctrl_host;  // [0]: push, [1]: colour-BW
     for (int round = 0;; ++round) {
-      LevelArgs p = a;
-      p.op = OP_PUSH;
-      p.nvalid = nullptr;
-      p.down_off = g.out_off; p.down_adj = g.out_adj;
-      {
-        Timed t(c, MSC_K_OTHER);
-        launch_seed_colors(L, false, p);
+      if (round > 0) {
+        const int nxt = cur == 1 ? 2 : 1;
+        Timed t(c, MSC_K_ROOTS);
+        launch_seed_list(L, p, c->lists[cur], c->lists[nxt], (int64_t)listed_prev);
+        cur = nxt;
       }
-      if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s, push_batch))) return rc;
-      uint32_t sweeps = c->ctrl_host->level, closes = c->ctrl_host->closes;
-      if (s) s->color_iters += (int32_t)sweeps;
-      push_batch = closes > 2 ? (closes < (uint32_t)c->tune.max_batch ? (int)closes : c->tune.max_batch) : 2;
-      LevelArgs r = a;
-      r.op = OP_CBW;
-      r.nvalid = nullptr;
-      r.down_off = g.in_off; r.down_adj = g.in_adj;
+      enqueue_levels(c, OP_PUSH, false, &p, 1, MSC_K_COLOR_PROP, push_batch);
       {
         Timed t(c, MSC_K_ROOTS);
-        launch_roots(L, r);
+        launch_roots_list(L, r, c->lists[cur], c->ctrl, (int64_t)before);
       }
-      if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch, true))) return rc;
-      const Ctrl* h = c->ctrl_host;
-      uint32_t cbw = h->level, cbw_closes = h->closes;
-      if (s) s->cbw_levels += (int32_t)cbw;
-      cbw_batch = cbw_closes > 2 ? (cbw_closes < (uint32_t)c->tune.max_batch ? (int)cbw_closes : c->tune.max_batch)
-                                 : 2;
-      if (round == 0) before = h->red_trim[0] - h->red_commit[0];
-      const uint64_t left = h->red_left[0];
+      enqueue_levels(c, OP_CBW, false, &r, 1, MSC_K_COLOR_BW, cbw_batch);
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->host_syncs += 1;
+      if (!h[0].done) {  // the roots above were skipped
+        if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s, push_batch * 2))) return rc;
+        {
+          Timed t(c, MSC_K_ROOTS);
+          launch_roots_list(L, r, c->lists[cur], c->ctrl, (int64_t)before);
+        }
+        if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch))) return rc;
+      } else if (!h[1].done) {
+        if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch * 2))) return rc;
+      }
+      const uint32_t listed = h[0].list_len, roots = h[1].list_len;
+      const uint64_t claimed = (uint64_t)h[1].found;
+      if (round == 0) list0 = listed;
+      if (roots == 0u && listed > 0u) return MSC_E_DEVICE;  // a settled fixpoint always has a root
+      const uint64_t left = (uint64_t)listed - roots - claimed;
       if (s) {
+        s->color_iters += (int32_t)h[0].level;
+        s->cbw_levels += (int32_t)h[1].level;
         s->color_rounds += 1;
-        s->k[MSC_K_ROOTS].bytes += 2.0 * 4.0 * nw + 4.0 * (double)(before - left);
+        // list scans (id + colour per entry) and per root its label, cmin, offsets and slot
+        s->k[MSC_K_ROOTS].bytes += 8.0 * listed + 24.0 * roots + (round > 0 ? 17.0 * listed : 0.0);
       }
+      push_batch = h[0].closes > 2 ? (h[0].closes < (uint32_t)c->tune.max_batch ? (int)h[0].closes : c->tune.max_batch)
+                                   : 2;
+      cbw_batch = h[1].closes > 2 ? (h[1].closes < (uint32_t)c->tune.max_batch ? (int)h[1].closes : c->tune.max_batch)
+                                  : 2;
       if (left == 0) break;
-      if (left >= before) return MSC_E_DEVICE;  // every round removes at least one SCC
+      if (left >= listed) return MSC_E_DEVICE;  // every round removes at least one SCC
       before = left;
+      listed_prev = listed;
     }
     if (fast_canon) {
       Timed t(c, MSC_K_OTHER);
-      launch_canon_colour(L, c->b2, nw, n, labels, c->cmin);
+      launch_canon_list(L, c->lists[0], list0, labels, c->cmin);
     }
   } else {
     // one max-label fixpoint written straight into the labels (multistep/cc_color.cpp:47-175)
-    LevelArgs p = a;
-    p.op = OP_PUSH;
-    p.nvalid = nullptr;
-    p.colors = labels;
-    p.down_off = g.out_off; p.down_adj = g.out_adj;
-    p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
-    {
-      Timed t(c, MSC_K_OTHER);
-      launch_seed_colors(L, true, p);
-    }
     if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s, 0))) return rc;
     if (s) {
       s->color_iters = (int32_t)c->ctrl_host->level;
@@ -907,7 +934,7 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (!c || !t) return MSC_E_INVALID;
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
-      t->concurrent_fwbw > 1)
+      t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;

@@ -197,9 +197,11 @@ typedef struct msc_tuning {
   int32_t small_cbw_edges;  /* same for the per-colour backward search (0: only empty ones) */
   int32_t first_batch;      /* levels enqueued before the first host check of a loop */
   int32_t max_batch;        /* levels enqueued between later host checks */
-  int32_t concurrent_fwbw;  /* 1: the backward search runs beside the forward one on a second
-                               stream, over the valid set (SCC = FW set & BW set); 0: after it,
-                               restricted to the FW set, the reference's order */
+  int32_t concurrent_fwbw;  /* 1: the backward search advances beside the forward one, both in
+                               one launch per level, over the valid set (SCC = FW set & BW set);
+                               0: after it, restricted to the FW set, the reference's order */
+  int32_t search_batch;     /* levels of a FW-BW / CC search enqueued before its first host
+                               check (colour loops start with first_batch) */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);
