@@ -13,7 +13,11 @@ constexpr int kBlock = 256;                    // threads per workgroup: 4 wave6
 constexpr uint32_t kChunk = 1024;              // edges per edge-balanced work item
 constexpr int kPer = (int)(kChunk / kBlock);   // edges per thread per work item
 constexpr uint32_t kStage = 2048;              // claims staged in LDS per workgroup
-constexpr int kBW = kBlock;                    // bitmap words per workgroup iteration (one per thread)
+#ifndef MSC_BU_WPT
+#define MSC_BU_WPT 1
+#endif
+constexpr int kWPT = MSC_BU_WPT;               // bitmap words per thread of a bottom-up word group
+constexpr int kBW = kBlock * kWPT;             // bitmap words per bottom-up word group
 constexpr int kMaxParts = 8192;                // per-workgroup partial results
 constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;
@@ -139,11 +143,9 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  //
 // reach searches (OP_FW, OP_BW, OP_CC): p.s[0 .. ns) advance by one level together
 void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns);  // work only
 void launch_close_reach(const Launch& L, const ReachArgs& p, int ns);        // one workgroup
-// Members of the pivot's component: an explicit bitmap, or valid & ~ca & ~cb, the valid
-// vertices both searches visited (ca, cb: their claimable bitmaps; cb may be null).  All
-// null: no member.
+// Members of the pivot's component: valid & ~ca & ~cb, the valid vertices both searches
+// visited (ca, cb: their claimable bitmaps; cb null: one search).  ca null: no member.
 struct MemberSrc {
-  const uint32_t* member;
   const uint32_t* ca;
   const uint32_t* cb;
 };

@@ -27,6 +27,11 @@
 
 #include "msc_internal.h"
 
+// bottom-up candidates per thread per round (all their loads in flight together)
+#ifndef MSC_BU_KR
+#define MSC_BU_KR 4
+#endif
+
 namespace msc {
 namespace {
 
@@ -124,19 +129,33 @@ struct BitIter {
   uint32_t wsum[kBlock / 64];
 };
 
-// Loads words [wb, wb + kBW) (one per thread) and the exclusive prefix of their popcounts;
-// returns the set bits in the group.  All threads call; ends with a barrier.
+// Loads words [wb, wb + kBW) (kWPT consecutive words per thread) and the exclusive prefix
+// of their popcounts; returns the set bits in the group.  All threads call; ends with a
+// barrier.
 __device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, int64_t nwords, int64_t wb) {
-  static_assert(kBW == kBlock, "one word per thread");
+  static_assert(kBW == kBlock * kWPT, "kWPT words per thread");
   const int t = threadIdx.x;
-  const uint32_t w = wb + t < nwords ? bits[wb + t] : 0u;
-  B.word[t] = w;
-  const uint32_t incl = wave_incl_sum((uint32_t)__popc(w));
+  const int64_t w0 = wb + (int64_t)t * kWPT;
+  uint32_t w[kWPT];
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) w[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) {
+    B.word[t * kWPT + j] = w[j];
+    cnt += (uint32_t)__popc(w[j]);
+  }
+  const uint32_t incl = wave_incl_sum(cnt);
   if (lane() == 63) B.wsum[wave_id()] = incl;
   __syncthreads();
   uint32_t carry = 0;
   for (int i = 0; i < wave_id(); ++i) carry += B.wsum[i];
-  B.pref[t + 1] = incl + carry;
+  uint32_t run = incl - cnt + carry;  // exclusive prefix of this thread's first word
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) {
+    run += (uint32_t)__popc(w[j]);
+    B.pref[t * kWPT + j + 1] = run;
+  }
   if (t == 0) B.pref[0] = 0u;
   __syncthreads();
   return B.pref[kBW];
@@ -146,7 +165,7 @@ __device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, 
 __device__ __forceinline__ uint32_t bits_pick(const BitIter& B, int64_t wb, uint32_t k) {
   int lo = 0, hi = kBW;  // pref[lo] <= k < pref[hi]
 #pragma unroll
-  for (int s = 0; s < 8; ++s) {
+  for (int s = 0; (1 << s) < kBW; ++s) {
     int mid = (lo + hi) >> 1;
     if (B.pref[mid] <= k) lo = mid;
     else hi = mid;
@@ -340,24 +359,38 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
   __shared__ unsigned long long s_red[kBlock / 64];
   const int64_t nw = ((int64_t)n + 31) / 32;
   unsigned long long kept = 0;
-  for (int64_t base = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); base < n; base += (int64_t)gridDim.x * kBlock) {
-    int64_t v = base + lane();
-    bool keep = false;
-    if (v < n) {
-      uint32_t od = oo[v + 1] - oo[v], id = io[v + 1] - io[v];
-      keep = MIRROR ? (od + id) > 0u : (od > 0u && id > 0u);
-      labels[v] = keep ? -1 : (int32_t)v;
+  // a wave takes kU consecutive 64-vertex slices per iteration, all offset loads first
+  constexpr int kU = 4;
+  const int64_t stride = (int64_t)gridDim.x * kBlock * kU;
+  for (int64_t b0 = ((int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63)) * kU; b0 < n; b0 += stride) {
+    uint32_t od[kU], id[kU];
+#pragma unroll
+    for (int r = 0; r < kU; ++r) {  // past n: reload vertex n - 1 (no branch between the loads)
+      const int64_t v0 = b0 + 64 * r + lane();
+      const int64_t v = v0 < n ? v0 : (int64_t)n - 1;
+      od[r] = oo[v + 1] - oo[v];
+      id[r] = io[v + 1] - io[v];
     }
-    unsigned long long mask = __ballot(keep);
-    int64_t word = base >> 5;
-    if ((lane() & 31) == 0 && word + (lane() >> 5) < nw) {
-      const int64_t wi = word + (lane() >> 5);
-      const uint32_t bits = (uint32_t)(mask >> (lane() & 32));
-      valid[wi] = bits;
-      if (cand1) cand1[wi] = bits;
-      if (cand2) cand2[wi] = bits;
+#pragma unroll
+    for (int r = 0; r < kU; ++r) {
+      const int64_t base = b0 + 64 * r;
+      const int64_t v = base + lane();
+      bool keep = false;
+      if (v < n) {
+        keep = MIRROR ? (od[r] + id[r]) > 0u : (od[r] > 0u && id[r] > 0u);
+        labels[v] = keep ? -1 : (int32_t)v;
+      }
+      unsigned long long mask = __ballot(keep);
+      int64_t word = base >> 5;
+      if ((lane() & 31) == 0 && word + (lane() >> 5) < nw) {
+        const int64_t wi = word + (lane() >> 5);
+        const uint32_t bits = (uint32_t)(mask >> (lane() & 32));
+        valid[wi] = bits;
+        if (cand1) cand1[wi] = bits;
+        if (cand2) cand2[wi] = bits;
+      }
+      kept += keep ? 1 : 0;
     }
-    kept += keep ? 1 : 0;
   }
   kept = block_sum64(kept, s_red);
   if (threadIdx.x == 0) {
@@ -366,10 +399,10 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
   }
 }
 
+// m as normalised by member_src (both pointers set): no branch between the loads, so a
+// thread's member words of several slices load together
 __device__ __forceinline__ uint32_t member_word(const MemberSrc& m, uint32_t valid_word, int64_t i) {
-  if (m.member) return m.member[i];
-  if (m.ca) return valid_word & ~m.ca[i] & (m.cb ? ~m.cb[i] : 0xffffffffu);
-  return 0u;
+  return valid_word & ~m.ca[i] & ~m.cb[i];
 }
 
 // part col 3 = 0xffffffff - smallest member in the workgroup's words (0 when none), so
@@ -635,9 +668,10 @@ template <bool MIRROR>
 __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, int64_t wb, Stage& S, BitIter& B,
                                          uint32_t* s_claim, unsigned long long& my_examined,
                                          unsigned long long& my_inspected) {
-  constexpr int kR = 4;
+  constexpr int kR = MSC_BU_KR;
   const int t = threadIdx.x;
-  s_claim[t] = 0u;
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) s_claim[t * kWPT + j] = 0u;
   const uint32_t total = bits_load(B, a.cand, a.nwords, wb);  // barrier: s_claim cleared
   for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
     uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
@@ -710,8 +744,12 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
     stage_flush(S, N, a.ctrl, kStage - kBlock * kR);  // also the barrier before the next round
   }
   // after the last round's barrier every claim of this group is in s_claim; each thread
-  // reads only its own word and mask, so the next group's loads need no barrier first
-  if (s_claim[t]) a.cand[wb + t] = B.word[t] & ~s_claim[t];
+  // reads only its own words and masks, so the next group's loads need no barrier first
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) {
+    const int i = t * kWPT + j;
+    if (s_claim[i]) a.cand[wb + i] = B.word[i] & ~s_claim[i];
+  }
 }
 
 // One colour sweep level (push or per-colour backward; work only, k_close_color ends it):
@@ -796,10 +834,19 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
 template <int NS, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   __shared__ Stage S;
-  __shared__ ChunkLds<OP_FW, MIRROR> L;
+  // a search's level is either top-down (ChunkLds) or bottom-up (BitIter + claim masks),
+  // and a workgroup moves between searches only across a barrier: the two share LDS
+  __shared__ union {
+    ChunkLds<OP_FW, MIRROR> L;
+    struct {
+      BitIter B;
+      uint32_t claim[kBW];
+    } bu;
+  } U;
   __shared__ unsigned long long s_red[kBlock / 64];
-  __shared__ BitIter B;
-  __shared__ uint32_t s_claim[kBW];
+  ChunkLds<OP_FW, MIRROR>& L = U.L;
+  BitIter& B = U.bu.B;
+  uint32_t* s_claim = U.bu.claim;
 
   const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
   const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
@@ -1571,41 +1618,46 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
     __syncthreads();
   }
   unsigned long long cnt = 0;
-  // the stage holds kStage claims, at most kBlock per iteration: flush every kEvery
-  // iterations, so the waves run unsynchronised in between
-  constexpr int kEvery = (int)(kStage / kBlock) - 1;
-  int since = 0;
-  for (int64_t wg = (int64_t)blockIdx.x * kBlock; wg < n; wg += (int64_t)gridDim.x * kBlock) {  // uniform
-    const int64_t wb = wg + (threadIdx.x & ~63);
-    const int64_t v = wg + threadIdx.x;
-    const int64_t wi = v >> 5;  // bitmaps carry slack words past n
-    const uint32_t vw = valid[wi];
-    const uint32_t w = member_word(m, vw, wi);
-    const uint32_t bit = 1u << (v & 31);
-    const bool in = v < n && (w & bit);
-    if (BATCH) {
-      const uint32_t g = wave_graph_of(base, ng, wb < n ? wb : n - 1, n);
-      if (in) labels[v] = piv[g];
-    } else if (in) {
-      labels[v] = label;
+  // kU slices of kBlock consecutive vertices per iteration, with all their loads issued
+  // before any store (one memory latency per iteration, not per slice)
+  constexpr int kU = 4;
+  static_assert(kU * kBlock <= (int)kStage / 2, "stage holds two iterations");
+  for (int64_t wg = (int64_t)blockIdx.x * kBlock * kU; wg < n; wg += (int64_t)gridDim.x * kBlock * kU) {  // uniform
+    uint32_t vw[kU], w[kU];
+#pragma unroll
+    for (int r = 0; r < kU; ++r) {
+      const int64_t wi = (wg + r * kBlock + threadIdx.x) >> 5;  // bitmaps carry slack words past n
+      vw[r] = valid[wi];
+      w[r] = member_word(m, vw[r], wi);
     }
-    if ((lane() & 31) == 0 && v < n && w) {
-      valid[wi] = vw & ~w;
-      cnt += (uint32_t)__popc(w);
-    }
-    if (SEED) {
-      const bool take = v < n && (vw & ~w & bit);
-      uint32_t deg = 0;
-      if (take) {
-        a.colors[v] = (int32_t)v;
-        deg = list_len(a.down_off, a.down_off2, (uint32_t)v);
+#pragma unroll
+    for (int r = 0; r < kU; ++r) {
+      const int64_t wb = wg + r * kBlock + (threadIdx.x & ~63);
+      const int64_t v = wg + r * kBlock + threadIdx.x;
+      const int64_t wi = v >> 5;
+      const uint32_t bit = 1u << (v & 31);
+      const bool in = v < n && (w[r] & bit);
+      if (BATCH) {
+        const uint32_t g = wave_graph_of(base, ng, wb < n ? wb : n - 1, n);
+        if (in) labels[v] = piv[g];
+      } else if (in) {
+        labels[v] = label;
       }
-      stage_push(S, take, (uint32_t)v, deg);
-      if (++since == kEvery) {
-        since = 0;
-        stage_flush(S, a.fr1, a.ctrl, kStage - kEvery * kBlock, list);
+      if ((lane() & 31) == 0 && v < n && w[r]) {
+        valid[wi] = vw[r] & ~w[r];
+        cnt += (uint32_t)__popc(w[r]);
+      }
+      if (SEED) {
+        const bool take = v < n && (vw[r] & ~w[r] & bit);
+        uint32_t deg = 0;
+        if (take) {
+          a.colors[v] = (int32_t)v;
+          deg = list_len(a.down_off, a.down_off2, (uint32_t)v);
+        }
+        stage_push(S, take, (uint32_t)v, deg);
       }
     }
+    if (SEED) stage_flush(S, a.fr1, a.ctrl, kStage - kU * kBlock, list);
   }
   if (SEED) stage_flush(S, a.fr1, a.ctrl, 0u, list);
   cnt = block_sum64(cnt, s_red);
@@ -1739,38 +1791,46 @@ void launch_close_reach(const Launch& L, const ReachArgs& p, int ns) {
   else hipLaunchKernelGGL(k_close_reach<1>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
 }
 
+// Both pointers set: no member (ca = valid), one bitmap (cb = ca) or two.
+MemberSrc member_src(const MemberSrc& m, const uint32_t* valid) {
+  MemberSrc r = m;
+  if (!r.ca) r.ca = valid;
+  if (!r.cb) r.cb = r.ca;
+  return r;
+}
+
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
                   int32_t* list) {
-  int g = grid_for(n, kBlock * 8, L.grid * 2);
+  int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
   if (seed)
-    hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, label,
-                       min_red, nullptr, nullptr, 0u, part, a, list);
+    hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
+                       labels, label, min_red, nullptr, nullptr, 0u, part, a, list);
   else
-    hipLaunchKernelGGL((k_commit<false, false>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, label,
-                       min_red, nullptr, nullptr, 0u, part, a, list);
+    hipLaunchKernelGGL((k_commit<false, false>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
+                       labels, label, min_red, nullptr, nullptr, 0u, part, a, list);
   return g;
 }
 
 int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
                         const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
                         const LevelArgs* seed, int32_t* list) {
-  int g = grid_for(n, kBlock * 8, L.grid * 2);
+  int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
   if (seed)
-    hipLaunchKernelGGL((k_commit<true, true>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, 0, nullptr,
-                       piv, base, ng, part, a, list);
+    hipLaunchKernelGGL((k_commit<true, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
+                       labels, 0, nullptr, piv, base, ng, part, a, list);
   else
-    hipLaunchKernelGGL((k_commit<true, false>), dim3(g), dim3(kBlock), 0, L.stream, m, valid, n, labels, 0, nullptr,
-                       piv, base, ng, part, a, list);
+    hipLaunchKernelGGL((k_commit<true, false>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
+                       labels, 0, nullptr, piv, base, ng, part, a, list);
   return g;
 }
 
 int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
                       unsigned long long* part) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, m, valid, nwords, part);
+  hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part);
   return g;
 }
 

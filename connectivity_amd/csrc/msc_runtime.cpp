@@ -622,7 +622,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
           s->bw_levels = (int32_t)c->ctrl_host[1].level;
         }
         // SCC = FW set & BW set = valid & ~b1 & ~b3
-        if (!c->ctrl_host[0].abandoned && !c->ctrl_host[1].abandoned) member = MemberSrc{nullptr, c->b1, c->b3};
+        if (!c->ctrl_host[0].abandoned && !c->ctrl_host[1].abandoned) member = MemberSrc{c->b1, c->b3};
       } else {
         if (!cand_ready) launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
         if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
@@ -638,7 +638,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
           if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
           if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
           // SCC = fw set & visited by BW = (valid & ~b1) & ~b3, as b3 started as the fw set
-          if (!c->ctrl_host->abandoned) member = MemberSrc{nullptr, c->b1, c->b3};
+          if (!c->ctrl_host->abandoned) member = MemberSrc{c->b1, c->b3};
         }
       }
     } else {
@@ -654,7 +654,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       else launch_search_init(L, bf, pivot, 2);
       if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
       if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
-      if (!c->ctrl_host->abandoned) member = MemberSrc{nullptr, c->b1, nullptr};  // valid & visited
+      if (!c->ctrl_host->abandoned) member = MemberSrc{c->b1, nullptr};  // valid & visited
     }
   }
 
@@ -672,7 +672,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
   }
   if (fast_canon) p.cmin = c->cmin;
-  const bool has_member = member.member || member.ca;
+  const bool has_member = member.ca != nullptr;
   launch_seed_begin(L, c->ctrl, 3);
   {
     Timed t(c, MSC_K_COMMIT);
