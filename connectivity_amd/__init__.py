@@ -554,6 +554,40 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
     return out, Stats._from(st)
 
 
+class Prepared:
+    """One decomposition call with its arguments marshalled once (for timing loops): each
+    call is a single C ABI call with no statistics requested.  ``kind``: "scc" or "cc";
+    ``batch``: (vert_base, pivots) for the batched entry points; ``out``: DeviceLabels."""
+
+    def __init__(self, kind, g, max_deg_vert=None, avg_degree=None, *, policy=POLICY_PLAIN,
+                 labels=LABELS_CANONICAL, ctx: Context, out: "DeviceLabels", batch=None):
+        if out.n != g.n or out.ctx is not ctx:
+            raise ValueError("out must be a DeviceLabels of the graph's size on the same context")
+        self._keep = (g, out)
+        self._csr = g.csr()
+        self._opts = _opts(policy, labels, False, False, False, ctx, labels_where=DEVICE)
+        self._dst = out.ptr.value
+        piv = -1 if max_deg_vert is None else int(max_deg_vert)
+        avg = -1.0 if avg_degree is None else float(avg_degree)
+        if batch is None:
+            fn = _lib().msc_scc_run if kind == "scc" else _lib().msc_cc_run
+            self._args = (C.byref(self._csr), piv, avg, self._dst, C.byref(self._opts), None)
+        else:
+            base, pivots = batch
+            self._base = np.ascontiguousarray(base, dtype=np.int64)
+            self._piv = None if pivots is None else np.ascontiguousarray(pivots, dtype=np.int32)
+            fn = _lib().msc_scc_run_batch if kind == "scc" else _lib().msc_cc_run_batch
+            self._args = (C.byref(self._csr), int(self._base.shape[0] - 1), _ptr(self._base),
+                          _ptr(self._piv) if self._piv is not None else None, self._dst, C.byref(self._opts), None)
+        self._fn = fn
+        self._name = fn.__name__
+
+    def __call__(self) -> None:
+        rc = self._fn(*self._args)
+        if rc != 0:
+            _check(rc, self._name)
+
+
 def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, *,
             policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
             record: bool = False, validate: bool = False, ctx: Context | None = None,

@@ -115,6 +115,21 @@ def test_device_resident_graph(msc, orc, gpu_ctx):
     dg.free()
 
 
+def test_prepared_call(msc, orc, gpu_ctx):
+    """The timing loop's prepared C ABI call writes the same labels as run_scc."""
+    _say("case: prepared calls s13 / grid 48")
+    for g in (msc.gen_rmat(13, 16, seed=9), msc.gen_grid(48, 48, seed=2)):
+        dg = msc.DeviceGraph(gpu_ctx, g)
+        out = msc.DeviceLabels(gpu_ctx, g.n)
+        for kind, want in (("scc", orc.canonical(orc.scc(g, vert_cutoff=0))), ("cc", orc.canonical(orc.cc(g)))):
+            call = msc.Prepared(kind, dg, policy=msc.POLICY_PLAIN, ctx=gpu_ctx, out=out)
+            for _ in range(2):
+                call()
+                assert np.array_equal(out.to_host(), want)
+        out.free()
+        dg.free()
+
+
 def test_device_pivot(msc, orc, gpu_ctx):
     for seed in range(3):
         g = msc.gen_rmat(12, 8, seed=seed)
