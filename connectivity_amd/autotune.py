@@ -1,6 +1,6 @@
 """Autotuner for a context's launch parameters (msc_tuning).
 
-The level loops have seven tunables that no rule of the reference fixes and that trade
+The level loops have nine tunables that no rule of the reference fixes and that trade
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
@@ -10,7 +10,9 @@ against each other differently on shallow power-law graphs and on deep meshes:
   first_batch       levels enqueued before a loop first reads its `done` word back;
   max_batch         levels enqueued between later reads (batches double up to this);
   concurrent_fwbw   backward search beside the forward one in the same launches (1) or after it;
-  search_batch      levels of a FW-BW / CC search enqueued before its first host check.
+  search_batch      levels of a FW-BW / CC search enqueued before its first host check;
+  local_rounds      levels a workgroup of a colour launch continues on its own claims;
+  local_us          ... while that launch has run for less than this many microseconds.
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -40,6 +42,8 @@ CANDIDATES = {
     "max_batch": [16, 32, 64],
     "concurrent_fwbw": [0, 1],
     "search_batch": [2, 4, 6, 8],
+    "local_rounds": [0, 4, 16, 32, 64],
+    "local_us": [10, 20, 40, 80, 160],
 }
 
 

@@ -19,6 +19,14 @@ constexpr uint32_t kStage = 2048;              // claims staged in LDS per workg
 constexpr int kWPT = MSC_BU_WPT;               // bitmap words per thread of a bottom-up word group
 constexpr int kBW = kBlock * kWPT;             // bitmap words per bottom-up word group
 constexpr int kMaxParts = 8192;                // per-workgroup partial results
+// workgroup-local continuation of colour levels: private frontiers of at most kLocalEdges
+// edges, two per workgroup (ids, offsets, chunk starts)
+#ifndef MSC_LOCAL_EDGES
+#define MSC_LOCAL_EDGES 4096
+#endif
+constexpr uint32_t kLocalEdges = MSC_LOCAL_EDGES;
+constexpr int kLocalHalf = (int)(2 * kStage + kLocalEdges / kChunk + 2);
+constexpr int kLocalStride = 2 * kLocalHalf;
 constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;
 // feature standardisation (mean, std) follows the weights
@@ -107,6 +115,9 @@ struct LevelArgs {
   uint32_t small_edges;       // colour levels over at most this many edges run inside the closer
   int32_t* cmin;              // canonical labels in flight: min member per colour root (null: off)
   int32_t op;                 // Op of this search or sweep (books its algorithmic bytes)
+  int32_t* loc;               // colour levels: kLocalStride ints per workgroup for local rounds (null: none)
+  int32_t local_rounds;       // colour levels: local continuation rounds per launch
+  uint64_t local_ticks;       // colour levels: no local round starts this long (wall clock) after a launch began
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or

@@ -195,6 +195,7 @@ struct Stage {
   uint32_t part[kBlock / 64];
   uint32_t cnt;
   uint32_t base_hi, base_lo;
+  uint32_t go;  // local rounds: thread 0's decision to run another one
 };
 
 // Wave-cooperative: appends the lanes with `take` to the stage (one LDS atomic per wave).
@@ -213,15 +214,22 @@ __device__ __forceinline__ void stage_push(Stage& S, bool take, uint32_t w, uint
   }
 }
 
-// Workgroup-wide (every thread calls it): when more than `keep_below` claims are staged,
-// reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
-// offsets and chunk starts to frontier `nx` (and the ids to `list` at the same slots, when
-// given).  Always a barrier.
-__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr) {
+// Sum of the list lengths of the n staged claims (every thread gets it; LDS only).
+__device__ __forceinline__ uint32_t stage_sum(Stage& S, uint32_t n) {
+  uint32_t sum = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) sum += S.d[i];
+  sum = wave_incl_sum(sum);
+  if (lane() == 63) S.part[wave_id()] = sum;
   __syncthreads();
-  const uint32_t n = S.cnt;
-  __syncthreads();  // no wave may push again before every wave has read the count
-  if (n <= keep_below || n == 0u) return;  // uniform
+  uint32_t total = 0;
+  for (int w = 0; w < kBlock / 64; ++w) total += S.part[w];
+  __syncthreads();  // S.part is reused by the next scan
+  return total;
+}
+
+// Turns the list lengths of the n staged claims into exclusive local edge offsets and
+// returns their sum (every thread gets it).  No trailing barrier.
+__device__ __forceinline__ uint32_t stage_scan(Stage& S, uint32_t n) {
   constexpr int per = (int)(kStage / kBlock);
   const int t = threadIdx.x;
   uint32_t loc[per];
@@ -246,14 +254,16 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
     uint32_t i = (uint32_t)t * per + r;
     if (i < n) S.d[i] = excl + loc[r];
   }
-  if (t == 0) {
-    unsigned long long b = atomicAdd(&c->next_packed, ((unsigned long long)n << 32) | (unsigned long long)total);
-    S.base_hi = (uint32_t)(b >> 32);
-    S.base_lo = (uint32_t)b;
-  }
-  __syncthreads();
-  const uint32_t bh = S.base_hi, bl = S.base_lo;
-  for (uint32_t i = (uint32_t)t; i < n; i += kBlock) {
+  return total;
+}
+
+// Writes the n scanned claims to frontier nx at queue slots [bh, bh + n) and edge offsets
+// from bl: ids, offsets, the chunk starts of every kChunk-edge chunk that begins inside a
+// claim's list, and the ids to `list` when given.  Empties the stage.  Call after a barrier
+// that follows stage_scan; ends with a barrier.
+__device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total, const Frontier& nx, uint32_t bh,
+                                            uint32_t bl, int32_t* list) {
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
     uint32_t lo = S.d[i];
     uint32_t hi = i + 1 < n ? S.d[i + 1] : total;
     uint32_t pos = bh + i, off = bl + lo;
@@ -264,8 +274,27 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
     for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch) nx.starts[ch] = pos;
   }
   __syncthreads();
-  if (t == 0) S.cnt = 0u;
+  if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
+}
+
+// Workgroup-wide (every thread calls it): when more than `keep_below` claims are staged,
+// reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
+// offsets and chunk starts to frontier `nx` (and the ids to `list` at the same slots, when
+// given).  Always a barrier.
+__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr) {
+  __syncthreads();
+  const uint32_t n = S.cnt;
+  __syncthreads();  // no wave may push again before every wave has read the count
+  if (n <= keep_below || n == 0u) return;  // uniform
+  const uint32_t total = stage_scan(S, n);
+  if (threadIdx.x == 0) {
+    unsigned long long b = atomicAdd(&c->next_packed, ((unsigned long long)n << 32) | (unsigned long long)total);
+    S.base_hi = (uint32_t)(b >> 32);
+    S.base_lo = (uint32_t)b;
+  }
+  __syncthreads();
+  stage_write(S, n, total, nx, S.base_hi, S.base_lo, list);
 }
 
 // ------------------------------------------------------------------------------------
@@ -538,7 +567,7 @@ struct ChunkLds {
 // adjacency is read coalesced, and claims / raises are staged for frontier N.  Ends with
 // a workgroup barrier.  Every queued vertex has a non-empty list, so at most len + 1
 // slots overlap a chunk.
-template <int OP, bool MIRROR>
+template <int OP, bool MIRROR, bool LOCAL = false>
 __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, const Frontier& N, uint32_t* inq_cur,
                                          uint32_t* inq_nxt, uint32_t ch, uint32_t nchunks, uint32_t q_size,
                                          uint32_t q_edges, Stage& S, ChunkLds<OP, MIRROR>& L,
@@ -570,7 +599,16 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
       L.split[j] = split;
       L.base2[j] = a.down_off2[v] - split;
     }
-    if (COLOR) L.col[j] = a.colors[v];
+    if (COLOR) {
+      if (LOCAL && OP == OP_PUSH) {
+        // a workgroup-local expansion: the colour after the worklist bit was cleared, read
+        // where atomics are performed (a raise that found the bit set relies on this read)
+        __builtin_amdgcn_s_waitcnt(0);
+        L.col[j] = __hip_atomic_load(&a.colors[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        L.col[j] = a.colors[v];
+      }
+    }
   }
   __syncthreads();
   // inclusive max-scan of own[0..len): thread t owns entries [t*kPer, t*kPer+kPer)
@@ -752,10 +790,55 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
   }
 }
 
+// Local continuation of a colour level (push or per-colour backward): the claims / raises
+// this workgroup staged are expanded by the workgroup itself, from a private frontier, for
+// up to a.local_rounds more levels while they have at most kLocalEdges edges and the
+// launch has run for less than a.local_ticks of the wall clock; whatever is staged when it
+// stops goes to the global next frontier.  A path of workgroup-local hops thus costs one
+// launch instead of one per hop, and the time budget keeps a long chain in a few
+// workgroups from outlasting the next launch, which spreads the work over the whole grid.
+// Push stays exact: a local expansion
+// clears the vertex's worklist bit before it reads the colour where atomics are performed,
+// so a raise that found the bit set is carried by that expansion (max-label propagation
+// reaches the same fixpoint in any order).  Colour-BW claims are final (each vertex's
+// valid bit is cleared once) and colours no longer change, so their expansion is the
+// global one moved earlier.  Every thread calls; (vertices, edges) expanded here are added
+// to loc_n / loc_e by thread 0.
+template <int OP, bool MIRROR>
+__device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* inq, Stage& S,
+                             ChunkLds<OP, MIRROR>& L, unsigned long long& raised, unsigned long long& loc_n,
+                             unsigned long long& loc_e, unsigned long long t_begin) {
+  int32_t* mine = a.loc + (size_t)blockIdx.x * kLocalStride;
+  for (int lr = 0; lr < a.local_rounds; ++lr) {
+    __syncthreads();
+    if (threadIdx.x == 0) S.go = (wall_clock64() - t_begin) < a.local_ticks ? 1u : 0u;
+    __syncthreads();
+    const uint32_t n = S.cnt;
+    const bool go = S.go != 0u;
+    __syncthreads();
+    if (n == 0u || !go) break;  // uniform
+    const uint32_t total = stage_sum(S, n);
+    if (total > kLocalEdges) break;  // uniform: too many for one workgroup; the grid takes them
+    int32_t* pb = mine + (lr & 1) * kLocalHalf;
+    const Frontier P{pb, (uint32_t*)(pb + kStage), (uint32_t*)(pb + 2 * kStage)};
+    stage_scan(S, n);
+    __syncthreads();
+    stage_write(S, n, total, P, 0u, 0u, nullptr);
+    if (threadIdx.x == 0) {
+      loc_n += n;
+      loc_e += total;
+    }
+    const uint32_t nch = (total + kChunk - 1) / kChunk;
+    for (uint32_t ch = 0; ch < nch; ++ch)
+      td_chunk<OP, MIRROR, true>(a, P, N, inq, inq, ch, nch, n, total, S, L, raised);
+  }
+}
+
 // One colour sweep level (push or per-colour backward; work only, k_close_color ends it):
 // edge-balanced expansion of the current frontier, one chunk per workgroup at a time;
 // workgroups without a chunk exit at once (the closer sums only the rows of the ones that
-// ran).  part row = (0, 0, raised, 0).
+// ran); then, with a.loc, the workgroup continues on its own claims (local_rounds).
+// part row = (vertices and edges expanded locally, raised, 0).
 template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   static_assert(OP == OP_PUSH || OP == OP_CBW, "colour ops only; searches run in k_reach");
@@ -775,19 +858,23 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
   const int t = threadIdx.x;
   unsigned long long my_raised = 0;
+  const unsigned long long t_begin = wall_clock64();
   if (t == 0) {
     S.cnt = 0u;
-    if (a.recs) atomicMin(&c->t_start, (unsigned long long)wall_clock64());
+    if (a.recs) atomicMin(&c->t_start, t_begin);
   }
   __syncthreads();
   for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
     td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
+  unsigned long long loc_n = 0, loc_e = 0;
+  if (a.loc && a.local_rounds > 0)
+    local_rounds<OP, MIRROR>(a, N, inq_nxt, S, L, my_raised, loc_n, loc_e, t_begin);
   stage_flush(S, N, c, 0u);
   unsigned long long ra = block_sum64(my_raised, s_red);
   if (t == 0) {
     unsigned long long* p = a.part + 4 * blockIdx.x;
-    p[0] = 0;
-    p[1] = 0;
+    p[0] = loc_n;
+    p[1] = loc_e;
     p[2] = ra;
     p[3] = 0;
   }
@@ -902,10 +989,11 @@ __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int a
                              uint32_t next, uint32_t nedges, const float* feat) {
   // algorithmic bytes of this level (unit costs documented in DESIGN.md)
   unsigned long long bytes;
+  // colour levels: ex / in are the vertices / edges a workgroup expanded locally
   if (op == OP_PUSH)
-    bytes = 12ull * q_size + 9ull * q_edges + 4ull * ra + 4ull * next;
+    bytes = 12ull * (q_size + ex) + 9ull * (q_edges + in) + 4ull * ra + 4ull * next;
   else if (op == OP_CBW)
-    bytes = 12ull * q_size + 9ull * q_edges + 5ull * next;
+    bytes = 12ull * (q_size + ex) + 9ull * (q_edges + in) + 5ull * next;
   else if (bu)
     bytes = 4ull * (unsigned long long)a.nwords + 8ull * ex + 5ull * in + 5ull * next;
   else
@@ -1138,16 +1226,22 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
   {
     const uint32_t q_size = c->q_size, q_edges = c->q_edges;
     const int rows = active_rows(nparts, q_edges);
-    unsigned long long ra = 0;
-    for (int i = t; i < rows; i += kBlock) ra += a.part[4 * i + 2];
+    unsigned long long ln = 0, le = 0, ra = 0;
+    for (int i = t; i < rows; i += kBlock) {
+      ln += a.part[4 * i];
+      le += a.part[4 * i + 1];
+      ra += a.part[4 * i + 2];
+    }
+    ln = block_sum64(ln, s_red);
+    le = block_sum64(le, s_red);
     ra = block_sum64(ra, s_red);
     if (t == 0) {
       const unsigned long long p = c->next_packed;
       const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
-      c->found += next;
+      c->found += next + (int64_t)ln;  // claims expanded locally never reach the frontier
       c->level += 1;
       c->closes += 1;
-      finish_level(c, a, OP, false, 0, q_size, q_edges, 0ull, 0ull, ra, next, nedges, nullptr);
+      finish_level(c, a, OP, false, 0, q_size, q_edges, ln, le, ra, next, nedges, nullptr);
       s_ctl[0] = c->done;
       s_ctl[1] = c->cur;
       s_ctl[2] = c->q_size;

@@ -37,7 +37,7 @@ struct msc_ctx {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 160};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -47,6 +47,7 @@ struct msc_ctx {
   int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
   Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the backward search's (beside the forward one)
   int32_t* lists[3] = {nullptr, nullptr, nullptr};  // vertex lists of the colouring rounds
+  int32_t* loc = nullptr;  // private frontiers of the colour launches' local rounds (per workgroup)
   unsigned long long* part = nullptr;   // kMaxParts x 4 per-workgroup partials
   unsigned long long* part2 = nullptr;  // the same for the backward search beside the forward one
   uint32_t* err = nullptr;              // validation flag
@@ -96,6 +97,12 @@ std::vector<msc_ctx*> g_default;  // one default context per device
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Largest level-kernel grid any tuning allows (blocks_per_cu <= 8, at most kMaxParts).
+int64_t max_grid(const msc_ctx* c) {
+  int64_t g = (int64_t)c->cus * 8;
+  return g < kMaxParts ? g : kMaxParts;
+}
+
 int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   if (n <= c->cap_n && edges <= c->cap_e) return MSC_OK;
   int64_t nn = n > c->cap_n ? n : c->cap_n;
@@ -122,6 +129,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr3
       (size_t)kMaxParts * 4 * 8,                                // partials of the backward search
       (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // colouring vertex lists
+      (size_t)kLocalStride * 4 * (size_t)max_grid(c),           // local-round frontiers
   };
   size_t total = 0;
   for (size_t s : sizes) total += align256(s);
@@ -157,6 +165,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->lists[0] = (int32_t*)parts[26];
   c->lists[1] = (int32_t*)parts[27];
   c->lists[2] = (int32_t*)parts[28];
+  c->loc = (int32_t*)parts[29];
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -672,6 +681,11 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
   }
   if (fast_canon) p.cmin = c->cmin;
+  if (c->tune.local_rounds > 0) {
+    p.loc = c->loc;
+    p.local_rounds = c->tune.local_rounds;
+    p.local_ticks = (uint64_t)((double)c->tune.local_us * c->clock_mhz);
+  }
   const bool has_member = member.ca != nullptr;
   launch_seed_begin(L, c->ctrl, 3);
   {
@@ -701,6 +715,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     r.part = c->part2;
     r.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
     if (fast_canon) r.cmin = c->cmin;
+    r.loc = p.loc;
+    r.local_rounds = p.local_rounds;
+    r.local_ticks = p.local_ticks;
     int push_batch = c->tune.first_batch, cbw_batch = c->tune.first_batch;
     uint64_t before = (uint64_t)n;  // bound of this round's list: the vertices left by the last round
     uint32_t list0 = 0, listed_prev = 0;
@@ -934,7 +951,8 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (!c || !t) return MSC_E_INVALID;
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
-      t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256)
+      t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
+      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;

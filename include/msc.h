@@ -202,6 +202,9 @@ typedef struct msc_tuning {
                                0: after it, restricted to the FW set, the reference's order */
   int32_t search_batch;     /* levels of a FW-BW / CC search enqueued before its first host
                                check (colour loops start with first_batch) */
+  int32_t local_rounds;     /* levels a workgroup of a colour launch continues on its own claims
+                               before handing them to the next launch (0..256; 0: off) */
+  int32_t local_us;         /* ... while the launch has run for less than this many us (1..100000) */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);
