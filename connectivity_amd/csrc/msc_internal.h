@@ -81,6 +81,30 @@ struct Ctrl {
   unsigned long long red_min[4];     // col 3: 0xffffffff - smallest member of the committed set
 };
 
+// Device state of the colouring rounds, which the host enqueues ahead of time: each
+// round's first kernel (k_round_begin) admits the round only if the steps before it have
+// finished, and every later step of the round checks `go`.  A step that finds its
+// predecessor unfinished sets `stall`; nothing after it runs until the host has finished
+// that step and cleared it.
+struct RoundCtl {
+  uint32_t go;        // the round admitted last may run its steps
+  uint32_t stall;     // a step found its predecessor unfinished
+  uint32_t finished;  // colouring complete: no valid vertex left
+  uint32_t admitted;  // rounds admitted so far (the current one is admitted - 1)
+  uint32_t roots_in;  // 1 + index of the last round whose roots ran (0: none)
+  uint32_t error;     // a round removed nothing, or its push settled without a root
+  uint32_t listed0;   // length of round 0's vertex list (the canonical pass walks it)
+  uint32_t pad_;
+  unsigned long long rounds;        // rounds completed
+  unsigned long long push_levels;   // push levels of the completed rounds
+  unsigned long long cbw_levels;    // colour-BW levels of the completed rounds
+  unsigned long long listed_total;  // vertices listed by the completed rounds
+  unsigned long long roots_total;   // roots of the completed rounds
+  unsigned long long push_closes;   // closer launches of the last completed round's push
+  unsigned long long cbw_closes;    // ... and of its colour-BW
+  unsigned long long left;          // valid vertices after the last completed round
+};
+
 struct LevelArgs {
   const uint32_t* down_off;   // top-down lists (first half for OP_CC / mirrored push)
   const int32_t* down_adj;
@@ -118,6 +142,7 @@ struct LevelArgs {
   int32_t* loc;               // colour levels: kLocalStride ints per workgroup for local rounds (null: none)
   int32_t local_rounds;       // colour levels: local continuation rounds per launch
   uint64_t local_ticks;       // colour levels: no local round starts this long (wall clock) after a launch began
+  const RoundCtl* rc;         // colour seeds and roots: run only when rc->go (null: always)
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or
@@ -155,39 +180,52 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a);  //
 void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns);  // work only
 void launch_close_reach(const Launch& L, const ReachArgs& p, int ns);        // one workgroup
 // Members of the pivot's component: valid & ~ca & ~cb, the valid vertices both searches
-// visited (ca, cb: their claimable bitmaps; cb null: one search).  ca null: no member.
+// visited (ca, cb: their claimable bitmaps; cb null: one search).  ca null: no member; none
+// either when a search whose control block is named in g0 / g1 was abandoned.
 struct MemberSrc {
   const uint32_t* ca;
   const uint32_t* cb;
+  const Ctrl* g0;
+  const Ctrl* g1;
 };
 // labels[v] = label and valid &= ~member for every member; part col 0 = count.  With
 // min_red, the label is the smallest member found by launch_min_member
 // (0xffffffff - min_red[3]).  With `seed` (the colouring's arguments), every vertex still
 // valid afterwards is seeded for colouring round 0 (colors[v] = v, queued in frontier 1 of
-// seed->ctrl, written to `list`): run launch_seed_begin on seed->ctrl first and
-// launch_seed_end after.
+// seed->ctrl, written to `list`), and launch_seed_end closes the seed.  With seed->rc, it
+// runs only in an admitted round (after launch_round_begin).
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
                   int32_t* list);
-// part col 3 = 0xffffffff - smallest member in each workgroup's words (0: none)
+// part col 3 = 0xffffffff - smallest member in each workgroup's words (0: none); with rc,
+// only in an admitted round
 int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
-                      unsigned long long* part);
-// labels[v] = cmin[labels[v]] for the `count` vertices of `list`
-void launch_canon_list(const Launch& L, const int32_t* list, uint32_t count, int32_t* labels, const int32_t* cmin);
+                      unsigned long long* part, const RoundCtl* rc);
+// once rc->finished: labels[v] = cmin[labels[v]] for the rc->listed0 vertices of `list`
+// (bound >= that count sizes the grid)
+void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, int32_t* labels,
+                       const int32_t* cmin);
+// the first kernel of a colouring round (k_round_begin); s0 / s1: the searches' control
+// blocks that round 0 (`first`) waits for (null: none); `single`: CC's one fixpoint, whose
+// closing check ends the colouring
+void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                        bool first, bool single);
+// host-directed round state: 0 fresh, 1 resume the admitted round, 2 clear a stall
+void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode);
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
-// control block reset before / frontier hand-over after a seeding kernel
-void launch_seed_begin(const Launch& L, Ctrl* c, int32_t phase);
-void launch_seed_end(const Launch& L, Ctrl* c);
-// colouring round > 0: reset, seed from the previous round's list `in` (length
-// a.ctrl->list_len) into `out`, hand-over; bound >= that length sizes the grid
+// frontier hand-over after a seeding kernel (with rc: only in an admitted round)
+void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc);
+// colouring round > 0, after launch_round_begin: seed from the previous round's list `in`
+// (length a.ctrl->list_len) into `out`, hand-over; bound >= that length sizes the grid
 void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound);
 // colour roots of this round's list (length gate->list_len) into a.ctrl's frontier, once
-// the push fixpoint of `gate` is done (else nothing); reset and hand-over included
+// the push fixpoint of `gate` is done (else the round stalls); reset and hand-over included
 void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list, const Ctrl* gate, int64_t bound);
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
-void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch);
+// canonical labels (minimum member per component); gate: only once *gate != 0 (null: always)
+void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate);
 // pivot: part col 3 = max key; results via launch_reduce then launch_pivot_first
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
                      unsigned long long* part);
@@ -212,7 +250,8 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
 int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
                         const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
                         const LevelArgs* seed, int32_t* list);
-void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng);
+void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng,
+                     const uint32_t* gate);
 // run only after launch_validate passed (reads adjacency through the offsets)
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,
                            uint32_t ng, uint32_t* err);

@@ -333,7 +333,8 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t value, int64_t count) {
     p[i] = value;
 }
 
-__global__ void k_iota(int32_t* p, int64_t count) {
+__global__ void k_iota(int32_t* p, int64_t count, const uint32_t* gate) {
+  if (gate && !*gate) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = (int32_t)i;
 }
@@ -430,18 +431,26 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
 
 // m as normalised by member_src (both pointers set): no branch between the loads, so a
 // thread's member words of several slices load together
-__device__ __forceinline__ uint32_t member_word(const MemberSrc& m, uint32_t valid_word, int64_t i) {
-  return valid_word & ~m.ca[i] & ~m.cb[i];
+__device__ __forceinline__ uint32_t member_word(const MemberSrc& m, uint32_t valid_word, int64_t i, uint32_t keep) {
+  return valid_word & ~m.ca[i] & ~m.cb[i] & keep;
+}
+
+// All ones unless a search named in m was abandoned (then its component is not committed).
+__device__ __forceinline__ uint32_t member_keep(const MemberSrc& m) {
+  return ((m.g0 && m.g0->abandoned) || (m.g1 && m.g1->abandoned)) ? 0u : 0xffffffffu;
 }
 
 // part col 3 = 0xffffffff - smallest member in the workgroup's words (0 when none), so
 // that k_reduce's max yields the smallest member overall.
 __global__ void __launch_bounds__(kBlock) k_min_member(MemberSrc m, const uint32_t* __restrict__ valid,
-                                                       int64_t nwords, unsigned long long* __restrict__ part) {
+                                                       int64_t nwords, unsigned long long* __restrict__ part,
+                                                       const RoundCtl* rc) {
   __shared__ unsigned long long s_red[kBlock / 64];
+  if (rc && !rc->go) return;  // uniform: the searches have not finished
+  const uint32_t keep = member_keep(m);
   unsigned long long best = 0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
-    uint32_t w = member_word(m, valid[i], i);
+    uint32_t w = member_word(m, valid[i], i, keep);
     if (w) {
       unsigned long long key = 0xffffffffull - (unsigned long long)(i * 32 + (__ffs(w) - 1));
       best = key > best ? key : best;
@@ -458,11 +467,13 @@ __global__ void __launch_bounds__(kBlock) k_min_member(MemberSrc m, const uint32
   }
 }
 
-// Canonical labels of the colouring phase: every vertex coloured (the list of colouring
-// round 0) gets the smallest member of its colour root's SCC, tracked in cmin by the
-// per-colour backward search.
-__global__ void __launch_bounds__(kBlock) k_canon_list(const int32_t* __restrict__ list, uint32_t count,
+// Canonical labels of the colouring phase, once it has ended: every vertex coloured (the
+// list of colouring round 0, rc->listed0 long) gets the smallest member of its colour
+// root's SCC, tracked in cmin by the per-colour backward search.
+__global__ void __launch_bounds__(kBlock) k_canon_list(const int32_t* __restrict__ list, const RoundCtl* rc,
                                                        int32_t* __restrict__ labels, const int32_t* __restrict__ cmin) {
+  if (!rc->finished) return;  // uniform: the colouring has not ended yet
+  const uint32_t count = rc->listed0;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
     const int32_t v = list[i];
     labels[v] = cmin[labels[v]];
@@ -514,6 +525,115 @@ __global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t pha
   for (uint32_t ch = lane(); ch < nch; ch += 64) a.fr0.starts[ch] = 0u;
 }
 
+// Thread 0: a fresh control block for a seeding kernel (list_len is kept: the previous
+// seed's list length).
+__device__ __forceinline__ void ctrl_fresh(Ctrl* c, int32_t phase) {
+  c->next_packed = 0ull;
+  c->q_size = 0u;
+  c->q_edges = 0u;
+  c->cur = 0u;
+  c->bottom_up = 0u;
+  c->switched_back = 0u;
+  c->done = 0u;
+  c->level = 0u;
+  c->closes = 0u;
+  c->found = 0;
+  c->phase = (uint32_t)phase;
+  c->abandoned = 0u;
+  c->t_start = ~0ull;
+}
+
+// First kernel of a colouring round (one thread).  Round 0 (flags bit 0) follows the
+// searches: it is admitted once their control blocks s0 / s1 (null: none) say done.  With
+// flags bit 1 (CC's single fixpoint) it ends the colouring once the push has settled.  A
+// later round is admitted once the previous round's push and colour-BW have settled and
+// its roots ran; it books that round (multistep/scc_color.cpp:340-373 loops while vertices
+// are left) and ends the colouring when none is.  An admitted round gets a fresh push
+// control block; a round that cannot be admitted sets rc->stall, which every later step
+// honours until the host clears it.
+__global__ void k_round_begin(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                              int32_t flags) {
+  if (threadIdx.x != 0) return;
+  rc->go = 0u;
+  if (rc->stall || rc->finished || rc->error) return;
+  if (flags & 1) {
+    if ((s0 && !s0->done) || (s1 && !s1->done)) {
+      rc->stall = 1u;
+      return;
+    }
+  } else if (flags & 2) {
+    // a single max-label fixpoint (CC, multistep/cc_color.cpp:47-175): it ends when it settles
+    if (!push->done) {
+      rc->stall = 1u;
+      return;
+    }
+    rc->rounds = 1;
+    rc->push_levels = push->level;
+    rc->finished = 1u;
+    return;
+  } else {
+    if (!push->done || !cbw->done || rc->roots_in != rc->admitted) {
+      rc->stall = 1u;
+      return;
+    }
+    const unsigned long long listed = push->list_len, roots = cbw->list_len;
+    const unsigned long long claimed = (unsigned long long)cbw->found;
+    rc->rounds += 1;
+    rc->push_levels += push->level;
+    rc->cbw_levels += cbw->level;
+    rc->listed_total += listed;
+    rc->roots_total += roots;
+    rc->push_closes = push->closes;
+    rc->cbw_closes = cbw->closes;
+    if (rc->admitted == 1u) rc->listed0 = (uint32_t)listed;
+    // a settled push fixpoint over a non-empty list always has a root (its largest vertex)
+    if ((roots == 0ull && listed > 0ull) || roots + claimed > listed) {
+      rc->error = 1u;
+      return;
+    }
+    rc->left = listed - roots - claimed;
+    if (rc->left == 0ull) {
+      rc->finished = 1u;
+      return;
+    }
+  }
+  ctrl_fresh(push, 3);
+  rc->admitted += 1u;
+  rc->go = 1u;
+}
+
+// Before the colour roots of the admitted round (one thread): the push fixpoint must have
+// settled, else the round stalls; then a fresh colour-BW control block.
+__global__ void k_roots_begin(RoundCtl* rc, const Ctrl* push, Ctrl* cbw) {
+  if (threadIdx.x != 0 || !rc->go) return;
+  if (!push->done) {
+    rc->go = 0u;
+    rc->stall = 1u;
+    return;
+  }
+  ctrl_fresh(cbw, 4);
+  rc->roots_in = rc->admitted;
+}
+
+// Host-directed round state changes: 0 = a fresh colouring (push / colour-BW blocks marked
+// done so that levels enqueued before anything is seeded do nothing), 1 = resume the
+// admitted round (its push was finished by the host), 2 = clear a stall whose round the
+// host finished.
+__global__ void k_rc_set(RoundCtl* rc, Ctrl* push, Ctrl* cbw, int32_t mode) {
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    *rc = RoundCtl{};
+    push->done = 1u;
+    cbw->done = 1u;
+  } else if (mode == 1) {
+    rc->stall = 0u;
+    rc->go = 1u;
+  } else {
+    rc->stall = 0u;
+    rc->go = 0u;
+  }
+}
+
 // Resets the control block before a seeding kernel fills frontier buffer 1.
 __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
   if (threadIdx.x == 0) {
@@ -534,8 +654,10 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
 }
 
 // Closes a seeding launch: the queue built in buffer 1 becomes the frontier (and the
-// length of the vertex list the seed wrote beside it, if any).
-__global__ void k_close_seed(Ctrl* c) {
+// length of the vertex list the seed wrote beside it, if any).  With rc, only in an
+// admitted round (rc->go).
+__global__ void k_close_seed(Ctrl* c, const RoundCtl* rc) {
+  if (rc && !rc->go) return;
   if (threadIdx.x == 0) {
     unsigned long long p = c->next_packed;
     uint32_t next = (uint32_t)(p >> 32);
@@ -1287,6 +1409,7 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
 __global__ void __launch_bounds__(kBlock) k_seed_list(LevelArgs a, const int32_t* __restrict__ in,
                                                       int32_t* __restrict__ out) {
   __shared__ Stage S;
+  if (a.rc && !a.rc->go) return;  // uniform
   const uint32_t in_n = a.ctrl->list_len;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
@@ -1316,7 +1439,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_list(LevelArgs a, const int32_t
 __global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_t* __restrict__ list,
                                                        const Ctrl* __restrict__ gate) {
   __shared__ Stage S;
-  if (!gate->done) return;  // uniform
+  if (!gate->done || (a.rc && !a.rc->go)) return;  // uniform
   const uint32_t nl = gate->list_len;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
@@ -1343,7 +1466,9 @@ __global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_
 // as the identity; a raw label is always a member of its component, so only vertices
 // below their label can lower it.  Lanes of a wave sharing a label combine into one
 // atomic (the pivot's SCC holds ~35% of all vertices at s24).
-__global__ void k_canon_min(int32_t n, const int32_t* __restrict__ labels, int32_t* __restrict__ mins) {
+__global__ void k_canon_min(int32_t n, const int32_t* __restrict__ labels, int32_t* __restrict__ mins,
+                            const uint32_t* gate) {
+  if (gate && !*gate) return;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t v = base + threadIdx.x;
     int32_t l = v < n ? labels[v] : -1;
@@ -1360,7 +1485,9 @@ __global__ void k_canon_min(int32_t n, const int32_t* __restrict__ labels, int32
     }
   }
 }
-__global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int32_t* __restrict__ mins) {
+__global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int32_t* __restrict__ mins,
+                              const uint32_t* gate) {
+  if (gate && !*gate) return;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     int32_t l = labels[v];
     if (l >= 0) labels[v] = mins[l];
@@ -1706,6 +1833,8 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
                                                    int32_t* __restrict__ list) {
   __shared__ unsigned long long s_red[kBlock / 64];
   __shared__ Stage S;
+  if (a.rc && !a.rc->go) return;  // uniform: the searches have not finished
+  const uint32_t keep = member_keep(m);
   if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
   if (SEED) {
     if (threadIdx.x == 0) S.cnt = 0u;
@@ -1722,7 +1851,7 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
     for (int r = 0; r < kU; ++r) {
       const int64_t wi = (wg + r * kBlock + threadIdx.x) >> 5;  // bitmaps carry slack words past n
       vw[r] = valid[wi];
-      w[r] = member_word(m, vw[r], wi);
+      w[r] = member_word(m, vw[r], wi, keep);
     }
 #pragma unroll
     for (int r = 0; r < kU; ++r) {
@@ -1764,7 +1893,9 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
 // Union ids -> ids local to each graph: labels[v] -= base[graph of v] (labels are vertices
 // of the same graph).
 __global__ void __launch_bounds__(kBlock) k_localise(int32_t n, int32_t* __restrict__ labels,
-                                                     const int64_t* __restrict__ base, uint32_t ng) {
+                                                     const int64_t* __restrict__ base, uint32_t ng,
+                                                     const uint32_t* gate) {
+  if (gate && !*gate) return;
   for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
     const int64_t v = wb + lane();
     const uint32_t g = wave_graph_of(base, ng, wb, n);
@@ -1922,16 +2053,28 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
 }
 
 int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
-                      unsigned long long* part) {
+                      unsigned long long* part, const RoundCtl* rc) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part);
+  hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part,
+                     rc);
   return g;
 }
 
-void launch_canon_list(const Launch& L, const int32_t* list, uint32_t count, int32_t* labels, const int32_t* cmin) {
-  if (count == 0u) return;
-  hipLaunchKernelGGL(k_canon_list, dim3(grid_for(count, kBlock, L.grid * 8)), dim3(kBlock), 0, L.stream, list,
-                     count, labels, cmin);
+void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, int32_t* labels,
+                       const int32_t* cmin) {
+  if (bound <= 0) return;
+  hipLaunchKernelGGL(k_canon_list, dim3(grid_for(bound, kBlock, L.grid * 8)), dim3(kBlock), 0, L.stream, list, rc,
+                     labels, cmin);
+}
+
+void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                        bool first, bool single) {
+  hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, s0, s1,
+                     (first ? 1 : 0) | (single ? 2 : 0));
+}
+
+void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode) {
+  hipLaunchKernelGGL(k_rc_set, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, mode);
 }
 
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {
@@ -1947,32 +2090,27 @@ void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_
                      b, words);
 }
 
-void launch_seed_begin(const Launch& L, Ctrl* c, int32_t phase) {
-  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, c, phase);
-}
-
-void launch_seed_end(const Launch& L, Ctrl* c) {
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, c);
+void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc) {
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, c, rc);
 }
 
 void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound) {
-  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 3);
   hipLaunchKernelGGL(k_seed_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, in, out);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc);
 }
 
 void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list, const Ctrl* gate, int64_t bound) {
-  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, 4);
+  hipLaunchKernelGGL(k_roots_begin, dim3(1), dim3(64), 0, L.stream, const_cast<RoundCtl*>(a.rc), gate, a.ctrl);
   hipLaunchKernelGGL(k_roots_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, list, gate);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc);
 }
 
-void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch) {
+void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate) {
   if (n <= 0) return;
   dim3 g(grid_for(n, kBlock, L.grid * 4));
-  hipLaunchKernelGGL(k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n);
-  hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
-  hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch);
+  hipLaunchKernelGGL(k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n, gate);
+  hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
+  hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
 }
 
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part) {
@@ -2012,13 +2150,14 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
   dim3 g((unsigned)grid_for(ng, kBlock, L.grid));
   if (mirror) hipLaunchKernelGGL(k_seed_pivots<true>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
   else hipLaunchKernelGGL(k_seed_pivots<false>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, nullptr);
 }
 
-void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng) {
+void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng,
+                     const uint32_t* gate) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_localise, dim3(grid_for(n, kBlock * 4, L.grid * 4)), dim3(kBlock), 0, L.stream, n, labels,
-                     base, ng);
+                     base, ng, gate);
 }
 
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,

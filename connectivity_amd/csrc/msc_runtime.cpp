@@ -22,6 +22,10 @@ using namespace msc;
 namespace {
 
 constexpr uint32_t kRecCap = 1u << 16;
+// control blocks: [0] forward / CC search, [1] backward search, [2] colour push,
+// [3] colour-BW, followed by the colouring rounds' state, all copied back together
+constexpr int kCtrls = 4;
+constexpr size_t kCtrlBytes = kCtrls * sizeof(Ctrl) + sizeof(RoundCtl);
 
 struct Ev {
   hipEvent_t a, b;
@@ -53,6 +57,7 @@ struct msc_ctx {
   uint32_t* err = nullptr;              // validation flag
   Ctrl* ctrl = nullptr;
   Ctrl* ctrl2 = nullptr;                // that backward search's control block: ctrl + 1
+  RoundCtl* rctl = nullptr;             // colouring rounds' state, after the kCtrls blocks
   msc_level_rec* recs = nullptr;
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
@@ -71,8 +76,9 @@ struct msc_ctx {
   void* in_buf = nullptr;
   size_t in_bytes = 0;
   // host side
-  Ctrl* ctrl_host = nullptr;   // pinned copies of ctrl and ctrl2 (adjacent, like on device)
+  Ctrl* ctrl_host = nullptr;   // pinned copies of the kCtrls blocks and the round state
   Ctrl* ctrl_host2 = nullptr;  // ctrl_host + 1
+  RoundCtl* rctl_host = nullptr;
   uint32_t* pinned_u32 = nullptr;
   std::vector<Ev> events;
   size_t ev_used = 0;
@@ -123,7 +129,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr1
       (size_t)kMaxParts * 4 * 8,                                // partials
       256,                                                      // err
-      2 * sizeof(Ctrl),                                         // ctrl, ctrl2
+      kCtrlBytes,                                               // control blocks, round state
       (size_t)kRecCap * sizeof(msc_level_rec),                  // records (two halves)
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr2
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr3
@@ -162,6 +168,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->fr3 = Frontier{(int32_t*)parts[22], (uint32_t*)parts[23], (uint32_t*)parts[24]};
   c->part2 = (unsigned long long*)parts[25];
   c->ctrl2 = c->ctrl + 1;
+  c->rctl = (RoundCtl*)(c->ctrl + kCtrls);
   c->lists[0] = (int32_t*)parts[26];
   c->lists[1] = (int32_t*)parts[27];
   c->lists[2] = (int32_t*)parts[28];
@@ -211,9 +218,9 @@ struct Timed {
 
 Launch launch_of(const msc_ctx* c) { return Launch{c->stream, c->grid}; }
 
-// Both control blocks (ctrl, ctrl2) back to their pinned host copies, one copy.
+// Every control block and the round state back to their pinned host copies, one copy.
 int read_ctrl(msc_ctx* c) {
-  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, 2 * sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, kCtrlBytes, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MSC_OK;
 }
@@ -260,13 +267,12 @@ int enqueue_levels(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kc
 
 // Level loop: enqueues levels in growing batches until the control blocks say done.  After
 // each batch the blocks come back with one copy and one synchronisation (into the host
-// copy of the same block: ctrl -> ctrl_host[0], ctrl2 -> ctrl_host[1]), and they are
+// copy of the same block: ctrl[i] -> ctrl_host[i]), and they are
 // current when this returns.  Every search and max-label fixpoint settles within n + 1
 // levels, so more launches than that mean a fault.
 int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, msc_stats* s, int first_batch) {
-  if (as[0].ctrl != c->ctrl && as[0].ctrl != c->ctrl2) return MSC_E_INVALID;
+  if (as[0].ctrl < c->ctrl || as[0].ctrl + k > c->ctrl + kCtrls) return MSC_E_INVALID;
   Ctrl* host = c->ctrl_host + (as[0].ctrl - c->ctrl);
-  if (host + k > c->ctrl_host + 2) return MSC_E_INVALID;
   int batch = first_batch > 0 ? first_batch : c->tune.first_batch;
   const int max_batch = c->tune.max_batch;
   const int64_t limit = (int64_t)as[0].n_total + 64;
@@ -538,8 +544,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   Launch L = launch_of(c);
   int32_t* labels = o->labels_where == MSC_DEVICE ? labels_out : c->labels;
   const int64_t nw = ((int64_t)n + 31) / 32;
-  HIP_TRY(hipMemsetAsync(c->ctrl, 0, sizeof(Ctrl), c->stream));
-  HIP_TRY(hipMemsetAsync(c->ctrl2, 0, sizeof(Ctrl), c->stream));
+  HIP_TRY(hipMemsetAsync(c->ctrl, 0, kCtrlBytes, c->stream));
 
   if (avg < 0.0) avg = cc ? 2.0 * (double)g.m / (double)n : (double)g.m / (double)n;
   if (bt) {
@@ -589,11 +594,24 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // its smallest member, colour SCCs track theirs in cmin; the batch and WCC paths
   // relabel at the end instead
   const bool fast_canon = !cc && !bt && o->label_mode == MSC_LABELS_CANONICAL;
+  // control blocks: [0] forward (or CC) search, [1] backward search, [2] colour push,
+  // [3] colour-BW; rcd: the colouring rounds' device state
+  Ctrl* cs = c->ctrl;
+  RoundCtl* rcd = c->rctl;
+  const Ctrl* hs = c->ctrl_host;
+  const RoundCtl* hr = c->rctl_host;
 
-  // FW-BW (SCC) or the CC search from the pivot; a search the policy abandons leaves
-  // every vertex to colouring
-  MemberSrc member{};  // none unless a search finished
+  // FW-BW (SCC) or the CC search from the pivot.  Only the searches' first batch of levels
+  // is enqueued here: the commit and the colouring rounds follow right behind it and wait
+  // on device for the searches to finish (k_round_begin), so the device never idles for a
+  // host round trip between the phases.  A search the policy abandons commits nothing and
+  // leaves every vertex to colouring.
+  MemberSrc member{};  // none unless a search runs
   const bool cand_ready = o->trim_level != 2;  // the degree trim wrote b1 (and b3) = valid
+  LevelArgs sa[2] = {a, a};
+  int nsearch = 0, sop = OP_FW;
+  bool smirror = false;
+  const Ctrl *s0 = nullptr, *s1 = nullptr;
   if (pivot >= 0) {
     if (!cc) {
       // FW: allow = valid, cand = b1, top-down on out-lists, bottom-up on in-lists
@@ -614,7 +632,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bw.up_off = g.out_off; bw.up_adj = g.out_adj;
       if (c->tune.concurrent_fwbw) {
         bw.allow = c->valid; bw.cand = c->b3;
-        bw.ctrl = c->ctrl2; bw.part = c->part2;
+        bw.ctrl = &cs[1]; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
         if (!cand_ready) launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);  // cand = allow, for both
@@ -624,30 +642,31 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         } else {
           launch_search_init_pair(L, fw, bw, pivot);
         }
-        LevelArgs both[2] = {fw, bw};
-        if ((rc = run_loop(c, OP_FW, false, both, 2, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
-        if (s) {
-          s->fw_levels = (int32_t)c->ctrl_host[0].level;
-          s->bw_levels = (int32_t)c->ctrl_host[1].level;
-        }
+        sa[0] = fw;
+        sa[1] = bw;
+        nsearch = 2;
+        s0 = &cs[0];
+        s1 = &cs[1];
         // SCC = FW set & BW set = valid & ~b1 & ~b3
-        if (!c->ctrl_host[0].abandoned && !c->ctrl_host[1].abandoned) member = MemberSrc{c->b1, c->b3};
+        member = MemberSrc{c->b1, c->b3, &cs[0], &cs[1]};
       } else {
         if (!cand_ready) launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
         if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
         else launch_search_init(L, fw, pivot, 0);
         if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
-        if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
-        if (!c->ctrl_host->abandoned) {
-          // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy)
+        if (s) s->fw_levels = (int32_t)hs[0].level;
+        if (!hs[0].abandoned) {
+          // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy), control block [0] again
           launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
           bw.allow = c->b2; bw.cand = c->b3;
           if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
           else launch_search_init(L, bw, pivot, 1);
-          if ((rc = run_levels(c, OP_BW, false, bw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
-          if (s) s->bw_levels = (int32_t)c->ctrl_host->level;
+          sa[0] = bw;
+          nsearch = 1;
+          sop = OP_BW;
+          s0 = &cs[0];
           // SCC = fw set & visited by BW = (valid & ~b1) & ~b3, as b3 started as the fw set
-          if (!c->ctrl_host->abandoned) member = MemberSrc{c->b1, c->b3};
+          member = MemberSrc{c->b1, c->b3, &cs[0], nullptr};
         }
       }
     } else {
@@ -661,20 +680,23 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bf.allow = c->valid; bf.cand = c->b1;
       if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
       else launch_search_init(L, bf, pivot, 2);
-      if ((rc = run_levels(c, OP_CC, true, bf, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
-      if (s) s->fw_levels = (int32_t)c->ctrl_host->level;
-      if (!c->ctrl_host->abandoned) member = MemberSrc{c->b1, nullptr};  // valid & visited
+      sa[0] = bf;
+      nsearch = 1;
+      sop = OP_CC;
+      smirror = true;
+      s0 = &cs[0];
+      member = MemberSrc{c->b1, nullptr, &cs[0], nullptr};  // valid & visited
     }
+    if ((rc = enqueue_levels(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, c->tune.search_batch))) return rc;
   }
 
-  // commit of the pivot's component fused with the seed of colouring round 0: one pass
-  // over the vertices labels the members, drops them from `valid`, and queues every vertex
-  // left (colors[v] = v) in the colouring's first frontier and in list 0 (multistep/
-  // scc_fwbw.cpp:412-466 then scc_color.cpp:56-63; cc_bfs.cpp:217-271 then
-  // cc_color.cpp:62-69)
+  // the colouring's arguments: push on control block [2], colour-BW on [3]
   LevelArgs p = a;
   p.op = OP_PUSH;
   p.nvalid = nullptr;
+  p.ctrl = &cs[2];
+  p.recs = nullptr;  // level records are the searches'
+  p.rc = rcd;
   p.down_off = g.out_off; p.down_adj = g.out_adj;
   if (cc) {
     p.colors = labels;  // the CC fixpoint writes straight into the labels
@@ -686,127 +708,162 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.local_rounds = c->tune.local_rounds;
     p.local_ticks = (uint64_t)((double)c->tune.local_us * c->clock_mhz);
   }
-  const bool has_member = member.ca != nullptr;
-  launch_seed_begin(L, c->ctrl, 3);
-  {
-    Timed t(c, MSC_K_COMMIT);
-    const bool want_min = fast_canon && has_member;
-    if (want_min) launch_reduce(L, c->part, launch_min_member(L, member, c->valid, nw, c->part), c->ctrl->red_min);
-    np = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p, c->lists[0])
-            : launch_commit(L, member, c->valid, n, labels, pivot, want_min ? c->ctrl->red_min : nullptr, c->part, &p,
-                            c->lists[0]);
-    launch_reduce(L, c->part, np, c->ctrl->red_commit);
-  }
-  launch_seed_end(L, c->ctrl);
+  LevelArgs r = p;
+  r.op = OP_CBW;
+  r.colors = c->colors;
+  r.down_off = g.in_off; r.down_adj = g.in_adj;
+  r.down_off2 = nullptr; r.down_adj2 = nullptr;
+  r.ctrl = &cs[3];
+  r.part = c->part2;
 
-  // colouring
-  if (!cc) {
-    // Rounds until nothing is left (kokkos_scc/scc_color.cpp:553-677).  A round enqueues
-    // its push levels, the colour roots (which run only once the push fixpoint is done) and
-    // the colour-BW levels, then reads both control blocks back once (push: ctrl, colour-BW:
-    // ctrl2); only a round that needed more levels than enqueued pays more round trips.
-    // The previous round's closer launches predict this round's level counts.  Round r works
-    // on list r: list 0 is kept for the canonical pass, later rounds alternate 1 and 2.
-    LevelArgs r = a;
-    r.op = OP_CBW;
-    r.nvalid = nullptr;
-    r.down_off = g.in_off; r.down_adj = g.in_adj;
-    r.ctrl = c->ctrl2;
-    r.part = c->part2;
-    r.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
-    if (fast_canon) r.cmin = c->cmin;
-    r.loc = p.loc;
-    r.local_rounds = p.local_rounds;
-    r.local_ticks = p.local_ticks;
-    int push_batch = c->tune.first_batch, cbw_batch = c->tune.first_batch;
-    uint64_t before = (uint64_t)n;  // bound of this round's list: the vertices left by the last round
-    uint32_t list0 = 0, listed_prev = 0;
-    int cur = 0;
-    const Ctrl* h = c->ctrl_host;  // [0]: push, [1]: colour-BW
-    for (int round = 0;; ++round) {
-      if (round > 0) {
-        const int nxt = cur == 1 ? 2 : 1;
-        Timed t(c, MSC_K_ROOTS);
-        launch_seed_list(L, p, c->lists[cur], c->lists[nxt], (int64_t)listed_prev);
-        cur = nxt;
-      }
-      enqueue_levels(c, OP_PUSH, false, &p, 1, MSC_K_COLOR_PROP, push_batch);
+  // Round r works on list r: list 0 (written by the commit) is kept for the canonical pass,
+  // later rounds alternate lists 1 and 2.
+  auto list_of = [&](int rr) { return c->lists[rr == 0 ? 0 : 1 + ((rr - 1) & 1)]; };
+  int push_batch = c->tune.first_batch, cbw_batch = c->tune.first_batch;
+  // the steps of admitted round rr: round 0 is the commit of the pivot's component fused
+  // with the seed of colouring round 0 (multistep/scc_fwbw.cpp:412-466 then
+  // scc_color.cpp:56-63; cc_bfs.cpp:217-271 then cc_color.cpp:62-69), a later round seeds
+  // from the previous round's list; then push levels, roots (scc_color.cpp:165-212) and
+  // colour-BW levels (scc_color.cpp:214-289)
+  auto round_steps = [&](int rr) -> int {
+    int e;
+    if (rr == 0) {
+      Timed t(c, MSC_K_COMMIT);
+      const bool want_min = fast_canon && member.ca;
+      if (want_min)
+        launch_reduce(L, c->part, launch_min_member(L, member, c->valid, nw, c->part, rcd), c->ctrl->red_min);
+      const int npc = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p,
+                                               list_of(0))
+                         : launch_commit(L, member, c->valid, n, labels, pivot, want_min ? c->ctrl->red_min : nullptr,
+                                         c->part, &p, list_of(0));
+      launch_reduce(L, c->part, npc, c->ctrl->red_commit);
+      launch_seed_end(L, p.ctrl, rcd);
+    } else {
+      Timed t(c, MSC_K_ROOTS);
+      launch_seed_list(L, p, list_of(rr - 1), list_of(rr), n);
+    }
+    if ((e = enqueue_levels(c, OP_PUSH, cc, &p, 1, MSC_K_COLOR_PROP, push_batch))) return e;
+    if (!cc) {
       {
         Timed t(c, MSC_K_ROOTS);
-        launch_roots_list(L, r, c->lists[cur], c->ctrl, (int64_t)before);
+        launch_roots_list(L, r, list_of(rr), p.ctrl, n);
       }
-      enqueue_levels(c, OP_CBW, false, &r, 1, MSC_K_COLOR_BW, cbw_batch);
-      if ((rc = read_ctrl(c))) return rc;
-      if (s) s->host_syncs += 1;
-      if (!h[0].done) {  // the roots above were skipped
-        if ((rc = run_levels(c, OP_PUSH, false, p, MSC_K_COLOR_PROP, s, push_batch * 2))) return rc;
-        {
-          Timed t(c, MSC_K_ROOTS);
-          launch_roots_list(L, r, c->lists[cur], c->ctrl, (int64_t)before);
-        }
-        if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch))) return rc;
-      } else if (!h[1].done) {
-        if ((rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch * 2))) return rc;
-      }
-      const uint32_t listed = h[0].list_len, roots = h[1].list_len;
-      const uint64_t claimed = (uint64_t)h[1].found;
-      if (round == 0) list0 = listed;
-      if (roots == 0u && listed > 0u) return MSC_E_DEVICE;  // a settled fixpoint always has a root
-      const uint64_t left = (uint64_t)listed - roots - claimed;
-      if (s) {
-        s->color_iters += (int32_t)h[0].level;
-        s->cbw_levels += (int32_t)h[1].level;
-        s->color_rounds += 1;
-        // list scans (id + colour per entry) and per root its label, cmin, offsets and slot
-        s->k[MSC_K_ROOTS].bytes += 8.0 * listed + 24.0 * roots + (round > 0 ? 17.0 * listed : 0.0);
-      }
-      push_batch = h[0].closes > 2 ? (h[0].closes < (uint32_t)c->tune.max_batch ? (int)h[0].closes : c->tune.max_batch)
-                                   : 2;
-      cbw_batch = h[1].closes > 2 ? (h[1].closes < (uint32_t)c->tune.max_batch ? (int)h[1].closes : c->tune.max_batch)
-                                  : 2;
-      if (left == 0) break;
-      if (left >= listed) return MSC_E_DEVICE;  // every round removes at least one SCC
-      before = left;
-      listed_prev = listed;
+      if ((e = enqueue_levels(c, OP_CBW, false, &r, 1, MSC_K_COLOR_BW, cbw_batch))) return e;
     }
-    if (fast_canon) {
+    return MSC_OK;
+  };
+  const uint32_t* done_gate = &rcd->finished;
+  // rounds [from, from + count), the check that books the last of them, and the final
+  // relabelling, which runs only once the colouring has ended
+  auto enqueue_rounds = [&](int from, int count) -> int {
+    int e;
+    for (int rr = from; rr < from + count; ++rr) {
+      launch_round_begin(L, rcd, p.ctrl, r.ctrl, rr == 0 ? s0 : nullptr, rr == 0 ? s1 : nullptr, rr == 0, false);
+      if ((e = round_steps(rr))) return e;
+    }
+    launch_round_begin(L, rcd, p.ctrl, r.ctrl, nullptr, nullptr, false, cc);
+    {
       Timed t(c, MSC_K_OTHER);
-      launch_canon_list(L, c->lists[0], list0, labels, c->cmin);
+      if (fast_canon) launch_canon_list(L, list_of(0), rcd, n, labels, c->cmin);
+      if (o->label_mode == MSC_LABELS_CANONICAL && !fast_canon) launch_canon(L, n, labels, c->colors, done_gate);
+      if (bt) launch_localise(L, n, labels, c->bbase, ng, done_gate);
     }
-  } else {
-    // one max-label fixpoint written straight into the labels (multistep/cc_color.cpp:47-175)
-    if ((rc = run_levels(c, OP_PUSH, true, p, MSC_K_COLOR_PROP, s, 0))) return rc;
-    if (s) {
-      s->color_iters = (int32_t)c->ctrl_host->level;
-      s->color_rounds = 1;
-    }
-  }
+    HIP_TRY(hipGetLastError());
+    return MSC_OK;
+  };
 
-  if (o->label_mode == MSC_LABELS_CANONICAL && !fast_canon) {
-    Timed t(c, MSC_K_OTHER);
-    launch_canon(L, n, labels, c->colors);
+  // Enqueue ahead, then read all control blocks back once.  Whatever a stalled step left
+  // unfinished the host completes (a search, a push fixpoint, the roots, a colour-BW
+  // search), then enqueues the following rounds again; the number of rounds enqueued
+  // ahead doubles up to 32.
+  launch_rc_set(L, rcd, p.ctrl, r.ctrl, 0);
+  int ahead = cc ? 0 : 2;
+  if ((rc = enqueue_rounds(0, cc ? 1 : ahead))) return rc;
+  const int64_t max_rounds = (int64_t)n + 2;
+  for (int64_t trips = 0;; ++trips) {
+    if (trips > max_rounds) return MSC_E_DEVICE;
+    if ((rc = read_ctrl(c))) return rc;
+    if (s) s->host_syncs += 1;
+    if (hr->error) return MSC_E_DEVICE;
+    if (hr->finished) break;
+    if (hr->rounds > 0 && !cc) {
+      // the last completed round predicts the next rounds' level counts
+      const int mb = c->tune.max_batch;
+      push_batch = hr->push_closes > 2 ? (int)(hr->push_closes < (unsigned long long)mb ? hr->push_closes : mb) : 2;
+      cbw_batch = hr->cbw_closes > 2 ? (int)(hr->cbw_closes < (unsigned long long)mb ? hr->cbw_closes : mb) : 2;
+    }
+    ahead = ahead * 2 < 32 ? (ahead < 1 ? 1 : ahead * 2) : 32;
+    int from;
+    if (hr->stall && hr->admitted == 0) {
+      // the searches needed more levels than enqueued: finish them, then start over
+      if ((rc = run_loop(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
+      launch_rc_set(L, rcd, p.ctrl, r.ctrl, 0);
+      from = 0;
+    } else if (hr->stall) {
+      // round admitted - 1 stalled: finish its push, its roots and its colour-BW search
+      const int rr = (int)hr->admitted - 1;
+      if (!hs[2].done && (rc = run_levels(c, OP_PUSH, cc, p, MSC_K_COLOR_PROP, s, push_batch * 2))) return rc;
+      if (!cc) {
+        bool cbw_left = !hs[3].done;
+        if (hr->roots_in != hr->admitted) {
+          launch_rc_set(L, rcd, p.ctrl, r.ctrl, 1);
+          Timed t(c, MSC_K_ROOTS);
+          launch_roots_list(L, r, list_of(rr), p.ctrl, n);
+          cbw_left = true;
+        }
+        if (cbw_left && (rc = run_levels(c, OP_CBW, false, r, MSC_K_COLOR_BW, s, cbw_batch))) return rc;
+      }
+      launch_rc_set(L, rcd, p.ctrl, r.ctrl, 2);
+      from = rr + 1;
+    } else {
+      // every enqueued round ran: the check after the last one admitted the next round
+      if (cc || hr->admitted == 0) return MSC_E_DEVICE;  // CC's closing check ends or stalls
+      from = (int)hr->admitted - 1;
+      if ((rc = round_steps(from))) return rc;
+      ++from;
+    }
+    if (cc) {
+      // one fixpoint: start it over after the search, else just check it again
+      if (from == 0) {
+        launch_round_begin(L, rcd, p.ctrl, r.ctrl, s0, s1, true, false);
+        if ((rc = round_steps(0))) return rc;
+      }
+      if ((rc = enqueue_rounds(1, 0))) return rc;
+    } else if ((rc = enqueue_rounds(from, ahead))) return rc;
   }
-  if (bt) {
-    Timed t(c, MSC_K_OTHER);
-    launch_localise(L, n, labels, c->bbase, ng);
+  if (s) {
+    s->color_rounds = (int32_t)hr->rounds;
+    s->color_iters = (int32_t)hr->push_levels;
+    s->cbw_levels = (int32_t)hr->cbw_levels;
+    if (nsearch == 2) {
+      s->fw_levels = (int32_t)hs[0].level;
+      s->bw_levels = (int32_t)hs[1].level;
+    } else if (nsearch == 1) {
+      if (sop == OP_BW) s->bw_levels = (int32_t)hs[0].level;
+      else s->fw_levels = (int32_t)hs[0].level;
+    }
+    // list scans (id + colour per entry), per root its label, cmin, offsets and slot, and
+    // the later rounds' seeds (id, valid bit, colour, offsets, slot, list entry)
+    s->k[MSC_K_ROOTS].bytes += 8.0 * (double)hr->listed_total + 24.0 * (double)hr->roots_total +
+                               17.0 * (double)(hr->listed_total - hr->listed0);
   }
   if (o->labels_where == MSC_HOST)
     HIP_TRY(hipMemcpyAsync(labels_out, labels, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipGetLastError());
-  if ((rc = read_ctrl(c))) return rc;
-  c->last_nrec = c->ctrl_host->nrec < kRecCap / 2 ? c->ctrl_host->nrec : kRecCap / 2;
-  c->last_nrec2 = c->ctrl_host2->nrec < kRecCap / 2 ? c->ctrl_host2->nrec : kRecCap / 2;
+  if (o->labels_where == MSC_HOST && (rc = read_ctrl(c))) return rc;
+  c->last_nrec = hs[0].nrec < kRecCap / 2 ? hs[0].nrec : kRecCap / 2;
+  c->last_nrec2 = hs[1].nrec < kRecCap / 2 ? hs[1].nrec : kRecCap / 2;
   auto t1 = std::chrono::steady_clock::now();
   if (s) {
     s->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     s->num_trim = n - (int32_t)c->ctrl_host->red_trim[0];
     s->num_fwbw = (int32_t)c->ctrl_host->red_commit[0];
     if (c->ctrl_host->red_commit[0]) s->k[MSC_K_COMMIT].bytes += 8.0 * nw + 4.0 * (double)c->ctrl_host->red_commit[0];
-    const bool two = !cc && c->tune.concurrent_fwbw;  // ctrl_host2 is this run's backward search
-    s->bu_levels = (int32_t)(c->ctrl_host->bu_levels + (two ? c->ctrl_host2->bu_levels : 0));
+    // every block was zeroed at the start: each holds this run's own levels only
+    s->bu_levels = (int32_t)(hs[0].bu_levels + hs[1].bu_levels);
     for (int op = 0; op < 5; ++op) {
       int cls = op == OP_PUSH ? MSC_K_COLOR_PROP : op == OP_CBW ? MSC_K_COLOR_BW : MSC_K_LEVEL;
-      s->k[cls].bytes += (double)c->ctrl_host->bytes_op[op] + (two ? (double)c->ctrl_host2->bytes_op[op] : 0.0);
+      for (int b = 0; b < kCtrls; ++b) s->k[cls].bytes += (double)hs[b].bytes_op[op];
     }
     ev_collect(c, s);
     s->bytes = 0;
@@ -896,7 +953,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     }
     c->own_stream = true;
   }
-  if (hipHostMalloc((void**)&c->ctrl_host, 2 * sizeof(Ctrl) + 64, hipHostMallocDefault) != hipSuccess ||
+  if (hipHostMalloc((void**)&c->ctrl_host, kCtrlBytes + 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->sched_host, 3 * MSC_SCHED_LEVELS, hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&c->small, align256((size_t)kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
@@ -905,6 +962,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     return MSC_E_OOM;
   }
   c->ctrl_host2 = c->ctrl_host + 1;
+  c->rctl_host = (RoundCtl*)(c->ctrl_host + kCtrls);
   c->mlp = (float*)c->small;
   c->sched = (int8_t*)c->small + align256((size_t)kMlpBlob * 4);
   int khz = 0;
