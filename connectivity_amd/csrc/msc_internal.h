@@ -188,12 +188,12 @@ struct MemberSrc {
   const Ctrl* g0;
   const Ctrl* g1;
 };
-// labels[v] = label and valid &= ~member for every member; part col 0 = count.  With
-// min_red, the label is the smallest member found by launch_min_member
-// (0xffffffff - min_red[3]).  With `seed` (the colouring's arguments), every vertex still
-// valid afterwards is seeded for colouring round 0 (colors[v] = v, queued in frontier 1 of
-// seed->ctrl, written to `list`), and launch_seed_end closes the seed.  With seed->rc, it
-// runs only in an admitted round (after launch_round_begin).
+// labels[v] = label and valid &= ~member for every member, labels[v] = v for every other
+// vertex; part col 0 = count.  With min_red, the label is the smallest member found by
+// launch_min_member (0xffffffff - min_red[3]).  With `seed` (the colouring's arguments),
+// every vertex still valid afterwards is then seeded for colouring round 0 (colors[v] = v,
+// queued in frontier 1 of seed->ctrl, written to `list`), and launch_seed_end closes the
+// seed.  With seed->rc, both run only in an admitted round (after launch_round_begin).
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
                   int32_t* list);

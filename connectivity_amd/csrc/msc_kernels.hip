@@ -123,26 +123,32 @@ __device__ __forceinline__ unsigned long long block_sum64(unsigned long long x, 
 // ------------------------------------------------------------------------------------
 // workgroup-level walk over the set bits of a bitmap
 
-struct BitIter {
-  uint32_t word[kBW];
-  uint32_t pref[kBW + 1];
+// A group of kBlock * WPT bitmap words: the words and the exclusive prefix of their popcounts.
+template <int WPT>
+struct BitIterT {
+  static constexpr int kWords = kBlock * WPT;
+  uint32_t word[kWords];
+  uint32_t pref[kWords + 1];
   uint32_t wsum[kBlock / 64];
 };
+using BitIter = BitIterT<kWPT>;  // the bottom-up search's word groups (kBW words)
+static_assert(BitIter::kWords == kBW, "bottom-up group width");
 
-// Loads words [wb, wb + kBW) (kWPT consecutive words per thread) and the exclusive prefix
-// of their popcounts; returns the set bits in the group.  All threads call; ends with a
-// barrier.
-__device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, int64_t nwords, int64_t wb) {
-  static_assert(kBW == kBlock * kWPT, "kWPT words per thread");
+// Loads words [wb, wb + kBlock * WPT) (WPT consecutive words per thread) and the exclusive
+// prefix of their popcounts; returns the set bits in the group.  All threads call; ends
+// with a barrier.
+template <int WPT>
+__device__ __forceinline__ uint32_t bits_load(BitIterT<WPT>& B, const uint32_t* bits, int64_t nwords, int64_t wb) {
+  constexpr int kW = BitIterT<WPT>::kWords;
   const int t = threadIdx.x;
-  const int64_t w0 = wb + (int64_t)t * kWPT;
-  uint32_t w[kWPT];
+  const int64_t w0 = wb + (int64_t)t * WPT;
+  uint32_t w[WPT];
 #pragma unroll
-  for (int j = 0; j < kWPT; ++j) w[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
+  for (int j = 0; j < WPT; ++j) w[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
   uint32_t cnt = 0;
 #pragma unroll
-  for (int j = 0; j < kWPT; ++j) {
-    B.word[t * kWPT + j] = w[j];
+  for (int j = 0; j < WPT; ++j) {
+    B.word[t * WPT + j] = w[j];
     cnt += (uint32_t)__popc(w[j]);
   }
   const uint32_t incl = wave_incl_sum(cnt);
@@ -152,20 +158,22 @@ __device__ __forceinline__ uint32_t bits_load(BitIter& B, const uint32_t* bits, 
   for (int i = 0; i < wave_id(); ++i) carry += B.wsum[i];
   uint32_t run = incl - cnt + carry;  // exclusive prefix of this thread's first word
 #pragma unroll
-  for (int j = 0; j < kWPT; ++j) {
+  for (int j = 0; j < WPT; ++j) {
     run += (uint32_t)__popc(w[j]);
-    B.pref[t * kWPT + j + 1] = run;
+    B.pref[t * WPT + j + 1] = run;
   }
   if (t == 0) B.pref[0] = 0u;
   __syncthreads();
-  return B.pref[kBW];
+  return B.pref[kW];
 }
 
 // Vertex id of the k-th set bit (k < total) of the loaded words.
-__device__ __forceinline__ uint32_t bits_pick(const BitIter& B, int64_t wb, uint32_t k) {
-  int lo = 0, hi = kBW;  // pref[lo] <= k < pref[hi]
+template <int WPT>
+__device__ __forceinline__ uint32_t bits_pick(const BitIterT<WPT>& B, int64_t wb, uint32_t k) {
+  constexpr int kW = BitIterT<WPT>::kWords;
+  int lo = 0, hi = kW;  // pref[lo] <= k < pref[hi]
 #pragma unroll
-  for (int s = 0; (1 << s) < kBW; ++s) {
+  for (int s = 0; (1 << s) < kW; ++s) {
     int mid = (lo + hi) >> 1;
     if (B.pref[mid] <= k) lo = mid;
     else hi = mid;
@@ -408,7 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
       bool keep = false;
       if (v < n) {
         keep = MIRROR ? (od[r] + id[r]) > 0u : (od[r] > 0u && id[r] > 0u);
-        labels[v] = keep ? -1 : (int32_t)v;
+        if (labels) labels[v] = keep ? -1 : (int32_t)v;
       }
       unsigned long long mask = __ballot(keep);
       int64_t word = base >> 5;
@@ -1431,6 +1439,38 @@ __global__ void __launch_bounds__(kBlock) k_seed_list(LevelArgs a, const int32_t
   stage_flush(S, a.fr1, a.ctrl, 0u, out);
 }
 
+// Colouring round 0's seed (multistep/scc_color.cpp:56-63, cc_color.cpp:62-69), from the
+// valid bitmap the commit left: every valid vertex gets colors[v] = v and is queued, and
+// the queue is also written to `list`.  A workgroup takes 1024 words (4 per thread) at a
+// time, so after FW-BW, when few vertices are left, a sweep costs one load per thread.
+// Needs a fresh control block (k_round_begin), k_close_seed after; with a.rc only in an
+// admitted round.
+__global__ void __launch_bounds__(kBlock) k_seed_bits(LevelArgs a, int32_t* __restrict__ list) {
+  __shared__ Stage S;
+  __shared__ BitIterT<4> B;
+  if (a.rc && !a.rc->go) return;  // uniform
+  if (threadIdx.x == 0) S.cnt = 0u;
+  __syncthreads();
+  constexpr int kW = BitIterT<4>::kWords;
+  for (int64_t wb = (int64_t)blockIdx.x * kW; wb < a.nwords; wb += (int64_t)gridDim.x * kW) {
+    const uint32_t total = bits_load(B, a.valid, a.nwords, wb);
+    for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
+      const uint32_t k = k0 + threadIdx.x;
+      const bool take = k < total;
+      uint32_t v = 0, deg = 0;
+      if (take) {
+        v = bits_pick(B, wb, k);
+        a.colors[v] = (int32_t)v;
+        deg = list_len(a.down_off, a.down_off2, v);
+      }
+      stage_push(S, take, v, deg);
+      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, list);  // also the barrier before the next round
+    }
+    __syncthreads();  // B is reloaded next
+  }
+  stage_flush(S, a.fr1, a.ctrl, 0u, list);
+}
+
 // Colour roots (multistep/scc_color.cpp:165-212) among this round's vertex list (length
 // gate->list_len): colors[v] == v labels v, removes it and queues it for the per-colour
 // backward search of control block a.ctrl.  Runs only once the push fixpoint of control
@@ -1814,38 +1854,30 @@ __global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }
 
-// Commit of the pivot's component (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271)
-// fused with the seed of the colouring that follows (multistep/scc_color.cpp:56-63,
-// cc_color.cpp:62-69): every member gets the label and leaves the valid set; with SEED,
-// every vertex still valid afterwards gets colors[v] = v and is queued in frontier 1 of
-// a.ctrl, and the queue is also written to `list` (the vertices colouring round 0 works
-// on).  part col 0 = members of the workgroup.  One vertex per thread, so the label stores
-// of a wave are one contiguous 256-byte run; the first lane of each word writes the word's
-// new valid bits (no other writer).  The label is `label`, or the smallest member when
-// min_red is given (0xffffffff - min_red[3], canonical labels), or in a batch the pivot of
-// the vertex's own graph.
-template <bool BATCH, bool SEED>
+// Commit of the pivot's component (multistep/scc_fwbw.cpp:412-466, cc_bfs.cpp:217-271):
+// every member gets the label and leaves the valid set, every other vertex gets its own id
+// (a trimmed vertex's final label; colouring overwrites the ones still valid).  part col 0
+// = members of the workgroup.  One vertex per thread, so the label stores of a wave are
+// one contiguous 256-byte run and every label is written in full lines; the first lane of
+// each word writes the word's new valid bits (no other writer).  The label is `label`, or
+// the smallest member when min_red is given (0xffffffff - min_red[3], canonical labels), or
+// in a batch the pivot of the vertex's own graph.  With a.rc, only in an admitted round.
+template <bool BATCH>
 __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __restrict__ valid, int32_t n,
                                                    int32_t* __restrict__ labels, int32_t label,
                                                    const unsigned long long* __restrict__ min_red,
                                                    const int32_t* __restrict__ piv, const int64_t* __restrict__ base,
-                                                   uint32_t ng, unsigned long long* __restrict__ part, LevelArgs a,
-                                                   int32_t* __restrict__ list) {
+                                                   uint32_t ng, unsigned long long* __restrict__ part,
+                                                   const RoundCtl* rc) {
   __shared__ unsigned long long s_red[kBlock / 64];
-  __shared__ Stage S;
-  if (a.rc && !a.rc->go) return;  // uniform: the searches have not finished
+  if (rc && !rc->go) return;  // uniform: the searches have not finished
   const uint32_t keep = member_keep(m);
   if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
-  if (SEED) {
-    if (threadIdx.x == 0) S.cnt = 0u;
-    __syncthreads();
-  }
   unsigned long long cnt = 0;
   // kU slices of kBlock consecutive vertices per iteration, with all their loads issued
   // before any store (one memory latency per iteration, not per slice)
   constexpr int kU = 4;
-  static_assert(kU * kBlock <= (int)kStage / 2, "stage holds two iterations");
-  for (int64_t wg = (int64_t)blockIdx.x * kBlock * kU; wg < n; wg += (int64_t)gridDim.x * kBlock * kU) {  // uniform
+  for (int64_t wg = (int64_t)blockIdx.x * kBlock * kU; wg < n; wg += (int64_t)gridDim.x * kBlock * kU) {
     uint32_t vw[kU], w[kU];
 #pragma unroll
     for (int r = 0; r < kU; ++r) {
@@ -1858,31 +1890,19 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
       const int64_t wb = wg + r * kBlock + (threadIdx.x & ~63);
       const int64_t v = wg + r * kBlock + threadIdx.x;
       const int64_t wi = v >> 5;
-      const uint32_t bit = 1u << (v & 31);
-      const bool in = v < n && (w[r] & bit);
+      const bool in = v < n && ((w[r] >> (v & 31)) & 1u);
       if (BATCH) {
         const uint32_t g = wave_graph_of(base, ng, wb < n ? wb : n - 1, n);
-        if (in) labels[v] = piv[g];
-      } else if (in) {
-        labels[v] = label;
+        if (v < n) labels[v] = in ? piv[g] : (int32_t)v;
+      } else if (v < n) {
+        labels[v] = in ? label : (int32_t)v;
       }
       if ((lane() & 31) == 0 && v < n && w[r]) {
         valid[wi] = vw[r] & ~w[r];
         cnt += (uint32_t)__popc(w[r]);
       }
-      if (SEED) {
-        const bool take = v < n && (vw[r] & ~w[r] & bit);
-        uint32_t deg = 0;
-        if (take) {
-          a.colors[v] = (int32_t)v;
-          deg = list_len(a.down_off, a.down_off2, (uint32_t)v);
-        }
-        stage_push(S, take, (uint32_t)v, deg);
-      }
     }
-    if (SEED) stage_flush(S, a.fr1, a.ctrl, kStage - kU * kBlock, list);
   }
-  if (SEED) stage_flush(S, a.fr1, a.ctrl, 0u, list);
   cnt = block_sum64(cnt, s_red);
   if (threadIdx.x == 0) {
     part[4 * blockIdx.x] = cnt;
@@ -2029,12 +2049,13 @@ int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t 
                   int32_t* list) {
   int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
+  // the seed runs as its own sparse sweep (k_seed_bits): fused into this pass it made
+  // every iteration wait on the stage (s24: 51 us fused, 22 + 12 us apart)
+  hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
+                     label, min_red, nullptr, nullptr, 0u, part, a.rc);
   if (seed)
-    hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
-                       labels, label, min_red, nullptr, nullptr, 0u, part, a, list);
-  else
-    hipLaunchKernelGGL((k_commit<false, false>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
-                       labels, label, min_red, nullptr, nullptr, 0u, part, a, list);
+    hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
+                       L.stream, a, list);
   return g;
 }
 
@@ -2043,12 +2064,11 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
                         const LevelArgs* seed, int32_t* list) {
   int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
+  hipLaunchKernelGGL(k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels, 0,
+                     nullptr, piv, base, ng, part, a.rc);
   if (seed)
-    hipLaunchKernelGGL((k_commit<true, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
-                       labels, 0, nullptr, piv, base, ng, part, a, list);
-  else
-    hipLaunchKernelGGL((k_commit<true, false>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
-                       labels, 0, nullptr, piv, base, ng, part, a, list);
+    hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
+                       L.stream, a, list);
   return g;
 }
 

@@ -582,11 +582,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   } else {
     Timed t(c, MSC_K_TRIM);
-    // the searches' claimable bitmaps start as the valid set: written by the trim as well
-    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, labels, c->part);
+    // the searches' claimable bitmaps start as the valid set: written by the trim as well;
+    // no labels yet (the commit writes every label in one pass)
+    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, nullptr, c->part);
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   }
-  if (s) s->k[MSC_K_TRIM].bytes += 8.0 * n + 4.0 * n + 1.0 * n;
+  if (s) s->k[MSC_K_TRIM].bytes += 8.0 * n + 1.0 * n;
 
   LevelArgs a = base_args(c, &g, labels, o, avg);
   a.nvalid = &c->ctrl->red_trim[0];
@@ -858,7 +859,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     s->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     s->num_trim = n - (int32_t)c->ctrl_host->red_trim[0];
     s->num_fwbw = (int32_t)c->ctrl_host->red_commit[0];
-    if (c->ctrl_host->red_commit[0]) s->k[MSC_K_COMMIT].bytes += 8.0 * nw + 4.0 * (double)c->ctrl_host->red_commit[0];
+    // the commit reads valid and both search bitmaps, writes valid and every label
+    s->k[MSC_K_COMMIT].bytes += 16.0 * nw + 4.0 * n;
     // every block was zeroed at the start: each holds this run's own levels only
     s->bu_levels = (int32_t)(hs[0].bu_levels + hs[1].bu_levels);
     for (int op = 0; op < 5; ++op) {
