@@ -107,11 +107,11 @@ class _Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("small_push_edges", C.c_int32),
                 ("small_cbw_edges", C.c_int32), ("first_batch", C.c_int32), ("max_batch", C.c_int32),
                 ("concurrent_fwbw", C.c_int32), ("search_batch", C.c_int32), ("local_rounds", C.c_int32),
-                ("local_us", C.c_int32)]
+                ("local_us", C.c_int32), ("pull_div", C.c_int32)]
 
 
 TUNING_FIELDS = ("blocks_per_cu", "small_push_edges", "small_cbw_edges", "first_batch", "max_batch",
-                 "concurrent_fwbw", "search_batch", "local_rounds", "local_us")
+                 "concurrent_fwbw", "search_batch", "local_rounds", "local_us", "pull_div")
 
 
 class _TrainOpts(C.Structure):

@@ -1,6 +1,6 @@
 """Autotuner for a context's launch parameters (msc_tuning).
 
-The level loops have nine tunables that no rule of the reference fixes and that trade
+The level loops have ten tunables that no rule of the reference fixes and that trade
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
@@ -12,7 +12,8 @@ against each other differently on shallow power-law graphs and on deep meshes:
   concurrent_fwbw   backward search beside the forward one in the same launches (1) or after it;
   search_batch      levels of a FW-BW / CC search enqueued before its first host check;
   local_rounds      levels a workgroup of a colour launch continues on its own claims;
-  local_us          ... while that launch has run for less than this many microseconds.
+  local_us          ... while that launch has run for less than this many microseconds;
+  pull_div          a colour level over at least 1/pull_div of its round's edges pulls (0: never).
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -44,6 +45,7 @@ CANDIDATES = {
     "search_batch": [2, 4, 6, 8],
     "local_rounds": [0, 4, 16, 32, 64],
     "local_us": [10, 20, 40, 80, 160],
+    "pull_div": [0, 2, 4, 8, 16],
 }
 
 

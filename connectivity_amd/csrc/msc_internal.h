@@ -60,7 +60,8 @@ struct Ctrl {
   uint32_t q_size;                 // frontier to expand
   uint32_t q_edges;                // its list lengths summed
   uint32_t cur;                    // Frontier buffer holding it
-  uint32_t bottom_up;              // direction of the coming level
+  uint32_t bottom_up;              // direction of the coming level (colour push: 0 push,
+                                   // 1 pull, 2 pull while colours are still the seed's ids)
   uint32_t switched_back;          // the one-way return to top-down happened
   uint32_t done;
   uint32_t level;                  // levels closed so far
@@ -74,6 +75,8 @@ struct Ctrl {
   unsigned long long t_start;      // earliest workgroup start of the running launch
   uint32_t closes;                 // closing-kernel launches that closed a level
   uint32_t list_len;               // vertex list written by the last seeding launch (kept by resets)
+  uint32_t round_edges;            // colour push: list lengths of the round's seeded frontier
+  uint32_t pad_;
   // counts the host reads with the next control-block copy instead of a round trip each
   unsigned long long red_trim[4];    // col 0: trim survivors
   unsigned long long red_commit[4];  // col 0: vertices committed by FW-BW / the CC search
@@ -143,6 +146,9 @@ struct LevelArgs {
   int32_t local_rounds;       // colour levels: local continuation rounds per launch
   uint64_t local_ticks;       // colour levels: no local round starts this long (wall clock) after a launch began
   const RoundCtl* rc;         // colour seeds and roots: run only when rc->go (null: always)
+  uint32_t pull_div;          // colour push: a level over at least 1/pull_div of the round's
+                              // edges runs as a pull level (0: never)
+  uint32_t pull_min;          // ... and over at least this many edges
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or
@@ -214,7 +220,7 @@ void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* c
 void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode);
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
 // frontier hand-over after a seeding kernel (with rc: only in an admitted round)
-void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc);
+void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc, uint32_t pull_min);
 // colouring round > 0, after launch_round_begin: seed from the previous round's list `in`
 // (length a.ctrl->list_len) into `out`, hand-over; bound >= that length sizes the grid
 void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound);

@@ -663,8 +663,9 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
 
 // Closes a seeding launch: the queue built in buffer 1 becomes the frontier (and the
 // length of the vertex list the seed wrote beside it, if any).  With rc, only in an
-// admitted round (rc->go).
-__global__ void k_close_seed(Ctrl* c, const RoundCtl* rc) {
+// admitted round (rc->go).  With pull_min (colour push seeds), a first level over at
+// least pull_min edges is a pull level over the seed's ids.
+__global__ void k_close_seed(Ctrl* c, const RoundCtl* rc, uint32_t pull_min) {
   if (rc && !rc->go) return;
   if (threadIdx.x == 0) {
     unsigned long long p = c->next_packed;
@@ -672,6 +673,8 @@ __global__ void k_close_seed(Ctrl* c, const RoundCtl* rc) {
     c->list_len = next;
     c->q_size = next;
     c->q_edges = (uint32_t)p;
+    c->round_edges = (uint32_t)p;
+    if (pull_min) c->bottom_up = (uint32_t)p >= pull_min ? 2u : 0u;
     c->cur = 1u;
     c->next_packed = 0ull;
     c->done = next == 0u ? 1u : 0u;
@@ -964,6 +967,133 @@ __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* in
   }
 }
 
+// LDS of a pull level: one bitmap word group and kBlock of its candidates' in-lists.
+template <bool MIRROR>
+struct PullLds {
+  BitIterT<1> B;
+  uint32_t pref[kBlock + 1];  // exclusive prefix of the candidates' list lengths
+  uint32_t beg[kBlock];
+  uint32_t beg2[MIRROR ? kBlock : 1];
+  uint32_t len1[MIRROR ? kBlock : 1];
+  int32_t acc[kBlock];
+  uint32_t scan[kBlock / 64];
+};
+
+// One word group [wb, wb + kBlock) of a pull level: a dense push level of max-label
+// propagation (multistep/scc_color.cpp:67-163) turned around.  Every valid vertex of the
+// group takes the largest colour among its valid in-neighbours (in- and out-neighbours
+// when mirrored) and, when that exceeds its own, stores it and is queued for frontier N.
+// Only the workgroup that owns a vertex's word writes its colour, so the level needs no
+// atomics beyond the queue reservations; a colour read while its owner raises it is
+// either value, and both are valid lower bounds of the fixpoint, so the level ends at the
+// same fixpoint as a push level.  With `ids` the colours are still the seed's
+// (colors[u] == u for every valid u) and a neighbour's id stands for its colour.  A
+// candidate's list is walked kChunk edges at a time by the whole workgroup, the owner of
+// an edge found by binary search over the candidates' prefix in LDS, so one long list
+// does not stall a thread.  The group's words of the level's worklist bitmap are cleared:
+// no pull level reads it, and the level after needs both worklist bitmaps clear.  Every
+// thread calls; raised / candidates / edges / colour reads are per-thread counts.
+template <bool MIRROR>
+__device__ __forceinline__ void pull_group(const LevelArgs& a, const Frontier& N, uint32_t* inq_cur, int64_t wb,
+                                           bool ids, Stage& S, PullLds<MIRROR>& P, unsigned long long& raised,
+                                           unsigned long long& cands, unsigned long long& edges,
+                                           unsigned long long& reads) {
+  const int t = threadIdx.x;
+  if (wb + t < a.nwords) inq_cur[wb + t] = 0u;
+  const uint32_t total = bits_load(P.B, a.valid, a.nwords, wb);  // ends with a barrier
+  if (t == 0) cands += total;
+  for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
+    const uint32_t nb = min(total - k0, (uint32_t)kBlock);
+    const bool has = (uint32_t)t < nb;
+    uint32_t v = 0, b1 = 0, l1 = 0, b2 = 0, l2 = 0;
+    int32_t own = 0;
+    if (has) {
+      v = bits_pick(P.B, wb, k0 + (uint32_t)t);
+      b1 = a.up_off[v];
+      l1 = a.up_off[v + 1] - b1;
+      if (MIRROR) {
+        b2 = a.up_off2[v];
+        l2 = a.up_off2[v + 1] - b2;
+      }
+      own = a.colors[v];
+    }
+    const uint32_t len = l1 + l2;
+    const uint32_t incl = wave_incl_sum(len);
+    if (lane() == 63) P.scan[wave_id()] = incl;
+    P.beg[t] = b1;
+    if (MIRROR) {
+      P.beg2[t] = b2;
+      P.len1[t] = l1;
+    }
+    P.acc[t] = own;
+    __syncthreads();
+    uint32_t carry = 0, T = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (w < wave_id()) carry += P.scan[w];
+      T += P.scan[w];
+    }
+    P.pref[t] = incl - len + carry;  // threads past nb have len 0: pref = T
+    if (t == 0) {
+      P.pref[kBlock] = T;
+      edges += T;
+    }
+    __syncthreads();
+    for (uint32_t e0 = 0; e0 < T; e0 += kChunk) {
+      uint32_t u[kPer], j[kPer], word[kPer];
+      bool ok[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const uint32_t e = e0 + (uint32_t)t + (uint32_t)r * kBlock;
+        ok[r] = e < T;
+        u[r] = 0;
+        j[r] = 0;
+        if (ok[r]) {
+          // the last candidate whose list starts at or before e (pref[0] = 0 <= e < T)
+          uint32_t lo = 0;
+#pragma unroll
+          for (uint32_t step = kBlock / 2; step >= 1; step >>= 1)
+            if (lo + step < nb && P.pref[lo + step] <= e) lo += step;
+          const uint32_t i = e - P.pref[lo];
+          j[r] = lo;
+          if (MIRROR && i >= P.len1[lo]) u[r] = (uint32_t)a.up_adj2[P.beg2[lo] + (i - P.len1[lo])];
+          else u[r] = (uint32_t)a.up_adj[P.beg[lo] + i];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) word[r] = ok[r] ? a.valid[u[r] >> 5] : 0u;
+      int32_t cu[kPer];
+      bool in[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        in[r] = ok[r] && ((word[r] >> (u[r] & 31u)) & 1u);
+        cu[r] = (int32_t)u[r];
+        if (in[r] && !ids) {
+          cu[r] = a.colors[u[r]];
+          ++reads;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r)
+        if (in[r] && cu[r] > P.acc[j[r]]) atomicMax(&P.acc[j[r]], cu[r]);
+    }
+    __syncthreads();
+    bool take = false;
+    uint32_t deg = 0;
+    if (has) {
+      const int32_t best = P.acc[t];
+      if (best > own) {
+        a.colors[v] = best;
+        take = true;
+        ++raised;
+        deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v);
+      }
+    }
+    stage_push(S, take, v, deg);
+    stage_flush(S, N, a.ctrl, kStage - kBlock);  // also the barrier before P is refilled
+  }
+  __syncthreads();  // B is reloaded next
+}
+
 // One colour sweep level (push or per-colour backward; work only, k_close_color ends it):
 // edge-balanced expansion of the current frontier, one chunk per workgroup at a time;
 // workgroups without a chunk exit at once (the closer sums only the rows of the ones that
@@ -973,11 +1103,44 @@ template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   static_assert(OP == OP_PUSH || OP == OP_CBW, "colour ops only; searches run in k_reach");
   __shared__ Stage S;
-  __shared__ ChunkLds<OP, MIRROR> L;
+  // a push level either expands its frontier (ChunkLds) or pulls (PullLds)
+  __shared__ union {
+    ChunkLds<OP, MIRROR> L;
+    PullLds<MIRROR> P;
+  } lds;
+  ChunkLds<OP, MIRROR>& L = lds.L;
   __shared__ unsigned long long s_red[kBlock / 64];
 
   Ctrl* c = a.ctrl;
   if (c->done) return;  // uniform: the sweep already ended
+  if (OP == OP_PUSH && c->bottom_up != 0u) {
+    // pull level: every word group of the bitmap; part row = (candidates, edges, raised,
+    // colour reads)
+    const int64_t groups = (a.nwords + kBlock - 1) / kBlock;
+    if ((int64_t)blockIdx.x >= groups) return;  // uniform
+    const Frontier N = c->cur ? a.fr0 : a.fr1;
+    uint32_t* inq_cur = c->cur ? a.inq1 : a.inq0;
+    const bool ids = c->bottom_up == 2u;
+    if (threadIdx.x == 0) {
+      S.cnt = 0u;
+      if (a.recs) atomicMin(&c->t_start, wall_clock64());
+    }
+    __syncthreads();
+    unsigned long long ra = 0, cn = 0, ed = 0, rd = 0;
+    for (int64_t g = blockIdx.x; g < groups; g += gridDim.x)
+      pull_group<MIRROR>(a, N, inq_cur, g * kBlock, ids, S, lds.P, ra, cn, ed, rd);
+    stage_flush(S, N, c, 0u);
+    ra = block_sum64(ra, s_red);
+    rd = block_sum64(rd, s_red);
+    if (threadIdx.x == 0) {
+      unsigned long long* p = a.part + 4 * blockIdx.x;
+      p[0] = cn;
+      p[1] = ed;
+      p[2] = ra;
+      p[3] = rd;
+    }
+    return;
+  }
   const uint32_t cur = c->cur;
   const uint32_t q_size = c->q_size, q_edges = c->q_edges;
   const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
@@ -1111,16 +1274,28 @@ __device__ __forceinline__ int active_rows(int nparts, uint32_t q_edges) {
   return (int)min((uint32_t)nparts, nchunks);
 }
 
+// Direction of the coming colour level: a push level whose frontier has at least
+// a.pull_min list entries and 1/a.pull_div of the round's runs as a pull level (1); else 0.
+__device__ __forceinline__ int colour_action(const Ctrl* c, const LevelArgs& a, int op, uint32_t nedges) {
+  if (op != OP_PUSH || a.pull_div == 0u || nedges < a.pull_min) return 0;
+  return (unsigned long long)nedges * a.pull_div >= (unsigned long long)c->round_edges ? 1 : 0;
+}
+
 // Thread 0 only: books the level that just closed (next frontier from next_packed,
 // algorithmic bytes, level record) and turns the control block to the next level with
 // direction `act` (0 top-down, 1 bottom-up, 2 abandon the search).
 __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int act, uint32_t q_size,
                              uint32_t q_edges, unsigned long long ex, unsigned long long in, unsigned long long ra,
-                             uint32_t next, uint32_t nedges, const float* feat) {
-  // algorithmic bytes of this level (unit costs documented in DESIGN.md)
+                             uint32_t next, uint32_t nedges, const float* feat, unsigned long long rd = 0ull) {
+  // algorithmic bytes of this level
   unsigned long long bytes;
-  // colour levels: ex / in are the vertices / edges a workgroup expanded locally
-  if (op == OP_PUSH)
+  // colour levels: ex / in are the vertices / edges a workgroup expanded locally; a pull
+  // level (bu) sweeps the valid and worklist bitmaps, reads each candidate's list bounds
+  // and colour (ex), every list entry and its valid bit (in), rd neighbour colours, and
+  // writes ra colours and the queue
+  if (op == OP_PUSH && bu)
+    bytes = 8ull * (unsigned long long)a.nwords + 12ull * ex + 5ull * in + 4ull * rd + 4ull * ra + 8ull * next;
+  else if (op == OP_PUSH)
     bytes = 12ull * (q_size + ex) + 9ull * (q_edges + in) + 4ull * ra + 4ull * next;
   else if (op == OP_CBW)
     bytes = 12ull * (q_size + ex) + 9ull * (q_edges + in) + 5ull * next;
@@ -1355,23 +1530,29 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
   const int t = threadIdx.x;
   {
     const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-    const int rows = active_rows(nparts, q_edges);
-    unsigned long long ln = 0, le = 0, ra = 0;
+    const bool pulled = OP == OP_PUSH && c->bottom_up != 0u;
+    const int rows = pulled ? (int)min((int64_t)nparts, (a.nwords + kBlock - 1) / kBlock)
+                            : active_rows(nparts, q_edges);
+    unsigned long long ln = 0, le = 0, ra = 0, rd = 0;
     for (int i = t; i < rows; i += kBlock) {
       ln += a.part[4 * i];
       le += a.part[4 * i + 1];
       ra += a.part[4 * i + 2];
+      if (pulled) rd += a.part[4 * i + 3];
     }
     ln = block_sum64(ln, s_red);
     le = block_sum64(le, s_red);
     ra = block_sum64(ra, s_red);
+    rd = block_sum64(rd, s_red);
     if (t == 0) {
       const unsigned long long p = c->next_packed;
       const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
-      c->found += next + (int64_t)ln;  // claims expanded locally never reach the frontier
+      // claims expanded locally never reach the frontier; a pull level's row is its sweep
+      c->found += next + (pulled ? 0 : (int64_t)ln);
       c->level += 1;
       c->closes += 1;
-      finish_level(c, a, OP, false, 0, q_size, q_edges, ln, le, ra, next, nedges, nullptr);
+      finish_level(c, a, OP, pulled, colour_action(c, a, OP, nedges), q_size, q_edges, ln, le, ra, next, nedges,
+                   nullptr, rd);
       s_ctl[0] = c->done;
       s_ctl[1] = c->cur;
       s_ctl[2] = c->q_size;
@@ -1399,7 +1580,8 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
       const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
       c->found += next;
       c->level += 1;
-      finish_level(c, a, OP, false, 0, q_size, q_edges, 0ull, 0ull, raised, next, nedges, nullptr);
+      finish_level(c, a, OP, false, colour_action(c, a, OP, nedges), q_size, q_edges, 0ull, 0ull, raised, next,
+                   nedges, nullptr);
       s_ctl[0] = c->done;
       s_ctl[1] = c->cur;
       s_ctl[2] = c->q_size;
@@ -2110,19 +2292,19 @@ void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_
                      b, words);
 }
 
-void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc) {
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, c, rc);
+void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc, uint32_t pull_min) {
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, c, rc, pull_min);
 }
 
 void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound) {
   hipLaunchKernelGGL(k_seed_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, in, out);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, a.pull_div ? a.pull_min : 0u);
 }
 
 void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list, const Ctrl* gate, int64_t bound) {
   hipLaunchKernelGGL(k_roots_begin, dim3(1), dim3(64), 0, L.stream, const_cast<RoundCtl*>(a.rc), gate, a.ctrl);
   hipLaunchKernelGGL(k_roots_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, list, gate);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, 0u);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate) {
@@ -2170,7 +2352,7 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
   dim3 g((unsigned)grid_for(ng, kBlock, L.grid));
   if (mirror) hipLaunchKernelGGL(k_seed_pivots<true>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
   else hipLaunchKernelGGL(k_seed_pivots<false>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, nullptr);
+  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, nullptr, 0u);
 }
 
 void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng,

@@ -41,7 +41,7 @@ struct msc_ctx {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 160};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 160, 4};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -699,9 +699,18 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   p.recs = nullptr;  // level records are the searches'
   p.rc = rcd;
   p.down_off = g.out_off; p.down_adj = g.out_adj;
+  p.up_off = g.in_off; p.up_adj = g.in_adj;  // pull levels
+  p.up_off2 = nullptr; p.up_adj2 = nullptr;
   if (cc) {
     p.colors = labels;  // the CC fixpoint writes straight into the labels
     p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
+    p.up_off2 = g.out_off; p.up_adj2 = g.out_adj;
+  }
+  if (c->tune.pull_div > 0) {
+    // a pull level sweeps every bitmap word and never runs inside the closing kernel
+    const int64_t floor_edges = nw > (int64_t)c->tune.small_push_edges ? nw : (int64_t)c->tune.small_push_edges + 1;
+    p.pull_div = (uint32_t)c->tune.pull_div;
+    p.pull_min = floor_edges > 0xffffffffll ? 0xffffffffu : (uint32_t)floor_edges;
   }
   if (fast_canon) p.cmin = c->cmin;
   if (c->tune.local_rounds > 0) {
@@ -738,7 +747,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
                          : launch_commit(L, member, c->valid, n, labels, pivot, want_min ? c->ctrl->red_min : nullptr,
                                          c->part, &p, list_of(0));
       launch_reduce(L, c->part, npc, c->ctrl->red_commit);
-      launch_seed_end(L, p.ctrl, rcd);
+      launch_seed_end(L, p.ctrl, rcd, p.pull_div ? p.pull_min : 0u);
     } else {
       Timed t(c, MSC_K_ROOTS);
       launch_seed_list(L, p, list_of(rr - 1), list_of(rr), n);
@@ -1012,7 +1021,7 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
       t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
-      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000)
+      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;

@@ -205,6 +205,10 @@ typedef struct msc_tuning {
   int32_t local_rounds;     /* levels a workgroup of a colour launch continues on its own claims
                                before handing them to the next launch (0..256; 0: off) */
   int32_t local_us;         /* ... while the launch has run for less than this many us (1..100000) */
+  int32_t pull_div;         /* a colour push level whose frontier holds at least 1/pull_div of
+                               the round's list entries runs as a pull level: every valid vertex
+                               takes the largest colour of its valid in-neighbours, no atomics
+                               (0..1024; 0: push only) */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);

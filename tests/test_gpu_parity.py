@@ -154,13 +154,13 @@ def test_stats_and_levels(msc, orc, gpu_ctx):
 
 @pytest.mark.parametrize("tuning", [
     dict(blocks_per_cu=1, small_push_edges=0, small_cbw_edges=0, first_batch=1, max_batch=1, concurrent_fwbw=0,
-         search_batch=1, local_rounds=0, local_us=1),
+         search_batch=1, local_rounds=0, local_us=1, pull_div=0),
     dict(blocks_per_cu=8, small_push_edges=1 << 20, small_cbw_edges=1 << 20, first_batch=8, max_batch=64,
-         concurrent_fwbw=1, search_batch=64, local_rounds=64, local_us=100000),
+         concurrent_fwbw=1, search_batch=64, local_rounds=64, local_us=100000, pull_div=1024),
     dict(blocks_per_cu=2, small_push_edges=1 << 12, small_cbw_edges=0, first_batch=2, max_batch=16,
-         concurrent_fwbw=0, search_batch=3, local_rounds=1, local_us=40),
+         concurrent_fwbw=0, search_batch=3, local_rounds=1, local_us=40, pull_div=1),
     dict(blocks_per_cu=5, small_push_edges=0, small_cbw_edges=0, first_batch=2, max_batch=32,
-         concurrent_fwbw=1, search_batch=2, local_rounds=8, local_us=10),
+         concurrent_fwbw=1, search_batch=2, local_rounds=8, local_us=10, pull_div=2),
 ])
 def test_tuning_keeps_labels(msc, orc, gpu_ctx, tuning):
     """Launch parameters change the schedule only: labels stay the oracle's."""
@@ -182,7 +182,7 @@ def test_tuning_rejects_bad_values(msc, gpu_ctx):
     for bad in (dict(blocks_per_cu=0), dict(blocks_per_cu=9), dict(small_push_edges=-1),
                 dict(first_batch=0), dict(first_batch=8, max_batch=4), dict(max_batch=257),
                 dict(concurrent_fwbw=2), dict(search_batch=0), dict(search_batch=257), dict(local_rounds=-1),
-                dict(local_rounds=257), dict(local_us=0)):
+                dict(local_rounds=257), dict(local_us=0), dict(pull_div=-1), dict(pull_div=1025)):
         with pytest.raises(msc.MscError):
             gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == defaults
