@@ -100,6 +100,9 @@ def main() -> int:
         return 1
     t0 = time.time()
     g, desc, cfg, base = build_graph(args, msc)
+    # every graph's pivot comes with the batch, as max_deg_vert comes with a single graph
+    # (create_csr computes it, untimed)
+    pivots = msc.batch_pivots(g, base) if base is not None else None
     log(f"{cfg['workload']}: n={g.n} m={g.m} pivot={g.max_deg_vert} built in {time.time() - t0:.1f}s")
     ctx = msc.Context(0)
     if args.calib_copy_mb:
@@ -122,7 +125,7 @@ def main() -> int:
 
     def step(pol, timing=True):
         if base is not None:
-            _, st = msc.run_scc_batch(dg, base, ctx=ctx, out=out, policy=pol, timing=timing, stats=True)
+            _, st = msc.run_scc_batch(dg, base, pivots, ctx=ctx, out=out, policy=pol, timing=timing, stats=True)
         else:
             _, st = msc.run_scc(dg, g.max_deg_vert, g.avg_degree, ctx=ctx, out=out, policy=pol, timing=timing,
                                 stats=True)
@@ -131,7 +134,7 @@ def main() -> int:
     def prepared(pol):
         # the arguments marshalled once: a step is one C ABI call, no statistics
         if base is not None:
-            return msc.Prepared("scc", dg, policy=pol, ctx=ctx, out=out, batch=(base, None))
+            return msc.Prepared("scc", dg, policy=pol, ctx=ctx, out=out, batch=(base, pivots))
         return msc.Prepared("scc", dg, g.max_deg_vert, g.avg_degree, policy=pol, ctx=ctx, out=out)
 
     def timed(pol, steps, timing):

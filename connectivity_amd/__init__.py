@@ -25,7 +25,7 @@ import numpy as np
 __all__ = [
     "MscError", "Graph", "DeviceGraph", "DeviceLabels", "Context", "Stats", "LevelRecord",
     "read_edge", "create_csr", "gen_rmat", "gen_grid", "gen_planted",
-    "run_scc", "run_cc", "scc_pivot", "scc_verify", "cc_verify", "verify_line",
+    "run_scc", "run_cc", "scc_pivot", "batch_pivots", "scc_verify", "cc_verify", "verify_line",
     "canonicalize", "write_labels", "device_count", "lib_path",
     "POLICY_PLAIN", "POLICY_LEARNED", "LABELS_CANONICAL", "LABELS_RAW", "schedule_array",
     "set_policy", "load_policy", "mlp_forward", "mlp_grad", "mlp_train", "POLICY_WEIGHTS",
@@ -679,6 +679,29 @@ def gen_batch(num_graphs: int = 8192, median_n: float = 4096, sigma: float = 1.4
     _check(_lib().msc_gen_batch(num_graphs, median_n, sigma, min_n, max_n, seed, C.byref(out), _ptr(base),
                                 _ptr(kinds)), "msc_gen_batch")
     return _wrap(out), base, kinds
+
+
+def batch_pivots(g, vert_base) -> np.ndarray:
+    """create_csr's pivot (multistep/scc_main.cpp:169-183) of every graph of a packed batch,
+    in the graph's own ids: the first vertex with the strictly largest out x in degree product
+    (as doubles), -1 when every product is 0.  These are the pivots run_scc_batch computes
+    on device when ``pivots`` is None; computing them once with the batch is the
+    counterpart of create_csr computing max_deg_vert with the graph."""
+    base = np.ascontiguousarray(vert_base, dtype=np.int64)
+    ng = base.shape[0] - 1
+    piv = np.full(ng, -1, dtype=np.int32)
+    prod = (np.diff(np.asarray(g.out_off, dtype=np.int64)).astype(np.float64)
+            * np.diff(np.asarray(g.in_off, dtype=np.int64)).astype(np.float64))
+    sizes = np.diff(base)
+    live = np.flatnonzero(sizes > 0)
+    if live.size == 0:
+        return piv
+    best = np.maximum.reduceat(prod, base[live])
+    hit = np.flatnonzero((prod > 0.0) & (prod == np.repeat(best, sizes[live])))
+    owner = np.searchsorted(base, hit, side="right") - 1
+    first_graphs, first_at = np.unique(owner, return_index=True)
+    piv[first_graphs] = (hit[first_at] - base[first_graphs]).astype(np.int32)
+    return piv
 
 
 def _batch_args(vert_base, pivots):

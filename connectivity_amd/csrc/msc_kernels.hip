@@ -699,8 +699,10 @@ struct ChunkLds {
 // kokkos_scc/fwbw_manhattan_global.cpp:162-289 without its per-edge binary search), the
 // adjacency is read coalesced, and claims / raises are staged for frontier N.  Ends with
 // a workgroup barrier.  Every queued vertex has a non-empty list, so at most len + 1
-// slots overlap a chunk.
-template <int OP, bool MIRROR, bool LOCAL = false>
+// slots overlap a chunk.  A push expansion takes the vertex off the current worklist
+// bitmap: its bit is cleared atomically, or with PLAIN_CLEAR its whole word is stored as
+// zero (only where nothing reads or sets that bitmap until the launch ends).
+template <int OP, bool MIRROR, bool LOCAL = false, bool PLAIN_CLEAR = false>
 __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, const Frontier& N, uint32_t* inq_cur,
                                          uint32_t* inq_nxt, uint32_t ch, uint32_t nchunks, uint32_t q_size,
                                          uint32_t q_edges, Stage& S, ChunkLds<OP, MIRROR>& L,
@@ -723,7 +725,10 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
     uint32_t s = qo > e0 ? qo - e0 : 0u;
     if (s < len) {
       atomicMax(&L.own[s], j);
-      if (OP == OP_PUSH) bit_clear(inq_cur, v);
+      if (OP == OP_PUSH) {
+        if (PLAIN_CLEAR) inq_cur[v >> 5] = 0u;
+        else bit_clear(inq_cur, v);
+      }
     }
     uint32_t b1 = a.down_off[v];
     L.base[j] = b1 - qo;
@@ -967,6 +972,11 @@ __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* in
   }
 }
 
+// Edges a pull level's workgroup walks per step (kPullPer per thread, all loads in flight
+// together).
+constexpr int kPullPer = 8;
+constexpr uint32_t kPullEdges = (uint32_t)kBlock * kPullPer;
+
 // LDS of a pull level: one bitmap word group and kBlock of its candidates' in-lists.
 template <bool MIRROR>
 struct PullLds {
@@ -988,7 +998,7 @@ struct PullLds {
 // either value, and both are valid lower bounds of the fixpoint, so the level ends at the
 // same fixpoint as a push level.  With `ids` the colours are still the seed's
 // (colors[u] == u for every valid u) and a neighbour's id stands for its colour.  A
-// candidate's list is walked kChunk edges at a time by the whole workgroup, the owner of
+// candidate's list is walked kPullEdges edges at a time by the whole workgroup, the owner of
 // an edge found by binary search over the candidates' prefix in LDS, so one long list
 // does not stall a thread.  The group's words of the level's worklist bitmap are cleared:
 // no pull level reads it, and the level after needs both worklist bitmaps clear.  Every
@@ -1038,11 +1048,12 @@ __device__ __forceinline__ void pull_group(const LevelArgs& a, const Frontier& N
       edges += T;
     }
     __syncthreads();
-    for (uint32_t e0 = 0; e0 < T; e0 += kChunk) {
-      uint32_t u[kPer], j[kPer], word[kPer];
-      bool ok[kPer];
+    for (uint32_t e0 = 0; e0 < T; e0 += kPullEdges) {
+      uint32_t u[kPullPer], j[kPullPer], word[kPullPer];
+      int32_t cu[kPullPer];
+      bool ok[kPullPer];
 #pragma unroll
-      for (int r = 0; r < kPer; ++r) {
+      for (int r = 0; r < kPullPer; ++r) {
         const uint32_t e = e0 + (uint32_t)t + (uint32_t)r * kBlock;
         ok[r] = e < T;
         u[r] = 0;
@@ -1059,22 +1070,22 @@ __device__ __forceinline__ void pull_group(const LevelArgs& a, const Frontier& N
           else u[r] = (uint32_t)a.up_adj[P.beg[lo] + i];
         }
       }
+      // valid words first, then the colours of valid neighbours only (reading every
+      // neighbour's colour alongside its word was slower: after FW-BW most neighbours are
+      // no longer valid and their colour lines are cold)
 #pragma unroll
-      for (int r = 0; r < kPer; ++r) word[r] = ok[r] ? a.valid[u[r] >> 5] : 0u;
-      int32_t cu[kPer];
-      bool in[kPer];
+      for (int r = 0; r < kPullPer; ++r) word[r] = ok[r] ? a.valid[u[r] >> 5] : 0u;
 #pragma unroll
-      for (int r = 0; r < kPer; ++r) {
-        in[r] = ok[r] && ((word[r] >> (u[r] & 31u)) & 1u);
-        cu[r] = (int32_t)u[r];
-        if (in[r] && !ids) {
-          cu[r] = a.colors[u[r]];
-          ++reads;
-        }
+      for (int r = 0; r < kPullPer; ++r) {
+        const bool in = ok[r] && ((word[r] >> (u[r] & 31u)) & 1u);
+        cu[r] = (in && !ids) ? a.colors[u[r]] : (int32_t)u[r];
       }
 #pragma unroll
-      for (int r = 0; r < kPer; ++r)
-        if (in[r] && cu[r] > P.acc[j[r]]) atomicMax(&P.acc[j[r]], cu[r]);
+      for (int r = 0; r < kPullPer; ++r) {
+        const bool in = ok[r] && ((word[r] >> (u[r] & 31u)) & 1u);
+        if (in && !ids) ++reads;
+        if (in && cu[r] > P.acc[j[r]]) atomicMax(&P.acc[j[r]], cu[r]);
+      }
     }
     __syncthreads();
     bool take = false;
@@ -1157,8 +1168,11 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
     if (a.recs) atomicMin(&c->t_start, t_begin);
   }
   __syncthreads();
+  // nothing reads the current worklist bitmap during a grid-wide level (local rounds use
+  // the next one) and the level must leave it clear: each expanded vertex's whole word is
+  // cleared with a plain store instead of an atomic on its bit
   for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
-    td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
+    td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
   unsigned long long loc_n = 0, loc_e = 0;
   if (a.loc && a.local_rounds > 0)
     local_rounds<OP, MIRROR>(a, N, inq_nxt, S, L, my_raised, loc_n, loc_e, t_begin);

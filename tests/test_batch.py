@@ -74,6 +74,21 @@ def test_gen_batch_structure(msc, orc):
     assert np.array_equal(orc.canonical(lab.astype(np.int32)), want)
 
 
+def test_batch_pivots_follow_create_csr(msc, orc):
+    """batch_pivots is create_csr's rule per graph: the oracle's pivot and the C++ builder's."""
+    gs = mixed_graphs(msc, seed=5)
+    u, base = msc.pack_graphs(gs)
+    piv = msc.batch_pivots(u, base)
+    assert piv.dtype == np.int32 and piv.shape == (len(gs),)
+    for i, g in enumerate(gs):
+        assert piv[i] == g.max_deg_vert == orc.pivot_scc(g)[0], f"graph {i}"
+    assert piv[1] == -1 and piv[2] == -1 and piv[3] == 1  # empty and edgeless graphs have none
+    u, base, _ = msc.gen_batch(30, median_n=400, sigma=1.0, min_n=64, max_n=3000, seed=6)
+    piv = msc.batch_pivots(u, base)
+    for i in range(30):
+        assert piv[i] == orc.pivot_scc(unpack(msc, u, base, i))[0], f"generated graph {i}"
+
+
 def test_gen_batch_other_seed_differs(msc):
     a = msc.gen_batch(8, median_n=300, sigma=0.5, min_n=64, max_n=2000, seed=1)
     b = msc.gen_batch(8, median_n=300, sigma=0.5, min_n=64, max_n=2000, seed=2)
@@ -131,6 +146,9 @@ def test_generated_batch(msc, orc, gpu_ctx):
     u, base, kinds = msc.gen_batch(96, median_n=1500, sigma=1.2, min_n=256, max_n=40000, seed=4)
     raw = msc.run_scc_batch(u, base, labels=msc.LABELS_RAW, ctx=gpu_ctx)
     wcc = msc.run_cc_batch(u, base, ctx=gpu_ctx)
+    # pivots computed once on the host are the ones the device computes per call
+    given = msc.run_scc_batch(u, base, msc.batch_pivots(u, base), labels=msc.LABELS_RAW, ctx=gpu_ctx)
+    assert np.array_equal(given, raw)
     for i in range(96):
         g = unpack(msc, u, base, i)
         sl = slice(int(base[i]), int(base[i + 1]))
