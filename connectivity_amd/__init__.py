@@ -716,20 +716,23 @@ def _batch_args(vert_base, pivots):
 
 def run_scc_batch(g, vert_base, pivots=None, *, policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL,
                   timing: bool = False, record: bool = False, validate: bool = False, ctx: Context | None = None,
-                  alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False, trim_level: int = 1):
+                  alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False, trim_level: int = 1,
+                  schedule: np.ndarray | None = None):
     """SCC labels of every graph of a packed batch (msc_scc_run_batch); each graph's slice
-    holds its own local labels.  ``pivots`` (local ids, -1 none) default to create_csr's rule."""
+    holds its own local labels.  ``pivots`` (local ids, -1 none) default to create_csr's rule.
+    ``schedule`` forces the actions of the batch's merged FW / BW searches (schedule_array)."""
     lab, st = _run("msc_scc_run_batch", g, None, None, policy, labels, timing, record, validate, ctx, alpha, beta,
-                   out, None, batch=_batch_args(vert_base, pivots), trim_level=trim_level)
+                   out, schedule, batch=_batch_args(vert_base, pivots), trim_level=trim_level)
     return (lab, st) if stats else lab
 
 
 def run_cc_batch(g, vert_base, pivots=None, *, policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL,
                  timing: bool = False, record: bool = False, validate: bool = False, ctx: Context | None = None,
-                 alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False):
+                 alpha: float = 15.0, beta: int = 25, out=None, stats: bool = False,
+                 schedule: np.ndarray | None = None):
     """Weakly connected components of every graph of a packed batch (msc_cc_run_batch)."""
     lab, st = _run("msc_cc_run_batch", g, None, None, policy, labels, timing, record, validate, ctx, alpha, beta,
-                   out, None, batch=_batch_args(vert_base, pivots))
+                   out, schedule, batch=_batch_args(vert_base, pivots))
     return (lab, st) if stats else lab
 
 

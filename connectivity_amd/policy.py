@@ -32,8 +32,9 @@ import time
 import numpy as np
 
 from . import (ACT_STOP, DeviceGraph, DeviceLabels, LABELS_CANONICAL, MLP_BLOB, MLP_H, MLP_IN, MLP_OUT,
-               MLP_PARAMS, POLICY_LEARNED, POLICY_PLAIN, POLICY_WEIGHTS, SCHED_LEVELS, Context, gen_grid,
-               gen_planted, gen_rmat, mlp_train, run_cc, run_scc, schedule_array, set_policy)
+               MLP_PARAMS, POLICY_LEARNED, POLICY_PLAIN, POLICY_WEIGHTS, SCHED_LEVELS, Context, batch_pivots,
+               gen_batch, gen_grid, gen_planted, gen_rmat, mlp_train, run_cc, run_cc_batch, run_scc, run_scc_batch,
+               schedule_array, set_policy)
 
 FEATURES = ["log(next/n)", "log(next_edges/(avg*n))", "log(unvisited/n)", "log1p(level)", "next/n", "next_edges/(avg*n)",
             "log1p(next_edges/next)", "log1p(avg_degree)", "bottom_up", "switched_back", "found/num_valid",
@@ -63,13 +64,18 @@ def init_blob(mean: np.ndarray, std: np.ndarray, seed: int = 0) -> np.ndarray:
 # ---- training graphs ---------------------------------------------------------------------
 def training_graphs(count: int, seed: int = 7):
     """`count` seeded specs: R-MAT (scale 14-20, edge factor 8-32), grid digraphs (side
-    128-1024) and planted-SCC graphs, in a fixed interleaved order."""
+    128-1024), planted-SCC graphs and packed batches (128-1024 R-MAT / planted graphs, whose
+    merged searches meet states no single graph shows: most unvisited vertices lie in graphs
+    whose search has ended), in a fixed interleaved order."""
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(count):
-        kind = ("rmat", "rmat", "grid", "rmat", "grid", "planted")[i % 6]
+        kind = ("rmat", "rmat", "grid", "rmat", "grid", "planted", "batch")[i % 7]
         s = int(rng.integers(1, 1 << 30))
-        if kind == "rmat":
+        if kind == "batch":
+            specs.append(("batch", {"graphs": int(2 ** rng.integers(7, 11)), "median_n": int(2 ** rng.integers(10, 13)),
+                                    "seed": s}))
+        elif kind == "rmat":
             specs.append(("rmat", {"scale": int(rng.integers(14, 21)), "edge_factor": int(rng.choice([8, 12, 16, 24, 32])),
                                    "seed": s}))
         elif kind == "grid":
@@ -84,7 +90,13 @@ def training_graphs(count: int, seed: int = 7):
 
 
 def build(spec):
+    """The spec's graph; a batch spec gives the packed Graph with `.batch` = (vert_base, pivots)."""
     kind, kw = spec
+    if kind == "batch":
+        g, base, _ = gen_batch(kw["graphs"], median_n=kw["median_n"], sigma=1.4, min_n=256, max_n=200_000,
+                               seed=kw["seed"])
+        g.batch = (base, batch_pivots(g, base))
+        return g
     if kind == "rmat":
         return gen_rmat(kw["scale"], kw["edge_factor"], seed=kw["seed"])
     if kind == "grid":
@@ -100,13 +112,23 @@ class Runner:
         self.ctx, self.g, self.kind = ctx, g, kind
         self.dg = DeviceGraph(ctx, g)
         self.out = DeviceLabels(ctx, g.n)
-        self.fn = run_scc if kind == "scc" else run_cc
+        self.batch = getattr(g, "batch", None)
+        if self.batch is not None:
+            self.fn = run_scc_batch if kind == "scc" else run_cc_batch
+        else:
+            self.fn = run_scc if kind == "scc" else run_cc
 
     def run(self, schedule=None, policy=POLICY_PLAIN, record=False):
-        piv = self.g.max_deg_vert if self.kind == "scc" else None
-        avg = self.g.avg_degree if self.kind == "scc" else None
-        _, st = self.fn(self.dg, piv, avg, ctx=self.ctx, out=self.out, schedule=schedule, policy=policy,
-                        record=record, stats=True, labels=LABELS_CANONICAL)
+        kw = dict(ctx=self.ctx, out=self.out, schedule=schedule, policy=policy, record=record, stats=True,
+                  labels=LABELS_CANONICAL)
+        if self.batch is not None:
+            # SCC takes the batch's create_csr pivots; CC applies its own rule per graph
+            base, pivots = self.batch
+            _, st = self.fn(self.dg, base, pivots if self.kind == "scc" else None, **kw)
+        else:
+            piv = self.g.max_deg_vert if self.kind == "scc" else None
+            avg = self.g.avg_degree if self.kind == "scc" else None
+            _, st = self.fn(self.dg, piv, avg, **kw)
         recs = self.ctx.levels() if record else None
         return st.wall_ms, recs
 
@@ -289,13 +311,15 @@ def _logits(ctx, blob, X):
 
 
 def heldout_specs():
-    """Graphs outside the training seeds: R-MAT s18-s22 and grids 1024^2 / 2048^2."""
+    """Graphs outside the training seeds: R-MAT s18-s22, grids 1024^2 / 2048^2, a planted
+    1 M graph and a batch of 2048 graphs."""
     return [("rmat", {"scale": 18, "edge_factor": 16, "seed": 424242}),
             ("rmat", {"scale": 20, "edge_factor": 16, "seed": 424243}),
             ("rmat", {"scale": 22, "edge_factor": 16, "seed": 424244}),
             ("grid", {"rows": 1024, "cols": 1024, "p_one": 0.5, "p_both": 0.1, "arc_frac": 0.005, "seed": 424245}),
             ("grid", {"rows": 2048, "cols": 2048, "p_one": 0.5, "p_both": 0.1, "arc_frac": 0.005, "seed": 424246}),
-            ("planted", {"n": 1 << 20, "max_size": 4096, "seed": 424247})]
+            ("planted", {"n": 1 << 20, "max_size": 4096, "seed": 424247}),
+            ("batch", {"graphs": 2048, "median_n": 4096, "seed": 424248})]
 
 
 if __name__ == "__main__":
