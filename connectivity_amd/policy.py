@@ -18,7 +18,8 @@ Training data come from the library's own runs:
 The network is trained on the GPU by msc_mlp_train (hand-written forward / backward / Adam
 kernels, msc_mlp.hip).
 
-CLI:  python -m connectivity_amd.policy all --graphs 120 --out connectivity_amd/policy_weights.npz
+CLI:  python -m connectivity_amd.policy all --graphs 210 --reps 2 --out connectivity_amd/policy_weights.npz
+      python -m connectivity_amd.policy train --data <collected.npz> --seed 2 --epochs 600 --out ...
 """
 from __future__ import annotations
 
@@ -272,6 +273,7 @@ def main(argv=None) -> int:
     ap.add_argument("--data", default="gpurun_out/policy_data.npz")
     ap.add_argument("--out", default=POLICY_WEIGHTS)
     ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=1, help="initialisation and minibatch order of the training")
     ap.add_argument("--eval-specs", default="heldout")
     args = ap.parse_args(argv)
     ctx = Context(0)
@@ -287,7 +289,7 @@ def main(argv=None) -> int:
         with np.load(args.data, allow_pickle=False) as z:
             X, Y, W = z["X"], z["Y"], z["W"]
         t0 = time.time()
-        blob, curve = fit(ctx, X, Y, W, epochs=args.epochs)
+        blob, curve = fit(ctx, X, Y, W, epochs=args.epochs, seed=args.seed)
         acc = float(np.mean(np.argmax(_logits(ctx, blob, X), axis=1) == Y))
         log(f"trained on {len(Y)} samples in {time.time() - t0:.1f}s: loss {curve[0]:.4f} -> {curve[-1]:.4f}, "
             f"train accuracy {acc:.3f}")
