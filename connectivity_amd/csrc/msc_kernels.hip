@@ -972,6 +972,19 @@ __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* in
   }
 }
 
+// Pull levels hand out their bitmap word groups XCD-aware.  Workgroups are dispatched to
+// the 8 XCDs round robin (workgroup b to XCD b % 8) and each XCD has its own L2, so with
+// the groups dealt out in order every XCD touches every region of the graph at once.  Instead the
+// positions 0, 1, 2, ... a grid-stride loop visits are padded to a multiple of 8 and
+// position p stands for group (p % 8) * ceil(groups / 8) + p / 8: with a grid that is a
+// multiple of 8, XCD x walks the x-th contiguous eighth of the groups, and neighbouring
+// vertices (one graph of a batch, one band of a mesh) share an L2.  The map is a
+// bijection onto [0, 8 * ceil(groups / 8)); positions past the last group do nothing.
+__device__ __forceinline__ uint32_t xcd_positions(uint32_t groups) { return (groups + 7u) & ~7u; }
+__device__ __forceinline__ uint32_t xcd_group(uint32_t pos, uint32_t groups) {
+  return (pos & 7u) * ((groups + 7u) >> 3) + (pos >> 3);
+}
+
 // Edges a pull level's workgroup walks per step (kPullPer per thread, all loads in flight
 // together).
 constexpr int kPullPer = 8;
@@ -1127,8 +1140,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   if (OP == OP_PUSH && c->bottom_up != 0u) {
     // pull level: every word group of the bitmap; part row = (candidates, edges, raised,
     // colour reads)
-    const int64_t groups = (a.nwords + kBlock - 1) / kBlock;
-    if ((int64_t)blockIdx.x >= groups) return;  // uniform
+    const uint32_t groups = (uint32_t)((a.nwords + kBlock - 1) / kBlock), pos = xcd_positions(groups);
+    if (blockIdx.x >= pos) return;  // uniform
     const Frontier N = c->cur ? a.fr0 : a.fr1;
     uint32_t* inq_cur = c->cur ? a.inq1 : a.inq0;
     const bool ids = c->bottom_up == 2u;
@@ -1138,8 +1151,10 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
     }
     __syncthreads();
     unsigned long long ra = 0, cn = 0, ed = 0, rd = 0;
-    for (int64_t g = blockIdx.x; g < groups; g += gridDim.x)
-      pull_group<MIRROR>(a, N, inq_cur, g * kBlock, ids, S, lds.P, ra, cn, ed, rd);
+    for (uint32_t p = blockIdx.x; p < pos; p += gridDim.x) {
+      const uint32_t g = xcd_group(p, groups);
+      if (g < groups) pull_group<MIRROR>(a, N, inq_cur, (int64_t)g * kBlock, ids, S, lds.P, ra, cn, ed, rd);
+    }
     stage_flush(S, N, c, 0u);
     ra = block_sum64(ra, s_red);
     rd = block_sum64(rd, s_red);
@@ -1171,6 +1186,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   // nothing reads the current worklist bitmap during a grid-wide level (local rounds use
   // the next one) and the level must leave it clear: each expanded vertex's whole word is
   // cleared with a plain store instead of an atomic on its bit
+  // (chunks dealt out XCD-aware, as pull groups are, made no difference on the grid or
+  // the batch)
   for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
     td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
   unsigned long long loc_n = 0, loc_e = 0;
@@ -1208,6 +1225,8 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
   const Frontier N = cur ? a.fr0 : a.fr1;
   unsigned long long unused = 0ull;
   if (bu) {
+    // groups in plain order: dealt out XCD-aware (as pull levels are) they cost R-MAT s24
+    // 2.5% and gave a batch nothing
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       bu_group<MIRROR>(a, N, (int64_t)lu * kBW, S, B, s_claim, ex, in);
   } else {
@@ -1545,7 +1564,7 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
   {
     const uint32_t q_size = c->q_size, q_edges = c->q_edges;
     const bool pulled = OP == OP_PUSH && c->bottom_up != 0u;
-    const int rows = pulled ? (int)min((int64_t)nparts, (a.nwords + kBlock - 1) / kBlock)
+    const int rows = pulled ? (int)min((uint32_t)nparts, xcd_positions((uint32_t)((a.nwords + kBlock - 1) / kBlock)))
                             : active_rows(nparts, q_edges);
     unsigned long long ln = 0, le = 0, ra = 0, rd = 0;
     for (int i = t; i < rows; i += kBlock) {
