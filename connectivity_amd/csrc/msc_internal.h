@@ -32,9 +32,12 @@ constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlp
 // feature standardisation (mean, std) follows the weights
 constexpr int kMlpBlob = kMlpParams + 2 * kMlpIn;
 
-// What one level kernel does with each edge it walks.  Reach ops keep one "claimable"
-// bitmap, cand = allow & ~visited: a top-down edge costs a single probe, a claim is one
-// atomicAnd that clears the bit, and the set bits of cand are the bottom-up candidates.
+// What one level kernel does with each edge it walks.  Reach ops keep one bitmap per
+// search, cand = ~visited (all ones at the start; a claim is one atomicAnd, or a plain
+// word store in a bottom-up level, that clears the bit).  A bottom-up probe of a
+// neighbour is then a single load (visited = cand bit clear; only allowed vertices and
+// the root are ever claimed), a top-down edge probes cand & allow, and the bottom-up
+// candidates are the set bits of cand & allow.
 enum Op : int {
   OP_FW = 0,   // forward reach:  allow = valid, top-down on out-lists, bottom-up on in-lists
   OP_BW = 1,   // backward reach: allow = fw set, top-down on in-lists, bottom-up on out-lists
@@ -118,7 +121,7 @@ struct LevelArgs {
   const uint32_t* up_off2;
   const int32_t* up_adj2;
   uint32_t* allow;            // bitmap: vertices the search may visit
-  uint32_t* cand;             // bitmap: allowed and not yet visited (cleared on claim)
+  uint32_t* cand;             // bitmap: not yet visited (cleared on claim; allowed or not)
   uint32_t* valid;            // bitmap: vertices without a label yet
   uint32_t* inq0;             // colour worklist membership, buffer 0 / 1
   uint32_t* inq1;
@@ -169,7 +172,8 @@ int ctx_set_policy(msc_ctx* ctx, const float* blob);  // kMlpBlob floats, valida
 // ---- kernel launchers (msc_kernels.hip) ----
 // Launchers that leave per-workgroup partials return how many workgroups wrote them.
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count);
-// valid bitmap (and copies in cand1 / cand2, when given), trimmed labels; part col 0 = survivors
+// valid bitmap (and the searches' cand1 / cand2 set to all ones, when given), trimmed
+// labels; part col 0 = survivors
 int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
                 uint32_t* valid, uint32_t* cand1, uint32_t* cand2, int32_t* labels, unsigned long long* part);
 // out[i] = sums (cols 0..2) and max (col 3) of part[0..nparts)[i]

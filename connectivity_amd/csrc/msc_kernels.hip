@@ -138,13 +138,24 @@ static_assert(BitIter::kWords == kBW, "bottom-up group width");
 // prefix of their popcounts; returns the set bits in the group.  All threads call; ends
 // with a barrier.
 template <int WPT>
+__device__ __forceinline__ uint32_t bits_scan(BitIterT<WPT>& B, const uint32_t* w);
+
+template <int WPT>
 __device__ __forceinline__ uint32_t bits_load(BitIterT<WPT>& B, const uint32_t* bits, int64_t nwords, int64_t wb) {
-  constexpr int kW = BitIterT<WPT>::kWords;
   const int t = threadIdx.x;
   const int64_t w0 = wb + (int64_t)t * WPT;
   uint32_t w[WPT];
 #pragma unroll
   for (int j = 0; j < WPT; ++j) w[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
+  return bits_scan(B, w);
+}
+
+// The group's words (WPT per thread, given) into B with the prefix of their popcounts;
+// returns the set bits in the group.  All threads call; ends with a barrier.
+template <int WPT>
+__device__ __forceinline__ uint32_t bits_scan(BitIterT<WPT>& B, const uint32_t* w) {
+  constexpr int kW = BitIterT<WPT>::kWords;
+  const int t = threadIdx.x;
   uint32_t cnt = 0;
 #pragma unroll
   for (int j = 0; j < WPT; ++j) {
@@ -165,6 +176,21 @@ __device__ __forceinline__ uint32_t bits_load(BitIterT<WPT>& B, const uint32_t* 
   if (t == 0) B.pref[0] = 0u;
   __syncthreads();
   return B.pref[kW];
+}
+
+// bits_load of (bits & mask): also returns this thread's words of `bits` in keep[].
+template <int WPT>
+__device__ __forceinline__ uint32_t bits_load_and(BitIterT<WPT>& B, const uint32_t* bits, const uint32_t* mask,
+                                                  int64_t nwords, int64_t wb, uint32_t* keep) {
+  const int t = threadIdx.x;
+  const int64_t w0 = wb + (int64_t)t * WPT;
+  uint32_t w[WPT];
+#pragma unroll
+  for (int j = 0; j < WPT; ++j) {
+    keep[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
+    w[j] = w0 + j < nwords ? keep[j] & mask[w0 + j] : 0u;
+  }
+  return bits_scan(B, w);
 }
 
 // Vertex id of the k-th set bit (k < total) of the loaded words.
@@ -387,8 +413,9 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const unsigned long long* __r
 
 // Degree trim (multistep/scc_trim.cpp:89-139; kokkos_scc/scc_trim.cpp:57-140) and the CC
 // variant (multistep/cc_trim.cpp:47-97).  Each wave round covers 64 consecutive vertices,
-// whose ballot is two words of the valid bitmap (also written to the searches' claimable
-// bitmaps cand1 / cand2 when given); part col 0 = survivors of the workgroup.
+// whose ballot is two words of the valid bitmap; the searches' bitmaps cand1 / cand2, when
+// given, start as all ones in the same words (nothing visited yet); part col 0 =
+// survivors of the workgroup.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __restrict__ oo,
                                                  const uint32_t* __restrict__ io, uint32_t* __restrict__ valid,
@@ -424,8 +451,8 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
         const int64_t wi = word + (lane() >> 5);
         const uint32_t bits = (uint32_t)(mask >> (lane() & 32));
         valid[wi] = bits;
-        if (cand1) cand1[wi] = bits;
-        if (cand2) cand2[wi] = bits;
+        if (cand1) cand1[wi] = 0xffffffffu;
+        if (cand2) cand2[wi] = 0xffffffffu;
       }
       kept += keep ? 1 : 0;
     }
@@ -801,7 +828,9 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
     word[r] = 0;
     wcol[r] = 0;
     if (ok[r]) {
-      word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
+      // reach: claimable = allowed and not yet visited (cand has a bit for both the
+      // unvisited and the not allowed)
+      word[r] = REACH ? (a.cand[w[r] >> 5] & a.allow[w[r] >> 5]) : a.valid[w[r] >> 5];
       if (COLOR) wcol[r] = a.colors[w[r]];
     }
   }
@@ -848,9 +877,11 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
   const int t = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < kWPT; ++j) s_claim[t * kWPT + j] = 0u;
-  const uint32_t total = bits_load(B, a.cand, a.nwords, wb);  // barrier: s_claim cleared
+  // candidates: allowed and unvisited; this thread's cand words are kept for the write-back
+  uint32_t cw[kWPT];
+  const uint32_t total = bits_load_and(B, a.cand, a.allow, a.nwords, wb, cw);  // barrier: s_claim cleared
   for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
-    uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR], wa[kR];
+    uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR];
     bool has[kR], hit[kR], first[kR];
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
@@ -884,11 +915,10 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
         }
       }
     }
+    // a parent is a visited vertex (the reference's `fw[in] && valid[in]`): its cand bit
+    // is clear, which only a claim (of an allowed vertex, or the root) does
 #pragma unroll
-    for (int r = 0; r < kR; ++r) {
-      wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
-      wa[r] = first[r] ? a.allow[u[r] >> 5] : 0u;
-    }
+    for (int r = 0; r < kR; ++r) wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
       hit[r] = false;
@@ -896,19 +926,16 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
       ++my_examined;
       if (first[r]) {
         ++my_inspected;
-        uint32_t b = 1u << (u[r] & 31u);
-        hit[r] = !(wc[r] & b) && (wa[r] & b);
+        hit[r] = !(wc[r] & (1u << (u[r] & 31u)));
       }
       for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
         ++my_inspected;
-        uint32_t x = (uint32_t)a.up_adj[j];
-        hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
+        hit[r] = !bit_test(a.cand, (uint32_t)a.up_adj[j]);
       }
       if (MIRROR)
         for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
           ++my_inspected;
-          uint32_t x = (uint32_t)a.up_adj2[j];
-          hit[r] = !bit_test(a.cand, x) && bit_test(a.allow, x);
+          hit[r] = !bit_test(a.cand, (uint32_t)a.up_adj2[j]);
         }
       if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
     }
@@ -924,7 +951,7 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
 #pragma unroll
   for (int j = 0; j < kWPT; ++j) {
     const int i = t * kWPT + j;
-    if (s_claim[i]) a.cand[wb + i] = B.word[i] & ~s_claim[i];
+    if (s_claim[i]) a.cand[wb + i] = cw[j] & ~s_claim[i];
   }
 }
 

@@ -582,7 +582,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   } else {
     Timed t(c, MSC_K_TRIM);
-    // the searches' claimable bitmaps start as the valid set: written by the trim as well;
+    // the searches' bitmaps start with nothing visited: written by the trim as well;
     // no labels yet (the commit writes every label in one pass)
     np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, nullptr, c->part);
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
@@ -636,7 +636,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         bw.ctrl = &cs[1]; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
-        if (!cand_ready) launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);  // cand = allow, for both
+        if (!cand_ready) {  // nothing visited yet, for both
+          launch_fill_u32(L, c->b1, 0xffffffffu, nw);
+          launch_fill_u32(L, c->b3, 0xffffffffu, nw);
+        }
         if (bt) {
           launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
           launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
@@ -651,14 +654,15 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         // SCC = FW set & BW set = valid & ~b1 & ~b3
         member = MemberSrc{c->b1, c->b3, &cs[0], &cs[1]};
       } else {
-        if (!cand_ready) launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
+        if (!cand_ready) launch_fill_u32(L, c->b1, 0xffffffffu, nw);  // nothing visited yet
         if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
         else launch_search_init(L, fw, pivot, 0);
         if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
         if (s) s->fw_levels = (int32_t)hs[0].level;
         if (!hs[0].abandoned) {
-          // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy), control block [0] again
-          launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
+          // BW: allow = fw set (b2 = valid & ~b1), nothing visited (b3), control block [0] again
+          launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);
+          launch_fill_u32(L, c->b3, 0xffffffffu, nw);
           bw.allow = c->b2; bw.cand = c->b3;
           if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
           else launch_search_init(L, bw, pivot, 1);
@@ -666,14 +670,14 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
           nsearch = 1;
           sop = OP_BW;
           s0 = &cs[0];
-          // SCC = fw set & visited by BW = (valid & ~b1) & ~b3, as b3 started as the fw set
+          // SCC = fw set & visited by BW = (valid & ~b1) & ~b3 (BW claims only fw-set vertices)
           member = MemberSrc{c->b1, c->b3, &cs[0], nullptr};
         }
       }
     } else {
       LevelArgs bf = a;
       bf.op = OP_CC;
-      if (!cand_ready) launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);  // cand = allow
+      if (!cand_ready) launch_fill_u32(L, c->b1, 0xffffffffu, nw);  // nothing visited yet
       bf.down_off = g.out_off; bf.down_adj = g.out_adj;
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
