@@ -18,6 +18,11 @@ constexpr uint32_t kStage = 2048;              // claims staged in LDS per workg
 #endif
 constexpr int kWPT = MSC_BU_WPT;               // bitmap words per thread of a bottom-up word group
 constexpr int kBW = kBlock * kWPT;             // bitmap words per bottom-up word group
+#ifndef MSC_BU_WPT_WIDE
+#define MSC_BU_WPT_WIDE 4
+#endif
+constexpr int kWPTWide = MSC_BU_WPT_WIDE;      // ... and of a sparse bottom-up level (few candidates)
+constexpr int kBWide = kBlock * kWPTWide;
 constexpr int kMaxParts = 8192;                // per-workgroup partial results
 // workgroup-local continuation of colour levels: private frontiers of at most kLocalEdges
 // edges, two per workgroup (ids, offsets, chunk starts)
@@ -79,7 +84,7 @@ struct Ctrl {
   uint32_t closes;                 // closing-kernel launches that closed a level
   uint32_t list_len;               // vertex list written by the last seeding launch (kept by resets)
   uint32_t round_edges;            // colour push: list lengths of the round's seeded frontier
-  uint32_t pad_;
+  uint32_t bu_wide;                // reach: the coming bottom-up level walks kBWide-word groups
   // counts the host reads with the next control-block copy instead of a round trip each
   unsigned long long red_trim[4];    // col 0: trim survivors
   unsigned long long red_commit[4];  // col 0: vertices committed by FW-BW / the CC search

@@ -543,6 +543,7 @@ __global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t pha
     c->q_edges = deg;
     c->cur = 0u;
     c->bottom_up = 0u;
+    c->bu_wide = 0u;
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
@@ -568,6 +569,7 @@ __device__ __forceinline__ void ctrl_fresh(Ctrl* c, int32_t phase) {
   c->q_edges = 0u;
   c->cur = 0u;
   c->bottom_up = 0u;
+  c->bu_wide = 0u;
   c->switched_back = 0u;
   c->done = 0u;
   c->level = 0u;
@@ -677,6 +679,7 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
     c->q_edges = 0u;
     c->cur = 0u;
     c->bottom_up = 0u;
+    c->bu_wide = 0u;
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
@@ -869,16 +872,16 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
 // the time); the rest scan on.  Claims collect in LDS word masks and are written back with
 // one plain store per word: no other workgroup clears bits of these words during a
 // bottom-up level.  Every thread calls; claims are staged for frontier N.
-template <bool MIRROR>
-__device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, int64_t wb, Stage& S, BitIter& B,
-                                         uint32_t* s_claim, unsigned long long& my_examined,
+template <bool MIRROR, int WPT>
+__device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, int64_t wb, Stage& S,
+                                         BitIterT<WPT>& B, uint32_t* s_claim, unsigned long long& my_examined,
                                          unsigned long long& my_inspected) {
   constexpr int kR = MSC_BU_KR;
   const int t = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < kWPT; ++j) s_claim[t * kWPT + j] = 0u;
+  for (int j = 0; j < WPT; ++j) s_claim[t * WPT + j] = 0u;
   // candidates: allowed and unvisited; this thread's cand words are kept for the write-back
-  uint32_t cw[kWPT];
+  uint32_t cw[WPT];
   const uint32_t total = bits_load_and(B, a.cand, a.allow, a.nwords, wb, cw);  // barrier: s_claim cleared
   for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
     uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR];
@@ -949,8 +952,8 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
   // after the last round's barrier every claim of this group is in s_claim; each thread
   // reads only its own words and masks, so the next group's loads need no barrier first
 #pragma unroll
-  for (int j = 0; j < kWPT; ++j) {
-    const int i = t * kWPT + j;
+  for (int j = 0; j < WPT; ++j) {
+    const int i = t * WPT + j;
     if (s_claim[i]) a.cand[wb + i] = cw[j] & ~s_claim[i];
   }
 }
@@ -1235,7 +1238,10 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
 // kChunk-edge chunks of the frontier (top-down); none once the search is done.
 __device__ __forceinline__ uint32_t reach_units(const Ctrl* c, int64_t nwords) {
   if (c->done) return 0u;
-  if (c->bottom_up) return (uint32_t)((nwords + kBW - 1) / kBW);
+  if (c->bottom_up) {
+    const int64_t gw = c->bu_wide ? kBWide : kBW;
+    return (uint32_t)((nwords + gw - 1) / gw);
+  }
   return (c->q_edges + kChunk - 1) / kChunk;
 }
 
@@ -1244,7 +1250,8 @@ __device__ __forceinline__ uint32_t reach_units(const Ctrl* c, int64_t nwords) {
 template <bool MIRROR>
 __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first, uint32_t units, Stage& S,
                                              ChunkLds<OP_FW, MIRROR>& L, BitIter& B, uint32_t* s_claim,
-                                             unsigned long long& ex, unsigned long long& in) {
+                                             BitIterT<kWPTWide>& BW, uint32_t* s_claimw, unsigned long long& ex,
+                                             unsigned long long& in) {
   Ctrl* c = a.ctrl;
   const uint32_t cur = c->cur, q_size = c->q_size, q_edges = c->q_edges;
   const bool bu = c->bottom_up != 0u;
@@ -1253,9 +1260,16 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
   unsigned long long unused = 0ull;
   if (bu) {
     // groups in plain order: dealt out XCD-aware (as pull levels are) they cost R-MAT s24
-    // 2.5% and gave a batch nothing
-    for (uint32_t lu = first; lu < units; lu += gridDim.x)
-      bu_group<MIRROR>(a, N, (int64_t)lu * kBW, S, B, s_claim, ex, in);
+    // 2.5% and gave a batch nothing.  A level with few candidates left walks wider
+    // groups: its time is the per-group chain of barriers and dependent loads, not the
+    // candidates, and the grid then covers the bitmap in one round of groups
+    if (c->bu_wide) {
+      for (uint32_t lu = first; lu < units; lu += gridDim.x)
+        bu_group<MIRROR, kWPTWide>(a, N, (int64_t)lu * kBWide, S, BW, s_claimw, ex, in);
+    } else {
+      for (uint32_t lu = first; lu < units; lu += gridDim.x)
+        bu_group<MIRROR, kWPT>(a, N, (int64_t)lu * kBW, S, B, s_claim, ex, in);
+    }
   } else {
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
@@ -1282,11 +1296,17 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
       BitIter B;
       uint32_t claim[kBW];
     } bu;
+    struct {
+      BitIterT<kWPTWide> B;
+      uint32_t claim[kBWide];
+    } buw;
   } U;
   __shared__ unsigned long long s_red[kBlock / 64];
   ChunkLds<OP_FW, MIRROR>& L = U.L;
   BitIter& B = U.bu.B;
+  BitIterT<kWPTWide>& BW = U.buw.B;
   uint32_t* s_claim = U.bu.claim;
+  uint32_t* s_claimw = U.buw.claim;
 
   const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
   const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
@@ -1300,11 +1320,11 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   }
   __syncthreads();
   unsigned long long ex0 = 0ull, in0 = 0ull, ex1 = 0ull, in1 = 0ull;
-  if (b < u0) reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, ex0, in0);
+  if (b < u0) reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, ex0, in0);
   if (NS == 2) {
     // this workgroup's first unit of the union at or past u0, in search 1's numbering
     const uint32_t first = b >= u0 ? b - u0 : b + (u0 - b + G - 1) / G * G - u0;
-    if (first < u1) reach_search<MIRROR>(p.s[1], first, u1, S, L, B, s_claim, ex1, in1);
+    if (first < u1) reach_search<MIRROR>(p.s[1], first, u1, S, L, B, s_claim, BW, s_claimw, ex1, in1);
   }
   unsigned long long e = block_sum64(ex0, s_red), i = block_sum64(in0, s_red);
   if (t == 0) {
@@ -1513,6 +1533,12 @@ __device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned
     if (f >= 0 && f <= 2) act = f;
   }
   finish_level(c, a, a.op, bu, act, q_size, q_edges, ex, in, 0ull, next, nedges, s_feat);
+  // a bottom-up level over fewer than one unvisited vertex in 8 walks wide word groups
+  // (unvisited: what the trim left, less what the search found; an estimate when the
+  // allowed set is smaller)
+  const unsigned long long nvl = a.nvalid ? *a.nvalid : (unsigned long long)a.n_total;
+  const long long left = (long long)nvl - (long long)c->found;
+  c->bu_wide = c->bottom_up && left * 8ll < (long long)a.nwords * 32ll ? 1u : 0u;
 }
 
 // Closes a level of NS reach searches (one workgroup): all threads sum the partial rows,
@@ -1864,6 +1890,7 @@ __global__ void k_trim2_init(LevelArgs a) {
     c->q_edges = 0u;
     c->cur = 0u;
     c->bottom_up = 0u;
+    c->bu_wide = 0u;
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;

@@ -120,7 +120,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
     (void)hipFree(c->ws);
     c->ws = nullptr;
   }
-  size_t nw = (size_t)((nn + 31) / 32) + kBW;  // slack: bitmap walks load whole kBW-word groups
+  size_t nw = (size_t)((nn + 31) / 32) + kBWide;  // slack past the last word group
   size_t nchunks = (size_t)((ee + kChunk - 1) / kChunk) + 2;
   size_t sizes[] = {
       nw * 4, nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,           // bitmaps
