@@ -37,12 +37,13 @@ constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlp
 // feature standardisation (mean, std) follows the weights
 constexpr int kMlpBlob = kMlpParams + 2 * kMlpIn;
 
-// What one level kernel does with each edge it walks.  Reach ops keep one bitmap per
-// search, cand = ~visited (all ones at the start; a claim is one atomicAnd, or a plain
-// word store in a bottom-up level, that clears the bit).  A bottom-up probe of a
-// neighbour is then a single load (visited = cand bit clear; only allowed vertices and
-// the root are ever claimed), a top-down edge probes cand & allow, and the bottom-up
-// candidates are the set bits of cand & allow.
+// What one level kernel does with each edge it walks.  Reach ops keep two bitmaps per
+// search: cand = allow & ~visited ("claimable": a top-down edge costs a single probe, a
+// claim is one atomicAnd that clears the bit, and the set bits of cand are the bottom-up
+// candidates) and seen = ~visited (all ones at the start; a bottom-up probe of a
+// neighbour is a single load of it, where allow & ~cand would take two).  A claim clears
+// both: an extra atomic without result in a top-down level, an extra plain word store in
+// a bottom-up level.
 enum Op : int {
   OP_FW = 0,   // forward reach:  allow = valid, top-down on out-lists, bottom-up on in-lists
   OP_BW = 1,   // backward reach: allow = fw set, top-down on in-lists, bottom-up on out-lists
@@ -126,7 +127,8 @@ struct LevelArgs {
   const uint32_t* up_off2;
   const int32_t* up_adj2;
   uint32_t* allow;            // bitmap: vertices the search may visit
-  uint32_t* cand;             // bitmap: not yet visited (cleared on claim; allowed or not)
+  uint32_t* cand;             // bitmap: allowed and not yet visited (cleared on claim)
+  uint32_t* seen;             // bitmap: not yet visited (cleared on claim; allowed or not)
   uint32_t* valid;            // bitmap: vertices without a label yet
   uint32_t* inq0;             // colour worklist membership, buffer 0 / 1
   uint32_t* inq1;
@@ -177,10 +179,11 @@ int ctx_set_policy(msc_ctx* ctx, const float* blob);  // kMlpBlob floats, valida
 // ---- kernel launchers (msc_kernels.hip) ----
 // Launchers that leave per-workgroup partials return how many workgroups wrote them.
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count);
-// valid bitmap (and the searches' cand1 / cand2 set to all ones, when given), trimmed
-// labels; part col 0 = survivors
+// valid bitmap (and, when given, the searches' cand1 / cand2 = valid, seen1 / seen2 = all
+// ones), trimmed labels; part col 0 = survivors
 int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
-                uint32_t* valid, uint32_t* cand1, uint32_t* cand2, int32_t* labels, unsigned long long* part);
+                uint32_t* valid, uint32_t* cand1, uint32_t* cand2, uint32_t* seen1, uint32_t* seen2,
+                int32_t* labels, unsigned long long* part);
 // out[i] = sums (cols 0..2) and max (col 3) of part[0..nparts)[i]
 void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out);
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);

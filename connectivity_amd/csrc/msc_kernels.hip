@@ -178,17 +178,17 @@ __device__ __forceinline__ uint32_t bits_scan(BitIterT<WPT>& B, const uint32_t* 
   return B.pref[kW];
 }
 
-// bits_load of (bits & mask): also returns this thread's words of `bits` in keep[].
+// bits_load that also loads this thread's words of a second bitmap into keep[].
 template <int WPT>
-__device__ __forceinline__ uint32_t bits_load_and(BitIterT<WPT>& B, const uint32_t* bits, const uint32_t* mask,
-                                                  int64_t nwords, int64_t wb, uint32_t* keep) {
+__device__ __forceinline__ uint32_t bits_load_keep(BitIterT<WPT>& B, const uint32_t* bits, const uint32_t* other,
+                                                   int64_t nwords, int64_t wb, uint32_t* keep) {
   const int t = threadIdx.x;
   const int64_t w0 = wb + (int64_t)t * WPT;
   uint32_t w[WPT];
 #pragma unroll
   for (int j = 0; j < WPT; ++j) {
-    keep[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
-    w[j] = w0 + j < nwords ? keep[j] & mask[w0 + j] : 0u;
+    w[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
+    keep[j] = w0 + j < nwords ? other[w0 + j] : 0u;
   }
   return bits_scan(B, w);
 }
@@ -413,13 +413,14 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const unsigned long long* __r
 
 // Degree trim (multistep/scc_trim.cpp:89-139; kokkos_scc/scc_trim.cpp:57-140) and the CC
 // variant (multistep/cc_trim.cpp:47-97).  Each wave round covers 64 consecutive vertices,
-// whose ballot is two words of the valid bitmap; the searches' bitmaps cand1 / cand2, when
-// given, start as all ones in the same words (nothing visited yet); part col 0 =
-// survivors of the workgroup.
+// whose ballot is two words of the valid bitmap; the searches' bitmaps start in the same
+// words, when given: claimable cand1 / cand2 = valid, unvisited seen1 / seen2 = all ones.
+// part col 0 = survivors of the workgroup.
 template <bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __restrict__ oo,
                                                  const uint32_t* __restrict__ io, uint32_t* __restrict__ valid,
                                                  uint32_t* __restrict__ cand1, uint32_t* __restrict__ cand2,
+                                                 uint32_t* __restrict__ seen1, uint32_t* __restrict__ seen2,
                                                  int32_t* __restrict__ labels, unsigned long long* __restrict__ part) {
   __shared__ unsigned long long s_red[kBlock / 64];
   const int64_t nw = ((int64_t)n + 31) / 32;
@@ -451,8 +452,10 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
         const int64_t wi = word + (lane() >> 5);
         const uint32_t bits = (uint32_t)(mask >> (lane() & 32));
         valid[wi] = bits;
-        if (cand1) cand1[wi] = 0xffffffffu;
-        if (cand2) cand2[wi] = 0xffffffffu;
+        if (cand1) cand1[wi] = bits;
+        if (cand2) cand2[wi] = bits;
+        if (seen1) seen1[wi] = 0xffffffffu;
+        if (seen2) seen2[wi] = 0xffffffffu;
       }
       kept += keep ? 1 : 0;
     }
@@ -554,6 +557,7 @@ __global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t pha
     c->t_start = ~0ull;
     uint32_t m = 1u << ((uint32_t)root & 31u);
     a.cand[(uint32_t)root >> 5] &= ~m;
+    a.seen[(uint32_t)root >> 5] &= ~m;
     a.fr0.q[0] = root;
     a.fr0.qo[0] = 0u;
   }
@@ -831,9 +835,7 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
     word[r] = 0;
     wcol[r] = 0;
     if (ok[r]) {
-      // reach: claimable = allowed and not yet visited (cand has a bit for both the
-      // unvisited and the not allowed)
-      word[r] = REACH ? (a.cand[w[r] >> 5] & a.allow[w[r] >> 5]) : a.valid[w[r] >> 5];
+      word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
       if (COLOR) wcol[r] = a.colors[w[r]];
     }
   }
@@ -843,6 +845,9 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
     bool got = false;  // claimed (reach, colour-BW) / raised (push) by this lane
     if (REACH) {
       got = in && bit_clear(a.cand, w[r]);
+      // and visited for bottom-up probes (no result needed: only this claim clears it)
+      if (got) __hip_atomic_fetch_and(&a.seen[w[r] >> 5], ~(1u << (w[r] & 31u)), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
     } else if (OP == OP_PUSH) {
       if (in && wcol[r] < col[r] && atomicMax(&a.colors[w[r]], col[r]) < col[r]) {
         ++my_raised;
@@ -880,9 +885,9 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
   const int t = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < WPT; ++j) s_claim[t * WPT + j] = 0u;
-  // candidates: allowed and unvisited; this thread's cand words are kept for the write-back
-  uint32_t cw[WPT];
-  const uint32_t total = bits_load_and(B, a.cand, a.allow, a.nwords, wb, cw);  // barrier: s_claim cleared
+  // candidates: the claimable bits; this thread's seen words are kept for the write-back
+  uint32_t sw[WPT];
+  const uint32_t total = bits_load_keep(B, a.cand, a.seen, a.nwords, wb, sw);  // barrier: s_claim cleared
   for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
     uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR];
     bool has[kR], hit[kR], first[kR];
@@ -918,10 +923,10 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
         }
       }
     }
-    // a parent is a visited vertex (the reference's `fw[in] && valid[in]`): its cand bit
+    // a parent is a visited vertex (the reference's `fw[in] && valid[in]`): its seen bit
     // is clear, which only a claim (of an allowed vertex, or the root) does
 #pragma unroll
-    for (int r = 0; r < kR; ++r) wc[r] = first[r] ? a.cand[u[r] >> 5] : 0xffffffffu;
+    for (int r = 0; r < kR; ++r) wc[r] = first[r] ? a.seen[u[r] >> 5] : 0xffffffffu;
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
       hit[r] = false;
@@ -933,12 +938,12 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
       }
       for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
         ++my_inspected;
-        hit[r] = !bit_test(a.cand, (uint32_t)a.up_adj[j]);
+        hit[r] = !bit_test(a.seen, (uint32_t)a.up_adj[j]);
       }
       if (MIRROR)
         for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
           ++my_inspected;
-          hit[r] = !bit_test(a.cand, (uint32_t)a.up_adj2[j]);
+          hit[r] = !bit_test(a.seen, (uint32_t)a.up_adj2[j]);
         }
       if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
     }
@@ -954,7 +959,10 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
 #pragma unroll
   for (int j = 0; j < WPT; ++j) {
     const int i = t * WPT + j;
-    if (s_claim[i]) a.cand[wb + i] = cw[j] & ~s_claim[i];
+    if (s_claim[i]) {
+      a.cand[wb + i] = B.word[i] & ~s_claim[i];
+      a.seen[wb + i] = sw[j] & ~s_claim[i];
+    }
   }
 }
 
@@ -2115,6 +2123,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32
     uint32_t deg = 0;
     if (p >= 0) {
       atomicAnd(&a.cand[(uint32_t)p >> 5], ~(1u << ((uint32_t)p & 31u)));
+      atomicAnd(&a.seen[(uint32_t)p >> 5], ~(1u << ((uint32_t)p & 31u)));
       deg = list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, (uint32_t)p);
     }
     stage_push(S, deg > 0u, (uint32_t)p, deg);  // a root without a list reaches nothing more
@@ -2234,14 +2243,15 @@ void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count
 }
 
 int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, const uint32_t* io, uint32_t* valid,
-                uint32_t* cand1, uint32_t* cand2, int32_t* labels, unsigned long long* part) {
+                uint32_t* cand1, uint32_t* cand2, uint32_t* seen1, uint32_t* seen2, int32_t* labels,
+                unsigned long long* part) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
   if (mirror)
-    hipLaunchKernelGGL(k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, labels,
-                       part);
+    hipLaunchKernelGGL(k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, seen1,
+                       seen2, labels, part);
   else
-    hipLaunchKernelGGL(k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, labels,
-                       part);
+    hipLaunchKernelGGL(k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, seen1,
+                       seen2, labels, part);
   return g;
 }
 

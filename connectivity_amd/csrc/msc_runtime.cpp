@@ -48,6 +48,7 @@ struct msc_ctx {
   int64_t cap_n = -1, cap_e = -1;
   // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
   uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
+  uint32_t *s1 = nullptr, *s3 = nullptr;  // the searches' unvisited sets (b1's and b3's)
   int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
   Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the backward search's (beside the forward one)
   int32_t* lists[3] = {nullptr, nullptr, nullptr};  // vertex lists of the colouring rounds
@@ -136,6 +137,7 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)kMaxParts * 4 * 8,                                // partials of the backward search
       (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // colouring vertex lists
       (size_t)kLocalStride * 4 * (size_t)max_grid(c),           // local-round frontiers
+      nw * 4, nw * 4,                                           // the searches' unvisited sets
   };
   size_t total = 0;
   for (size_t s : sizes) total += align256(s);
@@ -173,6 +175,8 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
   c->lists[1] = (int32_t*)parts[27];
   c->lists[2] = (int32_t*)parts[28];
   c->loc = (int32_t*)parts[29];
+  c->s1 = (uint32_t*)parts[30];
+  c->s3 = (uint32_t*)parts[31];
   c->cap_n = nn;
   c->cap_e = ee;
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
@@ -582,9 +586,11 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   } else {
     Timed t(c, MSC_K_TRIM);
-    // the searches' bitmaps start with nothing visited: written by the trim as well;
+    // the searches' claimable bitmaps start as the valid set and their unvisited sets as all
+    // ones: written by the trim as well;
     // no labels yet (the commit writes every label in one pass)
-    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, nullptr, c->part);
+    np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, c->s1,
+                     cc ? nullptr : c->s3, nullptr, c->part);
     launch_reduce(L, c->part, np, c->ctrl->red_trim);
   }
   if (s) s->k[MSC_K_TRIM].bytes += 8.0 * n + 1.0 * n;
@@ -620,7 +626,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       fw.op = OP_FW;
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
-      fw.allow = c->valid; fw.cand = c->b1;
+      fw.allow = c->valid; fw.cand = c->b1; fw.seen = c->s1;
       // BW: top-down on in-lists, bottom-up on out-lists.  Searched together with FW (the
       // default) it searches the valid set with its own control block, frontiers and
       // bitmap, both advancing one level per k_reach launch, and the SCC is FW set & BW
@@ -632,13 +638,14 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bw.down_off = g.in_off; bw.down_adj = g.in_adj;
       bw.up_off = g.out_off; bw.up_adj = g.out_adj;
       if (c->tune.concurrent_fwbw) {
-        bw.allow = c->valid; bw.cand = c->b3;
+        bw.allow = c->valid; bw.cand = c->b3; bw.seen = c->s3;
         bw.ctrl = &cs[1]; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
-        if (!cand_ready) {  // nothing visited yet, for both
-          launch_fill_u32(L, c->b1, 0xffffffffu, nw);
-          launch_fill_u32(L, c->b3, 0xffffffffu, nw);
+        if (!cand_ready) {  // cand = allow, nothing seen, for both
+          launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);
+          launch_fill_u32(L, c->s1, 0xffffffffu, nw);
+          launch_fill_u32(L, c->s3, 0xffffffffu, nw);
         }
         if (bt) {
           launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
@@ -654,35 +661,42 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         // SCC = FW set & BW set = valid & ~b1 & ~b3
         member = MemberSrc{c->b1, c->b3, &cs[0], &cs[1]};
       } else {
-        if (!cand_ready) launch_fill_u32(L, c->b1, 0xffffffffu, nw);  // nothing visited yet
+        if (!cand_ready) {  // cand = allow, nothing seen
+          launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
+          launch_fill_u32(L, c->s1, 0xffffffffu, nw);
+        }
         if (bt) launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
         else launch_search_init(L, fw, pivot, 0);
         if ((rc = run_levels(c, OP_FW, false, fw, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
         if (s) s->fw_levels = (int32_t)hs[0].level;
         if (!hs[0].abandoned) {
-          // BW: allow = fw set (b2 = valid & ~b1), nothing visited (b3), control block [0] again
-          launch_andnot(L, c->b2, nullptr, c->valid, c->b1, nw);
-          launch_fill_u32(L, c->b3, 0xffffffffu, nw);
-          bw.allow = c->b2; bw.cand = c->b3;
+          // BW: allow = fw set (b2 = valid & ~b1), cand = b3 (a copy), nothing seen (s3),
+          // control block [0] again
+          launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
+          launch_fill_u32(L, c->s3, 0xffffffffu, nw);
+          bw.allow = c->b2; bw.cand = c->b3; bw.seen = c->s3;
           if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
           else launch_search_init(L, bw, pivot, 1);
           sa[0] = bw;
           nsearch = 1;
           sop = OP_BW;
           s0 = &cs[0];
-          // SCC = fw set & visited by BW = (valid & ~b1) & ~b3 (BW claims only fw-set vertices)
+          // SCC = fw set & visited by BW = (valid & ~b1) & ~b3, as b3 started as the fw set
           member = MemberSrc{c->b1, c->b3, &cs[0], nullptr};
         }
       }
     } else {
       LevelArgs bf = a;
       bf.op = OP_CC;
-      if (!cand_ready) launch_fill_u32(L, c->b1, 0xffffffffu, nw);  // nothing visited yet
+      if (!cand_ready) {  // cand = allow, nothing seen
+        launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
+        launch_fill_u32(L, c->s1, 0xffffffffu, nw);
+      }
       bf.down_off = g.out_off; bf.down_adj = g.out_adj;
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
-      bf.allow = c->valid; bf.cand = c->b1;
+      bf.allow = c->valid; bf.cand = c->b1; bf.seen = c->s1;
       if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
       else launch_search_init(L, bf, pivot, 2);
       sa[0] = bf;
