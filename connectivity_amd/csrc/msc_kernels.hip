@@ -437,27 +437,28 @@ __global__ void __launch_bounds__(kBlock) k_trim(int32_t n, const uint32_t* __re
       od[r] = oo[v + 1] - oo[v];
       id[r] = io[v + 1] - io[v];
     }
+    // the kU ballots are the wave's 2 * kU consecutive bitmap words: lane j < 2 * kU takes
+    // word j, so each bitmap gets one contiguous store per wave and iteration
+    uint32_t mine = 0u;
 #pragma unroll
     for (int r = 0; r < kU; ++r) {
-      const int64_t base = b0 + 64 * r;
-      const int64_t v = base + lane();
+      const int64_t v = b0 + 64 * r + lane();
       bool keep = false;
       if (v < n) {
         keep = MIRROR ? (od[r] + id[r]) > 0u : (od[r] > 0u && id[r] > 0u);
         if (labels) labels[v] = keep ? -1 : (int32_t)v;
       }
-      unsigned long long mask = __ballot(keep);
-      int64_t word = base >> 5;
-      if ((lane() & 31) == 0 && word + (lane() >> 5) < nw) {
-        const int64_t wi = word + (lane() >> 5);
-        const uint32_t bits = (uint32_t)(mask >> (lane() & 32));
-        valid[wi] = bits;
-        if (cand1) cand1[wi] = bits;
-        if (cand2) cand2[wi] = bits;
-        if (seen1) seen1[wi] = 0xffffffffu;
-        if (seen2) seen2[wi] = 0xffffffffu;
-      }
+      const unsigned long long mask = __ballot(keep);
+      if ((lane() >> 1) == r) mine = (uint32_t)(mask >> (32 * (lane() & 1)));
       kept += keep ? 1 : 0;
+    }
+    const int64_t wi = (b0 >> 5) + lane();
+    if (lane() < 2 * kU && wi < nw) {
+      valid[wi] = mine;
+      if (cand1) cand1[wi] = mine;
+      if (cand2) cand2[wi] = mine;
+      if (seen1) seen1[wi] = 0xffffffffu;
+      if (seen2) seen2[wi] = 0xffffffffu;
     }
   }
   kept = block_sum64(kept, s_red);
