@@ -2141,7 +2141,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_pivots(LevelArgs a, const int32
 // each word writes the word's new valid bits (no other writer).  The label is `label`, or
 // the smallest member when min_red is given (0xffffffff - min_red[3], canonical labels), or
 // in a batch the pivot of the vertex's own graph.  With a.rc, only in an admitted round.
-template <bool BATCH>
+template <bool BATCH, bool VEC = false>
 __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __restrict__ valid, int32_t n,
                                                    int32_t* __restrict__ labels, int32_t label,
                                                    const unsigned long long* __restrict__ min_red,
@@ -2156,7 +2156,40 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
   // kU slices of kBlock consecutive vertices per iteration, with all their loads issued
   // before any store (one memory latency per iteration, not per slice)
   constexpr int kU = 4;
-  for (int64_t wg = (int64_t)blockIdx.x * kBlock * kU; wg < n; wg += (int64_t)gridDim.x * kBlock * kU) {
+  if (VEC) {
+    // four consecutive vertices per thread: one 16-byte label store (labels 16-byte aligned)
+    constexpr int64_t kStep = (int64_t)kBlock * 4 * kU;
+    for (int64_t wg = (int64_t)blockIdx.x * kStep; wg < n; wg += (int64_t)gridDim.x * kStep) {
+      uint32_t vw[kU], w[kU];
+#pragma unroll
+      for (int r = 0; r < kU; ++r) {
+        const int64_t wi = (wg + 4 * (r * kBlock + threadIdx.x)) >> 5;  // slack words past n
+        vw[r] = valid[wi];
+        w[r] = member_word(m, vw[r], wi, keep);
+      }
+#pragma unroll
+      for (int r = 0; r < kU; ++r) {
+        const int64_t v0 = wg + 4 * (r * kBlock + threadIdx.x);
+        const uint32_t sh = (uint32_t)(v0 & 31), bits = (w[r] >> sh) & 0xfu;
+        if (v0 + 3 < n) {
+          int4 o;
+          o.x = (bits & 1u) ? label : (int32_t)v0;
+          o.y = (bits & 2u) ? label : (int32_t)(v0 + 1);
+          o.z = (bits & 4u) ? label : (int32_t)(v0 + 2);
+          o.w = (bits & 8u) ? label : (int32_t)(v0 + 3);
+          *reinterpret_cast<int4*>(labels + v0) = o;
+        } else {
+          for (int k = 0; k < 4; ++k)
+            if (v0 + k < n) labels[v0 + k] = ((bits >> k) & 1u) ? label : (int32_t)(v0 + k);
+        }
+        if (sh == 0 && v0 < n && w[r]) {  // the word's first thread writes its valid bits
+          valid[v0 >> 5] = vw[r] & ~w[r];
+          cnt += (uint32_t)__popc(w[r]);
+        }
+      }
+    }
+  }
+  for (int64_t wg = (int64_t)blockIdx.x * kBlock * kU; !VEC && wg < n; wg += (int64_t)gridDim.x * kBlock * kU) {
     uint32_t vw[kU], w[kU];
 #pragma unroll
     for (int r = 0; r < kU; ++r) {
@@ -2327,12 +2360,19 @@ MemberSrc member_src(const MemberSrc& m, const uint32_t* valid) {
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
                   int32_t* list) {
-  int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
   // the seed runs as its own sparse sweep (k_seed_bits): fused into this pass it made
   // every iteration wait on the stage (s24: 51 us fused, 22 + 12 us apart)
-  hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
-                     label, min_red, nullptr, nullptr, 0u, part, a.rc);
+  int g;
+  if (((uintptr_t)labels & 15u) == 0) {
+    g = grid_for(n, kBlock * 64, L.grid * 2);
+    hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
+                       labels, label, min_red, nullptr, nullptr, 0u, part, a.rc);
+  } else {
+    g = grid_for(n, kBlock * 16, L.grid * 2);
+    hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
+                       label, min_red, nullptr, nullptr, 0u, part, a.rc);
+  }
   if (seed)
     hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);
