@@ -221,6 +221,9 @@ int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid
                       unsigned long long* part, const RoundCtl* rc);
 // once rc->finished: labels[v] = cmin[labels[v]] for the rc->listed0 vertices of `list`
 // (bound >= that count sizes the grid)
+// CC: the smallest member of every colouring component into cmin (list: colouring round 0's)
+void launch_cc_cmin(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, const int32_t* labels,
+                    int32_t* cmin);
 void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, int32_t* labels,
                        const int32_t* cmin);
 // the first kernel of a colouring round (k_round_begin); s0 / s1: the searches' control

@@ -519,6 +519,44 @@ __global__ void __launch_bounds__(kBlock) k_canon_list(const int32_t* __restrict
   }
 }
 
+// CC's canonical labels of the colouring phase, once it has ended (its components carry
+// their largest member as label, every listed vertex one of them): pass 1 starts each
+// component's minimum at its root, pass 2 lowers it to the smallest listed member (lanes
+// of a wave sharing a label combine into one atomic); k_canon_list then applies it.
+__global__ void __launch_bounds__(kBlock) k_cc_cmin_init(const int32_t* __restrict__ list, const RoundCtl* rc,
+                                                         const int32_t* __restrict__ labels, int32_t* __restrict__ cmin) {
+  if (!rc->finished) return;  // uniform
+  const uint32_t count = rc->listed0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+    const int32_t v = list[i];
+    if (labels[v] == v) cmin[v] = v;
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_cc_cmin(const int32_t* __restrict__ list, const RoundCtl* rc,
+                                                    const int32_t* __restrict__ labels, int32_t* __restrict__ cmin) {
+  if (!rc->finished) return;  // uniform
+  const uint32_t count = rc->listed0;
+  for (uint32_t i0 = blockIdx.x * kBlock; i0 < count; i0 += gridDim.x * kBlock) {
+    const uint32_t i = i0 + threadIdx.x;
+    int32_t v = 0, l = -1;
+    if (i < count) {
+      v = list[i];
+      l = labels[v];
+    }
+    bool active = l > v;
+    for (;;) {
+      const unsigned long long mk = __ballot(active);
+      if (mk == 0ull) break;
+      const int leader = __ffsll((long long)mk) - 1;
+      const int32_t key = __shfl(l, leader, 64);
+      const bool mine = active && l == key;
+      const uint32_t x = wave_min(mine ? (uint32_t)v : 0xffffffffu);
+      if (lane() == leader) atomicMin(&cmin[key], (int32_t)x);
+      active = active && !mine;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_count_bits(const uint32_t* __restrict__ bits, int64_t nwords,
                                                        unsigned long long* __restrict__ part) {
   __shared__ unsigned long long s_red[kBlock / 64];
@@ -611,6 +649,7 @@ __global__ void k_round_begin(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const C
     }
     rc->rounds = 1;
     rc->push_levels = push->level;
+    rc->listed0 = push->list_len;
     rc->finished = 1u;
     return;
   } else {
@@ -2398,6 +2437,14 @@ int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid
   hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part,
                      rc);
   return g;
+}
+
+void launch_cc_cmin(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, const int32_t* labels,
+                    int32_t* cmin) {
+  if (bound <= 0) return;
+  const dim3 g(grid_for(bound, kBlock, L.grid * 8));
+  hipLaunchKernelGGL(k_cc_cmin_init, g, dim3(kBlock), 0, L.stream, list, rc, labels, cmin);
+  hipLaunchKernelGGL(k_cc_cmin, g, dim3(kBlock), 0, L.stream, list, rc, labels, cmin);
 }
 
 void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, int32_t* labels,

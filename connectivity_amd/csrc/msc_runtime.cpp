@@ -597,10 +597,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
 
   LevelArgs a = base_args(c, &g, labels, o, avg);
   a.nvalid = &c->ctrl->red_trim[0];
-  // canonical SCC labels of one graph are written as the phases go: the pivot's SCC takes
-  // its smallest member, colour SCCs track theirs in cmin; the batch and WCC paths
-  // relabel at the end instead
-  const bool fast_canon = !cc && !bt && o->label_mode == MSC_LABELS_CANONICAL;
+  // canonical labels of one graph are written as the phases go: the pivot's component
+  // takes its smallest member, colour SCCs track theirs in cmin (CC finds them after its
+  // fixpoint, over the colouring's vertex list); the batch paths relabel at the end instead
+  const bool fast_canon = !bt && o->label_mode == MSC_LABELS_CANONICAL;
   // control blocks: [0] forward (or CC) search, [1] backward search, [2] colour push,
   // [3] colour-BW; rcd: the colouring rounds' device state
   Ctrl* cs = c->ctrl;
@@ -730,7 +730,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.pull_div = (uint32_t)c->tune.pull_div;
     p.pull_min = floor_edges > 0xffffffffll ? 0xffffffffu : (uint32_t)floor_edges;
   }
-  if (fast_canon) p.cmin = c->cmin;
+  if (fast_canon && !cc) p.cmin = c->cmin;  // tracked by the colour-BW searches
   if (c->tune.local_rounds > 0) {
     p.loc = c->loc;
     p.local_rounds = c->tune.local_rounds;
@@ -792,6 +792,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     launch_round_begin(L, rcd, p.ctrl, r.ctrl, nullptr, nullptr, false, cc);
     {
       Timed t(c, MSC_K_OTHER);
+      if (fast_canon && cc) launch_cc_cmin(L, list_of(0), rcd, n, labels, c->cmin);
       if (fast_canon) launch_canon_list(L, list_of(0), rcd, n, labels, c->cmin);
       if (o->label_mode == MSC_LABELS_CANONICAL && !fast_canon) launch_canon(L, n, labels, c->colors, done_gate);
       if (bt) launch_localise(L, n, labels, c->bbase, ng, done_gate);
