@@ -73,7 +73,7 @@ class _Opts(C.Structure):
                 ("policy", C.c_int32), ("label_mode", C.c_int32), ("labels_where", C.c_int32),
                 ("device", C.c_int32), ("validate", C.c_int32), ("timing", C.c_int32),
                 ("record", C.c_int32), ("ctx", C.c_void_p), ("schedule", C.c_void_p),
-                ("schedule_len", C.c_int32), ("pad_", C.c_int32)]
+                ("schedule_len", C.c_int32), ("fwbw_rounds", C.c_int32)]
 
 
 class _KStat(C.Structure):
@@ -522,7 +522,7 @@ def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alph
 
 
 def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
-         out, schedule, batch=None, trim_level=1):
+         out, schedule, batch=None, trim_level=1, fwbw_rounds=1):
     n = g.n
     where = HOST
     if isinstance(out, DeviceLabels):
@@ -541,6 +541,7 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
         schedule = np.ascontiguousarray(schedule, dtype=np.int8)
     o = _opts(policy, labels, timing, record, validate, ctx, labels_where=where, alpha=alpha, beta=beta,
               schedule=schedule, trim_level=trim_level)
+    o.fwbw_rounds = int(fwbw_rounds)
     st = _Stats()
     csr = g.csr()
     piv = -1 if max_deg_vert is None else int(max_deg_vert)
@@ -593,15 +594,17 @@ def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None,
             policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
             record: bool = False, validate: bool = False, ctx: Context | None = None,
             alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False,
-            schedule: np.ndarray | None = None, trim_level: int = 1):
+            schedule: np.ndarray | None = None, trim_level: int = 1, fwbw_rounds: int = 1):
     """SCC labels of ``g`` (a Graph or DeviceGraph) on the GPU.
 
     run_scc (multistep/scc_main.cpp:210).  ``max_deg_vert`` / ``avg_degree`` default to
     create_csr's rule computed on device.  Returns labels, or (labels, Stats) with
     ``stats=True``.  ``schedule`` (see :func:`schedule_array`) forces per-level decisions.
-    ``trim_level`` 2 is the reference's complete trim (TRIM_LEVEL 2, scc_trim.cpp:141-297)."""
+    ``trim_level`` 2 is the reference's complete trim (TRIM_LEVEL 2, scc_trim.cpp:141-297).
+    ``fwbw_rounds`` > 1 repeats FW-BW from new pivots while the SCC found is large
+    (msc_opts.fwbw_rounds); the labels do not change."""
     lab, st = _run("msc_scc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
-                   alpha, beta, out, schedule, trim_level=trim_level)
+                   alpha, beta, out, schedule, trim_level=trim_level, fwbw_rounds=fwbw_rounds)
     return (lab, st) if stats else lab
 
 

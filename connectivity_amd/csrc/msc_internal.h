@@ -207,23 +207,26 @@ struct MemberSrc {
   const Ctrl* g1;
 };
 // labels[v] = label and valid &= ~member for every member, labels[v] = v for every other
-// vertex; part col 0 = count.  With min_red, the label is the smallest member found by
-// launch_min_member (0xffffffff - min_red[3]).  With `seed` (the colouring's arguments),
+// vertex (with members_only: other labels untouched); part col 0 = count.  With min_red,
+// the label is the smallest member found by launch_min_member (0xffffffff - min_red[3]);
+// with max_red the largest (max_red[3] - 1).  With `seed` (the colouring's arguments),
 // every vertex still valid afterwards is then seeded for colouring round 0 (colors[v] = v,
 // queued in frontier 1 of seed->ctrl, written to `list`), and launch_seed_end closes the
 // seed.  With seed->rc, both run only in an admitted round (after launch_round_begin).
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
-                  int32_t* list);
-// part col 3 = 0xffffffff - smallest member in each workgroup's words (0: none); with rc,
-// only in an admitted round
+                  int32_t* list, const unsigned long long* max_red = nullptr, bool members_only = false);
+// colouring round 0's seed alone (k_seed_bits), after commits made without one
+void launch_seed_bits(const Launch& L, const LevelArgs& a, int32_t n, int32_t* list);
+// part col 3 = 0xffffffff - smallest member in each workgroup's words (0: none), or with
+// `largest` 1 + largest member; with rc, only in an admitted round
 int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
-                      unsigned long long* part, const RoundCtl* rc);
-// once rc->finished: labels[v] = cmin[labels[v]] for the rc->listed0 vertices of `list`
-// (bound >= that count sizes the grid)
+                      unsigned long long* part, const RoundCtl* rc, bool largest = false);
 // CC: the smallest member of every colouring component into cmin (list: colouring round 0's)
 void launch_cc_cmin(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, const int32_t* labels,
                     int32_t* cmin);
+// once rc->finished: labels[v] = cmin[labels[v]] for the rc->listed0 vertices of `list`
+// (bound >= that count sizes the grid)
 void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, int32_t* labels,
                        const int32_t* cmin);
 // the first kernel of a colouring round (k_round_begin); s0 / s1: the searches' control
@@ -247,11 +250,12 @@ void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_
                    int64_t words);
 // canonical labels (minimum member per component); gate: only once *gate != 0 (null: always)
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate);
-// pivot: part col 3 = max key; results via launch_reduce then launch_pivot_first
+// pivot: part col 3 = max key; results via launch_reduce then launch_pivot_first; with a
+// mask, among the vertices it holds only
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
-                     unsigned long long* part);
+                     unsigned long long* part, const uint32_t* mask = nullptr);
 void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
-                        unsigned long long* red);
+                        unsigned long long* red, const uint32_t* mask = nullptr);
 int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
                     unsigned long long* part);
 void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err);

@@ -481,9 +481,10 @@ __device__ __forceinline__ uint32_t member_keep(const MemberSrc& m) {
 
 // part col 3 = 0xffffffff - smallest member in the workgroup's words (0 when none), so
 // that k_reduce's max yields the smallest member overall.
+// With `largest`, col 3 = 1 + largest member instead (0 when none).
 __global__ void __launch_bounds__(kBlock) k_min_member(MemberSrc m, const uint32_t* __restrict__ valid,
                                                        int64_t nwords, unsigned long long* __restrict__ part,
-                                                       const RoundCtl* rc) {
+                                                       const RoundCtl* rc, int32_t largest) {
   __shared__ unsigned long long s_red[kBlock / 64];
   if (rc && !rc->go) return;  // uniform: the searches have not finished
   const uint32_t keep = member_keep(m);
@@ -491,7 +492,8 @@ __global__ void __launch_bounds__(kBlock) k_min_member(MemberSrc m, const uint32
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * kBlock) {
     uint32_t w = member_word(m, valid[i], i, keep);
     if (w) {
-      unsigned long long key = 0xffffffffull - (unsigned long long)(i * 32 + (__ffs(w) - 1));
+      unsigned long long key = largest ? 1ull + (unsigned long long)(i * 32 + (31 - __clz(w)))
+                                       : 0xffffffffull - (unsigned long long)(i * 32 + (__ffs(w) - 1));
       best = key > best ? key : best;
     }
   }
@@ -1855,10 +1857,12 @@ __global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int
 // (non-negative doubles order like their bits), reduced by k_reduce; pass 2: smallest
 // vertex attaining it.
 __global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t* __restrict__ oo,
-                                                      const uint32_t* __restrict__ io, unsigned long long* part) {
+                                                      const uint32_t* __restrict__ io, unsigned long long* part,
+                                                      const uint32_t* __restrict__ mask) {
   __shared__ unsigned long long s_red[kBlock / 64];
   unsigned long long best = 0;
   for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+    if (mask && !bit_test(mask, (uint32_t)v)) continue;
     double p = (double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]);
     unsigned long long b = (unsigned long long)__double_as_longlong(p);
     best = b > best ? b : best;
@@ -1873,12 +1877,14 @@ __global__ void __launch_bounds__(kBlock) k_pivot_max(int32_t n, const uint32_t*
     p[3] = best;
   }
 }
-// red[3] holds the max; red[1] receives the first vertex attaining it (~0 when none).
+// red[3] holds the max; red[1] receives the first vertex attaining it (~0 when none).  With
+// a mask, only the vertices it holds are candidates (both passes).
 __global__ void k_pivot_first(int32_t n, const uint32_t* __restrict__ oo, const uint32_t* __restrict__ io,
-                              unsigned long long* red) {
+                              unsigned long long* red, const uint32_t* __restrict__ mask) {
   const unsigned long long target = red[3];
   if (target == 0ull) return;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    if (mask && !bit_test(mask, (uint32_t)v)) continue;
     double p = (double)(oo[v + 1] - oo[v]) * (double)(io[v + 1] - io[v]);
     if ((unsigned long long)__double_as_longlong(p) == target) atomicMin(&red[1], (unsigned long long)v);
   }
@@ -2186,11 +2192,13 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
                                                    const unsigned long long* __restrict__ min_red,
                                                    const int32_t* __restrict__ piv, const int64_t* __restrict__ base,
                                                    uint32_t ng, unsigned long long* __restrict__ part,
-                                                   const RoundCtl* rc) {
+                                                   const RoundCtl* rc, const unsigned long long* __restrict__ max_red,
+                                                   int32_t members_only) {
   __shared__ unsigned long long s_red[kBlock / 64];
   if (rc && !rc->go) return;  // uniform: the searches have not finished
   const uint32_t keep = member_keep(m);
   if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
+  if (max_red) label = (int32_t)(max_red[3] - 1ull);
   unsigned long long cnt = 0;
   // kU slices of kBlock consecutive vertices per iteration, with all their loads issued
   // before any store (one memory latency per iteration, not per slice)
@@ -2245,7 +2253,7 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
       if (BATCH) {
         const uint32_t g = wave_graph_of(base, ng, wb < n ? wb : n - 1, n);
         if (v < n) labels[v] = in ? piv[g] : (int32_t)v;
-      } else if (v < n) {
+      } else if (v < n && (in || !members_only)) {
         labels[v] = in ? label : (int32_t)v;
       }
       if ((lane() & 31) == 0 && v < n && w[r]) {
@@ -2398,24 +2406,29 @@ MemberSrc member_src(const MemberSrc& m, const uint32_t* valid) {
 
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
-                  int32_t* list) {
+                  int32_t* list, const unsigned long long* max_red, bool members_only) {
   LevelArgs a = seed ? *seed : LevelArgs{};
   // the seed runs as its own sparse sweep (k_seed_bits): fused into this pass it made
   // every iteration wait on the stage (s24: 51 us fused, 22 + 12 us apart)
   int g;
-  if (((uintptr_t)labels & 15u) == 0) {
+  if (((uintptr_t)labels & 15u) == 0 && !members_only) {
     g = grid_for(n, kBlock * 64, L.grid * 2);
     hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
-                       labels, label, min_red, nullptr, nullptr, 0u, part, a.rc);
+                       labels, label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, 0);
   } else {
     g = grid_for(n, kBlock * 16, L.grid * 2);
     hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
-                       label, min_red, nullptr, nullptr, 0u, part, a.rc);
+                       label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, members_only ? 1 : 0);
   }
   if (seed)
     hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);
   return g;
+}
+
+void launch_seed_bits(const Launch& L, const LevelArgs& a, int32_t n, int32_t* list) {
+  hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
+                     L.stream, a, list);
 }
 
 int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
@@ -2424,7 +2437,7 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
   int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
   hipLaunchKernelGGL(k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels, 0,
-                     nullptr, piv, base, ng, part, a.rc);
+                     nullptr, piv, base, ng, part, a.rc, nullptr, 0);
   if (seed)
     hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);
@@ -2432,10 +2445,10 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
 }
 
 int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
-                      unsigned long long* part, const RoundCtl* rc) {
+                      unsigned long long* part, const RoundCtl* rc, bool largest) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
   hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part,
-                     rc);
+                     rc, largest ? 1 : 0);
   return g;
 }
 
@@ -2500,15 +2513,17 @@ void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch,
   hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
 }
 
-int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part) {
+int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part,
+                     const uint32_t* mask) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
-  hipLaunchKernelGGL(k_pivot_max, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part);
+  hipLaunchKernelGGL(k_pivot_max, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part, mask);
   return g;
 }
 
-void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* red) {
+void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* red,
+                        const uint32_t* mask) {
   hipLaunchKernelGGL(k_pivot_first, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, oo, io,
-                     red);
+                     red, mask);
 }
 
 int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part) {

@@ -709,6 +709,64 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     if ((rc = enqueue_levels(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, c->tune.search_batch))) return rc;
   }
 
+  // More FW-BW rounds before colouring (msc_opts.fwbw_rounds; the reference runs one): the
+  // searches are finished on the host's watch, the first SCC is committed as usual (every
+  // label written), and while the last SCC held at least 1/16 of the vertices left before
+  // it, a new pivot (create_csr's rule among the vertices left) starts another pair of
+  // searches whose SCC is committed labelled with its largest member (the label colouring
+  // gives it; its smallest with canonical labels), writing the members' labels only.
+  // Colouring round 0 then only seeds.
+  bool committed = false;
+  int64_t fwbw_total = -1;
+  if (o->fwbw_rounds > 1 && !cc && !bt && nsearch == 2) {
+    if ((rc = run_loop(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
+    {
+      Timed t(c, MSC_K_COMMIT);
+      if (fast_canon)
+        launch_reduce(L, c->part, launch_min_member(L, member, c->valid, nw, c->part, nullptr), c->ctrl->red_min);
+      const int npc = launch_commit(L, member, c->valid, n, labels, pivot, fast_canon ? c->ctrl->red_min : nullptr,
+                                    c->part, nullptr, nullptr);
+      launch_reduce(L, c->part, npc, c->ctrl->red_commit);
+    }
+    if ((rc = read_ctrl(c))) return rc;
+    if (s) s->host_syncs += 1;
+    int64_t before = (int64_t)hs[0].red_trim[0], found = (int64_t)hs[0].red_commit[0];
+    fwbw_total = found;
+    for (int round = 1; round < o->fwbw_rounds && found * 16 >= before && found < before; ++round) {
+      before -= found;
+      int32_t p2 = -1;
+      {
+        Timed t(c, MSC_K_OTHER);
+        const int np = launch_pivot_scc(L, n, g.out_off, g.in_off, c->part, c->valid);
+        launch_reduce(L, c->part, np, c->ctrl->reduced);
+        launch_fill_u32(L, (uint32_t*)&c->ctrl->reduced[1], 0xffffffffu, 2);
+        launch_pivot_first(L, n, g.out_off, g.in_off, c->ctrl->reduced, c->valid);
+      }
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->host_syncs += 1;
+      if (hs[0].reduced[3] == 0ull || hs[0].reduced[1] == ~0ull) break;  // no vertex with in- and out-edges
+      p2 = (int32_t)hs[0].reduced[1];
+      launch_andnot(L, c->b1, c->b3, c->valid, nullptr, nw);  // claimable = valid, nothing seen
+      launch_fill_u32(L, c->s1, 0xffffffffu, nw);
+      launch_fill_u32(L, c->s3, 0xffffffffu, nw);
+      launch_search_init_pair(L, sa[0], sa[1], p2);
+      if ((rc = run_loop(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
+      {
+        Timed t(c, MSC_K_COMMIT);
+        launch_reduce(L, c->part, launch_min_member(L, member, c->valid, nw, c->part, nullptr, !fast_canon),
+                      c->ctrl->red_min);
+        const int npc = launch_commit(L, member, c->valid, n, labels, p2, fast_canon ? c->ctrl->red_min : nullptr,
+                                      c->part, nullptr, nullptr, fast_canon ? nullptr : c->ctrl->red_min, true);
+        launch_reduce(L, c->part, npc, c->ctrl->red_commit);
+      }
+      if ((rc = read_ctrl(c))) return rc;
+      if (s) s->host_syncs += 1;
+      found = (int64_t)hs[0].red_commit[0];
+      fwbw_total += found;
+    }
+    committed = true;
+  }
+
   // the colouring's arguments: push on control block [2], colour-BW on [3]
   LevelArgs p = a;
   p.op = OP_PUSH;
@@ -755,7 +813,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // colour-BW levels (scc_color.cpp:214-289)
   auto round_steps = [&](int rr) -> int {
     int e;
-    if (rr == 0) {
+    if (rr == 0 && committed) {
+      // the FW-BW rounds committed already: only the seed of colouring round 0
+      Timed t(c, MSC_K_COMMIT);
+      launch_seed_bits(L, p, n, list_of(0));
+      launch_seed_end(L, p.ctrl, rcd, p.pull_div ? p.pull_min : 0u);
+    } else if (rr == 0) {
       Timed t(c, MSC_K_COMMIT);
       const bool want_min = fast_canon && member.ca;
       if (want_min)
@@ -886,7 +949,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   if (s) {
     s->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     s->num_trim = n - (int32_t)c->ctrl_host->red_trim[0];
-    s->num_fwbw = (int32_t)c->ctrl_host->red_commit[0];
+    s->num_fwbw = fwbw_total >= 0 ? (int32_t)fwbw_total : (int32_t)c->ctrl_host->red_commit[0];
     // the commit reads valid and both search bitmaps, writes valid and every label
     s->k[MSC_K_COMMIT].bytes += 16.0 * nw + 4.0 * n;
     // every block was zeroed at the start: each holds this run's own levels only

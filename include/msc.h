@@ -87,7 +87,13 @@ typedef struct msc_opts {
    * bottom-up, 2 abandon the search, -1 the policy's own choice.  NULL: none. */
   const int8_t* schedule;
   int32_t schedule_len;
-  int32_t pad_;
+  /* FW-BW rounds before colouring (no counterpart in the reference, which runs one):
+   * while the last round's SCC held at least 1/16 of the vertices left before it, up to
+   * this many rounds each take a new pivot (create_csr's rule among the vertices left) and
+   * commit its SCC labelled with its largest member (its smallest with canonical labels),
+   * the label colouring would give it, so every label is unchanged.  0 or 1: one round.
+   * SCC of one graph with concurrent_fwbw only; ignored elsewhere. */
+  int32_t fwbw_rounds;
 } msc_opts;
 #define MSC_SCHED_LEVELS 64
 

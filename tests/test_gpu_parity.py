@@ -45,6 +45,31 @@ def test_reference_fixture(msc, orc, gpu_ctx):
     assert list(msc.run_cc(g, labels=msc.LABELS_RAW, ctx=gpu_ctx)) == [0, 0, 0]
 
 
+@pytest.mark.parametrize("rounds", [2, 4, 16])
+@pytest.mark.parametrize("kind", ["union", "planted", "rmat"])
+def test_fwbw_rounds(msc, orc, gpu_ctx, kind, rounds):
+    """msc_opts.fwbw_rounds: more FW-BW rounds before colouring leave every label as the
+    reference's single round gives it (raw and canonical); on a union of four R-MAT graphs
+    (four giant SCCs) the later rounds commit the other three."""
+    _say(f"case: fwbw_rounds {kind} rounds={rounds}")
+    if kind == "union":
+        g, _ = msc.pack_graphs([msc.gen_rmat(11, 8, seed=40 + i) for i in range(4)])
+    elif kind == "planted":
+        g = msc.gen_planted(1 << 13, max_size=1024, seed=5)[0]
+    else:
+        g = msc.gen_rmat(13, 8, seed=6)
+    piv, avg = orc.pivot_scc(g)
+    want = orc.scc(g, piv, avg, vert_cutoff=0)
+    _, one = msc.run_scc(g, piv, avg, labels=msc.LABELS_RAW, ctx=gpu_ctx, stats=True)
+    got, st = msc.run_scc(g, piv, avg, labels=msc.LABELS_RAW, ctx=gpu_ctx, stats=True, fwbw_rounds=rounds)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"raw labels differ at {bad.size} vertices, first {bad[:5]}"
+    assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx, fwbw_rounds=rounds), orc.canonical(want))
+    assert st.num_fwbw >= one.num_fwbw
+    if kind == "union":
+        assert st.num_fwbw > one.num_fwbw, "the later rounds should commit the other graphs' giant SCCs"
+
+
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("n,avgdeg", [(50, 1.5), (300, 2.5), (2000, 4.0)])
 def test_random_digraphs(msc, orc, gpu_ctx, n, avgdeg, seed):
