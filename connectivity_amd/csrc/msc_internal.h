@@ -231,9 +231,10 @@ void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc,
                        const int32_t* cmin);
 // the first kernel of a colouring round (k_round_begin); s0 / s1: the searches' control
 // blocks that round 0 (`first`) waits for (null: none); `single`: CC's one fixpoint, whose
-// closing check ends the colouring
+// closing check ends the colouring; with `prog` (pinned host memory) the outcome is also
+// published there, tagged with `epoch`
 void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
-                        bool first, bool single);
+                        bool first, bool single, unsigned long long* prog = nullptr, uint32_t epoch = 0);
 // host-directed round state: 0 fresh, 1 resume the admitted round, 2 clear a stall
 void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode);
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);

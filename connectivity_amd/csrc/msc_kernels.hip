@@ -632,10 +632,11 @@ __device__ __forceinline__ void ctrl_fresh(Ctrl* c, int32_t phase) {
 // its roots ran; it books that round (multistep/scc_color.cpp:340-373 loops while vertices
 // are left) and ends the colouring when none is.  An admitted round gets a fresh push
 // control block; a round that cannot be admitted sets rc->stall, which every later step
-// honours until the host clears it.
-__global__ void k_round_begin(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
-                              int32_t flags) {
-  if (threadIdx.x != 0) return;
+// honours until the host clears it.  With `prog` (host memory) the outcome is published
+// there for the host that enqueues rounds ahead: epoch << 48 | (stall | finished << 1 |
+// error << 2) << 32 | rounds admitted.
+__device__ void round_decide(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                             int32_t flags) {
   rc->go = 0u;
   if (rc->stall || rc->finished || rc->error) return;
   if (flags & 1) {
@@ -683,6 +684,17 @@ __global__ void k_round_begin(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const C
   ctrl_fresh(push, 3);
   rc->admitted += 1u;
   rc->go = 1u;
+}
+
+__global__ void k_round_begin(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                              int32_t flags, unsigned long long* prog, uint32_t epoch) {
+  if (threadIdx.x != 0) return;
+  round_decide(rc, push, cbw, s0, s1, flags);
+  if (prog) {
+    const unsigned long long st = (rc->stall ? 1ull : 0ull) | (rc->finished ? 2ull : 0ull) | (rc->error ? 4ull : 0ull);
+    __hip_atomic_store(prog, ((unsigned long long)(epoch & 0xffffu) << 48) | (st << 32) | rc->admitted,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // Before the colour roots of the admitted round (one thread): the push fixpoint must have
@@ -2468,9 +2480,9 @@ void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc,
 }
 
 void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
-                        bool first, bool single) {
+                        bool first, bool single, unsigned long long* prog, uint32_t epoch) {
   hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, s0, s1,
-                     (first ? 1 : 0) | (single ? 2 : 0));
+                     (first ? 1 : 0) | (single ? 2 : 0), prog, epoch);
 }
 
 void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode) {

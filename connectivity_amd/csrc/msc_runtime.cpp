@@ -81,6 +81,11 @@ struct msc_ctx {
   Ctrl* ctrl_host2 = nullptr;  // ctrl_host + 1
   RoundCtl* rctl_host = nullptr;
   uint32_t* pinned_u32 = nullptr;
+  // colouring progress published by k_round_begin (coherent host memory; prog_dev is its
+  // device address) and the tag of the rounds last enqueued
+  unsigned long long* prog_host = nullptr;
+  unsigned long long* prog_dev = nullptr;
+  uint32_t epoch = 0;
   std::vector<Ev> events;
   size_t ev_used = 0;
   bool timing = false;
@@ -227,6 +232,25 @@ int read_ctrl(msc_ctx* c) {
   HIP_TRY(hipMemcpyAsync(c->ctrl_host, c->ctrl, kCtrlBytes, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MSC_OK;
+}
+
+// Colouring rounds the host keeps enqueued ahead of the device (enqueue_rounds).
+constexpr int kRoundLead = 3;
+
+// Waits until k_round_begin has published, for the rounds tagged `epoch`, at least `need`
+// admitted rounds (true) or a stalled / finished / failed colouring (false).  False too
+// after 2 s without either: the caller then stops enqueueing and reads the control blocks.
+bool device_admitted(const msc_ctx* c, uint32_t epoch, uint64_t need) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spin = 1;; ++spin) {
+    const unsigned long long v = __atomic_load_n(c->prog_host, __ATOMIC_ACQUIRE);
+    if ((uint32_t)(v >> 48) == (epoch & 0xffffu)) {
+      if ((v >> 32) & 7ull) return false;
+      if ((v & 0xffffffffull) >= need) return true;
+    }
+    if ((spin & 1023u) == 0u && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) return false;
+    __builtin_ia32_pause();
+  }
 }
 
 // Reduces `nparts` partial rows into ctrl->reduced and reads the result back.
@@ -845,11 +869,19 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   };
   const uint32_t* done_gate = &rcd->finished;
   // rounds [from, from + count), the check that books the last of them, and the final
-  // relabelling, which runs only once the colouring has ended
+  // relabelling, which runs only once the colouring has ended.  Past the first kRoundLead
+  // rounds, round rr is enqueued only once the device has admitted round rr - kRoundLead,
+  // and none once it has published that the colouring ended or stalled: the rounds left
+  // would find nothing to do.
   auto enqueue_rounds = [&](int from, int count) -> int {
     int e;
+    if ((++c->epoch & 0xffffu) == 0u) ++c->epoch;
+    const uint32_t ep = c->epoch;
     for (int rr = from; rr < from + count; ++rr) {
-      launch_round_begin(L, rcd, p.ctrl, r.ctrl, rr == 0 ? s0 : nullptr, rr == 0 ? s1 : nullptr, rr == 0, false);
+      if (rr >= from + kRoundLead && !device_admitted(c, ep, (uint64_t)(rr - kRoundLead) + 1u)) break;
+      // only the rounds the host may wait for publish (a store to host memory per round)
+      launch_round_begin(L, rcd, p.ctrl, r.ctrl, rr == 0 ? s0 : nullptr, rr == 0 ? s1 : nullptr, rr == 0, false,
+                         rr + kRoundLead < from + count ? c->prog_dev : nullptr, ep);
       if ((e = round_steps(rr))) return e;
     }
     launch_round_begin(L, rcd, p.ctrl, r.ctrl, nullptr, nullptr, false, cc);
@@ -1049,6 +1081,8 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
   if (hipHostMalloc((void**)&c->ctrl_host, kCtrlBytes + 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->pinned_u32, 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->sched_host, 3 * MSC_SCHED_LEVELS, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->prog_host, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->prog_dev, c->prog_host, 0) != hipSuccess ||
       hipMalloc(&c->small, align256((size_t)kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
     (void)hipGetLastError();
     msc_ctx_destroy(c);
@@ -1056,6 +1090,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
   }
   c->ctrl_host2 = c->ctrl_host + 1;
   c->rctl_host = (RoundCtl*)(c->ctrl_host + kCtrls);
+  *c->prog_host = 0ull;
   c->mlp = (float*)c->small;
   c->sched = (int8_t*)c->small + align256((size_t)kMlpBlob * 4);
   int khz = 0;
@@ -1080,6 +1115,7 @@ int msc_ctx_destroy(msc_ctx* c) {
   if (c->ctrl_host) (void)hipHostFree(c->ctrl_host);
   if (c->pinned_u32) (void)hipHostFree(c->pinned_u32);
   if (c->sched_host) (void)hipHostFree(c->sched_host);
+  if (c->prog_host) (void)hipHostFree(c->prog_host);
   if (c->small) (void)hipFree(c->small);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   {
