@@ -1034,10 +1034,20 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
 // valid bit is cleared once) and colours no longer change, so their expansion is the
 // global one moved earlier.  Every thread calls; (vertices, edges) expanded here are added
 // to loc_n / loc_e by thread 0.
+// A local frontier of at most kLocalLds claims and one chunk of edges is kept in LDS (a
+// chain of single-vertex hops then reads no frontier from memory); larger ones go to the
+// workgroup's part of a.loc.
+constexpr uint32_t kLocalLds = 256;
+struct LocalLds {
+  int32_t q[kLocalLds];
+  uint32_t qo[kLocalLds];
+  uint32_t starts[4];
+};
+
 template <int OP, bool MIRROR>
 __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* inq, Stage& S,
-                             ChunkLds<OP, MIRROR>& L, unsigned long long& raised, unsigned long long& loc_n,
-                             unsigned long long& loc_e, unsigned long long t_begin) {
+                             ChunkLds<OP, MIRROR>& L, LocalLds& Q, unsigned long long& raised,
+                             unsigned long long& loc_n, unsigned long long& loc_e, unsigned long long t_begin) {
   int32_t* mine = a.loc + (size_t)blockIdx.x * kLocalStride;
   for (int lr = 0; lr < a.local_rounds; ++lr) {
     __syncthreads();
@@ -1050,7 +1060,9 @@ __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* in
     const uint32_t total = stage_sum(S, n);
     if (total > kLocalEdges) break;  // uniform: too many for one workgroup; the grid takes them
     int32_t* pb = mine + (lr & 1) * kLocalHalf;
-    const Frontier P{pb, (uint32_t*)(pb + kStage), (uint32_t*)(pb + 2 * kStage)};
+    const bool in_lds = n <= kLocalLds && total <= kChunk;  // uniform
+    const Frontier P = in_lds ? Frontier{Q.q, Q.qo, Q.starts}
+                              : Frontier{pb, (uint32_t*)(pb + kStage), (uint32_t*)(pb + 2 * kStage)};
     stage_scan(S, n);
     __syncthreads();
     stage_write(S, n, total, P, 0u, 0u, nullptr);
@@ -1225,6 +1237,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
     PullLds<MIRROR> P;
   } lds;
   ChunkLds<OP, MIRROR>& L = lds.L;
+  __shared__ LocalLds Q;  // small local frontiers
   __shared__ unsigned long long s_red[kBlock / 64];
 
   Ctrl* c = a.ctrl;
@@ -1284,7 +1297,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
     td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
   unsigned long long loc_n = 0, loc_e = 0;
   if (a.loc && a.local_rounds > 0)
-    local_rounds<OP, MIRROR>(a, N, inq_nxt, S, L, my_raised, loc_n, loc_e, t_begin);
+    local_rounds<OP, MIRROR>(a, N, inq_nxt, S, L, Q, my_raised, loc_n, loc_e, t_begin);
   stage_flush(S, N, c, 0u);
   unsigned long long ra = block_sum64(my_raised, s_red);
   if (t == 0) {
