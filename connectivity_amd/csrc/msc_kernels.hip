@@ -1684,6 +1684,7 @@ template <int OP, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts) {
   __shared__ Stage S;
   __shared__ ChunkLds<OP, MIRROR> L;
+  __shared__ LocalLds Q;
   __shared__ unsigned long long s_red[kBlock / 64];
   __shared__ uint32_t s_ctl[4];  // done, cur, q_size, q_edges
   Ctrl* c = a.ctrl;
@@ -1731,17 +1732,21 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
     uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
     uint32_t* inq_nxt = cur ? a.inq0 : a.inq1;
     const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
-    unsigned long long raised = 0;
+    unsigned long long raised = 0, loc_n = 0, loc_e = 0;
+    const unsigned long long t_begin = wall_clock64();
     for (uint32_t ch = 0; ch < nchunks; ++ch)
       td_chunk<OP, MIRROR>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, raised);
+    // then, as a level launch's workgroup does, on from its own claims (nothing else runs)
+    if (a.loc && a.local_rounds > 0)
+      local_rounds<OP, MIRROR>(a, N, inq_nxt, S, L, Q, raised, loc_n, loc_e, t_begin);
     stage_flush(S, N, c, 0u);
     raised = block_sum64(raised, s_red);  // ends with a workgroup barrier
     if (t == 0) {
       const unsigned long long p = c->next_packed;
       const uint32_t next = (uint32_t)(p >> 32), nedges = (uint32_t)p;
-      c->found += next;
+      c->found += next + (int64_t)loc_n;
       c->level += 1;
-      finish_level(c, a, OP, false, colour_action(c, a, OP, nedges), q_size, q_edges, 0ull, 0ull, raised, next,
+      finish_level(c, a, OP, false, colour_action(c, a, OP, nedges), q_size, q_edges, loc_n, loc_e, raised, next,
                    nedges, nullptr);
       s_ctl[0] = c->done;
       s_ctl[1] = c->cur;
