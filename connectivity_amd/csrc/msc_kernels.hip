@@ -1044,6 +1044,110 @@ struct LocalLds {
   uint32_t starts[4];
 };
 
+// A tiny local frontier (at most 64 vertices and 64 edges, a chain's usual hop) is walked
+// by wave 0 alone, hop by hop in registers, with no LDS staging and no workgroup barrier
+// per hop.  The staged vertices come off the worklist as in any local expansion (bit
+// cleared, then the colour read where atomics are performed).  A vertex the walk raises
+// (push) or claims (colour-BW) is expanded at the next hop with the colour it was given,
+// without touching its worklist bit: a larger raise by anyone else later finds the bit
+// clear and stages an expansion of its own, and max-label propagation reaches the same
+// fixpoint in any order.  The walk stops when a hop has more than 64 edges, after
+// kWalkHops hops or once the launch's time budget is spent; the frontier it stops at is
+// staged again (push: each vertex whose worklist bit this call turns on, as for any
+// raise).  (vertices, edges) expanded are added to loc_n / loc_e by thread 0.  Every
+// thread calls; ends with a workgroup barrier.
+constexpr int kWalkHops = 1024;
+template <int OP>
+__device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t n, unsigned long long& raised,
+                          unsigned long long& loc_n, unsigned long long& loc_e, unsigned long long t_begin) {
+  if (wave_id() == 0) {
+    const uint32_t l = (uint32_t)lane();
+    uint32_t v = 0, beg = 0, len = 0;
+    int32_t col = 0;
+    if (l < n) {
+      v = (uint32_t)S.v[l];
+      if (OP == OP_PUSH) {
+        bit_clear(inq, v);
+        __builtin_amdgcn_s_waitcnt(0);
+        col = __hip_atomic_load(&a.colors[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        col = a.colors[v];
+      }
+      beg = a.down_off[v];
+      len = a.down_off[v + 1] - beg;
+    }
+    if (l == 0) S.cnt = 0u;  // the staged vertices are all in registers now
+    uint32_t fn = n;
+    for (int hop = 0;; ++hop) {
+      const uint32_t incl = wave_incl_sum(len);
+      const uint32_t total = __shfl(incl, 63, 64);
+      if (total == 0u) {
+        if (l == 0) loc_n += fn;
+        break;
+      }
+      const bool stop = __builtin_amdgcn_readfirstlane(
+                            (total > 64u || hop >= kWalkHops || (wall_clock64() - t_begin) >= a.local_ticks) ? 1
+                                                                                                           : 0) != 0;
+      if (stop) {
+        bool take = l < fn;
+        if (OP == OP_PUSH && take) take = bit_set(inq, v);
+        stage_push(S, take, v, len);
+        break;
+      }
+      if (l == 0) {
+        loc_n += fn;
+        loc_e += total;
+      }
+      // edge l of this hop belongs to the first lane whose inclusive prefix exceeds l
+      uint32_t j = 0;
+      for (uint32_t step = 32; step >= 1; step >>= 1)
+        if (__shfl(incl, (int)(j + step - 1), 64) <= l) j += step;
+      const uint32_t jb = __shfl(beg, (int)j, 64);
+      const uint32_t jx = __shfl(incl - len, (int)j, 64);
+      const int32_t jc = __shfl(col, (int)j, 64);
+      uint32_t w = 0;
+      bool got = false;
+      if (l < total) {
+        w = (uint32_t)a.down_adj[jb + (l - jx)];
+        const uint32_t word = a.valid[w >> 5];
+        const int32_t wc = a.colors[w];
+        const bool in = (word >> (w & 31u)) & 1u;
+        if (OP == OP_PUSH) {
+          got = in && wc < jc && atomicMax(&a.colors[w], jc) < jc;
+          if (got) ++raised;
+        } else {
+          got = in && wc == jc && bit_clear(a.valid, w);
+          if (got) {
+            a.labels[w] = jc;
+            if (a.cmin && (int32_t)w < a.cmin[jc]) atomicMin(&a.cmin[jc], (int32_t)w);
+          }
+        }
+      }
+      // the next frontier: lane k takes the (k+1)-th lane that raised / claimed
+      const unsigned long long m = __ballot(got);
+      fn = (uint32_t)__popcll(m);
+      uint32_t src = 0;
+      for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t c = src + step - 1;  // at most 62
+        if ((uint32_t)__popcll(m & ((2ull << c) - 1ull)) <= l) src += step;
+      }
+      const uint32_t nv = __shfl(w, (int)src, 64);
+      const int32_t nc = __shfl(jc, (int)src, 64);
+      v = 0;
+      beg = 0;
+      len = 0;
+      col = 0;
+      if (l < fn) {
+        v = nv;
+        col = nc;
+        beg = a.down_off[v];
+        len = a.down_off[v + 1] - beg;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <int OP, bool MIRROR>
 __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* inq, Stage& S,
                              ChunkLds<OP, MIRROR>& L, LocalLds& Q, unsigned long long& raised,
@@ -1058,6 +1162,10 @@ __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* in
     __syncthreads();
     if (n == 0u || !go) break;  // uniform
     const uint32_t total = stage_sum(S, n);
+    if (!MIRROR && n <= 64u && total <= 64u) {  // uniform
+      wave_walk<OP>(a, inq, S, n, raised, loc_n, loc_e, t_begin);
+      continue;
+    }
     if (total > kLocalEdges) break;  // uniform: too many for one workgroup; the grid takes them
     int32_t* pb = mine + (lr & 1) * kLocalHalf;
     const bool in_lds = n <= kLocalLds && total <= kChunk;  // uniform
