@@ -44,7 +44,7 @@ CANDIDATES = {
     "concurrent_fwbw": [0, 1],
     "search_batch": [2, 4, 6, 8],
     "local_rounds": [0, 4, 16, 32, 64],
-    "local_us": [10, 20, 40, 80, 160],
+    "local_us": [40, 80, 160, 240, 320],
     "pull_div": [0, 2, 4, 8, 16],
 }
 

@@ -41,7 +41,7 @@ struct msc_ctx {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 160, 2};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2};
   // device workspace
   void* ws = nullptr;
   size_t ws_bytes = 0;
