@@ -1105,12 +1105,15 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
       const uint32_t jb = __shfl(beg, (int)j, 64);
       const uint32_t jx = __shfl(incl - len, (int)j, 64);
       const int32_t jc = __shfl(col, (int)j, 64);
-      uint32_t w = 0;
+      uint32_t w = 0, wb = 0, we = 0;
       bool got = false;
       if (l < total) {
         w = (uint32_t)a.down_adj[jb + (l - jx)];
         const uint32_t word = a.valid[w >> 5];
         const int32_t wc = a.colors[w];
+        // the target's list bounds with its probes: the next hop needs no load to start
+        wb = a.down_off[w];
+        we = a.down_off[w + 1];
         const bool in = (word >> (w & 31u)) & 1u;
         if (OP == OP_PUSH) {
           got = in && wc < jc && atomicMax(&a.colors[w], jc) < jc;
@@ -1133,6 +1136,8 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
       }
       const uint32_t nv = __shfl(w, (int)src, 64);
       const int32_t nc = __shfl(jc, (int)src, 64);
+      const uint32_t nb = __shfl(wb, (int)src, 64);
+      const uint32_t ne = __shfl(we, (int)src, 64);
       v = 0;
       beg = 0;
       len = 0;
@@ -1140,8 +1145,8 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
       if (l < fn) {
         v = nv;
         col = nc;
-        beg = a.down_off[v];
-        len = a.down_off[v + 1] - beg;
+        beg = nb;
+        len = ne - nb;
       }
     }
   }
