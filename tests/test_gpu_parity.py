@@ -254,3 +254,73 @@ def test_complete_trim_rejected_for_wcc(msc, gpu_ctx):
     with pytest.raises(msc.MscError):
         msc._run("msc_cc_run", g, None, None, msc.POLICY_PLAIN, msc.LABELS_CANONICAL, False, False, False,
                  gpu_ctx, 15.0, 25, None, None, trim_level=2)
+
+
+
+_CHAINS = {}
+
+
+def _chain_cases(msc, orc):
+    """Graphs whose colouring is long chains of small frontiers (the one-wave walk of the
+    colour levels) with fan-outs past 64 edges (its hand-over to workgroup rounds), with the
+    oracle's raw labels (computed once: the oracle's colouring is quadratic in chain length)."""
+    if "cases" in _CHAINS:
+        return _CHAINS["cases"]
+    rng = np.random.default_rng(21)
+    gs = []
+    # a path of 3-cycles, ids decreasing along it: the largest colour walks the whole path
+    n = 1200
+    order = np.arange(n)[::-1]
+    src = list(order[:-1]) + [order[i + 2] for i in range(0, n - 2, 3)]
+    dst = list(order[1:]) + [order[i] for i in range(0, n - 2, 3)]
+    gs.append(msc.create_csr(n, src, dst))
+    # the same with permuted ids and a few long-range arcs forward along the path
+    perm = rng.permutation(n)
+    pa = rng.integers(0, n - 1, 12)
+    pb = pa + 1 + (rng.integers(0, n, 12) % (n - 1 - pa))
+    s2 = list(perm[np.asarray(src)]) + list(perm[order[pa]])
+    d2 = list(perm[np.asarray(dst)]) + list(perm[order[pb]])
+    gs.append(msc.create_csr(n, s2, d2))
+    # a path with 4-cycles, ids decreasing, where every 300 steps a vertex fans out to 100
+    # rungs that rejoin the path at the next vertex
+    m = 1200
+    src3, dst3 = [], []
+    for i in range(m - 1):
+        src3.append(i); dst3.append(i + 1)
+        if i % 7 == 3:
+            src3.append(i + 1); dst3.append(i - 2)
+    nxt = m
+    for h in range(150, m - 1, 300):
+        for k in range(100):
+            src3 += [h, nxt]; dst3 += [nxt, h + 1]
+            nxt += 1
+    rev = np.concatenate([np.arange(m)[::-1], np.arange(m, nxt)])
+    gs.append(msc.create_csr(nxt, list(rev[np.asarray(src3)]), list(rev[np.asarray(dst3)])))
+    cases = []
+    for g in gs:
+        piv, avg = orc.pivot_scc(g)
+        cases.append((g, piv, avg, orc.scc(g, piv, avg, vert_cutoff=0)))
+    _CHAINS["cases"] = cases
+    return cases
+
+
+@pytest.mark.parametrize("tuning", [None, dict(local_us=1), dict(local_rounds=256, local_us=100000)])
+def test_colour_chains(msc, orc, gpu_ctx, tuning):
+    """Colour levels along long chains (the one-wave walk, its time-budget stop and its
+    hand-over at fan-outs): raw and canonical labels equal the oracle's, alone and batched."""
+    _say(f"case: colour chains, tuning {tuning}")
+    cases = _chain_cases(msc, orc)
+    saved = gpu_ctx.tuning()
+    try:
+        if tuning:
+            gpu_ctx.set_tuning(**tuning)
+        for g, piv, avg, want in cases:
+            got = msc.run_scc(g, piv, avg, labels=msc.LABELS_RAW, ctx=gpu_ctx)
+            assert np.array_equal(got, want)
+            assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx), orc.canonical(want))
+        u, base = msc.pack_graphs([c[0] for c in cases])
+        raw = msc.run_scc_batch(u, base, labels=msc.LABELS_RAW, ctx=gpu_ctx)
+        for i, c in enumerate(cases):
+            assert np.array_equal(raw[int(base[i]):int(base[i + 1])], c[3])
+    finally:
+        gpu_ctx.set_tuning(**saved)
