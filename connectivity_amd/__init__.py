@@ -25,7 +25,7 @@ import numpy as np
 __all__ = [
     "MscError", "Graph", "DeviceGraph", "DeviceLabels", "Context", "Stats", "LevelRecord",
     "read_edge", "create_csr", "gen_rmat", "gen_grid", "gen_planted",
-    "run_scc", "run_cc", "scc_pivot", "batch_pivots", "scc_verify", "cc_verify", "verify_line",
+    "run_scc", "run_scc_noalloc", "run_cc", "scc_pivot", "batch_pivots", "scc_verify", "cc_verify", "verify_line",
     "canonicalize", "write_labels", "device_count", "lib_path",
     "POLICY_PLAIN", "POLICY_LEARNED", "LABELS_CANONICAL", "LABELS_RAW", "schedule_array",
     "set_policy", "load_policy", "mlp_forward", "mlp_grad", "mlp_train", "POLICY_WEIGHTS",
@@ -146,6 +146,8 @@ def _lib():
                                   C.POINTER(_Stats)]),
         "msc_cc_run": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
                                  C.POINTER(_Stats)]),
+        "msc_scc_run_noalloc": (C.c_int, [C.POINTER(_Csr), C.c_int32, C.c_double, P, C.POINTER(_Opts),
+                                          C.POINTER(_Stats)]),
         "msc_scc_run_batch": (C.c_int, [C.POINTER(_Csr), C.c_int32, P, P, P, C.POINTER(_Opts),
                                         C.POINTER(_Stats)]),
         "msc_cc_run_batch": (C.c_int, [C.POINTER(_Csr), C.c_int32, P, P, P, C.POINTER(_Opts),
@@ -605,6 +607,18 @@ def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None,
     (msc_opts.fwbw_rounds); the labels do not change."""
     lab, st = _run("msc_scc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
                    alpha, beta, out, schedule, trim_level=trim_level, fwbw_rounds=fwbw_rounds)
+    return (lab, st) if stats else lab
+
+
+def run_scc_noalloc(g: "DeviceGraph", max_deg_vert: int | None = None, avg_degree: float | None = None, *,
+                    ctx: Context, out: "DeviceLabels", policy: int = POLICY_PLAIN,
+                    labels: int = LABELS_CANONICAL, stats: bool = False):
+    """run_scc through msc_scc_run_noalloc: the same labels, computed in the library's static
+    per-device scratch array instead of a workspace it allocates (for callers that cannot
+    allocate).  ``g`` and ``out`` must be device-resident and ``ctx`` created beforehand;
+    n <= 2**20 and m <= 2**24.  Returns ``out`` (or (out, Stats))."""
+    lab, st = _run("msc_scc_run_noalloc", g, max_deg_vert, avg_degree, policy, labels, False, False, False, ctx,
+                   15.0, 25, out, None)
     return (lab, st) if stats else lab
 
 

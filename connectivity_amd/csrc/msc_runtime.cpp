@@ -26,6 +26,7 @@ constexpr uint32_t kRecCap = 1u << 16;
 // [3] colour-BW, followed by the colouring rounds' state, all copied back together
 constexpr int kCtrls = 4;
 constexpr size_t kCtrlBytes = kCtrls * sizeof(Ctrl) + sizeof(RoundCtl);
+static_assert(kCtrlBytes % 4 == 0, "control blocks are cleared as 32-bit words");
 
 struct Ev {
   hipEvent_t a, b;
@@ -35,14 +36,10 @@ struct Ev {
 
 }  // namespace
 
-struct msc_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  int grid = 1024;
-  int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2};
-  // device workspace
+// The device workspace of one call: one allocation carved into the parts below.  A
+// context keeps its own (grown on demand, reused across calls); msc_scc_run_noalloc
+// points the view at the device's static scratch array for the length of one call.
+struct WsView {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   int64_t cap_n = -1, cap_e = -1;
@@ -60,6 +57,15 @@ struct msc_ctx {
   Ctrl* ctrl2 = nullptr;                // that backward search's control block: ctrl + 1
   RoundCtl* rctl = nullptr;             // colouring rounds' state, after the kCtrls blocks
   msc_level_rec* recs = nullptr;
+};
+
+struct msc_ctx : WsView {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int grid = 1024;
+  int cus = 256;
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2};
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
   float* mlp = nullptr;
@@ -98,6 +104,18 @@ namespace {
 std::mutex g_default_mu;
 std::vector<msc_ctx*> g_default;  // one default context per device
 
+// Workspace of msc_scc_run_noalloc: one static array per device, held from the library's
+// load until the process ends, shared by every such call on that device (overlapping calls
+// on one device race).  Host code never passes `scratch` itself to a kernel: the name is a
+// host-side stand-in; the device address comes from hipGetSymbolAddress, once per device
+// when the first context on it is created (scratch_init), and is kept in g_scratch.
+constexpr size_t kScratchBytes = (size_t)136 << 20;
+constexpr int64_t kNoAllocMaxN = (int64_t)1 << 20;      // vertices
+constexpr int64_t kNoAllocMaxEdges = (int64_t)1 << 24;  // directed edges
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch;  // device address of `scratch` per device (null: not looked up)
+
 #define HIP_TRY(expr)                                              \
   do {                                                             \
     hipError_t _e = (expr);                                        \
@@ -115,20 +133,14 @@ int64_t max_grid(const msc_ctx* c) {
   return g < kMaxParts ? g : kMaxParts;
 }
 
-int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
-  if (n <= c->cap_n && edges <= c->cap_e) return MSC_OK;
-  int64_t nn = n > c->cap_n ? n : c->cap_n;
-  int64_t ee = edges > c->cap_e ? edges : c->cap_e;
-  if (nn < 1) nn = 1;
-  if (ee < 1) ee = 1;
-  if (c->ws) {
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    (void)hipFree(c->ws);
-    c->ws = nullptr;
-  }
+constexpr int kWsParts = 32;
+
+// Byte sizes of the workspace parts for nn vertices, ee walked edges and `grid` colour-level
+// workgroups, in carving order; returns the total with every part 256-byte aligned.
+size_t ws_layout(int64_t nn, int64_t ee, int64_t grid, size_t sizes[kWsParts]) {
   size_t nw = (size_t)((nn + 31) / 32) + kBWide;  // slack past the last word group
   size_t nchunks = (size_t)((ee + kChunk - 1) / kChunk) + 2;
-  size_t sizes[] = {
+  const size_t s[kWsParts] = {
       nw * 4, nw * 4, nw * 4, nw * 4, nw * 4, nw * 4,           // bitmaps
       (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // labels, colors, cmin
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr0
@@ -141,49 +153,74 @@ int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
       (size_t)(nn + 1) * 4, (size_t)(nn + 1) * 4, nchunks * 4,  // fr3
       (size_t)kMaxParts * 4 * 8,                                // partials of the backward search
       (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // colouring vertex lists
-      (size_t)kLocalStride * 4 * (size_t)max_grid(c),           // local-round frontiers
+      (size_t)kLocalStride * 4 * (size_t)grid,                  // local-round frontiers
       nw * 4, nw * 4,                                           // the searches' unvisited sets
   };
   size_t total = 0;
-  for (size_t s : sizes) total += align256(s);
-  void* p = nullptr;
-  HIP_TRY(hipMalloc(&p, total));
-  c->ws = p;
-  c->ws_bytes = total;
-  char* q = (char*)p;
-  void* parts[sizeof(sizes) / sizeof(sizes[0])];
-  for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
+  for (int i = 0; i < kWsParts; ++i) {
+    sizes[i] = s[i];
+    total += align256(s[i]);
+  }
+  return total;
+}
+
+// Points every workspace part of w into `base` (ws_layout's order and sizes).
+void ws_carve(WsView* w, void* base, const size_t sizes[kWsParts], size_t total, int64_t nn, int64_t ee) {
+  void* parts[kWsParts];
+  char* q = (char*)base;
+  for (int i = 0; i < kWsParts; ++i) {
     parts[i] = q;
     q += align256(sizes[i]);
   }
-  c->valid = (uint32_t*)parts[0];
-  c->b1 = (uint32_t*)parts[1];
-  c->b2 = (uint32_t*)parts[2];
-  c->b3 = (uint32_t*)parts[3];
-  c->inq0 = (uint32_t*)parts[4];
-  c->inq1 = (uint32_t*)parts[5];
-  c->labels = (int32_t*)parts[6];
-  c->colors = (int32_t*)parts[7];
-  c->cmin = (int32_t*)parts[8];
-  c->fr0 = Frontier{(int32_t*)parts[9], (uint32_t*)parts[10], (uint32_t*)parts[11]};
-  c->fr1 = Frontier{(int32_t*)parts[12], (uint32_t*)parts[13], (uint32_t*)parts[14]};
-  c->part = (unsigned long long*)parts[15];
-  c->err = (uint32_t*)parts[16];
-  c->ctrl = (Ctrl*)parts[17];
-  c->recs = (msc_level_rec*)parts[18];
-  c->fr2 = Frontier{(int32_t*)parts[19], (uint32_t*)parts[20], (uint32_t*)parts[21]};
-  c->fr3 = Frontier{(int32_t*)parts[22], (uint32_t*)parts[23], (uint32_t*)parts[24]};
-  c->part2 = (unsigned long long*)parts[25];
-  c->ctrl2 = c->ctrl + 1;
-  c->rctl = (RoundCtl*)(c->ctrl + kCtrls);
-  c->lists[0] = (int32_t*)parts[26];
-  c->lists[1] = (int32_t*)parts[27];
-  c->lists[2] = (int32_t*)parts[28];
-  c->loc = (int32_t*)parts[29];
-  c->s1 = (uint32_t*)parts[30];
-  c->s3 = (uint32_t*)parts[31];
-  c->cap_n = nn;
-  c->cap_e = ee;
+  w->ws = base;
+  w->ws_bytes = total;
+  w->valid = (uint32_t*)parts[0];
+  w->b1 = (uint32_t*)parts[1];
+  w->b2 = (uint32_t*)parts[2];
+  w->b3 = (uint32_t*)parts[3];
+  w->inq0 = (uint32_t*)parts[4];
+  w->inq1 = (uint32_t*)parts[5];
+  w->labels = (int32_t*)parts[6];
+  w->colors = (int32_t*)parts[7];
+  w->cmin = (int32_t*)parts[8];
+  w->fr0 = Frontier{(int32_t*)parts[9], (uint32_t*)parts[10], (uint32_t*)parts[11]};
+  w->fr1 = Frontier{(int32_t*)parts[12], (uint32_t*)parts[13], (uint32_t*)parts[14]};
+  w->part = (unsigned long long*)parts[15];
+  w->err = (uint32_t*)parts[16];
+  w->ctrl = (Ctrl*)parts[17];
+  w->recs = (msc_level_rec*)parts[18];
+  w->fr2 = Frontier{(int32_t*)parts[19], (uint32_t*)parts[20], (uint32_t*)parts[21]};
+  w->fr3 = Frontier{(int32_t*)parts[22], (uint32_t*)parts[23], (uint32_t*)parts[24]};
+  w->part2 = (unsigned long long*)parts[25];
+  w->ctrl2 = w->ctrl + 1;
+  w->rctl = (RoundCtl*)(w->ctrl + kCtrls);
+  w->lists[0] = (int32_t*)parts[26];
+  w->lists[1] = (int32_t*)parts[27];
+  w->lists[2] = (int32_t*)parts[28];
+  w->loc = (int32_t*)parts[29];
+  w->s1 = (uint32_t*)parts[30];
+  w->s3 = (uint32_t*)parts[31];
+  w->cap_n = nn;
+  w->cap_e = ee;
+}
+
+int ensure_workspace(msc_ctx* c, int64_t n, int64_t edges) {
+  if (n <= c->cap_n && edges <= c->cap_e) return MSC_OK;
+  int64_t nn = n > c->cap_n ? n : c->cap_n;
+  int64_t ee = edges > c->cap_e ? edges : c->cap_e;
+  if (nn < 1) nn = 1;
+  if (ee < 1) ee = 1;
+  if (c->ws) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->ws);
+    c->ws = nullptr;
+    c->cap_n = c->cap_e = -1;
+  }
+  size_t sizes[kWsParts];
+  const size_t total = ws_layout(nn, ee, max_grid(c), sizes);
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, total));
+  ws_carve(c, p, sizes, total, nn, ee);
   HIP_TRY(hipMemsetAsync(c->ws, 0, total, c->stream));
   return MSC_OK;
 }
@@ -530,11 +567,33 @@ msc_ctx* default_ctx(int device, int* rc) {
   return g_default[device];
 }
 
+// Looks up the device address of `scratch` on `device` (which must be current), once.
+int scratch_init(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((size_t)device >= g_scratch.size()) g_scratch.resize((size_t)device + 1, nullptr);
+  if (g_scratch[device]) return MSC_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return MSC_E_DEVICE;
+  }
+  g_scratch[device] = p;
+  return MSC_OK;
+}
+
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && (size_t)device < g_scratch.size() ? g_scratch[device] : nullptr;
+}
+
 enum class Kind { SCC, CC };
 
 // The whole hot path for one graph, or for a batch of graphs packed into one CSR (bt).
+// noalloc: the workspace is the device's scratch array for this call (msc_scc_run_noalloc,
+// which has checked that nothing else here allocates).
 int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t* labels_out, const msc_opts* o,
-             msc_stats* s, const BatchIn* bt = nullptr) {
+             msc_stats* s, const BatchIn* bt = nullptr, bool noalloc = false) {
   const bool cc = kind == Kind::CC;
   int rc = check_args(g_in, labels_out, o, cc);
   if (rc) return rc;
@@ -560,7 +619,31 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   }
   if ((rc = upload_schedule(c, o))) return rc;
   const int64_t edges = cc ? 2 * g_in->m : g_in->m;
-  if ((rc = ensure_workspace(c, n, edges))) return rc;
+  // the context's own workspace view, put back when a no-alloc call returns
+  struct Restore {
+    msc_ctx* c;
+    WsView v;
+    bool on;
+    ~Restore() {
+      if (!on) return;
+      static_cast<WsView&>(*c) = v;
+      c->last_nrec = c->last_nrec2 = 0;  // its records were in the scratch array
+    }
+  } restore{c, WsView{}, false};
+  if (noalloc) {
+    void* base = scratch_ptr(c->device);
+    if (!base) return MSC_E_DEVICE;
+    size_t sizes[kWsParts];
+    const size_t total = ws_layout(n, edges, c->grid, sizes);
+    if (total > kScratchBytes) return MSC_E_CAPACITY;
+    restore.v = static_cast<const WsView&>(*c);
+    restore.on = true;
+    ws_carve(c, base, sizes, total, n, edges);
+    // the last call's contents stay in the array: start from zero, as a fresh workspace does
+    launch_fill_u32(launch_of(c), (uint32_t*)base, 0u, (int64_t)(total / 4));
+  } else if ((rc = ensure_workspace(c, n, edges))) {
+    return rc;
+  }
   msc_csr g;
   if ((rc = stage_graph(c, g_in, &g))) return rc;
   if (o->validate && (rc = device_validate(c, &g))) return rc;
@@ -572,7 +655,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   Launch L = launch_of(c);
   int32_t* labels = o->labels_where == MSC_DEVICE ? labels_out : c->labels;
   const int64_t nw = ((int64_t)n + 31) / 32;
-  HIP_TRY(hipMemsetAsync(c->ctrl, 0, kCtrlBytes, c->stream));
+  launch_fill_u32(L, (uint32_t*)c->ctrl, 0u, (int64_t)(kCtrlBytes / 4));
 
   if (avg < 0.0) avg = cc ? 2.0 * (double)g.m / (double)n : (double)g.m / (double)n;
   if (bt) {
@@ -1088,6 +1171,8 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
     msc_ctx_destroy(c);
     return MSC_E_OOM;
   }
+  // the no-alloc entry point's scratch address on this device (checked again at its use)
+  (void)scratch_init(device);
   c->ctrl_host2 = c->ctrl_host + 1;
   c->rctl_host = (RoundCtl*)(c->ctrl_host + kCtrls);
   *c->prog_host = 0ull;
@@ -1173,6 +1258,22 @@ int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32
     return run_path(Kind::SCC, g, max_deg_vert, avg_degree, labels, opts, stats);
   } catch (const std::bad_alloc&) {
     return MSC_E_OOM;
+  } catch (...) {
+    return MSC_E_DEVICE;
+  }
+}
+
+int msc_scc_run_noalloc(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
+                        const msc_opts* opts, msc_stats* stats) {
+  // everything that would allocate is ruled out here: a context created beforehand (no
+  // default context made on the fly), device-resident graph and labels (no staging
+  // buffer), no per-kernel events, no level records kept past the call
+  if (!g || !opts || !opts->ctx) return MSC_E_INVALID;
+  if (g->where != MSC_DEVICE || opts->labels_where != MSC_DEVICE) return MSC_E_INVALID;
+  if (opts->timing || opts->record) return MSC_E_INVALID;
+  if ((int64_t)g->n > kNoAllocMaxN || g->m > kNoAllocMaxEdges) return MSC_E_CAPACITY;
+  try {
+    return run_path(Kind::SCC, g, max_deg_vert, avg_degree, labels, opts, stats, nullptr, true);
   } catch (...) {
     return MSC_E_DEVICE;
   }

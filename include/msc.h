@@ -237,6 +237,21 @@ int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);
 int msc_scc_run(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
                 const msc_opts* opts, msc_stats* stats);
 
+/* msc_scc_run without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, labels and statuses as msc_scc_run;
+ * every call uses one static device array of the library (136 MiB per device, held until
+ * the process ends) as its workspace instead of the context's, so it calls no hipMalloc,
+ * hipFree or other allocating routine.  Requirements (else MSC_E_INVALID):
+ * opts->ctx is a context made beforehand with msc_ctx_create (its creation is the set-up
+ * that allocates), the graph and labels are device-resident (g->where and
+ * opts->labels_where MSC_DEVICE), opts->timing = opts->record = 0.  Limits (else
+ * MSC_E_CAPACITY): n <= 1,048,576 vertices and m <= 16,777,216 edges.  All such calls on
+ * one device share that array, which keeps the last call's contents; each call starts by
+ * clearing it, and calls that overlap on one device race. */
+int msc_scc_run_noalloc(const msc_csr* g, int32_t max_deg_vert, double avg_degree, int32_t* labels,
+                        const msc_opts* opts, msc_stats* stats);
+
 /* Weakly connected components of a digraph (connected components of its mirror).
  * Replaces run_cc(graph&, int*& cc_maps, int max_deg_vert, double avg_degree)
  * (multistep/cc_main.cpp:184) over the mirrored edge list read_edge builds

@@ -78,6 +78,22 @@ def test_argument_errors_are_status_codes(msc):
     assert L.msc_scc_run(ctypes.byref(broken), -1, -1.0, lab.ctypes.data, ctypes.byref(o), None) == msc.E_INVALID
 
 
+def test_noalloc_refuses_what_would_allocate(msc):
+    # msc_scc_run_noalloc needs a context made beforehand and device-resident graph and
+    # labels; each refusal comes back before any device call
+    L = msc._lib()
+    o = msc._Opts()
+    L.msc_opts_default(ctypes.byref(o))
+    g = msc.create_csr(3, [0, 1], [1, 2])
+    csr = g.csr()
+    lab = np.zeros(3, np.int32)
+    assert L.msc_scc_run_noalloc(None, -1, -1.0, None, ctypes.byref(o), None) == msc.E_INVALID
+    # no context (the default context would be created, i.e. allocated, on the fly)
+    assert L.msc_scc_run_noalloc(ctypes.byref(csr), -1, -1.0, lab.ctypes.data, ctypes.byref(o), None) == msc.E_INVALID
+    o.labels_where = msc.DEVICE
+    assert L.msc_scc_run_noalloc(ctypes.byref(csr), -1, -1.0, lab.ctypes.data, ctypes.byref(o), None) == msc.E_INVALID
+
+
 def test_no_device_is_reported_not_hidden(msc):
     if msc.device_count() > 0:
         pytest.skip("a GPU is visible")
