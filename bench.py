@@ -179,6 +179,26 @@ def main() -> int:
         if not same:
             return 1
 
+    # the plain rule with the searches in the reference's order (FW, then BW restricted to the
+    # FW set: scc_fwbw.cpp:469-513), which the plain schedule of the reference follows level by
+    # level; the default runs BW beside FW over the whole valid set
+    ref_order = None
+    if not args.no_compare:
+        defaults = ctx.tuning()
+        try:
+            ctx.set_tuning(concurrent_fwbw=0)
+            step(msc.POLICY_PLAIN, timing=False)
+            ro_ms, _ = timed(msc.POLICY_PLAIN, args.steps, False)
+            ro_same = bool(np.array_equal(out.to_host(), labels))
+        finally:
+            ctx.set_tuning(**defaults)
+        ref_order = {"plain_ms": round(ro_ms, 4), "value_speedup": round(ro_ms / ms_per_step, 3),
+                     "identical_labels": ro_same}
+        log(f"plain rule, searches in the reference's order: {ro_ms:.3f} ms ({ro_ms / ms_per_step:.2f}x the "
+            f"reported run), labels identical: {ro_same}")
+        if not ro_same:
+            return 1
+
     # per-kernel-class event time and algorithmic bytes summed over the timed steps
     agg = {}
     for st in stats:
@@ -250,6 +270,7 @@ def main() -> int:
         "ms_per_step_with_kernel_events": round(ms_events, 4),
         "plain_ms": round(plain_ms, 4),
         "learned": learned,
+        "plain_reference_order": ref_order,
         "scc_counts": ({"num_sccs": counts[0], "nontrivial": counts[1], "max_scc": counts[2]}
                        if counts else None),
         "parity": check,
@@ -303,11 +324,7 @@ def cpu_baseline_batch(u, base, seconds: float):
 
     import connectivity_amd as msc
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = min(cores, 16)
+    cores = host_cores()
     use_ref = oracle.have_ref()
 
     def one(i):
@@ -332,7 +349,7 @@ def cpu_baseline_batch(u, base, seconds: float):
     log(f"cpu baseline ({'reference' if use_ref else 'port'}, {cores} threads x 1): {graphs} graphs, "
         f"{edges} edges in {wall:.2f} s")
     return {"value": round(edges / wall / 1e9, 5), "unit": "GTEPS", "cores": cores,
-            "kind": "reference" if use_ref else "port", "ms": round(wall * 1e3, 2),
+            "kind": "reference" if use_ref else "port", "ms": round(wall * 1e3, 2), "cpu_model": cpu_model(),
             "sample": f"run_scc (multistep, vert_cutoff 10000) on the first {i} graphs of the batch ({graphs} with "
                       f"a pivot), one single-threaded call per graph, {cores} graphs at a time; edges / wall time"}
 
@@ -350,28 +367,67 @@ def read_traffic(workload: str, policy: str, kernel_class: str):
         return None
 
 
-def cpu_baseline(g, runs: int):
-    """The reference's run_scc on this host's cores (oracle/_ref); the restatement if absent."""
-    import oracle
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def host_cores() -> int:
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    use_ref = oracle.have_ref()
-    times = []
-    for _ in range(runs):
-        if use_ref:
-            times.append(oracle.ref_time_scc(g, g.max_deg_vert, g.avg_degree))
-        else:
-            times.append(oracle.scc_time(g, g.max_deg_vert, g.avg_degree))
-    med = statistics.median(times)
-    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
-    log(f"cpu baseline ({'reference' if use_ref else 'port'}, {threads} threads): median {med * 1e3:.1f} ms "
-        f"over {runs} runs")
-    return {"value": round(g.m / med / 1e9, 5), "unit": "GTEPS", "cores": threads,
-            "kind": "reference" if use_ref else "port", "ms": round(med * 1e3, 2),
-            "sample": f"{runs} full run_scc calls (multistep, OpenMP, vert_cutoff 10000) on the same graph, median"}
+    return min(cores, 16)  # this host's share for one GPU
+
+
+def cpu_time_child(g, threads: int, runs: int) -> dict:
+    """oracle/cpu_time.py in a child process with OMP_NUM_THREADS = threads,
+    OMP_PROC_BIND = close, OMP_PLACES = cores (libgomp reads them when it loads, so they
+    cannot be changed inside this process, whose libgomp is already up); the graph goes to
+    it as .npy files in memory-backed temporary storage."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    tmp = tempfile.mkdtemp(prefix="msc_cpu_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        for k in ("out_off", "out_adj", "in_off", "in_adj"):
+            np.save(os.path.join(tmp, f"{k}.npy"), np.asarray(getattr(g, k)))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_time", "--dir", tmp, "--pivot", str(g.max_deg_vert),
+                            "--avg", repr(g.avg_degree), "--runs", str(runs)], cwd=ROOT, env=env,
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu baseline child failed: {r.stderr[-400:]}")
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_baseline(g, runs: int):
+    """The reference's run_scc (oracle/_ref; the restatement if absent) timed on this
+    host's cores with threads bound close to their cores, and once more on one thread."""
+    cores = host_cores()
+    full = cpu_time_child(g, cores, runs)
+    one = cpu_time_child(g, 1, max(1, min(3, runs)))
+    med, med1 = full["median_s"], one["median_s"]
+    log(f"cpu baseline ({full['kind']}, {cores} threads, {cpu_model()}): median {med * 1e3:.1f} ms over {runs} "
+        f"runs; 1 thread {med1 * 1e3:.1f} ms")
+    return {"value": round(g.m / med / 1e9, 5), "unit": "GTEPS", "cores": cores, "kind": full["kind"],
+            "ms": round(med * 1e3, 2), "cpu_model": cpu_model(),
+            "omp": {"OMP_NUM_THREADS": cores, "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"},
+            "one_thread": {"value": round(g.m / med1 / 1e9, 5), "ms": round(med1 * 1e3, 2),
+                           "runs": len(one["times_s"])},
+            "sample": f"{runs} full run_scc calls (multistep, OpenMP, vert_cutoff 10000) on the same graph, median; "
+                      f"each timed around the call as scc_main.cpp:368-378 does"}
 
 
 if __name__ == "__main__":

@@ -191,3 +191,40 @@ def test_recorded_features_follow_their_definition(msc, gpu_ctx):
             }
             for i, w in want.items():
                 assert abs(f[i] - w) <= 1e-4 * max(1.0, abs(w)), (r.phase, r.level, i, f[i], w)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scale,seed", [(20, 424243), (22, 424244)])
+def test_learned_policy_gain(msc, orc, gpu_ctx, scale, seed):
+    """What the trained network gains over the plain ALPHA/BETA rule on held-out R-MAT
+    graphs (seeds outside the training set): the FW-BW searches' summed level durations
+    must drop by at least 1.3x, with labels identical to the plain run and the oracle."""
+    import os
+
+    if not os.path.exists(msc.POLICY_WEIGHTS):
+        pytest.skip("no trained weights shipped yet")
+    msc.load_policy(gpu_ctx)
+    g = msc.gen_rmat(scale, 16, seed=seed)
+    dg = msc.DeviceGraph(gpu_ctx, g)
+    want = orc.canonical(orc.scc(g, g.max_deg_vert, g.avg_degree, vert_cutoff=0))
+    cost = {}
+    try:
+        for name, pol in (("plain", msc.POLICY_PLAIN), ("learned", msc.POLICY_LEARNED)):
+            best = None
+            for _ in range(3):
+                lab = msc.run_scc(dg, g.max_deg_vert, g.avg_degree, ctx=gpu_ctx, policy=pol, timing=True,
+                                  record=True)
+                assert np.array_equal(lab, want), name
+                recs = [r for r in gpu_ctx.levels() if r.phase in (0, 1)]
+                us = sum(r.level_us for r in recs)
+                edges = sum((r.inspected if r.bottom_up else r.q_edges) for r in recs)
+                if best is None or us < best[0]:
+                    best = (us, edges, len(recs))
+            cost[name] = best
+    finally:
+        dg.free()
+    gain = cost["plain"][0] / cost["learned"][0]
+    print(f"R-MAT s{scale} seed {seed}: plain {cost['plain'][0]:.0f} us over {cost['plain'][2]} levels "
+          f"({cost['plain'][1]} edges), learned {cost['learned'][0]:.0f} us over {cost['learned'][2]} levels "
+          f"({cost['learned'][1]} edges): {gain:.2f}x", flush=True)
+    assert gain >= 1.3, cost
