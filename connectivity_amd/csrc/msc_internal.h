@@ -86,6 +86,9 @@ struct Ctrl {
   uint32_t list_len;               // vertex list written by the last seeding launch (kept by resets)
   uint32_t round_edges;            // colour push: list lengths of the round's seeded frontier
   uint32_t bu_wide;                // reach: the coming bottom-up level walks kBWide-word groups
+  uint32_t from_bits;              // reach: the coming top-down level expands the vertices the
+                                   // bottom-up level before it claimed (LevelArgs::fresh), not a queue
+  uint32_t pad_ctrl_;
   // counts the host reads with the next control-block copy instead of a round trip each
   unsigned long long red_trim[4];    // col 0: trim survivors
   unsigned long long red_commit[4];  // col 0: vertices committed by FW-BW / the CC search
@@ -129,6 +132,7 @@ struct LevelArgs {
   uint32_t* allow;            // bitmap: vertices the search may visit
   uint32_t* cand;             // bitmap: allowed and not yet visited (cleared on claim)
   uint32_t* seen;             // bitmap: not yet visited (cleared on claim; allowed or not)
+  uint32_t* fresh;            // reach: bitmap of the vertices the last bottom-up level claimed
   uint32_t* valid;            // bitmap: vertices without a label yet
   uint32_t* inq0;             // colour worklist membership, buffer 0 / 1
   uint32_t* inq1;

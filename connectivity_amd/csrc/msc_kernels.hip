@@ -588,6 +588,7 @@ __global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t pha
     c->cur = 0u;
     c->bottom_up = 0u;
     c->bu_wide = 0u;
+    c->from_bits = 0u;
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
@@ -615,6 +616,7 @@ __device__ __forceinline__ void ctrl_fresh(Ctrl* c, int32_t phase) {
   c->cur = 0u;
   c->bottom_up = 0u;
   c->bu_wide = 0u;
+  c->from_bits = 0u;
   c->switched_back = 0u;
   c->done = 0u;
   c->level = 0u;
@@ -738,6 +740,7 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
     c->cur = 0u;
     c->bottom_up = 0u;
     c->bu_wide = 0u;
+    c->from_bits = 0u;
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;
@@ -930,11 +933,15 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
 // candidates are settled there (on R-MAT the first in-neighbour is already visited ~90% of
 // the time); the rest scan on.  Claims collect in LDS word masks and are written back with
 // one plain store per word: no other workgroup clears bits of these words during a
-// bottom-up level.  Every thread calls; claims are staged for frontier N.
+// bottom-up level.  Claims are not queued (queueing them cost R-MAT s24 1.5%, also when only
+// levels with few claims left did): a following bottom-up level needs no frontier, and a
+// following top-down level expands the claims from the group's words of a.fresh, which
+// every group writes in full (zero words included).  The thread's claims and their
+// down-list lengths are counted into my_next / my_nedges.  Every thread calls.
 template <bool MIRROR, int WPT>
-__device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, int64_t wb, Stage& S,
-                                         BitIterT<WPT>& B, uint32_t* s_claim, unsigned long long& my_examined,
-                                         unsigned long long& my_inspected) {
+__device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIterT<WPT>& B, uint32_t* s_claim,
+                                         unsigned long long& my_examined, unsigned long long& my_inspected,
+                                         unsigned long long& my_next, unsigned long long& my_nedges) {
   constexpr int kR = MSC_BU_KR;
   const int t = threadIdx.x;
 #pragma unroll
@@ -1001,22 +1008,119 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, const Frontier& N, 
         }
       if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
     }
+    // the claims, counted per wave from a ballot (a point where the wave's lanes reconverge
+    // after their divergent list walks: without one, -O3 builds of the mirrored search
+    // were seen to lose claims), and their down-list lengths: the next frontier's edges
+    // (the policy's features, and the size of a top-down level that expands these claims)
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
-      uint32_t deg = hit[r] ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]) : 0u;
-      stage_push(S, hit[r], v[r], deg);
+      const unsigned long long mask = __ballot(hit[r]);
+      if (lane() == 0) my_next += (unsigned long long)__popcll(mask);
+      if (hit[r]) my_nedges += list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, v[r]);
     }
-    stage_flush(S, N, a.ctrl, kStage - kBlock * kR);  // also the barrier before the next round
+    __syncthreads();  // every claim of the round is in s_claim before the next round's picks
   }
-  // after the last round's barrier every claim of this group is in s_claim; each thread
-  // reads only its own words and masks, so the next group's loads need no barrier first
+  // every claim of this group is in s_claim; each thread reads only its own words and
+  // masks, so the next group's loads need no barrier first
 #pragma unroll
   for (int j = 0; j < WPT; ++j) {
     const int i = t * WPT + j;
-    if (s_claim[i]) {
-      a.cand[wb + i] = B.word[i] & ~s_claim[i];
-      a.seen[wb + i] = sw[j] & ~s_claim[i];
+    const uint32_t c = s_claim[i];
+    a.fresh[wb + i] = c;
+    if (c) {
+      a.cand[wb + i] = B.word[i] & ~c;
+      a.seen[wb + i] = sw[j] & ~c;
     }
+  }
+}
+
+// LDS of a top-down level that expands the claims of the bottom-up level before it from the
+// fresh bitmap: the word group, and 256 of its vertices at a time with the exclusive prefix
+// of their list lengths.
+struct TdBitsLds {
+  BitIterT<kWPTWide> B;
+  uint32_t base[kBlock];
+  uint32_t beg[kBlock];
+  uint32_t len1[kBlock];
+  uint32_t beg2[kBlock];
+  uint32_t scan[kBlock / 64];
+};
+
+// A top-down level right after a bottom-up one (multistep/scc_fwbw.cpp:87-114 on the
+// vertices that level found), for one word group [wb, wb + kBWide) of a.fresh: the group's
+// vertices are taken 256 at a time, their list lengths prefix-summed in LDS, and their
+// edges expanded kBlock * kPer at a time (owner by binary search over the prefix), with the
+// same probes, claims and staging as td_chunk.  The edges of a group stay in its workgroup.
+// Every thread calls; claims are staged for frontier N.
+template <bool MIRROR>
+__device__ __forceinline__ void td_bits_group(const LevelArgs& a, const Frontier& N, int64_t wb, Stage& S,
+                                              TdBitsLds& T) {
+  const int t = threadIdx.x;
+  const uint32_t total = bits_load(T.B, a.fresh, a.nwords, wb);  // ends with a barrier
+  for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
+    const uint32_t k = k0 + (uint32_t)t;
+    uint32_t b1 = 0, d1 = 0, b2 = 0, d2 = 0;
+    if (k < total) {
+      const uint32_t v = bits_pick(T.B, wb, k);
+      b1 = a.down_off[v];
+      d1 = a.down_off[v + 1] - b1;
+      if (MIRROR) {
+        b2 = a.down_off2[v];
+        d2 = a.down_off2[v + 1] - b2;
+      }
+    }
+    const uint32_t d = d1 + d2;
+    const uint32_t incl = wave_incl_sum(d);
+    if (lane() == 63) T.scan[wave_id()] = incl;
+    __syncthreads();
+    uint32_t carry = 0, tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (w < wave_id()) carry += T.scan[w];
+      tot += T.scan[w];
+    }
+    T.base[t] = incl - d + carry;
+    T.beg[t] = b1;
+    T.len1[t] = d1;
+    T.beg2[t] = b2;
+    __syncthreads();
+    const int nv = (int)min((uint32_t)kBlock, total - k0);
+    for (uint32_t e0 = 0; e0 < tot; e0 += kBlock * kPer) {
+      uint32_t w[kPer], word[kPer];
+      bool ok[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const uint32_t i = e0 + (uint32_t)(r * kBlock + t);
+        ok[r] = i < tot;
+        w[r] = 0u;
+        if (ok[r]) {
+          int lo = 0, hi = nv;  // the last vertex whose edges start at or before i
+#pragma unroll
+          for (int st = 0; st < 8; ++st) {  // nv <= 256
+            const int mid = (lo + hi) >> 1;
+            if (hi - lo > 1) {
+              if (T.base[mid] <= i) lo = mid;
+              else hi = mid;
+            }
+          }
+          const uint32_t idx = i - T.base[lo];
+          if (MIRROR && idx >= T.len1[lo]) w[r] = (uint32_t)a.down_adj2[T.beg2[lo] + (idx - T.len1[lo])];
+          else w[r] = (uint32_t)a.down_adj[T.beg[lo] + idx];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) word[r] = ok[r] ? a.cand[w[r] >> 5] : 0u;
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const bool got = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u) && bit_clear(a.cand, w[r]);
+        if (got)
+          __hip_atomic_fetch_and(&a.seen[w[r] >> 5], ~(1u << (w[r] & 31u)), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t deg = got ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
+        stage_push(S, got, w[r], deg);
+      }
+      stage_flush(S, N, a.ctrl, kStage - kBlock * kPer);  // also the barrier before the next step
+    }
+    __syncthreads();  // T.base and T.scan are rewritten by the next 256 vertices
   }
 }
 
@@ -1426,8 +1530,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
 // kChunk-edge chunks of the frontier (top-down); none once the search is done.
 __device__ __forceinline__ uint32_t reach_units(const Ctrl* c, int64_t nwords) {
   if (c->done) return 0u;
-  if (c->bottom_up) {
-    const int64_t gw = c->bu_wide ? kBWide : kBW;
+  if (c->bottom_up || c->from_bits) {
+    const int64_t gw = (c->bu_wide || !c->bottom_up) ? kBWide : kBW;
     return (uint32_t)((nwords + gw - 1) / gw);
   }
   return (c->q_edges + kChunk - 1) / kChunk;
@@ -1438,8 +1542,9 @@ __device__ __forceinline__ uint32_t reach_units(const Ctrl* c, int64_t nwords) {
 template <bool MIRROR>
 __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first, uint32_t units, Stage& S,
                                              ChunkLds<OP_FW, MIRROR>& L, BitIter& B, uint32_t* s_claim,
-                                             BitIterT<kWPTWide>& BW, uint32_t* s_claimw, unsigned long long& ex,
-                                             unsigned long long& in) {
+                                             BitIterT<kWPTWide>& BW, uint32_t* s_claimw, TdBitsLds& T,
+                                             unsigned long long& ex, unsigned long long& in,
+                                             unsigned long long& nx, unsigned long long& ne) {
   Ctrl* c = a.ctrl;
   const uint32_t cur = c->cur, q_size = c->q_size, q_edges = c->q_edges;
   const bool bu = c->bottom_up != 0u;
@@ -1453,11 +1558,16 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
     // candidates, and the grid then covers the bitmap in one round of groups
     if (c->bu_wide) {
       for (uint32_t lu = first; lu < units; lu += gridDim.x)
-        bu_group<MIRROR, kWPTWide>(a, N, (int64_t)lu * kBWide, S, BW, s_claimw, ex, in);
+        bu_group<MIRROR, kWPTWide>(a, (int64_t)lu * kBWide, BW, s_claimw, ex, in, nx, ne);
     } else {
       for (uint32_t lu = first; lu < units; lu += gridDim.x)
-        bu_group<MIRROR, kWPT>(a, N, (int64_t)lu * kBW, S, B, s_claim, ex, in);
+        bu_group<MIRROR, kWPT>(a, (int64_t)lu * kBW, B, s_claim, ex, in, nx, ne);
     }
+    __syncthreads();  // nothing staged; the workgroup may move on to the other search's LDS
+    return;
+  }
+  if (c->from_bits) {
+    for (uint32_t lu = first; lu < units; lu += gridDim.x) td_bits_group<MIRROR>(a, N, (int64_t)lu * kBWide, S, T);
   } else {
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
@@ -1472,12 +1582,14 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
 // compute unit behind the other search's level.  Each search keeps its own direction,
 // control block, frontiers and bitmaps; a workgroup flushes the claims staged for one
 // search before it moves on to the other's units.  Workgroups without a unit exit at once;
-// every other one writes one partial row per search, (examined, inspected, 0, 0).
+// every other one writes one partial row per search, (examined, inspected, claims, their
+// down-list lengths) -- the last two counted by bottom-up levels, which queue nothing.
 template <int NS, bool MIRROR>
 __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   __shared__ Stage S;
-  // a search's level is either top-down (ChunkLds) or bottom-up (BitIter + claim masks),
-  // and a workgroup moves between searches only across a barrier: the two share LDS
+  // a search's level is either top-down (ChunkLds, or TdBitsLds right after a bottom-up
+  // level) or bottom-up (BitIter + claim masks), and a workgroup moves between searches only
+  // across a barrier: they share LDS
   __shared__ union {
     ChunkLds<OP_FW, MIRROR> L;
     struct {
@@ -1488,6 +1600,7 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
       BitIterT<kWPTWide> B;
       uint32_t claim[kBWide];
     } buw;
+    TdBitsLds T;
   } U;
   __shared__ unsigned long long s_red[kBlock / 64];
   ChunkLds<OP_FW, MIRROR>& L = U.L;
@@ -1507,30 +1620,21 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
     if (NS == 2 && u1 && p.s[1].recs) atomicMin(&p.s[1].ctrl->t_start, (unsigned long long)wall_clock64());
   }
   __syncthreads();
-  unsigned long long ex0 = 0ull, in0 = 0ull, ex1 = 0ull, in1 = 0ull;
-  if (b < u0) reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, ex0, in0);
+  unsigned long long c0[4] = {0ull, 0ull, 0ull, 0ull}, c1[4] = {0ull, 0ull, 0ull, 0ull};
+  if (b < u0) reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, U.T, c0[0], c0[1], c0[2], c0[3]);
   if (NS == 2) {
     // this workgroup's first unit of the union at or past u0, in search 1's numbering
     const uint32_t first = b >= u0 ? b - u0 : b + (u0 - b + G - 1) / G * G - u0;
-    if (first < u1) reach_search<MIRROR>(p.s[1], first, u1, S, L, B, s_claim, BW, s_claimw, ex1, in1);
+    if (first < u1)
+      reach_search<MIRROR>(p.s[1], first, u1, S, L, B, s_claim, BW, s_claimw, U.T, c1[0], c1[1], c1[2], c1[3]);
   }
-  unsigned long long e = block_sum64(ex0, s_red), i = block_sum64(in0, s_red);
-  if (t == 0) {
-    unsigned long long* r = p.s[0].part + 4 * b;
-    r[0] = e;
-    r[1] = i;
-    r[2] = 0;
-    r[3] = 0;
-  }
-  if (NS == 2) {
-    e = block_sum64(ex1, s_red);
-    i = block_sum64(in1, s_red);
+  for (int s = 0; s < NS; ++s) {
+    unsigned long long* cs = s == 0 ? c0 : c1;
+    unsigned long long r4[4];
+    for (int k = 0; k < 4; ++k) r4[k] = block_sum64(cs[k], s_red);
     if (t == 0) {
-      unsigned long long* r = p.s[1].part + 4 * b;
-      r[0] = e;
-      r[1] = i;
-      r[2] = 0;
-      r[3] = 0;
+      unsigned long long* r = p.s[s].part + 4 * b;
+      for (int k = 0; k < 4; ++k) r[k] = r4[k];
     }
   }
 }
@@ -1567,8 +1671,10 @@ __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int a
     bytes = 12ull * (q_size + ex) + 9ull * (q_edges + in) + 4ull * ra + 4ull * next;
   else if (op == OP_CBW)
     bytes = 12ull * (q_size + ex) + 9ull * (q_edges + in) + 5ull * next;
-  else if (bu)
-    bytes = 4ull * (unsigned long long)a.nwords + 8ull * ex + 5ull * in + 5ull * next;
+  else if (bu)  // candidate and fresh words, list bounds, probes, claims' list bounds (+ queue)
+    bytes = 8ull * (unsigned long long)a.nwords + 8ull * ex + 5ull * in + 8ull * next;
+  else if (c->from_bits)  // fresh words instead of queue ids
+    bytes = 4ull * (unsigned long long)a.nwords + 8ull * q_size + 5ull * q_edges + 5ull * next;
   else
     bytes = 12ull * q_size + 5ull * q_edges + 5ull * next;
   c->bytes_op[op] += bytes;
@@ -1599,6 +1705,7 @@ __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int a
   c->q_size = next;
   c->q_edges = nedges;
   c->cur ^= 1u;
+  c->from_bits = bu ? 1u : 0u;  // a top-down level next expands this level's claims from a.fresh
   c->bottom_up = act == 1 ? 1u : 0u;
   c->next_packed = 0ull;
   c->done = (next == 0u || act == 2) ? 1u : 0u;
@@ -1683,8 +1790,9 @@ __device__ __forceinline__ int mlp_eval(const MlpLane& m, const float* s_feat, f
 // Closes one reach search's level; one whole wave calls it.  Lane 0 books the level and
 // applies the direction rule -- the reference's (multistep/scc_fwbw.cpp:168-195,
 // cc_bfs.cpp:167-194), the MLP (when `learned`), or a forced schedule.
-__device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned long long in, bool learned,
-                             const MlpLane& m, float* s_feat, float* s_z, float* s_h) {
+__device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned long long in, unsigned long long nx,
+                             unsigned long long ne, bool learned, const MlpLane& m, float* s_feat, float* s_z,
+                             float* s_h) {
   Ctrl* c = a.ctrl;
   int act = 0;
   uint32_t q_size = 0, q_edges = 0, next = 0, nedges = 0;
@@ -1693,9 +1801,11 @@ __device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned
     q_size = c->q_size;
     q_edges = c->q_edges;
     bu = c->bottom_up != 0u;
+    // a bottom-up level counted its claims in the partial rows (it queues nothing); a
+    // top-down level's queue counter holds them
     const unsigned long long np = c->next_packed;
-    next = (uint32_t)(np >> 32);
-    nedges = (uint32_t)np;
+    next = bu ? (uint32_t)nx : (uint32_t)(np >> 32);
+    nedges = bu ? (uint32_t)ne : (uint32_t)np;
     c->found += next;
     c->level += 1;
     c->closes += 1;
@@ -1736,7 +1846,7 @@ __device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned
 // registers first, so their latency overlaps the partial sums.
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts) {
-  __shared__ unsigned long long s_sum[2][2][kBlock / 64];
+  __shared__ unsigned long long s_sum[2][4][kBlock / 64];
   __shared__ float s_feat[2][kMlpIn];
   __shared__ float s_z[2][kMlpIn];
   __shared__ float s_h[2][kMlpH];
@@ -1750,36 +1860,22 @@ __global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts)
   if (w == 0 && learned0) mlp_lane_load(p.s[0].mlp, m);
   if (NS == 2 && w == 1 && learned1) mlp_lane_load(p.s[1].mlp, m);
   const int rows = (int)min((uint32_t)nparts, u0 + u1);
-  unsigned long long e0 = 0, i0 = 0, e1 = 0, i1 = 0;
-  for (int r = threadIdx.x; r < rows; r += kBlock) {
-    e0 += p.s[0].part[4 * r];
-    i0 += p.s[0].part[4 * r + 1];
-    if (NS == 2) {
-      e1 += p.s[1].part[4 * r];
-      i1 += p.s[1].part[4 * r + 1];
+  unsigned long long acc[2][4] = {{0ull, 0ull, 0ull, 0ull}, {0ull, 0ull, 0ull, 0ull}};
+  for (int r = threadIdx.x; r < rows; r += kBlock)
+    for (int s = 0; s < NS; ++s)
+      for (int k = 0; k < 4; ++k) acc[s][k] += p.s[s].part[4 * r + k];
+  for (int s = 0; s < NS; ++s)
+    for (int k = 0; k < 4; ++k) {
+      const unsigned long long x = wave_sum64(acc[s][k]);
+      if (lane() == 0) s_sum[s][k][w] = x;
     }
-  }
-  e0 = wave_sum64(e0);
-  i0 = wave_sum64(i0);
-  if (NS == 2) {
-    e1 = wave_sum64(e1);
-    i1 = wave_sum64(i1);
-  }
-  if (lane() == 0) {
-    s_sum[0][0][w] = e0;
-    s_sum[0][1][w] = i0;
-    s_sum[1][0][w] = e1;
-    s_sum[1][1][w] = i1;
-  }
   __syncthreads();  // also: every thread has read the control blocks
   if (w < NS && (w == 0 ? u0 : u1)) {
-    unsigned long long ex = 0, in = 0;
-    for (int k = 0; k < kBlock / 64; ++k) {
-      ex += s_sum[w][0][k];
-      in += s_sum[w][1][k];
-    }
-    if (w == 0) close_search(p.s[0], ex, in, learned0, m, s_feat[0], s_z[0], s_h[0]);
-    else close_search(p.s[1], ex, in, learned1, m, s_feat[1], s_z[1], s_h[1]);
+    unsigned long long tot[4] = {0ull, 0ull, 0ull, 0ull};
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < kBlock / 64; ++j) tot[k] += s_sum[w][k][j];
+    if (w == 0) close_search(p.s[0], tot[0], tot[1], tot[2], tot[3], learned0, m, s_feat[0], s_z[0], s_h[0]);
+    else close_search(p.s[1], tot[0], tot[1], tot[2], tot[3], learned1, m, s_feat[1], s_z[1], s_h[1]);
   }
 }
 
@@ -2088,6 +2184,7 @@ __global__ void k_trim2_init(LevelArgs a) {
     c->cur = 0u;
     c->bottom_up = 0u;
     c->bu_wide = 0u;
+    c->from_bits = 0u;
     c->switched_back = 0u;
     c->done = 0u;
     c->level = 0u;

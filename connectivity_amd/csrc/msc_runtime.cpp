@@ -46,6 +46,7 @@ struct WsView {
   // bitmaps: valid set, three search bitmaps (claimable / fw set / claimable), worklists
   uint32_t *valid = nullptr, *b1 = nullptr, *b2 = nullptr, *b3 = nullptr, *inq0 = nullptr, *inq1 = nullptr;
   uint32_t *s1 = nullptr, *s3 = nullptr;  // the searches' unvisited sets (b1's and b3's)
+  uint32_t *f1 = nullptr, *f3 = nullptr;  // the searches' last bottom-up claims (b1's and b3's)
   int32_t *labels = nullptr, *colors = nullptr, *cmin = nullptr;
   Frontier fr0{}, fr1{}, fr2{}, fr3{};  // fr2 / fr3: the backward search's (beside the forward one)
   int32_t* lists[3] = {nullptr, nullptr, nullptr};  // vertex lists of the colouring rounds
@@ -133,7 +134,7 @@ int64_t max_grid(const msc_ctx* c) {
   return g < kMaxParts ? g : kMaxParts;
 }
 
-constexpr int kWsParts = 32;
+constexpr int kWsParts = 34;
 
 // Byte sizes of the workspace parts for nn vertices, ee walked edges and `grid` colour-level
 // workgroups, in carving order; returns the total with every part 256-byte aligned.
@@ -155,6 +156,7 @@ size_t ws_layout(int64_t nn, int64_t ee, int64_t grid, size_t sizes[kWsParts]) {
       (size_t)nn * 4, (size_t)nn * 4, (size_t)nn * 4,           // colouring vertex lists
       (size_t)kLocalStride * 4 * (size_t)grid,                  // local-round frontiers
       nw * 4, nw * 4,                                           // the searches' unvisited sets
+      nw * 4, nw * 4,                                           // ... and their last bottom-up claims
   };
   size_t total = 0;
   for (int i = 0; i < kWsParts; ++i) {
@@ -200,6 +202,8 @@ void ws_carve(WsView* w, void* base, const size_t sizes[kWsParts], size_t total,
   w->loc = (int32_t*)parts[29];
   w->s1 = (uint32_t*)parts[30];
   w->s3 = (uint32_t*)parts[31];
+  w->f1 = (uint32_t*)parts[32];
+  w->f3 = (uint32_t*)parts[33];
   w->cap_n = nn;
   w->cap_e = ee;
 }
@@ -733,7 +737,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       fw.op = OP_FW;
       fw.down_off = g.out_off; fw.down_adj = g.out_adj;
       fw.up_off = g.in_off; fw.up_adj = g.in_adj;
-      fw.allow = c->valid; fw.cand = c->b1; fw.seen = c->s1;
+      fw.allow = c->valid; fw.cand = c->b1; fw.seen = c->s1; fw.fresh = c->f1;
       // BW: top-down on in-lists, bottom-up on out-lists.  Searched together with FW (the
       // default) it searches the valid set with its own control block, frontiers and
       // bitmap, both advancing one level per k_reach launch, and the SCC is FW set & BW
@@ -745,7 +749,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bw.down_off = g.in_off; bw.down_adj = g.in_adj;
       bw.up_off = g.out_off; bw.up_adj = g.out_adj;
       if (c->tune.concurrent_fwbw) {
-        bw.allow = c->valid; bw.cand = c->b3; bw.seen = c->s3;
+        bw.allow = c->valid; bw.cand = c->b3; bw.seen = c->s3; bw.fresh = c->f3;
         bw.ctrl = &cs[1]; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
@@ -781,7 +785,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
           // control block [0] again
           launch_andnot(L, c->b2, c->b3, c->valid, c->b1, nw);
           launch_fill_u32(L, c->s3, 0xffffffffu, nw);
-          bw.allow = c->b2; bw.cand = c->b3; bw.seen = c->s3;
+          bw.allow = c->b2; bw.cand = c->b3; bw.seen = c->s3; bw.fresh = c->f3;
           if (bt) launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
           else launch_search_init(L, bw, pivot, 1);
           sa[0] = bw;
@@ -803,7 +807,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bf.down_off2 = g.in_off; bf.down_adj2 = g.in_adj;
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
-      bf.allow = c->valid; bf.cand = c->b1; bf.seen = c->s1;
+      bf.allow = c->valid; bf.cand = c->b1; bf.seen = c->s1; bf.fresh = c->f1;
       if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
       else launch_search_init(L, bf, pivot, 2);
       sa[0] = bf;
