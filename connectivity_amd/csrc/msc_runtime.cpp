@@ -964,6 +964,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     int e;
     if ((++c->epoch & 0xffffu) == 0u) ++c->epoch;
     const uint32_t ep = c->epoch;
+    // nothing of this context runs now (every earlier batch ended with a synchronisation), so
+    // the word can be cleared: a value left by an earlier call can never carry this tag, even
+    // once the 16-bit tags wrap
+    __atomic_store_n(c->prog_host, 0ull, __ATOMIC_RELEASE);
     for (int rr = from; rr < from + count; ++rr) {
       if (rr >= from + kRoundLead && !device_admitted(c, ep, (uint64_t)(rr - kRoundLead) + 1u)) break;
       // only the rounds the host may wait for publish (a store to host memory per round)
