@@ -253,8 +253,9 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
-// canonical labels (minimum member per component); gate: only once *gate != 0 (null: always)
-void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate);
+// canonical labels (minimum member per component); gate: only once *gate != 0 (null: always).
+// Without `apply` only the minima: scratch[l] = smallest v with labels[v] == l.
+void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply);
 // pivot: part col 3 = max key; results via launch_reduce then launch_pivot_first; with a
 // mask, among the vertices it holds only
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,
@@ -280,8 +281,9 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
 int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
                         const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
                         const LevelArgs* seed, int32_t* list);
-void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng,
-                     const uint32_t* gate);
+// labels[v] = (mins ? mins[labels[v]] : labels[v]) - base of v's graph, once *gate != 0
+void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int32_t* mins, const int64_t* base,
+                     uint32_t ng, const uint32_t* gate);
 // run only after launch_validate passed (reads adjacency through the offsets)
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,
                            uint32_t ng, uint32_t* err);

@@ -2509,19 +2509,29 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
   }
 }
 
-// Union ids -> ids local to each graph: labels[v] -= base[graph of v] (labels are vertices
-// of the same graph).
+// Final labels of a batch, once *gate != 0: labels[v] = mins[labels[v]] (canonical; mins
+// null: raw) minus the base of v's graph (its own ids).  Each wave takes a contiguous run of
+// 64-vertex chunks and finds the graph of its first vertex with one binary search over
+// `base`; from there every lane steps forward over the graph boundaries its vertices cross
+// (graphs are contiguous id ranges in order), so no vertex needs a search of its own.
 __global__ void __launch_bounds__(kBlock) k_localise(int32_t n, int32_t* __restrict__ labels,
+                                                     const int32_t* __restrict__ mins,
                                                      const int64_t* __restrict__ base, uint32_t ng,
                                                      const uint32_t* gate) {
   if (gate && !*gate) return;
-  for (int64_t wb = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63); wb < n; wb += (int64_t)gridDim.x * kBlock) {
-    const int64_t v = wb + lane();
-    const uint32_t g = wave_graph_of(base, ng, wb, n);
-    if (v < n) {
-      int32_t l = labels[v];
-      if (l >= 0) labels[v] = (int32_t)((int64_t)l - base[g]);
-    }
+  const int64_t chunks = ((int64_t)n + 63) / 64;
+  const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t per = (chunks + waves - 1) / waves;
+  const int64_t w = (int64_t)blockIdx.x * (kBlock / 64) + wave_id();
+  const int64_t c0 = w * per, c1 = c0 + per < chunks ? c0 + per : chunks;
+  if (c0 >= c1) return;
+  uint32_t g = graph_of(base, 0u, ng, c0 * 64);
+  for (int64_t c = c0; c < c1; ++c) {
+    const int64_t v = c * 64 + lane();
+    if (v >= n) break;
+    while (base[g + 1] <= v) ++g;  // base[ng] = n > v ends it
+    const int32_t l = labels[v];
+    if (l >= 0) labels[v] = (int32_t)((int64_t)(mins ? mins[l] : l) - base[g]);
   }
 }
 
@@ -2745,12 +2755,12 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
   hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, 0u);
 }
 
-void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate) {
+void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply) {
   if (n <= 0) return;
   dim3 g(grid_for(n, kBlock, L.grid * 4));
   hipLaunchKernelGGL(k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n, gate);
   hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
-  hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
+  if (apply) hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
 }
 
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part,
@@ -2795,11 +2805,11 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
   hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, nullptr, 0u);
 }
 
-void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int64_t* base, uint32_t ng,
-                     const uint32_t* gate) {
+void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int32_t* mins, const int64_t* base,
+                     uint32_t ng, const uint32_t* gate) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_localise, dim3(grid_for(n, kBlock * 4, L.grid * 4)), dim3(kBlock), 0, L.stream, n, labels,
-                     base, ng, gate);
+  hipLaunchKernelGGL(k_localise, dim3(grid_for(n, kBlock * 16, L.grid * 2)), dim3(kBlock), 0, L.stream, n, labels,
+                     mins, base, ng, gate);
 }
 
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,

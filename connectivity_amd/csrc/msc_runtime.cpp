@@ -980,8 +980,11 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       Timed t(c, MSC_K_OTHER);
       if (fast_canon && cc) launch_cc_cmin(L, list_of(0), rcd, n, labels, c->cmin);
       if (fast_canon) launch_canon_list(L, list_of(0), rcd, n, labels, c->cmin);
-      if (o->label_mode == MSC_LABELS_CANONICAL && !fast_canon) launch_canon(L, n, labels, c->colors, done_gate);
-      if (bt) launch_localise(L, n, labels, c->bbase, ng, done_gate);
+      // batches: the minimum member of every label class, then one pass that applies it
+      // and turns union ids into each graph's own ids
+      const bool gen_canon = o->label_mode == MSC_LABELS_CANONICAL && !fast_canon;
+      if (gen_canon) launch_canon(L, n, labels, c->colors, done_gate, !bt);
+      if (bt) launch_localise(L, n, labels, gen_canon ? c->colors : nullptr, c->bbase, ng, done_gate);
     }
     HIP_TRY(hipGetLastError());
     return MSC_OK;
