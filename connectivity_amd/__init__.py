@@ -73,7 +73,7 @@ class _Opts(C.Structure):
                 ("policy", C.c_int32), ("label_mode", C.c_int32), ("labels_where", C.c_int32),
                 ("device", C.c_int32), ("validate", C.c_int32), ("timing", C.c_int32),
                 ("record", C.c_int32), ("ctx", C.c_void_p), ("schedule", C.c_void_p),
-                ("schedule_len", C.c_int32), ("fwbw_rounds", C.c_int32)]
+                ("schedule_len", C.c_int32), ("fwbw_rounds", C.c_int32), ("replay", C.c_int32)]
 
 
 class _KStat(C.Structure):
@@ -504,7 +504,7 @@ def schedule_array(fw=None, bw=None, cc=None) -> np.ndarray:
 
 
 def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alpha=15.0, beta=25,
-          device=0, schedule=None, trim_level=1) -> _Opts:
+          device=0, schedule=None, trim_level=1, replay=False) -> _Opts:
     o = _Opts()
     _lib().msc_opts_default(C.byref(o))
     if schedule is not None:
@@ -520,11 +520,12 @@ def _opts(policy, labels, timing, record, validate, ctx, labels_where=HOST, alph
     o.validate = 1 if validate else 0
     o.ctx = ctx.handle if ctx is not None else None
     o.device = ctx.device if ctx is not None else device
+    o.replay = 1 if replay else 0
     return o
 
 
 def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx, alpha, beta,
-         out, schedule, batch=None, trim_level=1, fwbw_rounds=1):
+         out, schedule, batch=None, trim_level=1, fwbw_rounds=1, replay=False):
     n = g.n
     where = HOST
     if isinstance(out, DeviceLabels):
@@ -542,7 +543,7 @@ def _run(fn_name, g, max_deg_vert, avg_degree, policy, labels, timing, record, v
     if schedule is not None:
         schedule = np.ascontiguousarray(schedule, dtype=np.int8)
     o = _opts(policy, labels, timing, record, validate, ctx, labels_where=where, alpha=alpha, beta=beta,
-              schedule=schedule, trim_level=trim_level)
+              schedule=schedule, trim_level=trim_level, replay=replay)
     o.fwbw_rounds = int(fwbw_rounds)
     st = _Stats()
     csr = g.csr()
@@ -564,12 +565,12 @@ class Prepared:
     ``batch``: (vert_base, pivots) for the batched entry points; ``out``: DeviceLabels."""
 
     def __init__(self, kind, g, max_deg_vert=None, avg_degree=None, *, policy=POLICY_PLAIN,
-                 labels=LABELS_CANONICAL, ctx: Context, out: "DeviceLabels", batch=None):
+                 labels=LABELS_CANONICAL, ctx: Context, out: "DeviceLabels", batch=None, replay=False):
         if out.n != g.n or out.ctx is not ctx:
             raise ValueError("out must be a DeviceLabels of the graph's size on the same context")
         self._keep = (g, out)
         self._csr = g.csr()
-        self._opts = _opts(policy, labels, False, False, False, ctx, labels_where=DEVICE)
+        self._opts = _opts(policy, labels, False, False, False, ctx, labels_where=DEVICE, replay=replay)
         self._dst = out.ptr.value
         piv = -1 if max_deg_vert is None else int(max_deg_vert)
         avg = -1.0 if avg_degree is None else float(avg_degree)
@@ -596,7 +597,8 @@ def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None,
             policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
             record: bool = False, validate: bool = False, ctx: Context | None = None,
             alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False,
-            schedule: np.ndarray | None = None, trim_level: int = 1, fwbw_rounds: int = 1):
+            schedule: np.ndarray | None = None, trim_level: int = 1, fwbw_rounds: int = 1,
+            replay: bool = False):
     """SCC labels of ``g`` (a Graph or DeviceGraph) on the GPU.
 
     run_scc (multistep/scc_main.cpp:210).  ``max_deg_vert`` / ``avg_degree`` default to
@@ -604,9 +606,10 @@ def run_scc(g, max_deg_vert: int | None = None, avg_degree: float | None = None,
     ``stats=True``.  ``schedule`` (see :func:`schedule_array`) forces per-level decisions.
     ``trim_level`` 2 is the reference's complete trim (TRIM_LEVEL 2, scc_trim.cpp:141-297).
     ``fwbw_rounds`` > 1 repeats FW-BW from new pivots while the SCC found is large
-    (msc_opts.fwbw_rounds); the labels do not change."""
+    (msc_opts.fwbw_rounds); the labels do not change.  ``replay`` lets the context capture the
+    call's launches in a HIP graph and replay it on later identical calls (msc_opts.replay)."""
     lab, st = _run("msc_scc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
-                   alpha, beta, out, schedule, trim_level=trim_level, fwbw_rounds=fwbw_rounds)
+                   alpha, beta, out, schedule, trim_level=trim_level, fwbw_rounds=fwbw_rounds, replay=replay)
     return (lab, st) if stats else lab
 
 
@@ -626,10 +629,10 @@ def run_cc(g, max_deg_vert: int | None = None, avg_degree: float | None = None, 
            policy: int = POLICY_PLAIN, labels: int = LABELS_CANONICAL, timing: bool = False,
            record: bool = False, validate: bool = False, ctx: Context | None = None,
            alpha: float = 15.0, beta: int = 25, out: np.ndarray | None = None, stats: bool = False,
-           schedule: np.ndarray | None = None):
+           schedule: np.ndarray | None = None, replay: bool = False):
     """Weakly connected components (run_cc, multistep/cc_main.cpp:184, on the mirror)."""
     lab, st = _run("msc_cc_run", g, max_deg_vert, avg_degree, policy, labels, timing, record, validate, ctx,
-                   alpha, beta, out, schedule)
+                   alpha, beta, out, schedule, replay=replay)
     return (lab, st) if stats else lab
 
 

@@ -173,7 +173,8 @@ struct ReachArgs {
 
 struct Launch {
   hipStream_t stream;
-  int grid;  // workgroups for level kernels
+  int grid;          // workgroups for level kernels
+  bool dry = false;  // the launches are already in a captured graph that replays them: skip
 };
 
 // ---- context accessors for the policy code (msc_runtime.cpp) ----

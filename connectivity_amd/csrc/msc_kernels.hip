@@ -2555,6 +2555,13 @@ __global__ void __launch_bounds__(kBlock) k_validate_batch(int32_t n, const uint
   if (bad) atomicOr(err, 1u);
 }
 
+// Every launch goes through here: while the runtime replays a HIP graph captured from the
+// same launches (msc_opts.replay), the launchers only rebuild the host-side state.
+#define MSC_LAUNCH(L, ...)                            \
+  do {                                                \
+    if (!(L).dry) hipLaunchKernelGGL(__VA_ARGS__); \
+  } while (0)
+
 inline int grid_for(int64_t items, int64_t per_block, int cap) {
   int64_t b = (items + per_block - 1) / per_block;
   if (b < 1) b = 1;
@@ -2569,7 +2576,7 @@ inline int grid_for(int64_t items, int64_t per_block, int cap) {
 
 void launch_fill_u32(const Launch& L, uint32_t* p, uint32_t value, int64_t count) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(count, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, p, value,
+  MSC_LAUNCH(L, k_fill_u32, dim3(grid_for(count, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, p, value,
                      count);
 }
 
@@ -2578,45 +2585,45 @@ int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* oo, con
                 unsigned long long* part) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
   if (mirror)
-    hipLaunchKernelGGL(k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, seen1,
+    MSC_LAUNCH(L, k_trim<true>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, seen1,
                        seen2, labels, part);
   else
-    hipLaunchKernelGGL(k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, seen1,
+    MSC_LAUNCH(L, k_trim<false>, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, valid, cand1, cand2, seen1,
                        seen2, labels, part);
   return g;
 }
 
 void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, L.stream, part, nparts, out);
+  MSC_LAUNCH(L, k_reduce, dim3(1), dim3(kBlock), 0, L.stream, part, nparts, out);
 }
 
 void launch_trim2_init(const Launch& L, const LevelArgs& a) {
-  hipLaunchKernelGGL(k_trim2_init, dim3(grid_for(a.n_total > a.nwords ? a.n_total : a.nwords, kBlock, L.grid * 4)),
+  MSC_LAUNCH(L, k_trim2_init, dim3(grid_for(a.n_total > a.nwords ? a.n_total : a.nwords, kBlock, L.grid * 4)),
                      dim3(kBlock), 0, L.stream, a);
 }
 
 void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
   ReachArgs p{};
   p.s[0] = a;
-  hipLaunchKernelGGL(k_search_init, dim3(1), dim3(64), 0, L.stream, p, 1, root, phase);
+  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(64), 0, L.stream, p, 1, root, phase);
 }
 
 void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root) {
   ReachArgs p{};
   p.s[0] = a0;
   p.s[1] = a1;
-  hipLaunchKernelGGL(k_search_init, dim3(1), dim3(128), 0, L.stream, p, 2, root, 0);
+  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(128), 0, L.stream, p, 2, root, 0);
 }
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   dim3 g((unsigned)L.grid), b(kBlock);
   switch (op) {
     case OP_PUSH:
-      if (mirror) hipLaunchKernelGGL((k_level<OP_PUSH, true>), g, b, 0, L.stream, a);
-      else hipLaunchKernelGGL((k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
+      if (mirror) MSC_LAUNCH(L, (k_level<OP_PUSH, true>), g, b, 0, L.stream, a);
+      else MSC_LAUNCH(L, (k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
       break;
-    case OP_CBW: hipLaunchKernelGGL((k_level<OP_CBW, false>), g, b, 0, L.stream, a); break;
-    case OP_TRIM2: hipLaunchKernelGGL(k_trim2, g, b, 0, L.stream, a); break;
+    case OP_CBW: MSC_LAUNCH(L, (k_level<OP_CBW, false>), g, b, 0, L.stream, a); break;
+    case OP_TRIM2: MSC_LAUNCH(L, k_trim2, g, b, 0, L.stream, a); break;
     default: break;
   }
 }
@@ -2625,25 +2632,25 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   dim3 g(1), b(kBlock);
   switch (op) {
     case OP_PUSH:
-      if (mirror) hipLaunchKernelGGL((k_close_color<OP_PUSH, true>), g, b, 0, L.stream, a, L.grid);
-      else hipLaunchKernelGGL((k_close_color<OP_PUSH, false>), g, b, 0, L.stream, a, L.grid);
+      if (mirror) MSC_LAUNCH(L, (k_close_color<OP_PUSH, true>), g, b, 0, L.stream, a, L.grid);
+      else MSC_LAUNCH(L, (k_close_color<OP_PUSH, false>), g, b, 0, L.stream, a, L.grid);
       break;
-    case OP_CBW: hipLaunchKernelGGL((k_close_color<OP_CBW, false>), g, b, 0, L.stream, a, L.grid); break;
-    case OP_TRIM2: hipLaunchKernelGGL(k_trim2_close, g, dim3(64), 0, L.stream, a.ctrl); break;
+    case OP_CBW: MSC_LAUNCH(L, (k_close_color<OP_CBW, false>), g, b, 0, L.stream, a, L.grid); break;
+    case OP_TRIM2: MSC_LAUNCH(L, k_trim2_close, g, dim3(64), 0, L.stream, a.ctrl); break;
     default: break;
   }
 }
 
 void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns) {
   dim3 g((unsigned)L.grid), b(kBlock);
-  if (ns == 2) hipLaunchKernelGGL((k_reach<2, false>), g, b, 0, L.stream, p);
-  else if (mirror) hipLaunchKernelGGL((k_reach<1, true>), g, b, 0, L.stream, p);
-  else hipLaunchKernelGGL((k_reach<1, false>), g, b, 0, L.stream, p);
+  if (ns == 2) MSC_LAUNCH(L, (k_reach<2, false>), g, b, 0, L.stream, p);
+  else if (mirror) MSC_LAUNCH(L, (k_reach<1, true>), g, b, 0, L.stream, p);
+  else MSC_LAUNCH(L, (k_reach<1, false>), g, b, 0, L.stream, p);
 }
 
 void launch_close_reach(const Launch& L, const ReachArgs& p, int ns) {
-  if (ns == 2) hipLaunchKernelGGL(k_close_reach<2>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
-  else hipLaunchKernelGGL(k_close_reach<1>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
+  if (ns == 2) MSC_LAUNCH(L, k_close_reach<2>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
+  else MSC_LAUNCH(L, k_close_reach<1>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
 }
 
 // Both pointers set: no member (ca = valid), one bitmap (cb = ca) or two.
@@ -2663,21 +2670,21 @@ int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t 
   int g;
   if (((uintptr_t)labels & 15u) == 0 && !members_only) {
     g = grid_for(n, kBlock * 64, L.grid * 2);
-    hipLaunchKernelGGL((k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
+    MSC_LAUNCH(L, (k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
                        labels, label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, 0);
   } else {
     g = grid_for(n, kBlock * 16, L.grid * 2);
-    hipLaunchKernelGGL(k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
+    MSC_LAUNCH(L, k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
                        label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, members_only ? 1 : 0);
   }
   if (seed)
-    hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
+    MSC_LAUNCH(L, k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);
   return g;
 }
 
 void launch_seed_bits(const Launch& L, const LevelArgs& a, int32_t n, int32_t* list) {
-  hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
+  MSC_LAUNCH(L, k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                      L.stream, a, list);
 }
 
@@ -2686,10 +2693,10 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
                         const LevelArgs* seed, int32_t* list) {
   int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
-  hipLaunchKernelGGL(k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels, 0,
+  MSC_LAUNCH(L, k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels, 0,
                      nullptr, piv, base, ng, part, a.rc, nullptr, 0);
   if (seed)
-    hipLaunchKernelGGL(k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
+    MSC_LAUNCH(L, k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);
   return g;
 }
@@ -2697,7 +2704,7 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
 int launch_min_member(const Launch& L, const MemberSrc& m, const uint32_t* valid, int64_t nwords,
                       unsigned long long* part, const RoundCtl* rc, bool largest) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part,
+  MSC_LAUNCH(L, k_min_member, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, nwords, part,
                      rc, largest ? 1 : 0);
   return g;
 }
@@ -2706,79 +2713,79 @@ void launch_cc_cmin(const Launch& L, const int32_t* list, const RoundCtl* rc, in
                     int32_t* cmin) {
   if (bound <= 0) return;
   const dim3 g(grid_for(bound, kBlock, L.grid * 8));
-  hipLaunchKernelGGL(k_cc_cmin_init, g, dim3(kBlock), 0, L.stream, list, rc, labels, cmin);
-  hipLaunchKernelGGL(k_cc_cmin, g, dim3(kBlock), 0, L.stream, list, rc, labels, cmin);
+  MSC_LAUNCH(L, k_cc_cmin_init, g, dim3(kBlock), 0, L.stream, list, rc, labels, cmin);
+  MSC_LAUNCH(L, k_cc_cmin, g, dim3(kBlock), 0, L.stream, list, rc, labels, cmin);
 }
 
 void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc, int64_t bound, int32_t* labels,
                        const int32_t* cmin) {
   if (bound <= 0) return;
-  hipLaunchKernelGGL(k_canon_list, dim3(grid_for(bound, kBlock, L.grid * 8)), dim3(kBlock), 0, L.stream, list, rc,
+  MSC_LAUNCH(L, k_canon_list, dim3(grid_for(bound, kBlock, L.grid * 8)), dim3(kBlock), 0, L.stream, list, rc,
                      labels, cmin);
 }
 
 void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
                         bool first, bool single, unsigned long long* prog, uint32_t epoch) {
-  hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, s0, s1,
+  MSC_LAUNCH(L, k_round_begin, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, s0, s1,
                      (first ? 1 : 0) | (single ? 2 : 0), prog, epoch);
 }
 
 void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode) {
-  hipLaunchKernelGGL(k_rc_set, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, mode);
+  MSC_LAUNCH(L, k_rc_set, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, mode);
 }
 
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part) {
   int g = grid_for(nwords, kBlock * 4, L.grid * 2);
-  hipLaunchKernelGGL(k_count_bits, dim3(g), dim3(kBlock), 0, L.stream, bits, nwords, part);
+  MSC_LAUNCH(L, k_count_bits, dim3(g), dim3(kBlock), 0, L.stream, bits, nwords, part);
   return g;
 }
 
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words) {
   if (words <= 0) return;
-  hipLaunchKernelGGL(k_andnot, dim3(grid_for(words, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, out, out2, a,
+  MSC_LAUNCH(L, k_andnot, dim3(grid_for(words, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, out, out2, a,
                      b, words);
 }
 
 void launch_seed_end(const Launch& L, Ctrl* c, const RoundCtl* rc, uint32_t pull_min) {
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, c, rc, pull_min);
+  MSC_LAUNCH(L, k_close_seed, dim3(1), dim3(64), 0, L.stream, c, rc, pull_min);
 }
 
 void launch_seed_list(const Launch& L, const LevelArgs& a, const int32_t* in, int32_t* out, int64_t bound) {
-  hipLaunchKernelGGL(k_seed_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, in, out);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, a.pull_div ? a.pull_min : 0u);
+  MSC_LAUNCH(L, k_seed_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, in, out);
+  MSC_LAUNCH(L, k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, a.pull_div ? a.pull_min : 0u);
 }
 
 void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list, const Ctrl* gate, int64_t bound) {
-  hipLaunchKernelGGL(k_roots_begin, dim3(1), dim3(64), 0, L.stream, const_cast<RoundCtl*>(a.rc), gate, a.ctrl);
-  hipLaunchKernelGGL(k_roots_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, list, gate);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, 0u);
+  MSC_LAUNCH(L, k_roots_begin, dim3(1), dim3(64), 0, L.stream, const_cast<RoundCtl*>(a.rc), gate, a.ctrl);
+  MSC_LAUNCH(L, k_roots_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, list, gate);
+  MSC_LAUNCH(L, k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, 0u);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply) {
   if (n <= 0) return;
   dim3 g(grid_for(n, kBlock, L.grid * 4));
-  hipLaunchKernelGGL(k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n, gate);
-  hipLaunchKernelGGL(k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
-  if (apply) hipLaunchKernelGGL(k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
+  MSC_LAUNCH(L, k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n, gate);
+  MSC_LAUNCH(L, k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
+  if (apply) MSC_LAUNCH(L, k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
 }
 
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part,
                      const uint32_t* mask) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
-  hipLaunchKernelGGL(k_pivot_max, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part, mask);
+  MSC_LAUNCH(L, k_pivot_max, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part, mask);
   return g;
 }
 
 void launch_pivot_first(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* red,
                         const uint32_t* mask) {
-  hipLaunchKernelGGL(k_pivot_first, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, oo, io,
+  MSC_LAUNCH(L, k_pivot_first, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, oo, io,
                      red, mask);
 }
 
 int launch_pivot_cc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part) {
   int g = grid_for(n, kBlock * 8, L.grid * 2);
-  hipLaunchKernelGGL(k_pivot_cc, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part);
+  MSC_LAUNCH(L, k_pivot_cc, dim3(g), dim3(kBlock), 0, L.stream, n, oo, io, part);
   return g;
 }
 
@@ -2787,40 +2794,40 @@ void launch_batch_pivots(const Launch& L, bool cc, int32_t n, const uint32_t* oo
   launch_fill_u32(L, (uint32_t*)keys, 0u, 2 * (int64_t)ng);
   dim3 g(grid_for(n, kBlock * 8, L.grid * 2)), b(kBlock);
   if (cc) {
-    hipLaunchKernelGGL(k_bpivot_cc, g, b, 0, L.stream, n, oo, io, base, ng, keys);
-    hipLaunchKernelGGL(k_bpivot_cc_final, dim3(grid_for(ng, kBlock, L.grid)), b, 0, L.stream, ng, keys, piv);
+    MSC_LAUNCH(L, k_bpivot_cc, g, b, 0, L.stream, n, oo, io, base, ng, keys);
+    MSC_LAUNCH(L, k_bpivot_cc_final, dim3(grid_for(ng, kBlock, L.grid)), b, 0, L.stream, ng, keys, piv);
   } else {
     launch_fill_u32(L, (uint32_t*)piv, 0xffffffffu, ng);
-    hipLaunchKernelGGL(k_bpivot_max, g, b, 0, L.stream, n, oo, io, base, ng, keys);
-    hipLaunchKernelGGL(k_bpivot_first, g, b, 0, L.stream, n, oo, io, base, ng, keys, (uint32_t*)piv);
+    MSC_LAUNCH(L, k_bpivot_max, g, b, 0, L.stream, n, oo, io, base, ng, keys);
+    MSC_LAUNCH(L, k_bpivot_first, g, b, 0, L.stream, n, oo, io, base, ng, keys, (uint32_t*)piv);
   }
 }
 
 void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const int32_t* piv, uint32_t ng,
                         int32_t phase) {
-  hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, phase);
+  MSC_LAUNCH(L, k_ctrl_reset, dim3(1), dim3(64), 0, L.stream, a.ctrl, phase);
   dim3 g((unsigned)grid_for(ng, kBlock, L.grid));
-  if (mirror) hipLaunchKernelGGL(k_seed_pivots<true>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
-  else hipLaunchKernelGGL(k_seed_pivots<false>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
-  hipLaunchKernelGGL(k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, nullptr, 0u);
+  if (mirror) MSC_LAUNCH(L, k_seed_pivots<true>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
+  else MSC_LAUNCH(L, k_seed_pivots<false>, g, dim3(kBlock), 0, L.stream, a, piv, ng);
+  MSC_LAUNCH(L, k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, nullptr, 0u);
 }
 
 void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int32_t* mins, const int64_t* base,
                      uint32_t ng, const uint32_t* gate) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_localise, dim3(grid_for(n, kBlock * 16, L.grid * 2)), dim3(kBlock), 0, L.stream, n, labels,
+  MSC_LAUNCH(L, k_localise, dim3(grid_for(n, kBlock * 16, L.grid * 2)), dim3(kBlock), 0, L.stream, n, labels,
                      mins, base, ng, gate);
 }
 
 void launch_validate_batch(const Launch& L, int32_t n, const uint32_t* off, const int32_t* adj, const int64_t* base,
                            uint32_t ng, uint32_t* err) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_validate_batch, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, off,
+  MSC_LAUNCH(L, k_validate_batch, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, off,
                      adj, base, ng, err);
 }
 
 void launch_validate(const Launch& L, int32_t n, int64_t m, const uint32_t* off, const int32_t* adj, uint32_t* err) {
-  hipLaunchKernelGGL(k_validate, dim3(grid_for(m > n ? m : n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, m,
+  MSC_LAUNCH(L, k_validate, dim3(grid_for(m > n ? m : n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, n, m,
                      off, adj, err);
 }
 

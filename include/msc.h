@@ -94,6 +94,13 @@ typedef struct msc_opts {
    * the label colouring would give it, so every label is unchanged.  0 or 1: one round.
    * SCC of one graph with concurrent_fwbw only; ignored elsewhere. */
   int32_t fwbw_rounds;
+  /* Nonzero: the context may capture the call's launches in a HIP graph and replay that graph
+   * on later calls with the same arguments, tuning and workspace (no reference counterpart).
+   * Applies to one graph in device memory with a given pivot (max_deg_vert >= 0), labels in
+   * device memory, trim_level 1, one FW-BW round, searches side by side (tuning
+   * concurrent_fwbw = 1, SCC) and no timing, record, schedule or validate; other calls run
+   * as usual.  Labels and statistics are those of an ordinary call. */
+  int32_t replay;
 } msc_opts;
 #define MSC_SCHED_LEVELS 64
 
