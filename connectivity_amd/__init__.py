@@ -86,7 +86,7 @@ class _Stats(C.Structure):
                 ("color_rounds", C.c_int32), ("color_iters", C.c_int32),
                 ("fw_levels", C.c_int32), ("bw_levels", C.c_int32), ("bu_levels", C.c_int32),
                 ("cbw_levels", C.c_int32), ("host_syncs", C.c_int32), ("bytes", C.c_double),
-                ("k", _KStat * 7)]
+                ("k", _KStat * 7), ("color_tail", C.c_int32)]
 
 
 class _Verify(C.Structure):
@@ -107,11 +107,13 @@ class _Tuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("small_push_edges", C.c_int32),
                 ("small_cbw_edges", C.c_int32), ("first_batch", C.c_int32), ("max_batch", C.c_int32),
                 ("concurrent_fwbw", C.c_int32), ("search_batch", C.c_int32), ("local_rounds", C.c_int32),
-                ("local_us", C.c_int32), ("pull_div", C.c_int32)]
+                ("local_us", C.c_int32), ("pull_div", C.c_int32),
+                ("tail_vertices", C.c_int32)]
 
 
 TUNING_FIELDS = ("blocks_per_cu", "small_push_edges", "small_cbw_edges", "first_batch", "max_batch",
-                 "concurrent_fwbw", "search_batch", "local_rounds", "local_us", "pull_div")
+                 "concurrent_fwbw", "search_batch", "local_rounds", "local_us", "pull_div",
+                 "tail_vertices")
 
 
 class _TrainOpts(C.Structure):
@@ -479,6 +481,7 @@ class Stats:
     host_syncs: int
     bytes: float
     kernels: dict
+    color_tail: int
 
     @classmethod
     def _from(cls, s: _Stats) -> "Stats":

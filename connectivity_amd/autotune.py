@@ -1,6 +1,6 @@
 """Autotuner for a context's launch parameters (msc_tuning).
 
-The level loops have ten tunables that no rule of the reference fixes and that trade
+The level loops have eleven tunables that no rule of the reference fixes and that trade
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
@@ -13,7 +13,9 @@ against each other differently on shallow power-law graphs and on deep meshes:
   search_batch      levels of a FW-BW / CC search enqueued before its first host check;
   local_rounds      levels a workgroup of a colour launch continues on its own claims;
   local_us          ... while that launch has run for less than this many microseconds;
-  pull_div          a colour level over at least 1/pull_div of its round's edges pulls (0: never).
+  pull_div          a colour level over at least 1/pull_div of its round's edges pulls (0: never);
+  tail_vertices     a colouring of at most this many vertices is settled by one workgroup when
+                    the edges joining them fit its LDS (0: always colour in rounds).
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -46,6 +48,7 @@ CANDIDATES = {
     "local_rounds": [0, 4, 16, 32, 64],
     "local_us": [40, 80, 160, 240, 320],
     "pull_div": [0, 2, 4, 8, 16],
+    "tail_vertices": [0, 4096, 65536],
 }
 
 

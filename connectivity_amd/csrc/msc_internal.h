@@ -32,6 +32,16 @@ constexpr int kMaxParts = 8192;                // per-workgroup partial results
 constexpr uint32_t kLocalEdges = MSC_LOCAL_EDGES;
 constexpr int kLocalHalf = (int)(2 * kStage + kLocalEdges / kChunk + 2);
 constexpr int kLocalStride = 2 * kLocalHalf;
+// colour tail (k_tail_edges, k_tail_solve): one workgroup settles colouring round 0's list
+// when it has at most kTailV vertices (slots fit 16 bits), their lists at most kTailScan
+// entries, at most kTailE edges join two of them, those edges at most kTailN endpoints,
+// and the colouring of those takes at most kTailSweeps sweeps
+constexpr uint32_t kTailV = 65536;
+constexpr uint32_t kTailScan = 1u << 18;
+constexpr uint32_t kTailE = 8192;
+constexpr uint32_t kTailN = 8192;
+constexpr uint32_t kTailSweeps = 4096;
+constexpr int kTailBlock = 1024;
 constexpr int kMlpIn = 12, kMlpH = 32, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH * kMlpOut + kMlpOut;
 // feature standardisation (mean, std) follows the weights
@@ -109,7 +119,7 @@ struct RoundCtl {
   uint32_t roots_in;  // 1 + index of the last round whose roots ran (0: none)
   uint32_t error;     // a round removed nothing, or its push settled without a root
   uint32_t listed0;   // length of round 0's vertex list (the canonical pass walks it)
-  uint32_t pad_;
+  uint32_t tail;      // colour tail: 0 not run, 1 settled the colouring, 2 declined it
   unsigned long long rounds;        // rounds completed
   unsigned long long push_levels;   // push levels of the completed rounds
   unsigned long long cbw_levels;    // colour-BW levels of the completed rounds
@@ -118,6 +128,8 @@ struct RoundCtl {
   unsigned long long push_closes;   // closer launches of the last completed round's push
   unsigned long long cbw_closes;    // ... and of its colour-BW
   unsigned long long left;          // valid vertices after the last completed round
+  uint32_t tail_edges;              // colour tail: edges joining two listed vertices
+  uint32_t tail_nodes;              // ... and their endpoints
 };
 
 struct LevelArgs {
@@ -163,6 +175,8 @@ struct LevelArgs {
   uint32_t pull_div;          // colour push: a level over at least 1/pull_div of the round's
                               // edges runs as a pull level (0: never)
   uint32_t pull_min;          // ... and over at least this many edges
+  int32_t* slots;             // colouring round 0's seed: slots[v] = v's slot in the round's list
+                              // for the first kTailV slots (null: not written)
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or
@@ -254,6 +268,13 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
+// Colour tail (after colouring round 0's seed, which wrote a.slots): k_tail_edges collects
+// the edges joining two listed vertices into `pairs` (cap entries), k_tail_solve colours
+// them in one workgroup and writes the labels of every component with more than one
+// vertex (largest id; smallest with `canonical`), or declines (rc->tail = 2); a list of
+// more than max_vertices vertices is declined at once
+void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, uint32_t* pairs,
+                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical);
 // canonical labels (minimum member per component); gate: only once *gate != 0 (null: always).
 // Without `apply` only the minima: scratch[l] = smallest v with labels[v] == l.
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply);

@@ -293,10 +293,11 @@ __device__ __forceinline__ uint32_t stage_scan(Stage& S, uint32_t n) {
 
 // Writes the n scanned claims to frontier nx at queue slots [bh, bh + n) and edge offsets
 // from bl: ids, offsets, the chunk starts of every kChunk-edge chunk that begins inside a
-// claim's list, and the ids to `list` when given.  Empties the stage.  Call after a barrier
-// that follows stage_scan; ends with a barrier.
+// claim's list, and the ids to `list` when given (with `slots`, also slots[id] = slot for
+// the slots below kTailV).  Empties the stage.  Call after a barrier that follows
+// stage_scan; ends with a barrier.
 __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total, const Frontier& nx, uint32_t bh,
-                                            uint32_t bl, int32_t* list) {
+                                            uint32_t bl, int32_t* list, int32_t* slots) {
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
     uint32_t lo = S.d[i];
     uint32_t hi = i + 1 < n ? S.d[i + 1] : total;
@@ -304,6 +305,7 @@ __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total
     nx.q[pos] = S.v[i];
     nx.qo[pos] = off;
     if (list) list[pos] = S.v[i];
+    if (slots && pos < kTailV) slots[S.v[i]] = (int32_t)pos;
     uint64_t end = (uint64_t)off + (hi - lo);
     for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch) nx.starts[ch] = pos;
   }
@@ -316,7 +318,8 @@ __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total
 // reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
 // offsets and chunk starts to frontier `nx` (and the ids to `list` at the same slots, when
 // given).  Always a barrier.
-__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr) {
+__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr,
+                            int32_t* slots = nullptr) {
   __syncthreads();
   const uint32_t n = S.cnt;
   __syncthreads();  // no wave may push again before every wave has read the count
@@ -328,7 +331,7 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
     S.base_lo = (uint32_t)b;
   }
   __syncthreads();
-  stage_write(S, n, total, nx, S.base_hi, S.base_lo, list);
+  stage_write(S, n, total, nx, S.base_hi, S.base_lo, list, slots);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1282,7 +1285,7 @@ __device__ void local_rounds(const LevelArgs& a, const Frontier& N, uint32_t* in
                               : Frontier{pb, (uint32_t*)(pb + kStage), (uint32_t*)(pb + 2 * kStage)};
     stage_scan(S, n);
     __syncthreads();
-    stage_write(S, n, total, P, 0u, 0u, nullptr);
+    stage_write(S, n, total, P, 0u, 0u, nullptr, nullptr);
     if (threadIdx.x == 0) {
       loc_n += n;
       loc_e += total;
@@ -2021,11 +2024,11 @@ __global__ void __launch_bounds__(kBlock) k_seed_bits(LevelArgs a, int32_t* __re
         deg = list_len(a.down_off, a.down_off2, v);
       }
       stage_push(S, take, v, deg);
-      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, list);  // also the barrier before the next round
+      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, list, a.slots);  // also the barrier before the next round
     }
     __syncthreads();  // B is reloaded next
   }
-  stage_flush(S, a.fr1, a.ctrl, 0u, list);
+  stage_flush(S, a.fr1, a.ctrl, 0u, list, a.slots);
 }
 
 // Colour roots (multistep/scc_color.cpp:165-212) among this round's vertex list (length
@@ -2057,6 +2060,321 @@ __global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_
     stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
+}
+
+// ------------------------------------------------------------------------------------
+// colour tail: when FW-BW leaves few vertices (R-MAT s24: 12 390, joined by 61 edges), one
+// workgroup settles all of them, instead of the ~30 launches of the colouring rounds
+// (multistep/scc_color.cpp:340-373 loops rounds; scc_main.cpp:257-262 hands the last
+// vert_cutoff vertices to a serial tail).  A listed vertex that no edge joins to another
+// one is its own component and already carries its id as label (the commit wrote it);
+// only the endpoints of the joining edges can share a component, and they are coloured in
+// LDS.
+
+// Colour tail, step 1 (one kChunk-edge chunk of the seeded frontier per workgroup and
+// iteration): every list entry u -> w of a listed vertex u whose target w is still valid
+// (so listed as well: the seed lists every valid vertex) and is not u itself is appended
+// to `pairs` as slot(u) << 16 | slot(w); rc->tail_edges counts them, also past `cap`.
+// With mirrored lists (CC) only the out-list half is walked: every joining edge is some
+// listed vertex's out-entry.  Runs only in an admitted round whose list fits the tail.
+__global__ void __launch_bounds__(kBlock) k_tail_edges(LevelArgs a, RoundCtl* rc, uint32_t* __restrict__ pairs,
+                                                       uint32_t cap, uint32_t maxv) {
+  __shared__ uint32_t own[kChunk];
+  __shared__ uint32_t s_part[kBlock / 64];
+  if (!rc->go) return;  // uniform
+  const Ctrl* c = a.ctrl;
+  const uint32_t nv = c->list_len, qe = c->q_edges;
+  if (nv == 0u || nv > maxv || qe > kTailScan) return;  // uniform: k_tail_solve declines
+  const int t = threadIdx.x;
+  const uint32_t nch = (qe + kChunk - 1) / kChunk;
+  for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const uint32_t cbase = ch * kChunk;
+    // the slots whose lists overlap the chunk, and the owner of every edge by a max-scan
+    // over their start positions (the first one starts at or before the chunk)
+    const uint32_t s0 = a.fr1.starts[ch];
+    const uint32_t s1 = ch + 1 < nch ? a.fr1.starts[ch + 1] : nv - 1;
+    for (int k = t; k < (int)kChunk; k += kBlock) own[k] = 0u;
+    __syncthreads();
+    for (uint32_t i = s0 + (uint32_t)t; i <= s1; i += kBlock) {
+      const uint32_t q = a.fr1.qo[i];
+      const uint32_t st = q > cbase ? q - cbase : 0u;
+      if (st < kChunk) atomicMax(&own[st], i);
+    }
+    __syncthreads();
+    uint32_t o[kPer];
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      run = max(run, own[t * kPer + k]);
+      o[k] = run;
+    }
+    const uint32_t incl = wave_incl_max(run);
+    uint32_t before = __shfl_up(incl, 1, 64);
+    if (lane() == 0) before = 0u;
+    if (lane() == 63) s_part[wave_id()] = incl;
+    __syncthreads();
+    for (int w = 0; w < wave_id(); ++w) before = max(before, s_part[w]);
+    int32_t v[kPer], w[kPer];
+    uint32_t j[kPer];
+    bool in[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      o[k] = max(o[k], before);
+      const uint32_t e = cbase + (uint32_t)(t * kPer + k);
+      in[k] = e < qe;
+      v[k] = in[k] ? a.fr1.q[o[k]] : 0;
+      j[k] = in[k] ? e - a.fr1.qo[o[k]] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      w[k] = -1;
+      if (in[k]) {
+        const uint32_t beg = a.down_off[v[k]], end = a.down_off[v[k] + 1];
+        if (j[k] < end - beg) w[k] = a.down_adj[beg + j[k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const bool ok = w[k] >= 0 && w[k] != v[k] && bit_test(a.valid, (uint32_t)w[k]);
+      const uint32_t ds = ok ? (uint32_t)a.slots[w[k]] : 0u;
+      const unsigned long long mask = __ballot(ok);
+      if (mask == 0ull) continue;  // uniform over the wave
+      const int leader = __ffsll((long long)mask) - 1;
+      uint32_t at = 0;
+      if (lane() == leader) at = atomicAdd(&rc->tail_edges, (uint32_t)__popcll(mask));
+      at = __shfl(at, leader, 64) + (uint32_t)__popcll(mask & lanes_below());
+      if (ok && at < cap) pairs[at] = (o[k] << 16) | ds;
+    }
+    __syncthreads();  // own and s_part are rewritten for the next chunk
+  }
+}
+
+struct TailLds {
+  uint32_t mark[kTailV / 32];  // list slots that are endpoints of a joining edge
+  uint32_t pre[kTailV / 32];   // exclusive prefix of mark's popcounts: node of a slot
+  uint32_t edge[kTailE];       // node(u) << 16 | node(w)
+  int32_t gid[kTailN];         // vertex id of every node
+  uint32_t col[kTailN];        // colour (a node); once settled, the node's final label
+  int32_t rl[kTailN];          // per root: its component's label (largest or smallest id)
+  uint8_t st[kTailN];          // bit 0: still valid, bit 1: in the current root's component
+  uint32_t part[kTailBlock / 64];
+  uint32_t flag[3];            // sweep i sets flag[i % 3] on a change and clears flag[(i + 1) % 3]
+  uint32_t count;
+};
+
+// One sweep's end: every thread learns whether the sweep changed anything.  flag[i % 3]
+// is cleared two sweeps after it was read, so one barrier per sweep suffices.
+__device__ __forceinline__ bool tail_sweep_end(TailLds& T, uint32_t i) {
+  __syncthreads();
+  return T.flag[i % 3u] != 0u;
+}
+
+// Colour tail, step 2 (one workgroup of kTailBlock threads): the components among the
+// endpoints of the joining edges k_tail_edges found.  SCC: rounds of max-colour push to a
+// fixpoint, roots (colour == self), and the per-colour backward search from the roots
+// (multistep/scc_color.cpp:47-289), each a sweep over the edges in LDS; colours are node
+// indices, and each component is labelled with its largest vertex id (raw: what colouring
+// gives) or its smallest (canonical).  CC: max-colour propagation over the edges both ways
+// (multistep/cc_color.cpp:47-175).  Labels are written only once everything is settled, so
+// a declined tail (too many edges or endpoints, or more than kTailSweeps sweeps) leaves
+// every buffer as the seed left it: rc->tail = 2, and the host runs the colouring rounds.
+// A settled tail ends the colouring: rc->tail = 1, rc->finished.
+template <bool CC>
+__global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, const Ctrl* push,
+                                                           const int32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ pairs, uint32_t cap,
+                                                           uint32_t maxv, int32_t* __restrict__ labels,
+                                                           int32_t canonical) {
+  __shared__ TailLds T;
+  if (!rc->go) return;  // uniform: round 0 was not admitted
+  const int t = threadIdx.x;
+  const uint32_t nv = push->list_len, qe = push->q_edges, ne = rc->tail_edges;
+  if (nv > maxv || qe > kTailScan || ne > kTailE || ne > cap) {  // uniform
+    if (t == 0) rc->tail = 2u;
+    return;
+  }
+  const uint32_t nwd = (nv + 31) / 32;
+  for (uint32_t i = t; i < nwd; i += kTailBlock) T.mark[i] = 0u;
+  if (t < 3) T.flag[t] = 0u;
+  if (t == 0) T.count = 0u;
+  __syncthreads();
+  for (uint32_t e = t; e < ne; e += kTailBlock) {
+    const uint32_t p = pairs[e], u = p >> 16, w = p & 0xffffu;
+    T.edge[e] = p;
+    atomicOr(&T.mark[u >> 5], 1u << (u & 31));
+    atomicOr(&T.mark[w >> 5], 1u << (w & 31));
+  }
+  __syncthreads();
+  {
+    // nodes: the marked slots in slot order (kTailV / 32 words, two per thread)
+    const uint32_t i0 = 2u * (uint32_t)t, i1 = i0 + 1u;
+    const uint32_t c0 = i0 < nwd ? (uint32_t)__popc(T.mark[i0]) : 0u;
+    const uint32_t c1 = i1 < nwd ? (uint32_t)__popc(T.mark[i1]) : 0u;
+    const uint32_t incl = wave_incl_sum(c0 + c1);
+    if (lane() == 63) T.part[wave_id()] = incl;
+    __syncthreads();
+    uint32_t carry = 0, total = 0;
+    for (int w = 0; w < kTailBlock / 64; ++w) {
+      if (w < wave_id()) carry += T.part[w];
+      total += T.part[w];
+    }
+    const uint32_t excl = incl - (c0 + c1) + carry;
+    if (i0 < nwd) T.pre[i0] = excl;
+    if (i1 < nwd) T.pre[i1] = excl + c0;
+    if (total > kTailN) {  // uniform
+      if (t == 0) rc->tail = 2u;
+      return;
+    }
+    if (t == 0) T.count = total;
+  }
+  __syncthreads();
+  const uint32_t nn = T.count;
+  for (uint32_t e = t; e < ne; e += kTailBlock) {
+    const uint32_t p = T.edge[e], u = p >> 16, w = p & 0xffffu;
+    const uint32_t nu = T.pre[u >> 5] + (uint32_t)__popc(T.mark[u >> 5] & ((1u << (u & 31)) - 1u));
+    const uint32_t nw = T.pre[w >> 5] + (uint32_t)__popc(T.mark[w >> 5] & ((1u << (w & 31)) - 1u));
+    T.edge[e] = (nu << 16) | nw;
+    T.gid[nu] = list[u];
+    T.gid[nw] = list[w];
+  }
+  for (uint32_t x = t; x < nn; x += kTailBlock) {
+    T.st[x] = 1u;
+    T.col[x] = x;
+  }
+  __syncthreads();
+  uint32_t sweeps = 0, push_sweeps = 0, bw_sweeps = 0, rounds = 0;
+  bool settled = true;
+  if (CC) {
+    for (;;) {
+      const uint32_t i = sweeps++;
+      if (t == 0) T.flag[(i + 1) % 3u] = 0u;
+      for (uint32_t e = t; e < ne; e += kTailBlock) {
+        const uint32_t p = T.edge[e], u = p >> 16, w = p & 0xffffu;
+        const uint32_t cu = T.col[u], cw = T.col[w];
+        if (cu > cw) {
+          atomicMax(&T.col[w], cu);
+          T.flag[i % 3u] = 1u;
+        } else if (cw > cu) {
+          atomicMax(&T.col[u], cw);
+          T.flag[i % 3u] = 1u;
+        }
+      }
+      ++push_sweeps;
+      if (!tail_sweep_end(T, i)) break;
+      if (sweeps >= kTailSweeps) {
+        settled = false;
+        break;
+      }
+    }
+    if (settled) {
+      rounds = 1;
+      for (uint32_t x = t; x < nn; x += kTailBlock) T.rl[x] = T.gid[x];
+      __syncthreads();
+      for (uint32_t x = t; x < nn; x += kTailBlock) {
+        if (canonical) atomicMin(&T.rl[T.col[x]], T.gid[x]);
+        else atomicMax(&T.rl[T.col[x]], T.gid[x]);
+      }
+      __syncthreads();
+      for (uint32_t x = t; x < nn; x += kTailBlock) T.col[x] = (uint32_t)T.rl[T.col[x]];
+      __syncthreads();
+    }
+  } else {
+    for (;;) {
+      ++rounds;
+      // push: the largest colour along u -> w among valid nodes, to the fixpoint
+      for (;;) {
+        const uint32_t i = sweeps++;
+        if (t == 0) T.flag[(i + 1) % 3u] = 0u;
+        for (uint32_t e = t; e < ne; e += kTailBlock) {
+          const uint32_t p = T.edge[e], u = p >> 16, w = p & 0xffffu;
+          if (T.st[u] & T.st[w] & 1u) {
+            const uint32_t cu = T.col[u];
+            if (cu > T.col[w]) {
+              atomicMax(&T.col[w], cu);
+              T.flag[i % 3u] = 1u;
+            }
+          }
+        }
+        ++push_sweeps;
+        if (!tail_sweep_end(T, i)) break;
+        if (sweeps >= kTailSweeps) {
+          settled = false;
+          break;
+        }
+      }
+      if (!settled) break;
+      // roots, then their components: valid u with an edge u -> w into the component and
+      // w's colour
+      for (uint32_t x = t; x < nn; x += kTailBlock) {
+        if (T.st[x] == 1u && T.col[x] == x) {
+          T.st[x] = 3u;
+          T.rl[x] = T.gid[x];
+        }
+      }
+      __syncthreads();
+      for (;;) {
+        const uint32_t i = sweeps++;
+        if (t == 0) T.flag[(i + 1) % 3u] = 0u;
+        for (uint32_t e = t; e < ne; e += kTailBlock) {
+          const uint32_t p = T.edge[e], u = p >> 16, w = p & 0xffffu;
+          if (T.st[w] == 3u && T.st[u] == 1u && T.col[u] == T.col[w]) {
+            T.st[u] = 3u;
+            T.flag[i % 3u] = 1u;
+          }
+        }
+        ++bw_sweeps;
+        if (!tail_sweep_end(T, i)) break;
+        if (sweeps >= kTailSweeps) {
+          settled = false;
+          break;
+        }
+      }
+      if (!settled) break;
+      for (uint32_t x = t; x < nn; x += kTailBlock) {
+        if (T.st[x] == 3u) {
+          if (canonical) atomicMin(&T.rl[T.col[x]], T.gid[x]);
+          else atomicMax(&T.rl[T.col[x]], T.gid[x]);
+        }
+      }
+      if (t == 0) T.count = 0u;
+      __syncthreads();
+      // the component found settles with its label; the rest start the next round
+      bool left = false;
+      for (uint32_t x = t; x < nn; x += kTailBlock) {
+        const uint32_t s = T.st[x];
+        if (s == 3u) {
+          T.col[x] = (uint32_t)T.rl[T.col[x]];
+          T.st[x] = 0u;
+        } else if (s == 1u) {
+          T.col[x] = x;
+          left = true;
+        }
+      }
+      if (left) T.count = 1u;
+      __syncthreads();
+      const bool more = T.count != 0u;
+      __syncthreads();  // T.count is cleared again before the next round's read
+      if (!more) break;
+    }
+  }
+  if (!settled) {  // uniform: nothing was written outside LDS
+    if (t == 0) rc->tail = 2u;
+    return;
+  }
+  for (uint32_t x = t; x < nn; x += kTailBlock) {
+    const int32_t lab = (int32_t)T.col[x];
+    if (lab != T.gid[x]) labels[T.gid[x]] = lab;
+  }
+  if (t == 0) {
+    rc->tail = 1u;
+    rc->tail_nodes = nn;
+    rc->rounds = rounds;
+    rc->push_levels = push_sweeps;
+    rc->cbw_levels = bw_sweeps;
+    rc->listed0 = nv;
+    rc->left = 0ull;
+    rc->finished = 1u;
+  }
 }
 
 // Canonical labels: every component relabelled to its minimum vertex id.  mins[] starts
@@ -2760,6 +3078,19 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
   MSC_LAUNCH(L, k_roots_begin, dim3(1), dim3(64), 0, L.stream, const_cast<RoundCtl*>(a.rc), gate, a.ctrl);
   MSC_LAUNCH(L, k_roots_list, dim3(grid_for(bound, kBlock, L.grid)), dim3(kBlock), 0, L.stream, a, list, gate);
   MSC_LAUNCH(L, k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, 0u);
+}
+
+void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, uint32_t* pairs,
+                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical) {
+  const uint32_t maxv = max_vertices < kTailV ? max_vertices : kTailV;
+  const int g = (int)(kTailScan / kChunk) < L.grid ? (int)(kTailScan / kChunk) : L.grid;
+  MSC_LAUNCH(L, k_tail_edges, dim3(g), dim3(kBlock), 0, L.stream, a, rc, pairs, cap, maxv);
+  if (cc)
+    MSC_LAUNCH(L, k_tail_solve<true>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, pairs, cap, maxv,
+               labels, canonical ? 1 : 0);
+  else
+    MSC_LAUNCH(L, k_tail_solve<false>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, pairs, cap, maxv,
+               labels, canonical ? 1 : 0);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply) {

@@ -66,7 +66,7 @@ struct msc_ctx : WsView {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2, 65536};
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
   float* mlp = nullptr;
@@ -959,6 +959,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.pull_min = floor_edges > 0xffffffffll ? 0xffffffffu : (uint32_t)floor_edges;
   }
   if (fast_canon && !cc) p.cmin = c->cmin;  // tracked by the colour-BW searches
+  // the colour tail (batches colour their union in rounds); its slot map borrows cmin,
+  // whose entries the rounds only read after writing them, and its edge list frontier 0's
+  // queue, which nothing reads before round 0's first push level
+  bool use_tail = c->tune.tail_vertices > 0 && !bt;
+  const uint32_t tail_cap = (uint32_t)(n < (int64_t)kTailE ? n : (int64_t)kTailE);
+  if (use_tail) p.slots = c->cmin;
   if (c->tune.local_rounds > 0) {
     p.loc = c->loc;
     p.local_rounds = c->tune.local_rounds;
@@ -981,8 +987,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // scc_color.cpp:56-63; cc_bfs.cpp:217-271 then cc_color.cpp:62-69), a later round seeds
   // from the previous round's list; then push levels, roots (scc_color.cpp:165-212) and
   // colour-BW levels (scc_color.cpp:214-289)
-  auto round_steps = [&](int rr) -> int {
-    int e;
+  auto seed_steps = [&](int rr) {
     if (rr == 0 && committed) {
       // the FW-BW rounds committed already: only the seed of colouring round 0
       Timed t(c, MSC_K_COMMIT);
@@ -1003,6 +1008,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       Timed t(c, MSC_K_ROOTS);
       launch_seed_list(L, p, list_of(rr - 1), list_of(rr), n);
     }
+  };
+  auto level_steps = [&](int rr) -> int {
+    int e;
     if ((e = enqueue_levels(c, OP_PUSH, cc, &p, 1, MSC_K_COLOR_PROP, push_batch))) return e;
     if (!cc) {
       {
@@ -1012,6 +1020,18 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if ((e = enqueue_levels(c, OP_CBW, false, &r, 1, MSC_K_COLOR_BW, cbw_batch))) return e;
     }
     return MSC_OK;
+  };
+  auto round_steps = [&](int rr) -> int {
+    seed_steps(rr);
+    return level_steps(rr);
+  };
+  // Colour tail: round 0's seed, then one workgroup settles the whole colouring if its
+  // joining edges fit; the host reads the outcome before it enqueues anything else.
+  auto tail_steps = [&]() {
+    seed_steps(0);
+    Timed t(c, MSC_K_COLOR_PROP);
+    launch_tail(L, cc, p, rcd, list_of(0), (uint32_t*)c->fr0.q, tail_cap, (uint32_t)c->tune.tail_vertices, labels,
+                fast_canon);
   };
   const uint32_t* done_gate = &rcd->finished;
   // rounds [from, from + count), the check that books the last of them, and the final
@@ -1032,6 +1052,11 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       // only the rounds the host may wait for publish (a store to host memory per round)
       launch_round_begin(L, rcd, p.ctrl, r.ctrl, rr == 0 ? s0 : nullptr, rr == 0 ? s1 : nullptr, rr == 0, false,
                          rr + kRoundLead < from + count ? c->prog_dev : nullptr, ep);
+      if (rr == 0 && use_tail) {
+        tail_steps();
+        HIP_TRY(hipGetLastError());
+        return MSC_OK;
+      }
       if ((e = round_steps(rr))) return e;
     }
     launch_round_begin(L, rcd, p.ctrl, r.ctrl, nullptr, nullptr, false, cc);
@@ -1086,6 +1111,14 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       cbw_batch = hr->cbw_closes > 2 ? (int)(hr->cbw_closes < (unsigned long long)mb ? hr->cbw_closes : mb) : 2;
     }
     ahead = ahead * 2 < 32 ? (ahead < 1 ? 1 : ahead * 2) : 32;
+    if (use_tail && hr->tail == 2u && !hr->stall) {
+      // the tail declined colouring round 0 (too many vertices or joining edges): its seed
+      // stands, so its levels and the following rounds run as usual
+      use_tail = false;
+      if ((rc = level_steps(0))) return rc;
+      if ((rc = enqueue_rounds(1, cc ? 0 : ahead))) return rc;
+      continue;
+    }
     int from;
     if (hr->stall && hr->admitted == 0) {
       // the searches needed more levels than enqueued: finish them, then start over
@@ -1119,6 +1152,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       // one fixpoint: start it over after the search, else just check it again
       if (from == 0) {
         launch_round_begin(L, rcd, p.ctrl, r.ctrl, s0, s1, true, false);
+        if (use_tail) {
+          tail_steps();
+          continue;
+        }
         if ((rc = round_steps(0))) return rc;
       }
       if ((rc = enqueue_rounds(1, 0))) return rc;
@@ -1126,6 +1163,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   }
   if (s) {
     s->color_rounds = (int32_t)hr->rounds;
+    s->color_tail = hr->tail == 1u ? 1 : 0;
     s->color_iters = (int32_t)hr->push_levels;
     s->cbw_levels = (int32_t)hr->cbw_levels;
     if (nsearch == 2) {
@@ -1139,6 +1177,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     // the later rounds' seeds (id, valid bit, colour, offsets, slot, list entry)
     s->k[MSC_K_ROOTS].bytes += 8.0 * (double)hr->listed_total + 24.0 * (double)hr->roots_total +
                                17.0 * (double)(hr->listed_total - hr->listed0);
+    // colour tail: per listed vertex its slot, edge offset and list bounds; per list entry
+    // the id and the target's valid word; per joining edge the target's slot and the pair
+    // written and read back, per endpoint its id and label
+    if (hr->tail == 1u)
+      s->k[MSC_K_COLOR_PROP].bytes += 16.0 * (double)hs[2].list_len + 8.0 * (double)hs[2].q_edges +
+                                      12.0 * (double)hr->tail_edges + 8.0 * (double)hr->tail_nodes;
   }
   if (o->labels_where == MSC_HOST)
     HIP_TRY(hipMemcpyAsync(labels_out, labels, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1311,7 +1355,8 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
       t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
-      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024)
+      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024 || t->tail_vertices < 0 ||
+      t->tail_vertices > (int32_t)kTailV)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;

@@ -135,6 +135,8 @@ typedef struct msc_stats {
   int32_t host_syncs;  /* host round trips taken by the level loops */
   double bytes;        /* algorithmic bytes over all kernels */
   msc_kstat k[MSC_K_COUNT];
+  int32_t color_tail;  /* 1: one workgroup settled the colouring (msc_tuning.tail_vertices);
+                          color_rounds / color_iters / cbw_levels then count its sweeps */
 } msc_stats;
 
 /* Defaults equal the reference's compile-time settings. */
@@ -222,6 +224,9 @@ typedef struct msc_tuning {
                                the round's list entries runs as a pull level: every valid vertex
                                takes the largest colour of its valid in-neighbours, no atomics
                                (0..1024; 0: push only) */
+  int32_t tail_vertices;    /* colouring round 0 with at most this many vertices is settled by
+                               one workgroup (joining edges coloured in LDS) instead of rounds of
+                               level launches, when its joining edges fit (0..65536; 0: never) */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);

@@ -14,6 +14,16 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
+@pytest.fixture(autouse=True, params=["tail", "rounds"])
+def colouring(request, gpu_ctx):
+    """Every test runs twice: with the colour tail (one workgroup settles a small colouring)
+    and with it off, so that the colouring rounds are checked on the same small graphs."""
+    saved = gpu_ctx.tuning()
+    gpu_ctx.set_tuning(tail_vertices=65536 if request.param == "tail" else 0)
+    yield request.param
+    gpu_ctx.set_tuning(**saved)
+
+
 def _say(msg):
     print(msg, flush=True)
 
@@ -192,8 +202,8 @@ def test_tuning_keeps_labels(msc, orc, gpu_ctx, tuning):
     _say(f"case: tuning {tuning}")
     defaults = gpu_ctx.tuning()
     try:
-        gpu_ctx.set_tuning(**tuning)
-        assert gpu_ctx.tuning() == tuning
+        got = gpu_ctx.set_tuning(**tuning)
+        assert {k: got[k] for k in tuning} == tuning
         for g in (msc.gen_grid(64, 64, seed=3), msc.gen_rmat(12, 8, seed=2),
                   msc.gen_planted(1 << 13, max_size=512, seed=4)[0]):
             _check_scc(msc, orc, gpu_ctx, g)
@@ -207,7 +217,8 @@ def test_tuning_rejects_bad_values(msc, gpu_ctx):
     for bad in (dict(blocks_per_cu=0), dict(blocks_per_cu=9), dict(small_push_edges=-1),
                 dict(first_batch=0), dict(first_batch=8, max_batch=4), dict(max_batch=257),
                 dict(concurrent_fwbw=2), dict(search_batch=0), dict(search_batch=257), dict(local_rounds=-1),
-                dict(local_rounds=257), dict(local_us=0), dict(pull_div=-1), dict(pull_div=1025)):
+                dict(local_rounds=257), dict(local_us=0), dict(pull_div=-1), dict(pull_div=1025),
+                dict(tail_vertices=-1), dict(tail_vertices=65537)):
         with pytest.raises(msc.MscError):
             gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == defaults
