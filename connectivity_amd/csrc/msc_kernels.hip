@@ -342,9 +342,8 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
 // (vertices of n, edges of avg * n), so a policy trained on small graphs sees the same
 // values on a large graph in the same state.
 __device__ void level_features(const Ctrl* c, const LevelArgs& a, uint32_t next, uint32_t nedges, bool bu,
-                               float* f) {
+                               float* f, unsigned long long nvl) {
   float n = a.n_total > 0 ? (float)a.n_total : 1.0f;
-  const unsigned long long nvl = a.nvalid ? *a.nvalid : 0ull;
   float nv = nvl > 0ull ? (float)nvl : 1.0f;
   float m = fmaxf((float)a.avg * n, 1.0f);
   float found = (float)c->found;
@@ -1615,7 +1614,10 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
   const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
   const uint32_t b = blockIdx.x, G = gridDim.x;
-  if (b >= u0 + u1) return;  // uniform
+  if (b >= u0 + u1) {  // uniform: no unit; zero rows, so that the closer can sum every row
+    if (threadIdx.x < 4 * NS) p.s[threadIdx.x >> 2].part[4 * b + (threadIdx.x & 3)] = 0ull;
+    return;
+  }
   const int t = threadIdx.x;
   if (t == 0) {
     S.cnt = 0u;
@@ -1661,7 +1663,8 @@ __device__ __forceinline__ int colour_action(const Ctrl* c, const LevelArgs& a, 
 // direction `act` (0 top-down, 1 bottom-up, 2 abandon the search).
 __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int act, uint32_t q_size,
                              uint32_t q_edges, unsigned long long ex, unsigned long long in, unsigned long long ra,
-                             uint32_t next, uint32_t nedges, const float* feat, unsigned long long rd = 0ull) {
+                             uint32_t next, uint32_t nedges, const float* feat, unsigned long long rd = 0ull,
+                             unsigned long long nvl = 0ull) {
   // algorithmic bytes of this level
   unsigned long long bytes;
   // colour levels: ex / in are the vertices / edges a workgroup expanded locally; a pull
@@ -1695,7 +1698,7 @@ __device__ void finish_level(Ctrl* c, const LevelArgs& a, int op, bool bu, int a
     r.inspected = in;
     r.found = c->found;
     r.n_total = a.n_total;
-    r.num_valid = a.nvalid ? (int32_t)*a.nvalid : 0;
+    r.num_valid = (int32_t)nvl;
     r.action = act;
     r.pad = 0;
     for (int i = 0; i < kMlpIn; ++i) r.feat[i] = feat ? feat[i] : 0.0f;
@@ -1793,10 +1796,9 @@ __device__ __forceinline__ int mlp_eval(const MlpLane& m, const float* s_feat, f
 // Closes one reach search's level; one whole wave calls it.  Lane 0 books the level and
 // applies the direction rule -- the reference's (multistep/scc_fwbw.cpp:168-195,
 // cc_bfs.cpp:167-194), the MLP (when `learned`), or a forced schedule.
-__device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned long long in, unsigned long long nx,
-                             unsigned long long ne, bool learned, const MlpLane& m, float* s_feat, float* s_z,
-                             float* s_h) {
-  Ctrl* c = a.ctrl;
+__device__ void close_search(const LevelArgs& a, Ctrl* c, unsigned long long nvalid, unsigned long long ex,
+                             unsigned long long in, unsigned long long nx, unsigned long long ne, bool learned,
+                             const MlpLane& m, float* s_feat, float* s_z, float* s_h) {
   int act = 0;
   uint32_t q_size = 0, q_edges = 0, next = 0, nedges = 0;
   bool bu = false;
@@ -1821,7 +1823,7 @@ __device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned
       act = 0;
       c->switched_back = 1;
     }
-    level_features(c, a, next, nedges, bu, s_feat);
+    level_features(c, a, next, nedges, bu, s_feat, nvalid);
   }
   __builtin_amdgcn_wave_barrier();
   if (learned) {
@@ -1833,38 +1835,43 @@ __device__ void close_search(const LevelArgs& a, unsigned long long ex, unsigned
     int f = a.sched[c->phase * MSC_SCHED_LEVELS + c->level - 1];
     if (f >= 0 && f <= 2) act = f;
   }
-  finish_level(c, a, a.op, bu, act, q_size, q_edges, ex, in, 0ull, next, nedges, s_feat);
+  finish_level(c, a, a.op, bu, act, q_size, q_edges, ex, in, 0ull, next, nedges, s_feat, 0ull, nvalid);
   // a bottom-up level over fewer than one unvisited vertex in 8 walks wide word groups
   // (unvisited: what the trim left, less what the search found; an estimate when the
   // allowed set is smaller)
-  const unsigned long long nvl = a.nvalid ? *a.nvalid : (unsigned long long)a.n_total;
+  const unsigned long long nvl = a.nvalid ? nvalid : (unsigned long long)a.n_total;
   const long long left = (long long)nvl - (long long)c->found;
   c->bu_wide = c->bottom_up && left * 8ll < (long long)a.nwords * 32ll ? 1u : 0u;
 }
 
-// Closes a level of NS reach searches (one workgroup): all threads sum the partial rows,
-// then wave s closes search s, the searches in parallel.  The level kernel wrote rows
-// [0, min(nparts, units)) of every search's partials, where units counts both searches'
-// work units of that level (unchanged until this closer).  The MLP weights are loaded into
-// registers first, so their latency overlaps the partial sums.
+// Closes a level of NS reach searches (one workgroup).  Everything it reads comes in one
+// round trip: both control blocks (copied into LDS, worked on there and copied back once),
+// the valid counts, every partial row (a level kernel's idle workgroups write zero rows) and
+// the MLP weights (into registers).  Then all threads sum the rows and wave s closes search
+// s, the searches in parallel.
 template <int NS>
 __global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts) {
+  constexpr int kCW = (int)(sizeof(Ctrl) / 8);
+  static_assert(sizeof(Ctrl) % 8 == 0 && NS * kCW <= kBlock, "control blocks copied as 8-byte words");
+  __shared__ unsigned long long s_ctl[2][kCW];
+  __shared__ unsigned long long s_nv[2];
   __shared__ unsigned long long s_sum[2][4][kBlock / 64];
   __shared__ float s_feat[2][kMlpIn];
   __shared__ float s_z[2][kMlpIn];
   __shared__ float s_h[2][kMlpH];
-  const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
-  const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
-  if (u0 + u1 == 0u) return;  // uniform: every search is done
-  const int w = wave_id();
-  const bool learned0 = u0 && p.s[0].policy == MSC_POLICY_LEARNED && p.s[0].mlp;
-  const bool learned1 = NS == 2 && u1 && p.s[1].policy == MSC_POLICY_LEARNED && p.s[1].mlp;
+  const int t = threadIdx.x, w = wave_id();
+  if (t < NS * kCW) s_ctl[t / kCW][t % kCW] = reinterpret_cast<const unsigned long long*>(p.s[t / kCW].ctrl)[t % kCW];
+  if (t >= kBlock - NS) {
+    const LevelArgs& a = p.s[kBlock - 1 - t];
+    s_nv[kBlock - 1 - t] = a.nvalid ? *a.nvalid : 0ull;
+  }
+  const bool learned0 = p.s[0].policy == MSC_POLICY_LEARNED && p.s[0].mlp;
+  const bool learned1 = NS == 2 && p.s[1].policy == MSC_POLICY_LEARNED && p.s[1].mlp;
   MlpLane m;
   if (w == 0 && learned0) mlp_lane_load(p.s[0].mlp, m);
   if (NS == 2 && w == 1 && learned1) mlp_lane_load(p.s[1].mlp, m);
-  const int rows = (int)min((uint32_t)nparts, u0 + u1);
   unsigned long long acc[2][4] = {{0ull, 0ull, 0ull, 0ull}, {0ull, 0ull, 0ull, 0ull}};
-  for (int r = threadIdx.x; r < rows; r += kBlock)
+  for (int r = t; r < nparts; r += kBlock)
     for (int s = 0; s < NS; ++s)
       for (int k = 0; k < 4; ++k) acc[s][k] += p.s[s].part[4 * r + k];
   for (int s = 0; s < NS; ++s)
@@ -1872,14 +1879,24 @@ __global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts)
       const unsigned long long x = wave_sum64(acc[s][k]);
       if (lane() == 0) s_sum[s][k][w] = x;
     }
-  __syncthreads();  // also: every thread has read the control blocks
+  __syncthreads();
+  Ctrl* c0 = reinterpret_cast<Ctrl*>(s_ctl[0]);
+  Ctrl* c1 = reinterpret_cast<Ctrl*>(s_ctl[1]);
+  const uint32_t u0 = reach_units(c0, p.s[0].nwords);
+  const uint32_t u1 = NS == 2 ? reach_units(c1, p.s[1].nwords) : 0u;
+  if (u0 + u1 == 0u) return;  // uniform: every search is done
   if (w < NS && (w == 0 ? u0 : u1)) {
     unsigned long long tot[4] = {0ull, 0ull, 0ull, 0ull};
     for (int k = 0; k < 4; ++k)
       for (int j = 0; j < kBlock / 64; ++j) tot[k] += s_sum[w][k][j];
-    if (w == 0) close_search(p.s[0], tot[0], tot[1], tot[2], tot[3], learned0, m, s_feat[0], s_z[0], s_h[0]);
-    else close_search(p.s[1], tot[0], tot[1], tot[2], tot[3], learned1, m, s_feat[1], s_z[1], s_h[1]);
+    if (w == 0)
+      close_search(p.s[0], c0, s_nv[0], tot[0], tot[1], tot[2], tot[3], learned0, m, s_feat[0], s_z[0], s_h[0]);
+    else
+      close_search(p.s[1], c1, s_nv[1], tot[0], tot[1], tot[2], tot[3], learned1, m, s_feat[1], s_z[1], s_h[1]);
   }
+  __syncthreads();
+  if (t < NS * kCW && (t < kCW ? u0 : u1))
+    reinterpret_cast<unsigned long long*>(p.s[t / kCW].ctrl)[t % kCW] = s_ctl[t / kCW][t % kCW];
 }
 
 // Closes a colour sweep (push or per-colour backward level), then keeps going in this one
