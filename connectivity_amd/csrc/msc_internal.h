@@ -232,9 +232,12 @@ struct MemberSrc {
 // every vertex still valid afterwards is then seeded for colouring round 0 (colors[v] = v,
 // queued in frontier 1 of seed->ctrl, written to `list`), and launch_seed_end closes the
 // seed.  With seed->rc, both run only in an admitted round (after launch_round_begin).
+// With min_part (min_np rows of launch_min_member, which must not be `part`), every
+// workgroup reduces them itself instead of reading min_red.
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
-                  int32_t* list, const unsigned long long* max_red = nullptr, bool members_only = false);
+                  int32_t* list, const unsigned long long* max_red = nullptr, bool members_only = false,
+                  const unsigned long long* min_part = nullptr, int min_np = 0);
 // colouring round 0's seed alone (k_seed_bits), after commits made without one
 void launch_seed_bits(const Launch& L, const LevelArgs& a, int32_t n, int32_t* list);
 // part col 3 = 0xffffffff - smallest member in each workgroup's words (0: none), or with
@@ -251,9 +254,11 @@ void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc,
 // the first kernel of a colouring round (k_round_begin); s0 / s1: the searches' control
 // blocks that round 0 (`first`) waits for (null: none); `single`: CC's one fixpoint, whose
 // closing check ends the colouring; with `prog` (pinned host memory) the outcome is also
-// published there, tagged with `epoch`
-void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
-                        bool first, bool single, unsigned long long* prog = nullptr, uint32_t epoch = 0);
+// published there, tagged with `epoch`; `fresh`: the round state is reset first (as
+// launch_rc_set mode 0)
+void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                        bool first, bool single, unsigned long long* prog = nullptr, uint32_t epoch = 0,
+                        bool fresh = false);
 // host-directed round state: 0 fresh, 1 resume the admitted round, 2 clear a stall
 void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode);
 int launch_count_bits(const Launch& L, const uint32_t* bits, int64_t nwords, unsigned long long* part);
@@ -268,13 +273,16 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
-// Colour tail (after colouring round 0's seed, which wrote a.slots): k_tail_edges collects
-// the edges joining two listed vertices into `pairs` (cap entries), k_tail_solve colours
-// them in one workgroup and writes the labels of every component with more than one
-// vertex (largest id; smallest with `canonical`), or declines (rc->tail = 2); a list of
-// more than max_vertices vertices is declined at once
+// Colour tail (after colouring round 0's seed, which wrote a.slots, with no seed hand-over
+// and no reduce of the commit's count: this does both, the count from commit_np rows of
+// commit_part into commit_red): k_tail_edges collects the edges joining two listed vertices
+// into `pairs` (cap entries), k_tail_solve colours them in one workgroup and writes the
+// labels of every component with more than one vertex (largest id; smallest with
+// `canonical`), or declines (rc->tail = 2); a list of more than max_vertices vertices is
+// declined at once
 void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, uint32_t* pairs,
-                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical);
+                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical,
+                 const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red);
 // canonical labels (minimum member per component); gate: only once *gate != 0 (null: always).
 // Without `apply` only the minima: scratch[l] = smallest v with labels[v] == l.
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply);

@@ -690,10 +690,15 @@ __device__ void round_decide(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ct
   rc->go = 1u;
 }
 
-__global__ void k_round_begin(RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+__global__ void k_round_begin(RoundCtl* rc, Ctrl* push, Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
                               int32_t flags, unsigned long long* prog, uint32_t epoch) {
   if (threadIdx.x != 0) return;
-  round_decide(rc, push, cbw, s0, s1, flags);
+  if (flags & 4) {  // a fresh colouring first (k_rc_set mode 0)
+    *rc = RoundCtl{};
+    push->done = 1u;
+    cbw->done = 1u;
+  }
+  round_decide(rc, push, cbw, s0, s1, flags & 3);
   if (prog) {
     const unsigned long long st = (rc->stall ? 1ull : 0ull) | (rc->finished ? 2ull : 0ull) | (rc->error ? 4ull : 0ull);
     __hip_atomic_store(prog, ((unsigned long long)(epoch & 0xffffu) << 48) | (st << 32) | rc->admitted,
@@ -2099,8 +2104,9 @@ __global__ void __launch_bounds__(kBlock) k_tail_edges(LevelArgs a, RoundCtl* rc
   __shared__ uint32_t own[kChunk];
   __shared__ uint32_t s_part[kBlock / 64];
   if (!rc->go) return;  // uniform
-  const Ctrl* c = a.ctrl;
-  const uint32_t nv = c->list_len, qe = c->q_edges;
+  // the seed's queue counter: no k_close_seed has run yet (k_tail_solve does its work)
+  const unsigned long long pk = a.ctrl->next_packed;
+  const uint32_t nv = (uint32_t)(pk >> 32), qe = (uint32_t)pk;
   if (nv == 0u || nv > maxv || qe > kTailScan) return;  // uniform: k_tail_solve declines
   const int t = threadIdx.x;
   const uint32_t nch = (qe + kChunk - 1) / kChunk;
@@ -2186,7 +2192,10 @@ __device__ __forceinline__ bool tail_sweep_end(TailLds& T, uint32_t i) {
   return T.flag[i % 3u] != 0u;
 }
 
-// Colour tail, step 2 (one workgroup of kTailBlock threads): the components among the
+// Colour tail, step 2 (one workgroup of kTailBlock threads).  First the two steps the
+// tail path does not launch: the seed's hand-over (k_close_seed: the queue becomes the push
+// frontier, which a declined tail leaves to the colouring rounds) and, when cnp > 0, the
+// commit's count (k_reduce of its cnp rows into cred).  Then the components among the
 // endpoints of the joining edges k_tail_edges found.  SCC: rounds of max-colour push to a
 // fixpoint, roots (colour == self), and the per-colour backward search from the roots
 // (multistep/scc_color.cpp:47-289), each a sweep over the edges in LDS; colours are node
@@ -2197,15 +2206,54 @@ __device__ __forceinline__ bool tail_sweep_end(TailLds& T, uint32_t i) {
 // every buffer as the seed left it: rc->tail = 2, and the host runs the colouring rounds.
 // A settled tail ends the colouring: rc->tail = 1, rc->finished.
 template <bool CC>
-__global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, const Ctrl* push,
-                                                           const int32_t* __restrict__ list,
+__global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, Ctrl* push, const int32_t* __restrict__ list,
                                                            const uint32_t* __restrict__ pairs, uint32_t cap,
                                                            uint32_t maxv, int32_t* __restrict__ labels,
-                                                           int32_t canonical) {
+                                                           int32_t canonical, uint32_t pull_min,
+                                                           const unsigned long long* __restrict__ cpart, int32_t cnp,
+                                                           unsigned long long* __restrict__ cred) {
   __shared__ TailLds T;
+  __shared__ unsigned long long s_red[4][kTailBlock / 64];
+  __shared__ uint32_t s_seed[2];
   if (!rc->go) return;  // uniform: round 0 was not admitted
   const int t = threadIdx.x;
-  const uint32_t nv = push->list_len, qe = push->q_edges, ne = rc->tail_edges;
+  {
+    unsigned long long x[4] = {0ull, 0ull, 0ull, 0ull};
+    for (int r = t; r < cnp; r += kTailBlock) {
+      x[0] += cpart[4 * r];
+      x[1] += cpart[4 * r + 1];
+      x[2] += cpart[4 * r + 2];
+      x[3] = cpart[4 * r + 3] > x[3] ? cpart[4 * r + 3] : x[3];
+    }
+    for (int k = 0; k < 3; ++k) x[k] = wave_sum64(x[k]);
+    x[3] = wave_max64(x[3]);
+    if (lane() == 0)
+      for (int k = 0; k < 4; ++k) s_red[k][wave_id()] = x[k];
+    if (t == 0) {
+      const unsigned long long p = push->next_packed;
+      const uint32_t next = (uint32_t)(p >> 32);
+      push->list_len = next;
+      push->q_size = next;
+      push->q_edges = (uint32_t)p;
+      push->round_edges = (uint32_t)p;
+      if (pull_min) push->bottom_up = (uint32_t)p >= pull_min ? 2u : 0u;
+      push->cur = 1u;
+      push->next_packed = 0ull;
+      push->done = next == 0u ? 1u : 0u;
+      s_seed[0] = next;
+      s_seed[1] = (uint32_t)p;
+    }
+    __syncthreads();
+    if (t == 0 && cnp > 0) {
+      unsigned long long o[4] = {0ull, 0ull, 0ull, 0ull};
+      for (int w = 0; w < kTailBlock / 64; ++w) {
+        for (int k = 0; k < 3; ++k) o[k] += s_red[k][w];
+        o[3] = s_red[3][w] > o[3] ? s_red[3][w] : o[3];
+      }
+      for (int k = 0; k < 4; ++k) cred[k] = o[k];
+    }
+  }
+  const uint32_t nv = s_seed[0], qe = s_seed[1], ne = rc->tail_edges;
   if (nv > maxv || qe > kTailScan || ne > kTailE || ne > cap) {  // uniform
     if (t == 0) rc->tail = 2u;
     return;
@@ -2768,11 +2816,23 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
                                                    const int32_t* __restrict__ piv, const int64_t* __restrict__ base,
                                                    uint32_t ng, unsigned long long* __restrict__ part,
                                                    const RoundCtl* rc, const unsigned long long* __restrict__ max_red,
-                                                   int32_t members_only) {
+                                                   int32_t members_only, const unsigned long long* __restrict__ min_part,
+                                                   int32_t min_np) {
   __shared__ unsigned long long s_red[kBlock / 64];
   if (rc && !rc->go) return;  // uniform: the searches have not finished
   const uint32_t keep = member_keep(m);
   if (min_red) label = (int32_t)(0xffffffffu - (uint32_t)min_red[3]);
+  if (min_np > 0) {
+    // k_min_member's rows, reduced by every workgroup (col 3: 0xffffffff - smallest member)
+    unsigned long long mx = 0;
+    for (int r = threadIdx.x; r < min_np; r += kBlock) mx = min_part[4 * r + 3] > mx ? min_part[4 * r + 3] : mx;
+    mx = wave_max64(mx);
+    if (lane() == 0) s_red[wave_id()] = mx;
+    __syncthreads();
+    for (int w = 0; w < kBlock / 64; ++w) mx = s_red[w] > mx ? s_red[w] : mx;
+    __syncthreads();  // s_red is reused below
+    label = (int32_t)(0xffffffffu - (uint32_t)mx);
+  }
   if (max_red) label = (int32_t)(max_red[3] - 1ull);
   unsigned long long cnt = 0;
   // kU slices of kBlock consecutive vertices per iteration, with all their loads issued
@@ -2998,7 +3058,8 @@ MemberSrc member_src(const MemberSrc& m, const uint32_t* valid) {
 
 int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels, int32_t label,
                   const unsigned long long* min_red, unsigned long long* part, const LevelArgs* seed,
-                  int32_t* list, const unsigned long long* max_red, bool members_only) {
+                  int32_t* list, const unsigned long long* max_red, bool members_only,
+                  const unsigned long long* min_part, int min_np) {
   LevelArgs a = seed ? *seed : LevelArgs{};
   // the seed runs as its own sparse sweep (k_seed_bits): fused into this pass it made
   // every iteration wait on the stage (s24: 51 us fused, 22 + 12 us apart)
@@ -3006,11 +3067,11 @@ int launch_commit(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t 
   if (((uintptr_t)labels & 15u) == 0 && !members_only) {
     g = grid_for(n, kBlock * 64, L.grid * 2);
     MSC_LAUNCH(L, (k_commit<false, true>), dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n,
-                       labels, label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, 0);
+                       labels, label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, 0, min_part, min_np);
   } else {
     g = grid_for(n, kBlock * 16, L.grid * 2);
     MSC_LAUNCH(L, k_commit<false>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels,
-                       label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, members_only ? 1 : 0);
+               label, min_red, nullptr, nullptr, 0u, part, a.rc, max_red, members_only ? 1 : 0, min_part, min_np);
   }
   if (seed)
     MSC_LAUNCH(L, k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
@@ -3029,7 +3090,7 @@ int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, in
   int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
   MSC_LAUNCH(L, k_commit<true>, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels, 0,
-                     nullptr, piv, base, ng, part, a.rc, nullptr, 0);
+                     nullptr, piv, base, ng, part, a.rc, nullptr, 0, nullptr, 0);
   if (seed)
     MSC_LAUNCH(L, k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);
@@ -3059,10 +3120,10 @@ void launch_canon_list(const Launch& L, const int32_t* list, const RoundCtl* rc,
                      labels, cmin);
 }
 
-void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, const Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
-                        bool first, bool single, unsigned long long* prog, uint32_t epoch) {
+void launch_round_begin(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, const Ctrl* s0, const Ctrl* s1,
+                        bool first, bool single, unsigned long long* prog, uint32_t epoch, bool fresh) {
   MSC_LAUNCH(L, k_round_begin, dim3(1), dim3(64), 0, L.stream, rc, push, cbw, s0, s1,
-                     (first ? 1 : 0) | (single ? 2 : 0), prog, epoch);
+             (first ? 1 : 0) | (single ? 2 : 0) | (fresh ? 4 : 0), prog, epoch);
 }
 
 void launch_rc_set(const Launch& L, RoundCtl* rc, Ctrl* push, Ctrl* cbw, int mode) {
@@ -3098,16 +3159,18 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
 }
 
 void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, uint32_t* pairs,
-                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical) {
+                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical,
+                 const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red) {
   const uint32_t maxv = max_vertices < kTailV ? max_vertices : kTailV;
+  const uint32_t pull_min = a.pull_div ? a.pull_min : 0u;
   const int g = (int)(kTailScan / kChunk) < L.grid ? (int)(kTailScan / kChunk) : L.grid;
   MSC_LAUNCH(L, k_tail_edges, dim3(g), dim3(kBlock), 0, L.stream, a, rc, pairs, cap, maxv);
   if (cc)
     MSC_LAUNCH(L, k_tail_solve<true>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, pairs, cap, maxv,
-               labels, canonical ? 1 : 0);
+               labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
   else
     MSC_LAUNCH(L, k_tail_solve<false>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, pairs, cap, maxv,
-               labels, canonical ? 1 : 0);
+               labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply) {

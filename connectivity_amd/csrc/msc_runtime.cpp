@@ -1009,27 +1009,33 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // scc_color.cpp:56-63; cc_bfs.cpp:217-271 then cc_color.cpp:62-69), a later round seeds
   // from the previous round's list; then push levels, roots (scc_color.cpp:165-212) and
   // colour-BW levels (scc_color.cpp:214-289)
-  auto seed_steps = [&](int rr) {
+  // With `tail` the seed's hand-over and the reduce of the commit's count are left to the
+  // colour tail (launch_tail), which does both first; returns the commit's partial rows.
+  auto seed_steps = [&](int rr, bool tail) -> int {
+    int npc = 0;
     if (rr == 0 && committed) {
       // the FW-BW rounds committed already: only the seed of colouring round 0
       Timed t(c, MSC_K_COMMIT);
       launch_seed_bits(L, p, n, list_of(0));
-      launch_seed_end(L, p.ctrl, rcd, p.pull_div ? p.pull_min : 0u);
+      if (!tail) launch_seed_end(L, p.ctrl, rcd, p.pull_div ? p.pull_min : 0u);
     } else if (rr == 0) {
       Timed t(c, MSC_K_COMMIT);
+      // the smallest member into the backward search's partials (free once the searches
+      // ended), which every commit workgroup reduces for itself
       const bool want_min = fast_canon && member.ca;
-      if (want_min)
-        launch_reduce(L, c->part, launch_min_member(L, member, c->valid, nw, c->part, rcd), c->ctrl->red_min);
-      const int npc = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p,
-                                               list_of(0))
-                         : launch_commit(L, member, c->valid, n, labels, pivot, want_min ? c->ctrl->red_min : nullptr,
-                                         c->part, &p, list_of(0));
-      launch_reduce(L, c->part, npc, c->ctrl->red_commit);
-      launch_seed_end(L, p.ctrl, rcd, p.pull_div ? p.pull_min : 0u);
+      const int npm = want_min ? launch_min_member(L, member, c->valid, nw, c->part2, rcd) : 0;
+      npc = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p, list_of(0))
+               : launch_commit(L, member, c->valid, n, labels, pivot, nullptr, c->part, &p, list_of(0), nullptr, false,
+                               want_min ? c->part2 : nullptr, npm);
+      if (!tail) {
+        launch_reduce(L, c->part, npc, c->ctrl->red_commit);
+        launch_seed_end(L, p.ctrl, rcd, p.pull_div ? p.pull_min : 0u);
+      }
     } else {
       Timed t(c, MSC_K_ROOTS);
       launch_seed_list(L, p, list_of(rr - 1), list_of(rr), n);
     }
+    return npc;
   };
   auto level_steps = [&](int rr) -> int {
     int e;
@@ -1044,16 +1050,16 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     return MSC_OK;
   };
   auto round_steps = [&](int rr) -> int {
-    seed_steps(rr);
+    seed_steps(rr, false);
     return level_steps(rr);
   };
   // Colour tail: round 0's seed, then one workgroup settles the whole colouring if its
   // joining edges fit; the host reads the outcome before it enqueues anything else.
   auto tail_steps = [&]() {
-    seed_steps(0);
+    const int npc = seed_steps(0, true);
     Timed t(c, MSC_K_COLOR_PROP);
     launch_tail(L, cc, p, rcd, list_of(0), (uint32_t*)c->fr0.q, tail_cap, (uint32_t)c->tune.tail_vertices, labels,
-                fast_canon);
+                fast_canon, c->part, npc, c->ctrl->red_commit);
   };
   const uint32_t* done_gate = &rcd->finished;
   // rounds [from, from + count), the check that books the last of them, and the final
@@ -1072,8 +1078,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     for (int rr = from; rr < from + count; ++rr) {
       if (rr >= from + kRoundLead && !device_admitted(c, ep, (uint64_t)(rr - kRoundLead) + 1u)) break;
       // only the rounds the host may wait for publish (a store to host memory per round)
+      // round 0 starts a fresh colouring (its begin resets the round state)
       launch_round_begin(L, rcd, p.ctrl, r.ctrl, rr == 0 ? s0 : nullptr, rr == 0 ? s1 : nullptr, rr == 0, false,
-                         rr + kRoundLead < from + count ? c->prog_dev : nullptr, ep);
+                         rr + kRoundLead < from + count ? c->prog_dev : nullptr, ep, rr == 0);
       if (rr == 0 && use_tail) {
         tail_steps();
         HIP_TRY(hipGetLastError());
@@ -1100,7 +1107,6 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // unfinished the host completes (a search, a push fixpoint, the roots, a colour-BW
   // search), then enqueues the following rounds again; the number of rounds enqueued
   // ahead doubles up to 32.
-  launch_rc_set(L, rcd, p.ctrl, r.ctrl, 0);
   int ahead = cc ? 0 : 2;
   if ((rc = enqueue_rounds(0, cc ? 1 : ahead))) return rc;
   if (capturing) {
@@ -1145,8 +1151,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     if (hr->stall && hr->admitted == 0) {
       // the searches needed more levels than enqueued: finish them, then start over
       if ((rc = run_loop(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, s, c->tune.search_batch))) return rc;
-      launch_rc_set(L, rcd, p.ctrl, r.ctrl, 0);
-      from = 0;
+      from = 0;  // round 0's begin starts the colouring afresh
     } else if (hr->stall) {
       // round admitted - 1 stalled: finish its push, its roots and its colour-BW search
       const int rr = (int)hr->admitted - 1;
@@ -1173,7 +1178,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     if (cc) {
       // one fixpoint: start it over after the search, else just check it again
       if (from == 0) {
-        launch_round_begin(L, rcd, p.ctrl, r.ctrl, s0, s1, true, false);
+        launch_round_begin(L, rcd, p.ctrl, r.ctrl, s0, s1, true, false, nullptr, 0, true);
         if (use_tail) {
           tail_steps();
           continue;
