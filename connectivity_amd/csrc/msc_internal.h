@@ -32,12 +32,11 @@ constexpr int kMaxParts = 8192;                // per-workgroup partial results
 constexpr uint32_t kLocalEdges = MSC_LOCAL_EDGES;
 constexpr int kLocalHalf = (int)(2 * kStage + kLocalEdges / kChunk + 2);
 constexpr int kLocalStride = 2 * kLocalHalf;
-// colour tail (k_tail_edges, k_tail_solve): one workgroup settles colouring round 0's list
-// when it has at most kTailV vertices (slots fit 16 bits), their lists at most kTailScan
-// entries, at most kTailE edges join two of them, those edges at most kTailN endpoints,
-// and the colouring of those takes at most kTailSweeps sweeps
+// colour tail (the seed of colouring round 0, then k_tail_solve): one workgroup settles
+// colouring round 0's list when it has at most kTailV vertices (slots fit 16 bits), at most
+// kTailE edges join two of them, those edges have at most kTailN endpoints, and the
+// colouring of those takes at most kTailSweeps sweeps
 constexpr uint32_t kTailV = 65536;
-constexpr uint32_t kTailScan = 1u << 18;
 constexpr uint32_t kTailE = 8192;
 constexpr uint32_t kTailN = 8192;
 constexpr uint32_t kTailSweeps = 4096;
@@ -132,6 +131,21 @@ struct RoundCtl {
   uint32_t tail_nodes;              // ... and their endpoints
 };
 
+// What colouring round 0's seed collects for the colour tail, for the vertices in the first
+// maxv slots of the round's list: slots[v] = v's slot, and every out-list entry u -> w whose
+// target is still valid (listed as well) and not u itself, as src = slot(u), dst = w (cap
+// entries; *count counts them all).
+struct TailSeed {
+  int32_t* slots;
+  uint32_t* src;
+  int32_t* dst;
+  uint32_t* count;
+  uint32_t cap, maxv;
+  const uint32_t* off;
+  const int32_t* adj;
+  const uint32_t* valid;
+};
+
 struct LevelArgs {
   const uint32_t* down_off;   // top-down lists (first half for OP_CC / mirrored push)
   const int32_t* down_adj;
@@ -175,8 +189,7 @@ struct LevelArgs {
   uint32_t pull_div;          // colour push: a level over at least 1/pull_div of the round's
                               // edges runs as a pull level (0: never)
   uint32_t pull_min;          // ... and over at least this many edges
-  int32_t* slots;             // colouring round 0's seed: slots[v] = v's slot in the round's list
-                              // for the first kTailV slots (null: not written)
+  TailSeed tail;              // colouring round 0's seed: what the colour tail needs (tail.slots null: off)
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or
@@ -205,9 +218,13 @@ int launch_trim(const Launch& L, bool mirror, int32_t n, const uint32_t* out_off
                 int32_t* labels, unsigned long long* part);
 // out[i] = sums (cols 0..2) and max (col 3) of part[0..nparts)[i]
 void launch_reduce(const Launch& L, const unsigned long long* part, int nparts, unsigned long long* out);
-void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase);
+// (with rnp > 0, first reduces rnp rows of rpart into rout, as launch_reduce)
+void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase,
+                        const unsigned long long* rpart = nullptr, int rnp = 0, unsigned long long* rout = nullptr);
 // both searches of a pivot at once: a0 gets phase 0 (FW), a1 phase 1 (BW)
-void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root);
+void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root,
+                             const unsigned long long* rpart = nullptr, int rnp = 0,
+                             unsigned long long* rout = nullptr);
 // complete trim: every vertex valid and unlabelled, control block reset (round 1 examines all)
 void launch_trim2_init(const Launch& L, const LevelArgs& a);
 // colour sweeps (OP_PUSH, OP_CBW) and complete-trim rounds (OP_TRIM2)
@@ -273,16 +290,13 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
 // out[w] = a[w] & ~b[w] (b may be null: copy), and when out2 is given, out2[w] = out[w]
 void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_t* a, const uint32_t* b,
                    int64_t words);
-// Colour tail (after colouring round 0's seed, which wrote a.slots, with no seed hand-over
-// and no reduce of the commit's count: this does both, the count from commit_np rows of
-// commit_part into commit_red): k_tail_edges collects the edges joining two listed vertices
-// into `pairs` (cap entries), k_tail_solve colours them in one workgroup and writes the
-// labels of every component with more than one vertex (largest id; smallest with
-// `canonical`), or declines (rc->tail = 2); a list of more than max_vertices vertices is
-// declined at once
-void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, uint32_t* pairs,
-                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical,
-                 const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red);
+// Colour tail, after colouring round 0's seed (which collected a.tail) with no seed
+// hand-over and no reduce of the commit's count: k_tail_solve does both (the count from
+// commit_np rows of commit_part into commit_red), then colours the endpoints of the joining
+// edges in one workgroup and writes the labels of every component with more than one vertex
+// (largest id; smallest with `canonical`), or declines (rc->tail = 2)
+void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, int32_t* labels,
+                 bool canonical, const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red);
 // canonical labels (minimum member per component); gate: only once *gate != 0 (null: always).
 // Without `apply` only the minima: scratch[l] = smallest v with labels[v] == l.
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply);

@@ -291,13 +291,58 @@ __device__ __forceinline__ uint32_t stage_scan(Stage& S, uint32_t n) {
   return total;
 }
 
+// The colour tail's share of a seed flush (TailSeed): the n scanned entries at slots
+// [bh, bh + n) whose slot is below ts.maxv record their slots, and every list entry of
+// theirs (one thread per entry, its owner by binary search over the scanned offsets) whose
+// target is valid and not the vertex itself is appended to ts.src / ts.dst.
+__device__ __forceinline__ void tail_collect(const Stage& S, uint32_t n, uint32_t total, uint32_t bh,
+                                             const TailSeed& ts) {
+  if (bh >= ts.maxv) return;  // uniform
+  const uint32_t ne = min(n, ts.maxv - bh);
+  for (uint32_t i = threadIdx.x; i < ne; i += kBlock) ts.slots[S.v[i]] = (int32_t)(bh + i);
+  if (!ts.src) return;  // uniform
+  const uint32_t edges = ne < n ? S.d[ne] : total;
+  for (uint32_t e0 = 0; e0 < edges; e0 += kBlock) {
+    const uint32_t e = e0 + threadIdx.x;
+    bool ok = false;
+    uint32_t pos = 0;
+    int32_t w = 0;
+    if (e < edges) {
+      // the last entry whose offset is at most e (offsets rise: listed vertices have lists)
+      uint32_t lo = 0, hi = ne;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S.d[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      const int32_t v = S.v[lo];
+      const uint32_t j = e - S.d[lo];
+      const uint32_t beg = ts.off[v], len = ts.off[v + 1] - beg;  // out-list part (CC: then in-list)
+      if (j < len) {
+        w = ts.adj[beg + j];
+        ok = w != v && bit_test(ts.valid, (uint32_t)w);
+      }
+      pos = bh + lo;
+    }
+    const unsigned long long mask = __ballot(ok);
+    if (mask == 0ull) continue;  // uniform over the wave
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t at = 0;
+    if (lane() == leader) at = atomicAdd(ts.count, (uint32_t)__popcll(mask));
+    at = __shfl(at, leader, 64) + (uint32_t)__popcll(mask & lanes_below());
+    if (ok && at < ts.cap) {
+      ts.src[at] = pos;
+      ts.dst[at] = w;
+    }
+  }
+}
+
 // Writes the n scanned claims to frontier nx at queue slots [bh, bh + n) and edge offsets
 // from bl: ids, offsets, the chunk starts of every kChunk-edge chunk that begins inside a
-// claim's list, and the ids to `list` when given (with `slots`, also slots[id] = slot for
-// the slots below kTailV).  Empties the stage.  Call after a barrier that follows
-// stage_scan; ends with a barrier.
+// claim's list, and the ids to `list` when given (and the colour tail's share, with ts).
+// Empties the stage.  Call after a barrier that follows stage_scan; ends with a barrier.
 __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total, const Frontier& nx, uint32_t bh,
-                                            uint32_t bl, int32_t* list, int32_t* slots) {
+                                            uint32_t bl, int32_t* list, const TailSeed* ts) {
   for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
     uint32_t lo = S.d[i];
     uint32_t hi = i + 1 < n ? S.d[i + 1] : total;
@@ -305,10 +350,10 @@ __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total
     nx.q[pos] = S.v[i];
     nx.qo[pos] = off;
     if (list) list[pos] = S.v[i];
-    if (slots && pos < kTailV) slots[S.v[i]] = (int32_t)pos;
     uint64_t end = (uint64_t)off + (hi - lo);
     for (uint64_t ch = ((uint64_t)off + kChunk - 1) / kChunk; ch * kChunk < end; ++ch) nx.starts[ch] = pos;
   }
+  if (ts) tail_collect(S, n, total, bh, *ts);
   __syncthreads();
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
@@ -319,7 +364,7 @@ __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total
 // offsets and chunk starts to frontier `nx` (and the ids to `list` at the same slots, when
 // given).  Always a barrier.
 __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr,
-                            int32_t* slots = nullptr) {
+                            const TailSeed* ts = nullptr) {
   __syncthreads();
   const uint32_t n = S.cnt;
   __syncthreads();  // no wave may push again before every wave has read the count
@@ -331,7 +376,7 @@ __device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep
     S.base_lo = (uint32_t)b;
   }
   __syncthreads();
-  stage_write(S, n, total, nx, S.base_hi, S.base_lo, list, slots);
+  stage_write(S, n, total, nx, S.base_hi, S.base_lo, list, ts);
 }
 
 // ------------------------------------------------------------------------------------
@@ -576,8 +621,28 @@ __global__ void __launch_bounds__(kBlock) k_count_bits(const uint32_t* __restric
 
 // Starts searches p.s[0 .. ns) from `root` (wave s starts search s, phase phase0 + s):
 // fresh control block, root claimed and queued alone.  Follows scc_fwbw.cpp:72-73
-// (queue = {root}, fw[root] = true, set even if root is invalid).
-__global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t phase0) {
+// (queue = {root}, fw[root] = true, set even if root is invalid).  With rnp > 0 it first
+// reduces rnp partial rows into rout as k_reduce does (the trim's survivor count).
+__global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t phase0,
+                              const unsigned long long* __restrict__ rpart, int32_t rnp,
+                              unsigned long long* __restrict__ rout) {
+  __shared__ unsigned long long s_r[4][2];
+  if (rnp > 0) {  // uniform
+    unsigned long long x[4] = {0ull, 0ull, 0ull, 0ull};
+    for (int i = threadIdx.x; i < rnp; i += blockDim.x) {
+      for (int k = 0; k < 3; ++k) x[k] += rpart[4 * i + k];
+      x[3] = rpart[4 * i + 3] > x[3] ? rpart[4 * i + 3] : x[3];
+    }
+    for (int k = 0; k < 3; ++k) x[k] = wave_sum64(x[k]);
+    x[3] = wave_max64(x[3]);
+    if (lane() == 0)
+      for (int k = 0; k < 4; ++k) s_r[k][wave_id()] = x[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 3; ++k) rout[k] = s_r[k][0] + s_r[k][1];
+      rout[3] = s_r[3][0] > s_r[3][1] ? s_r[3][0] : s_r[3][1];
+    }
+  }
   const int s = wave_id();
   if (s >= ns || s >= 2) return;
   const LevelArgs& a = p.s[s];
@@ -2046,11 +2111,11 @@ __global__ void __launch_bounds__(kBlock) k_seed_bits(LevelArgs a, int32_t* __re
         deg = list_len(a.down_off, a.down_off2, v);
       }
       stage_push(S, take, v, deg);
-      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, list, a.slots);  // also the barrier before the next round
+      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, list, a.tail.slots ? &a.tail : nullptr);  // also a barrier
     }
     __syncthreads();  // B is reloaded next
   }
-  stage_flush(S, a.fr1, a.ctrl, 0u, list, a.slots);
+  stage_flush(S, a.fr1, a.ctrl, 0u, list, a.tail.slots ? &a.tail : nullptr);
 }
 
 // Colour roots (multistep/scc_color.cpp:165-212) among this round's vertex list (length
@@ -2091,86 +2156,8 @@ __global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_
 // vert_cutoff vertices to a serial tail).  A listed vertex that no edge joins to another
 // one is its own component and already carries its id as label (the commit wrote it);
 // only the endpoints of the joining edges can share a component, and they are coloured in
-// LDS.
-
-// Colour tail, step 1 (one kChunk-edge chunk of the seeded frontier per workgroup and
-// iteration): every list entry u -> w of a listed vertex u whose target w is still valid
-// (so listed as well: the seed lists every valid vertex) and is not u itself is appended
-// to `pairs` as slot(u) << 16 | slot(w); rc->tail_edges counts them, also past `cap`.
-// With mirrored lists (CC) only the out-list half is walked: every joining edge is some
-// listed vertex's out-entry.  Runs only in an admitted round whose list fits the tail.
-__global__ void __launch_bounds__(kBlock) k_tail_edges(LevelArgs a, RoundCtl* rc, uint32_t* __restrict__ pairs,
-                                                       uint32_t cap, uint32_t maxv) {
-  __shared__ uint32_t own[kChunk];
-  __shared__ uint32_t s_part[kBlock / 64];
-  if (!rc->go) return;  // uniform
-  // the seed's queue counter: no k_close_seed has run yet (k_tail_solve does its work)
-  const unsigned long long pk = a.ctrl->next_packed;
-  const uint32_t nv = (uint32_t)(pk >> 32), qe = (uint32_t)pk;
-  if (nv == 0u || nv > maxv || qe > kTailScan) return;  // uniform: k_tail_solve declines
-  const int t = threadIdx.x;
-  const uint32_t nch = (qe + kChunk - 1) / kChunk;
-  for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-    const uint32_t cbase = ch * kChunk;
-    // the slots whose lists overlap the chunk, and the owner of every edge by a max-scan
-    // over their start positions (the first one starts at or before the chunk)
-    const uint32_t s0 = a.fr1.starts[ch];
-    const uint32_t s1 = ch + 1 < nch ? a.fr1.starts[ch + 1] : nv - 1;
-    for (int k = t; k < (int)kChunk; k += kBlock) own[k] = 0u;
-    __syncthreads();
-    for (uint32_t i = s0 + (uint32_t)t; i <= s1; i += kBlock) {
-      const uint32_t q = a.fr1.qo[i];
-      const uint32_t st = q > cbase ? q - cbase : 0u;
-      if (st < kChunk) atomicMax(&own[st], i);
-    }
-    __syncthreads();
-    uint32_t o[kPer];
-    uint32_t run = 0;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      run = max(run, own[t * kPer + k]);
-      o[k] = run;
-    }
-    const uint32_t incl = wave_incl_max(run);
-    uint32_t before = __shfl_up(incl, 1, 64);
-    if (lane() == 0) before = 0u;
-    if (lane() == 63) s_part[wave_id()] = incl;
-    __syncthreads();
-    for (int w = 0; w < wave_id(); ++w) before = max(before, s_part[w]);
-    int32_t v[kPer], w[kPer];
-    uint32_t j[kPer];
-    bool in[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      o[k] = max(o[k], before);
-      const uint32_t e = cbase + (uint32_t)(t * kPer + k);
-      in[k] = e < qe;
-      v[k] = in[k] ? a.fr1.q[o[k]] : 0;
-      j[k] = in[k] ? e - a.fr1.qo[o[k]] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      w[k] = -1;
-      if (in[k]) {
-        const uint32_t beg = a.down_off[v[k]], end = a.down_off[v[k] + 1];
-        if (j[k] < end - beg) w[k] = a.down_adj[beg + j[k]];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const bool ok = w[k] >= 0 && w[k] != v[k] && bit_test(a.valid, (uint32_t)w[k]);
-      const uint32_t ds = ok ? (uint32_t)a.slots[w[k]] : 0u;
-      const unsigned long long mask = __ballot(ok);
-      if (mask == 0ull) continue;  // uniform over the wave
-      const int leader = __ffsll((long long)mask) - 1;
-      uint32_t at = 0;
-      if (lane() == leader) at = atomicAdd(&rc->tail_edges, (uint32_t)__popcll(mask));
-      at = __shfl(at, leader, 64) + (uint32_t)__popcll(mask & lanes_below());
-      if (ok && at < cap) pairs[at] = (o[k] << 16) | ds;
-    }
-    __syncthreads();  // own and s_part are rewritten for the next chunk
-  }
-}
+// LDS.  The seed of colouring round 0 collects those edges (tail_collect); k_tail_solve
+// does the rest.
 
 struct TailLds {
   uint32_t mark[kTailV / 32];  // list slots that are endpoints of a joining edge
@@ -2192,11 +2179,11 @@ __device__ __forceinline__ bool tail_sweep_end(TailLds& T, uint32_t i) {
   return T.flag[i % 3u] != 0u;
 }
 
-// Colour tail, step 2 (one workgroup of kTailBlock threads).  First the two steps the
-// tail path does not launch: the seed's hand-over (k_close_seed: the queue becomes the push
-// frontier, which a declined tail leaves to the colouring rounds) and, when cnp > 0, the
-// commit's count (k_reduce of its cnp rows into cred).  Then the components among the
-// endpoints of the joining edges k_tail_edges found.  SCC: rounds of max-colour push to a
+// Colour tail (one workgroup of kTailBlock threads).  First the two steps the tail path
+// does not launch: the seed's hand-over (k_close_seed: the queue becomes the push frontier,
+// which a declined tail leaves to the colouring rounds) and, when cnp > 0, the commit's
+// count (k_reduce of its cnp rows into cred).  Then the components among the endpoints of
+// the joining edges the seed collected (src slot, dst vertex: slots[dst] is its slot).  SCC: rounds of max-colour push to a
 // fixpoint, roots (colour == self), and the per-colour backward search from the roots
 // (multistep/scc_color.cpp:47-289), each a sweep over the edges in LDS; colours are node
 // indices, and each component is labelled with its largest vertex id (raw: what colouring
@@ -2207,7 +2194,9 @@ __device__ __forceinline__ bool tail_sweep_end(TailLds& T, uint32_t i) {
 // A settled tail ends the colouring: rc->tail = 1, rc->finished.
 template <bool CC>
 __global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, Ctrl* push, const int32_t* __restrict__ list,
-                                                           const uint32_t* __restrict__ pairs, uint32_t cap,
+                                                           const uint32_t* __restrict__ psrc,
+                                                           const int32_t* __restrict__ pdst,
+                                                           const int32_t* __restrict__ slots, uint32_t cap,
                                                            uint32_t maxv, int32_t* __restrict__ labels,
                                                            int32_t canonical, uint32_t pull_min,
                                                            const unsigned long long* __restrict__ cpart, int32_t cnp,
@@ -2253,8 +2242,8 @@ __global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, Ctrl* p
       for (int k = 0; k < 4; ++k) cred[k] = o[k];
     }
   }
-  const uint32_t nv = s_seed[0], qe = s_seed[1], ne = rc->tail_edges;
-  if (nv > maxv || qe > kTailScan || ne > kTailE || ne > cap) {  // uniform
+  const uint32_t nv = s_seed[0], ne = rc->tail_edges;
+  if (nv > maxv || ne > kTailE || ne > cap) {  // uniform
     if (t == 0) rc->tail = 2u;
     return;
   }
@@ -2264,12 +2253,20 @@ __global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, Ctrl* p
   if (t == 0) T.count = 0u;
   __syncthreads();
   for (uint32_t e = t; e < ne; e += kTailBlock) {
-    const uint32_t p = pairs[e], u = p >> 16, w = p & 0xffffu;
-    T.edge[e] = p;
+    const uint32_t u = psrc[e], w = (uint32_t)slots[pdst[e]];
+    if (u >= nv || w >= nv) {  // cannot happen: every listed vertex's slot is below nv <= maxv
+      T.count = 1u;
+      continue;
+    }
+    T.edge[e] = (u << 16) | w;
     atomicOr(&T.mark[u >> 5], 1u << (u & 31));
     atomicOr(&T.mark[w >> 5], 1u << (w & 31));
   }
   __syncthreads();
+  if (T.count) {  // uniform
+    if (t == 0) rc->tail = 2u;
+    return;
+  }
   {
     // nodes: the marked slots in slot order (kTailV / 32 words, two per thread)
     const uint32_t i0 = 2u * (uint32_t)t, i1 = i0 + 1u;
@@ -2997,17 +2994,19 @@ void launch_trim2_init(const Launch& L, const LevelArgs& a) {
                      dim3(kBlock), 0, L.stream, a);
 }
 
-void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase) {
+void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32_t phase,
+                        const unsigned long long* rpart, int rnp, unsigned long long* rout) {
   ReachArgs p{};
   p.s[0] = a;
-  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(64), 0, L.stream, p, 1, root, phase);
+  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(128), 0, L.stream, p, 1, root, phase, rpart, rnp, rout);
 }
 
-void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root) {
+void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root,
+                             const unsigned long long* rpart, int rnp, unsigned long long* rout) {
   ReachArgs p{};
   p.s[0] = a0;
   p.s[1] = a1;
-  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(128), 0, L.stream, p, 2, root, 0);
+  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(128), 0, L.stream, p, 2, root, 0, rpart, rnp, rout);
 }
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
@@ -3158,19 +3157,16 @@ void launch_roots_list(const Launch& L, const LevelArgs& a, const int32_t* list,
   MSC_LAUNCH(L, k_close_seed, dim3(1), dim3(64), 0, L.stream, a.ctrl, a.rc, 0u);
 }
 
-void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, uint32_t* pairs,
-                 uint32_t cap, uint32_t max_vertices, int32_t* labels, bool canonical,
-                 const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red) {
-  const uint32_t maxv = max_vertices < kTailV ? max_vertices : kTailV;
+void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, int32_t* labels,
+                 bool canonical, const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red) {
+  const TailSeed& ts = a.tail;
   const uint32_t pull_min = a.pull_div ? a.pull_min : 0u;
-  const int g = (int)(kTailScan / kChunk) < L.grid ? (int)(kTailScan / kChunk) : L.grid;
-  MSC_LAUNCH(L, k_tail_edges, dim3(g), dim3(kBlock), 0, L.stream, a, rc, pairs, cap, maxv);
   if (cc)
-    MSC_LAUNCH(L, k_tail_solve<true>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, pairs, cap, maxv,
-               labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
+    MSC_LAUNCH(L, k_tail_solve<true>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, ts.src, ts.dst,
+               ts.slots, ts.cap, ts.maxv, labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
   else
-    MSC_LAUNCH(L, k_tail_solve<false>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, pairs, cap, maxv,
-               labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
+    MSC_LAUNCH(L, k_tail_solve<false>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, ts.src, ts.dst,
+               ts.slots, ts.cap, ts.maxv, labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
 }
 
 void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply) {

@@ -761,8 +761,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   }
 
   // trim (multistep/scc_trim.cpp:89-139, cc_trim.cpp:47-97); the survivor count stays on
-  // device (the searches read it there) and reaches the host with a later copy
-  int np;
+  // device (the searches read it there) and reaches the host with a later copy.  The degree
+  // trim's rows are reduced by the searches' start kernel where there is one (trim_np
+  // pending until then), else by k_reduce.
+  int np, trim_np = 0;
   if (o->trim_level == 2) {
     // complete trim (multistep/scc_trim.cpp:141-297): worklist rounds until nothing leaves;
     // the pivot stays the caller's choice even when it is trimmed, as in the reference
@@ -784,8 +786,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     // no labels yet (the commit writes every label in one pass)
     np = launch_trim(L, cc, n, g.out_off, g.in_off, c->valid, c->b1, cc ? nullptr : c->b3, c->s1,
                      cc ? nullptr : c->s3, nullptr, c->part);
-    launch_reduce(L, c->part, np, c->ctrl->red_trim);
+    trim_np = np;
   }
+  auto flush_trim = [&]() {
+    if (trim_np > 0) launch_reduce(L, c->part, trim_np, c->ctrl->red_trim);
+    trim_np = 0;
+  };
   if (s) s->k[MSC_K_TRIM].bytes += 8.0 * n + 1.0 * n;
 
   LevelArgs a = base_args(c, &g, labels, o, avg);
@@ -841,10 +847,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
           launch_fill_u32(L, c->s3, 0xffffffffu, nw);
         }
         if (bt) {
+          flush_trim();
           launch_seed_pivots(L, false, fw, c->bpiv, ng, 0);
           launch_seed_pivots(L, false, bw, c->bpiv, ng, 1);
         } else {
-          launch_search_init_pair(L, fw, bw, pivot);
+          launch_search_init_pair(L, fw, bw, pivot, c->part, trim_np, c->ctrl->red_trim);
+          trim_np = 0;
         }
         sa[0] = fw;
         sa[1] = bw;
@@ -854,6 +862,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
         // SCC = FW set & BW set = valid & ~b1 & ~b3
         member = MemberSrc{c->b1, c->b3, &cs[0], &cs[1]};
       } else {
+        flush_trim();
         if (!cand_ready) {  // cand = allow, nothing seen
           launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
           launch_fill_u32(L, c->s1, 0xffffffffu, nw);
@@ -890,8 +899,13 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bf.up_off = g.out_off; bf.up_adj = g.out_adj;
       bf.up_off2 = g.in_off; bf.up_adj2 = g.in_adj;
       bf.allow = c->valid; bf.cand = c->b1; bf.seen = c->s1; bf.fresh = c->f1;
-      if (bt) launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
-      else launch_search_init(L, bf, pivot, 2);
+      if (bt) {
+        flush_trim();
+        launch_seed_pivots(L, true, bf, c->bpiv, ng, 2);
+      } else {
+        launch_search_init(L, bf, pivot, 2, c->part, trim_np, c->ctrl->red_trim);
+        trim_np = 0;
+      }
       sa[0] = bf;
       nsearch = 1;
       sop = OP_CC;
@@ -901,6 +915,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     }
     if ((rc = enqueue_levels(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, c->tune.search_batch))) return rc;
   }
+  flush_trim();  // no search (no pivot)
 
   // More FW-BW rounds before colouring (msc_opts.fwbw_rounds; the reference runs one): the
   // searches are finished on the host's watch, the first SCC is committed as usual (every
@@ -984,9 +999,12 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   if (fast_canon && !cc) p.cmin = c->cmin;  // tracked by the colour-BW searches
   // the colour tail (batches colour their union in rounds); its slot map borrows cmin,
   // whose entries the rounds only read after writing them, and its edge list frontier 0's
-  // queue, which nothing reads before round 0's first push level
-  const uint32_t tail_cap = (uint32_t)(n < (int64_t)kTailE ? n : (int64_t)kTailE);
-  if (use_tail) p.slots = c->cmin;
+  // queue and offsets, which nothing reads before round 0's first push level
+  if (use_tail) {
+    const uint32_t maxv = (uint32_t)c->tune.tail_vertices < kTailV ? (uint32_t)c->tune.tail_vertices : kTailV;
+    p.tail = TailSeed{c->cmin, c->fr0.qo, c->fr0.q, &rcd->tail_edges, (uint32_t)(n < (int64_t)kTailE ? n : (int64_t)kTailE),
+                      maxv, g.out_off, g.out_adj, c->valid};
+  }
   if (c->tune.local_rounds > 0) {
     p.loc = c->loc;
     p.local_rounds = c->tune.local_rounds;
@@ -1058,8 +1076,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   auto tail_steps = [&]() {
     const int npc = seed_steps(0, true);
     Timed t(c, MSC_K_COLOR_PROP);
-    launch_tail(L, cc, p, rcd, list_of(0), (uint32_t*)c->fr0.q, tail_cap, (uint32_t)c->tune.tail_vertices, labels,
-                fast_canon, c->part, npc, c->ctrl->red_commit);
+    launch_tail(L, cc, p, rcd, list_of(0), labels, fast_canon, c->part, npc, c->ctrl->red_commit);
   };
   const uint32_t* done_gate = &rcd->finished;
   // rounds [from, from + count), the check that books the last of them, and the final
