@@ -368,14 +368,21 @@ def read_traffic(workload: str, policy: str, kernel_class: str):
 
 
 def cpu_model() -> str:
+    """The host CPU's model name, with its family / model numbers (a virtual machine may
+    report only a generic name) and the logical CPUs the machine has."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = (x.strip() for x in line.split(":", 1))
+                    if k in ("model name", "cpu family", "model", "stepping") and k not in fields:
+                        fields[k] = v
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = fields.get("model name", "unknown")
+    ids = ", ".join(f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields)
+    return f"{name} ({ids}; {os.cpu_count()} logical CPUs)" if ids else name
 
 
 def host_cores() -> int:
