@@ -31,6 +31,14 @@
 #ifndef MSC_BU_KR
 #define MSC_BU_KR 4
 #endif
+// list entries a bottom-up candidate probes at a time after its first probe missed
+#ifndef MSC_BU_SCAN
+#define MSC_BU_SCAN 2
+#endif
+// 1: a level where both searches of a pivot are bottom-up runs as one fused sweep (bu_pair)
+#ifndef MSC_BU_PAIR
+#define MSC_BU_PAIR 1
+#endif
 
 namespace msc {
 namespace {
@@ -1003,13 +1011,38 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
 // workgroup takes the candidates in rounds of kR per thread; each candidate's first up-list
 // entry is checked with all loads of the thread's candidates in flight together, and most
 // candidates are settled there (on R-MAT the first in-neighbour is already visited ~90% of
-// the time); the rest scan on.  Claims collect in LDS word masks and are written back with
+// the time); the rest scan on, MSC_BU_SCAN entries at a time.  Claims collect in LDS word masks and are written back with
 // one plain store per word: no other workgroup clears bits of these words during a
 // bottom-up level.  Claims are not queued (queueing them cost R-MAT s24 1.5%, also when only
 // levels with few claims left did): a following bottom-up level needs no frontier, and a
 // following top-down level expands the claims from the group's words of a.fresh, which
 // every group writes in full (zero words included).  The thread's claims and their
 // down-list lengths are counted into my_next / my_nedges.  Every thread calls.
+// The rest [j, end) of a bottom-up candidate's up-list after its first probe missed:
+// MSC_BU_SCAN entries at a time, their adjacency loads and then their seen probes in flight
+// together, instead of one dependent load pair per entry (a wave's round lasts as long as
+// its longest scan).  `inspected` counts the entries up to the first visited one, as the
+// reference's scan with its break does (multistep/scc_fwbw.cpp:124-133).
+__device__ __forceinline__ bool bu_scan(const int32_t* __restrict__ adj, const uint32_t* seen, uint32_t j,
+                                        uint32_t end, unsigned long long& inspected) {
+  constexpr uint32_t K = MSC_BU_SCAN;
+  for (; j < end; j += K) {
+    uint32_t x[K], w[K];
+#pragma unroll
+    for (uint32_t i = 0; i < K; ++i) x[i] = j + i < end ? (uint32_t)adj[j + i] : 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < K; ++i) w[i] = j + i < end ? seen[x[i] >> 5] : 0xffffffffu;
+#pragma unroll
+    for (uint32_t i = 0; i < K; ++i) {
+      if (j + i < end) {
+        ++inspected;
+        if (!((w[i] >> (x[i] & 31u)) & 1u)) return true;
+      }
+    }
+  }
+  return false;
+}
+
 template <bool MIRROR, int WPT>
 __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIterT<WPT>& B, uint32_t* s_claim,
                                          unsigned long long& my_examined, unsigned long long& my_inspected,
@@ -1069,15 +1102,8 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
         ++my_inspected;
         hit[r] = !(wc[r] & (1u << (u[r] & 31u)));
       }
-      for (uint32_t j = beg[r]; !hit[r] && j < end[r]; ++j) {
-        ++my_inspected;
-        hit[r] = !bit_test(a.seen, (uint32_t)a.up_adj[j]);
-      }
-      if (MIRROR)
-        for (uint32_t j = beg2[r]; !hit[r] && j < end2[r]; ++j) {
-          ++my_inspected;
-          hit[r] = !bit_test(a.seen, (uint32_t)a.up_adj2[j]);
-        }
+      if (!hit[r]) hit[r] = bu_scan(a.up_adj, a.seen, beg[r], end[r], my_inspected);
+      if (MIRROR && !hit[r]) hit[r] = bu_scan(a.up_adj2, a.seen, beg2[r], end2[r], my_inspected);
       if (hit[r]) atomicOr(&s_claim[(uint32_t)(v[r] - (uint32_t)(wb * 32)) >> 5], 1u << (v[r] & 31u));
     }
     // the claims, counted per wave from a ballot (a point where the wave's lanes reconverge
@@ -1102,6 +1128,144 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
     if (c) {
       a.cand[wb + i] = B.word[i] & ~c;
       a.seen[wb + i] = sw[j] & ~c;
+    }
+  }
+}
+
+// LDS of the fused bottom-up group of two searches (bu_pair): the union of their candidate
+// words (walked as one), each search's candidate words and its claim masks.
+struct BuPairLds {
+  BitIter B;
+  uint32_t cf[kBW];
+  uint32_t cb[kBW];
+  uint32_t claim_f[kBW];
+  uint32_t claim_b[kBW];
+};
+
+// Whether the two searches of a k_reach<2> launch run this level as one fused bottom-up
+// sweep: both bottom-up over kBW-word groups of one bitmap, the forward search's up-lists
+// being the backward search's down-lists and vice versa (a pivot's FW and BW search).
+__device__ __forceinline__ bool bu_pairable(const LevelArgs& f, const LevelArgs& b) {
+  const Ctrl* cf = f.ctrl;
+  const Ctrl* cb = b.ctrl;
+  return !cf->done && !cb->done && cf->bottom_up && cb->bottom_up && !cf->bu_wide && !cb->bu_wide &&
+         f.nwords == b.nwords && f.up_off == b.down_off && f.down_off == b.up_off && f.up_off2 == nullptr;
+}
+
+// One bottom-up word group [wb, wb + kBW) of a forward search f and a backward search b
+// together (each as bu_group does it alone, multistep/scc_fwbw.cpp:118-148 and 301-331).
+// The candidates of either search are walked as one set, so a vertex's in- and out-list
+// bounds are loaded once for both: f probes its in-list head (f.up), b its out-list head
+// (b.up = f.down), and a claim's down-list length, counted for the next frontier's edges,
+// is the other pair of bounds.  On R-MAT nearly every unvisited vertex is a candidate of
+// both searches at the giant level, and the two offset arrays are each read once instead
+// of twice.  Counts, claims and write-backs per search are those of two bu_group calls.
+// Every thread calls.
+__device__ __forceinline__ void bu_pair(const LevelArgs& f, const LevelArgs& b, int64_t wb, BuPairLds& L,
+                                        unsigned long long* cf4, unsigned long long* cb4) {
+  constexpr int kR = MSC_BU_KR;
+  const int t = threadIdx.x;
+  uint32_t wf[kWPT], wbk[kWPT], sf[kWPT], sb[kWPT], un[kWPT];
+  const int64_t w0 = wb + (int64_t)t * kWPT;
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) {
+    const bool in = w0 + j < f.nwords;
+    wf[j] = in ? f.cand[w0 + j] : 0u;
+    wbk[j] = in ? b.cand[w0 + j] : 0u;
+    sf[j] = in ? f.seen[w0 + j] : 0u;
+    sb[j] = in ? b.seen[w0 + j] : 0u;
+    un[j] = wf[j] | wbk[j];
+    L.cf[t * kWPT + j] = wf[j];
+    L.cb[t * kWPT + j] = wbk[j];
+    L.claim_f[t * kWPT + j] = 0u;
+    L.claim_b[t * kWPT + j] = 0u;
+  }
+  const uint32_t total = bits_scan(L.B, un);  // barrier: words and cleared masks visible
+  for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
+    uint32_t v[kR], ib[kR], ie[kR], ob[kR], oe[kR], uf[kR], ub[kR], pf[kR], pb[kR];
+    bool has[kR], isf[kR], isb[kR], hf[kR], hb[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const uint32_t k = k0 + (uint32_t)(r * kBlock + t);
+      has[r] = k < total;
+      v[r] = has[r] ? bits_pick(L.B, wb, k) : 0u;
+      const uint32_t i = (v[r] - (uint32_t)(wb * 32)) >> 5, bit = 1u << (v[r] & 31u);
+      isf[r] = has[r] && (L.cf[i] & bit);
+      isb[r] = has[r] && (L.cb[i] & bit);
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      ib[r] = ie[r] = ob[r] = oe[r] = 0u;
+      if (has[r]) {
+        ib[r] = f.up_off[v[r]];
+        ie[r] = f.up_off[v[r] + 1];
+        ob[r] = f.down_off[v[r]];
+        oe[r] = f.down_off[v[r] + 1];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      pf[r] = isf[r] && ib[r] < ie[r];
+      pb[r] = isb[r] && ob[r] < oe[r];
+      uf[r] = pf[r] ? (uint32_t)f.up_adj[ib[r]] : 0u;
+      ub[r] = pb[r] ? (uint32_t)b.up_adj[ob[r]] : 0u;
+    }
+    uint32_t xf[kR], xb[kR];
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      xf[r] = pf[r] ? f.seen[uf[r] >> 5] : 0xffffffffu;
+      xb[r] = pb[r] ? b.seen[ub[r] >> 5] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      hf[r] = hb[r] = false;
+      if (isf[r]) {
+        ++cf4[0];
+        if (pf[r]) {
+          ++cf4[1];
+          hf[r] = !(xf[r] & (1u << (uf[r] & 31u)));
+          if (!hf[r]) hf[r] = bu_scan(f.up_adj, f.seen, ib[r] + 1u, ie[r], cf4[1]);
+        }
+      }
+      if (isb[r]) {
+        ++cb4[0];
+        if (pb[r]) {
+          ++cb4[1];
+          hb[r] = !(xb[r] & (1u << (ub[r] & 31u)));
+          if (!hb[r]) hb[r] = bu_scan(b.up_adj, b.seen, ob[r] + 1u, oe[r], cb4[1]);
+        }
+      }
+      const uint32_t i = (v[r] - (uint32_t)(wb * 32)) >> 5, bit = 1u << (v[r] & 31u);
+      if (hf[r]) atomicOr(&L.claim_f[i], bit);
+      if (hb[r]) atomicOr(&L.claim_b[i], bit);
+    }
+    // per-search claim counts from ballots (the reconvergence point after the scans, as in
+    // bu_group) and the claims' down-list lengths: the other search's up-list bounds
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const unsigned long long mf = __ballot(hf[r]), mb = __ballot(hb[r]);
+      if (lane() == 0) {
+        cf4[2] += (unsigned long long)__popcll(mf);
+        cb4[2] += (unsigned long long)__popcll(mb);
+      }
+      if (hf[r]) cf4[3] += oe[r] - ob[r];
+      if (hb[r]) cb4[3] += ie[r] - ib[r];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kWPT; ++j) {
+    const int i = t * kWPT + j;
+    const uint32_t c1 = L.claim_f[i], c2 = L.claim_b[i];
+    f.fresh[wb + i] = c1;
+    b.fresh[wb + i] = c2;
+    if (c1) {
+      f.cand[wb + i] = wf[j] & ~c1;
+      f.seen[wb + i] = sf[j] & ~c1;
+    }
+    if (c2) {
+      b.cand[wb + i] = wbk[j] & ~c2;
+      b.seen[wb + i] = sb[j] & ~c2;
     }
   }
 }
@@ -1673,6 +1837,7 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
       uint32_t claim[kBWide];
     } buw;
     TdBitsLds T;
+    BuPairLds pair;
   } U;
   __shared__ unsigned long long s_red[kBlock / 64];
   ChunkLds<OP_FW, MIRROR>& L = U.L;
@@ -1683,8 +1848,10 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
 
   const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
   const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
+  // both searches bottom-up over the same word groups: one fused sweep (bu_pair), u0 units
+  const bool pair = MSC_BU_PAIR && NS == 2 && !MIRROR && bu_pairable(p.s[0], p.s[NS - 1]);
   const uint32_t b = blockIdx.x, G = gridDim.x;
-  if (b >= u0 + u1) {  // uniform: no unit; zero rows, so that the closer can sum every row
+  if (b >= (pair ? u0 : u0 + u1)) {  // uniform: no unit; zero rows, so that the closer can sum every row
     if (threadIdx.x < 4 * NS) p.s[threadIdx.x >> 2].part[4 * b + (threadIdx.x & 3)] = 0ull;
     return;
   }
@@ -1696,8 +1863,13 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   }
   __syncthreads();
   unsigned long long c0[4] = {0ull, 0ull, 0ull, 0ull}, c1[4] = {0ull, 0ull, 0ull, 0ull};
-  if (b < u0) reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, U.T, c0[0], c0[1], c0[2], c0[3]);
-  if (NS == 2) {
+  if (pair) {
+    for (uint32_t lu = b; lu < u0; lu += G) bu_pair(p.s[0], p.s[NS - 1], (int64_t)lu * kBW, U.pair, c0, c1);
+    __syncthreads();  // the LDS is reused by the partial sums below
+  } else if (b < u0) {
+    reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, U.T, c0[0], c0[1], c0[2], c0[3]);
+  }
+  if (NS == 2 && !pair) {
     // this workgroup's first unit of the union at or past u0, in search 1's numbering
     const uint32_t first = b >= u0 ? b - u0 : b + (u0 - b + G - 1) / G * G - u0;
     if (first < u1)
