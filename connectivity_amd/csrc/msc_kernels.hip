@@ -631,15 +631,30 @@ __global__ void __launch_bounds__(kBlock) k_count_bits(const uint32_t* __restric
 // fresh control block, root claimed and queued alone.  Follows scc_fwbw.cpp:72-73
 // (queue = {root}, fw[root] = true, set even if root is invalid).  With rnp > 0 it first
 // reduces rnp partial rows into rout as k_reduce does (the trim's survivor count).
-__global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t phase0,
+constexpr int kInitBlock = 512;  // k_search_init's workgroup: waves 0 and 1 start the searches
+
+__global__ void __launch_bounds__(kInitBlock) k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t phase0,
                               const unsigned long long* __restrict__ rpart, int32_t rnp,
                               unsigned long long* __restrict__ rout) {
-  __shared__ unsigned long long s_r[4][2];
+  constexpr int kWaves = kInitBlock / 64;
+  __shared__ unsigned long long s_r[4][kWaves];
   if (rnp > 0) {  // uniform
+    // a few rows per thread, their loads in flight together (a 2560-row trim took 10 us
+    // with 20 dependent rows per thread)
     unsigned long long x[4] = {0ull, 0ull, 0ull, 0ull};
-    for (int i = threadIdx.x; i < rnp; i += blockDim.x) {
-      for (int k = 0; k < 3; ++k) x[k] += rpart[4 * i + k];
-      x[3] = rpart[4 * i + 3] > x[3] ? rpart[4 * i + 3] : x[3];
+    for (int i0 = threadIdx.x; i0 < rnp; i0 += 4 * kInitBlock) {
+      unsigned long long r[4][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = i0 + j * kInitBlock;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[j][k] = i < rnp ? rpart[4 * i + k] : 0ull;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        for (int k = 0; k < 3; ++k) x[k] += r[j][k];
+        x[3] = r[j][3] > x[3] ? r[j][3] : x[3];
+      }
     }
     for (int k = 0; k < 3; ++k) x[k] = wave_sum64(x[k]);
     x[3] = wave_max64(x[3]);
@@ -647,8 +662,12 @@ __global__ void k_search_init(ReachArgs p, int32_t ns, int32_t root, int32_t pha
       for (int k = 0; k < 4; ++k) s_r[k][wave_id()] = x[k];
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int k = 0; k < 3; ++k) rout[k] = s_r[k][0] + s_r[k][1];
-      rout[3] = s_r[3][0] > s_r[3][1] ? s_r[3][0] : s_r[3][1];
+      unsigned long long y[4] = {0ull, 0ull, 0ull, 0ull};
+      for (int w = 0; w < kWaves; ++w) {
+        for (int k = 0; k < 3; ++k) y[k] += s_r[k][w];
+        y[3] = s_r[3][w] > y[3] ? s_r[3][w] : y[3];
+      }
+      for (int k = 0; k < 4; ++k) rout[k] = y[k];
     }
   }
   const int s = wave_id();
@@ -2091,31 +2110,49 @@ __device__ void close_search(const LevelArgs& a, Ctrl* c, unsigned long long nva
 // the valid counts, every partial row (a level kernel's idle workgroups write zero rows) and
 // the MLP weights (into registers).  Then all threads sum the rows and wave s closes search
 // s, the searches in parallel.
+constexpr int kCloseBlock = 512;  // k_close_reach's workgroup
+
 template <int NS>
-__global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts) {
+__global__ void __launch_bounds__(kCloseBlock) k_close_reach(ReachArgs p, int nparts) {
   constexpr int kCW = (int)(sizeof(Ctrl) / 8);
-  static_assert(sizeof(Ctrl) % 8 == 0 && NS * kCW <= kBlock, "control blocks copied as 8-byte words");
+  static_assert(sizeof(Ctrl) % 8 == 0 && NS * kCW <= kCloseBlock, "control blocks copied as 8-byte words");
   __shared__ unsigned long long s_ctl[2][kCW];
   __shared__ unsigned long long s_nv[2];
-  __shared__ unsigned long long s_sum[2][4][kBlock / 64];
+  __shared__ unsigned long long s_sum[2][4][kCloseBlock / 64];
   __shared__ float s_feat[2][kMlpIn];
   __shared__ float s_z[2][kMlpIn];
   __shared__ float s_h[2][kMlpH];
   const int t = threadIdx.x, w = wave_id();
   if (t < NS * kCW) s_ctl[t / kCW][t % kCW] = reinterpret_cast<const unsigned long long*>(p.s[t / kCW].ctrl)[t % kCW];
-  if (t >= kBlock - NS) {
-    const LevelArgs& a = p.s[kBlock - 1 - t];
-    s_nv[kBlock - 1 - t] = a.nvalid ? *a.nvalid : 0ull;
+  if (t >= kCloseBlock - NS) {
+    const LevelArgs& a = p.s[kCloseBlock - 1 - t];
+    s_nv[kCloseBlock - 1 - t] = a.nvalid ? *a.nvalid : 0ull;
   }
   const bool learned0 = p.s[0].policy == MSC_POLICY_LEARNED && p.s[0].mlp;
   const bool learned1 = NS == 2 && p.s[1].policy == MSC_POLICY_LEARNED && p.s[1].mlp;
   MlpLane m;
   if (w == 0 && learned0) mlp_lane_load(p.s[0].mlp, m);
   if (NS == 2 && w == 1 && learned1) mlp_lane_load(p.s[1].mlp, m);
+  // up to four rows per thread in flight together: with the level grid's 1280 rows one
+  // round trip instead of five dependent ones
   unsigned long long acc[2][4] = {{0ull, 0ull, 0ull, 0ull}, {0ull, 0ull, 0ull, 0ull}};
-  for (int r = t; r < nparts; r += kBlock)
-    for (int s = 0; s < NS; ++s)
-      for (int k = 0; k < 4; ++k) acc[s][k] += p.s[s].part[4 * r + k];
+  for (int r0 = t; r0 < nparts; r0 += 4 * kCloseBlock) {
+    unsigned long long x[4][2][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + j * kCloseBlock;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[j][s][k] = r < nparts ? p.s[s].part[4 * r + k] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[s][k] += x[j][s][k];
+  }
   for (int s = 0; s < NS; ++s)
     for (int k = 0; k < 4; ++k) {
       const unsigned long long x = wave_sum64(acc[s][k]);
@@ -2130,7 +2167,7 @@ __global__ void __launch_bounds__(kBlock) k_close_reach(ReachArgs p, int nparts)
   if (w < NS && (w == 0 ? u0 : u1)) {
     unsigned long long tot[4] = {0ull, 0ull, 0ull, 0ull};
     for (int k = 0; k < 4; ++k)
-      for (int j = 0; j < kBlock / 64; ++j) tot[k] += s_sum[w][k][j];
+      for (int j = 0; j < kCloseBlock / 64; ++j) tot[k] += s_sum[w][k][j];
     if (w == 0)
       close_search(p.s[0], c0, s_nv[0], tot[0], tot[1], tot[2], tot[3], learned0, m, s_feat[0], s_z[0], s_h[0]);
     else
@@ -3170,7 +3207,7 @@ void launch_search_init(const Launch& L, const LevelArgs& a, int32_t root, int32
                         const unsigned long long* rpart, int rnp, unsigned long long* rout) {
   ReachArgs p{};
   p.s[0] = a;
-  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(128), 0, L.stream, p, 1, root, phase, rpart, rnp, rout);
+  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(kInitBlock), 0, L.stream, p, 1, root, phase, rpart, rnp, rout);
 }
 
 void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelArgs& a1, int32_t root,
@@ -3178,7 +3215,7 @@ void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelAr
   ReachArgs p{};
   p.s[0] = a0;
   p.s[1] = a1;
-  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(128), 0, L.stream, p, 2, root, 0, rpart, rnp, rout);
+  MSC_LAUNCH(L, k_search_init, dim3(1), dim3(kInitBlock), 0, L.stream, p, 2, root, 0, rpart, rnp, rout);
 }
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
@@ -3215,8 +3252,8 @@ void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns) {
 }
 
 void launch_close_reach(const Launch& L, const ReachArgs& p, int ns) {
-  if (ns == 2) MSC_LAUNCH(L, k_close_reach<2>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
-  else MSC_LAUNCH(L, k_close_reach<1>, dim3(1), dim3(kBlock), 0, L.stream, p, L.grid);
+  if (ns == 2) MSC_LAUNCH(L, k_close_reach<2>, dim3(1), dim3(kCloseBlock), 0, L.stream, p, L.grid);
+  else MSC_LAUNCH(L, k_close_reach<1>, dim3(1), dim3(kCloseBlock), 0, L.stream, p, L.grid);
 }
 
 // Both pointers set: no member (ca = valid), one bitmap (cb = ca) or two.
