@@ -1,6 +1,6 @@
 """Autotuner for a context's launch parameters (msc_tuning).
 
-The level loops have eleven tunables that no rule of the reference fixes and that trade
+The level loops have twelve tunables that no rule of the reference fixes and that trade
 against each other differently on shallow power-law graphs and on deep meshes:
 
   blocks_per_cu     workgroups per compute unit of every level kernel (grid = CUs x this);
@@ -15,7 +15,11 @@ against each other differently on shallow power-law graphs and on deep meshes:
   local_us          ... while that launch has run for less than this many microseconds;
   pull_div          a colour level over at least 1/pull_div of its round's edges pulls (0: never);
   tail_vertices     a colouring of at most this many vertices is settled by one workgroup when
-                    the edges joining them fit its LDS (0: always colour in rounds).
+                    the edges joining them fit its LDS (0: always colour in rounds);
+  balance           top-down load balance of grid-wide levels: 1 a workgroup deals its 256
+                    frontier vertices' edges, 2 chunks of the whole frontier's edges, 3 one or
+                    the other per level by the frontier's mean list length (0, a thread per
+                    frontier vertex, is left out: it walks a hub's list on one thread).
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -49,6 +53,7 @@ CANDIDATES = {
     "local_us": [40, 80, 160, 240, 320],
     "pull_div": [0, 2, 4, 8, 16],
     "tail_vertices": [0, 4096, 65536],
+    "balance": [1, 2, 3],
 }
 
 

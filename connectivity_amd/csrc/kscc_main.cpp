@@ -7,11 +7,13 @@
 // three variants are run `iterations` times each and "<name>, Baseline|ManhattanLocal|
 // ManhattanGlobal, <average s>" is printed per variant (scc_run.cpp:171-180).
 //
-// The reference's variants differ only in how their top-down kernels spread work (a thread
-// per vertex, team-local or global edge balancing) and compute the same labels.  Here every
-// variant runs msc's edge-balanced kernels, so the selector only names the output line; it
-// is kept so scripts written for the reference run unchanged.  An alg outside 0..2 prints
-// the usage line and exits, as the reference does.
+// The reference's variants differ only in how their top-down kernels spread work and
+// compute the same labels.  The selector picks msc's matching load balance
+// (msc_tuning.balance) for the grid-wide search and colour levels: 0 a thread per frontier
+// vertex (fwbw_baseline / color_baseline), 1 a workgroup deals the edges of its 256 frontier
+// vertices (fwbw_manhattan_local / color_manhattan_local), 2 fixed-size chunks of the whole
+// frontier's edges (fwbw_manhattan_global / color_manhattan_global, msc's default).  An alg
+// outside 0..2 prints the usage line and exits, as the reference does.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -67,7 +69,17 @@ int main(int argc, char** argv) {
 
   // every variant in the TIMING build, only the chosen one otherwise
   const int first = timing ? 0 : alg, last = timing ? 2 : alg;
+  msc_tuning tune;
+  if ((rc = msc_ctx_get_tuning(ctx, &tune))) {
+    std::fprintf(stderr, "kscc: cannot read the tuning (error %d)\n", rc);
+    return 1;
+  }
   for (int a = first; a <= last; ++a) {
+    tune.balance = a;
+    if ((rc = msc_ctx_set_tuning(ctx, &tune))) {
+      std::fprintf(stderr, "kscc: cannot select variant %d (error %d)\n", a, rc);
+      return 1;
+    }
     double total = 0.0;
     for (int t = 0; t < iters; ++t) {
       auto t0 = std::chrono::steady_clock::now();

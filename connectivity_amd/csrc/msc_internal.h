@@ -190,6 +190,7 @@ struct LevelArgs {
                               // edges runs as a pull level (0: never)
   uint32_t pull_min;          // ... and over at least this many edges
   TailSeed tail;              // colouring round 0's seed: what the colour tail needs (tail.slots null: off)
+  int32_t balance;            // top-down load balance of grid-wide levels (msc_tuning.balance)
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or

@@ -872,6 +872,57 @@ __global__ void k_close_seed(Ctrl* c, const RoundCtl* rc, uint32_t pull_min) {
 }
 
 // LDS of one top-down chunk.
+// The probes, claims and staging of kPer top-down edges per thread (targets w with the
+// source's colour col; ok: the lane has that edge): one state probe per edge with all
+// loads in flight together, then per edge a claim (reach: cand and seen bits; colour-BW:
+// the valid bit, the label and the colour's minimum member) or a raise (push: atomicMax
+// of the colour, then the next worklist bit), and every winner staged with its down-list
+// length.  Every thread calls (stage_push is wave-cooperative); the caller flushes.
+template <int OP, bool MIRROR>
+__device__ __forceinline__ void td_claim(const LevelArgs& a, uint32_t* inq_nxt, const uint32_t (&w)[kPer],
+                                         const int32_t (&col)[kPer], const bool (&ok)[kPer], Stage& S,
+                                         unsigned long long& my_raised) {
+  constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
+  constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
+  uint32_t word[kPer];
+  int32_t wcol[kPer];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    word[r] = 0;
+    wcol[r] = 0;
+    if (ok[r]) {
+      word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
+      if (COLOR) wcol[r] = a.colors[w[r]];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
+    bool got = false;  // claimed (reach, colour-BW) / raised (push) by this lane
+    if (REACH) {
+      got = in && bit_clear(a.cand, w[r]);
+      // and visited for bottom-up probes (no result needed: only this claim clears it)
+      if (got) __hip_atomic_fetch_and(&a.seen[w[r] >> 5], ~(1u << (w[r] & 31u)), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    } else if (OP == OP_PUSH) {
+      if (in && wcol[r] < col[r] && atomicMax(&a.colors[w[r]], col[r]) < col[r]) {
+        ++my_raised;
+        got = true;
+      }
+    } else {  // OP_CBW
+      if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
+        a.labels[w[r]] = col[r];
+        got = true;
+        if (a.cmin && (int32_t)w[r] < a.cmin[col[r]]) atomicMin(&a.cmin[col[r]], (int32_t)w[r]);
+      }
+    }
+    bool take = got;
+    if (OP == OP_PUSH && got) take = bit_set(inq_nxt, w[r]);
+    uint32_t deg = take ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
+    stage_push(S, take, w[r], deg);
+  }
+}
+
 template <int OP, bool MIRROR>
 struct ChunkLds {
   static constexpr bool kColor = OP == OP_PUSH || OP == OP_CBW;
@@ -897,7 +948,6 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
                                          uint32_t* inq_nxt, uint32_t ch, uint32_t nchunks, uint32_t q_size,
                                          uint32_t q_edges, Stage& S, ChunkLds<OP, MIRROR>& L,
                                          unsigned long long& my_raised) {
-  constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
   constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
   const int t = threadIdx.x;
   const uint32_t e0 = ch * kChunk;
@@ -967,8 +1017,8 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
   __syncthreads();
   // batched per thread so the loads of all kPer edges are in flight together:
   // adjacency (coalesced), then one state probe per edge, then the claims
-  uint32_t w[kPer], word[kPer];
-  int32_t col[kPer], wcol[kPer];
+  uint32_t w[kPer];
+  int32_t col[kPer];
   bool ok[kPer];
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
@@ -986,57 +1036,139 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
       if (COLOR) col[r] = L.col[j];
     }
   }
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    word[r] = 0;
-    wcol[r] = 0;
-    if (ok[r]) {
-      word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
-      if (COLOR) wcol[r] = a.colors[w[r]];
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
-    bool got = false;  // claimed (reach, colour-BW) / raised (push) by this lane
-    if (REACH) {
-      got = in && bit_clear(a.cand, w[r]);
-      // and visited for bottom-up probes (no result needed: only this claim clears it)
-      if (got) __hip_atomic_fetch_and(&a.seen[w[r] >> 5], ~(1u << (w[r] & 31u)), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-    } else if (OP == OP_PUSH) {
-      if (in && wcol[r] < col[r] && atomicMax(&a.colors[w[r]], col[r]) < col[r]) {
-        ++my_raised;
-        got = true;
-      }
-    } else {  // OP_CBW
-      if (in && wcol[r] == col[r] && bit_clear(a.valid, w[r])) {
-        a.labels[w[r]] = col[r];
-        got = true;
-        if (a.cmin && (int32_t)w[r] < a.cmin[col[r]]) atomicMin(&a.cmin[col[r]], (int32_t)w[r]);
-      }
-    }
-    bool take = got;
-    if (OP == OP_PUSH && got) take = bit_set(inq_nxt, w[r]);
-    uint32_t deg = take ? list_len(a.down_off, MIRROR ? a.down_off2 : nullptr, w[r]) : 0u;
-    stage_push(S, take, w[r], deg);
-  }
+  td_claim<OP, MIRROR>(a, inq_nxt, w, col, ok, S, my_raised);
   stage_flush(S, N, a.ctrl, kStage - kChunk);  // also the end-of-chunk barrier
 }
 
-// One bottom-up word group of a reach search (multistep/scc_fwbw.cpp:118-148): every set bit
-// of cand words [wb, wb + kBW) is a candidate, claimed when some vertex of its up-lists is
-// visited (allowed and no longer claimable, the reference's `fw[in] && valid[in]`).  The
-// workgroup takes the candidates in rounds of kR per thread; each candidate's first up-list
-// entry is checked with all loads of the thread's candidates in flight together, and most
-// candidates are settled there (on R-MAT the first in-neighbour is already visited ~90% of
-// the time); the rest scan on, MSC_BU_SCAN entries at a time.  Claims collect in LDS word masks and are written back with
-// one plain store per word: no other workgroup clears bits of these words during a
-// bottom-up level.  Claims are not queued (queueing them cost R-MAT s24 1.5%, also when only
-// levels with few claims left did): a following bottom-up level needs no frontier, and a
-// following top-down level expands the claims from the group's words of a.fresh, which
-// every group writes in full (zero words included).  The thread's claims and their
-// down-list lengths are counted into my_next / my_nedges.  Every thread calls.
+// Top-down load balance of a grid-wide level (msc_tuning.balance): how the frontier's edges
+// are dealt to threads, as the Kokkos variants selected by kokkos_scc/scc_fwbw.cpp:337-364
+// deal them.  Every variant probes and claims the same edges, so labels never depend on it.
+enum Balance : int32_t {
+  kBalanceVertex = 0,  // a thread per frontier vertex walks its whole list (fwbw_baseline.cpp)
+  kBalanceLocal = 1,   // a workgroup deals the edges of its 256 frontier vertices evenly
+                       // (fwbw_manhattan_local.cpp)
+  kBalanceGlobal = 2,  // kChunk-edge chunks of the whole frontier's edge space, td_chunk
+                       // (fwbw_manhattan_global.cpp)
+  kBalanceAuto = 3,    // per level: kBalanceLocal while the frontier's lists average at most
+                       // MSC_AUTO_DEG entries, else kBalanceGlobal
+};
+#ifndef MSC_AUTO_DEG
+#define MSC_AUTO_DEG 4
+#endif
+
+// The variant a level of frontier (q_size vertices, q_edges list entries) runs.  With short
+// lists a chunk's owner table (an LDS scatter and max-scan over kChunk entries) costs more
+// than the per-slot prefix of a workgroup's own 256 vertices.
+__device__ __forceinline__ int32_t td_variant(int32_t balance, uint32_t q_size, uint32_t q_edges) {
+  if (balance != kBalanceAuto) return balance;
+  return (unsigned long long)q_edges <= (unsigned long long)MSC_AUTO_DEG * q_size ? kBalanceLocal : kBalanceGlobal;
+}
+
+// Work units of a queue-based top-down level: chunks of the frontier's edges, or blocks of
+// kBlock frontier vertices.
+__device__ __forceinline__ uint32_t td_units(int32_t balance, uint32_t q_size, uint32_t q_edges) {
+  return td_variant(balance, q_size, q_edges) == kBalanceGlobal ? (q_edges + kChunk - 1) / kChunk
+                                                                : (q_size + kBlock - 1) / kBlock;
+}
+
+// Expands frontier slots [blk * kBlock, blk * kBlock + kBlock) with the whole workgroup, as
+// the balance variants VAR (kBalanceVertex or kBalanceLocal) deal them; the probes, claims
+// and staging are td_chunk's (td_claim), and a push expansion clears each expanded vertex's
+// worklist word with a plain store, as a grid-wide td_chunk does.  kBalanceVertex: thread t
+// walks slot t's lists kPer entries at a time, the workgroup for as many steps as its
+// longest list needs.  kBalanceLocal: the slots' list lengths are prefix-summed in LDS and
+// the block's edges taken kChunk at a time, an edge's slot found by binary search over the
+// prefix (the reference's highest_less_than over its team's trip counts).  Ends with a
+// workgroup barrier.
+template <int OP, bool MIRROR, int VAR>
+__device__ __forceinline__ void td_slots(const LevelArgs& a, const Frontier& F, const Frontier& N, uint32_t* inq_cur,
+                                         uint32_t* inq_nxt, uint32_t blk, uint32_t q_size, Stage& S,
+                                         ChunkLds<OP, MIRROR>& L, unsigned long long& my_raised) {
+  constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
+  const int t = threadIdx.x;
+  const uint32_t k0 = blk * kBlock;
+  const uint32_t nb = min(q_size - k0, (uint32_t)kBlock);
+  const bool has = (uint32_t)t < nb;
+  uint32_t b1 = 0, l1 = 0, b2 = 0, l2 = 0;
+  int32_t own = 0;
+  if (has) {
+    const uint32_t v = (uint32_t)F.q[k0 + (uint32_t)t];
+    if (OP == OP_PUSH) inq_cur[v >> 5] = 0u;
+    b1 = a.down_off[v];
+    l1 = a.down_off[v + 1] - b1;
+    if (MIRROR) {
+      b2 = a.down_off2[v];
+      l2 = a.down_off2[v + 1] - b2;
+    }
+    if (COLOR) own = a.colors[v];
+  }
+  const uint32_t len = l1 + l2;
+  if (VAR == kBalanceVertex) {
+    const uint32_t wmax = wave_incl_max(len);
+    if (lane() == 63) L.scan[wave_id()] = wmax;
+    __syncthreads();
+    uint32_t steps = 0;
+    for (int w = 0; w < kBlock / 64; ++w) steps = max(steps, L.scan[w]);
+    for (uint32_t i0 = 0; i0 < steps; i0 += kPer) {
+      uint32_t w[kPer];
+      int32_t col[kPer];
+      bool ok[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const uint32_t i = i0 + (uint32_t)r;
+        ok[r] = i < len;
+        w[r] = ok[r] ? (uint32_t)((MIRROR && i >= l1) ? a.down_adj2[b2 + (i - l1)] : a.down_adj[b1 + i]) : 0u;
+        col[r] = own;
+      }
+      td_claim<OP, MIRROR>(a, inq_nxt, w, col, ok, S, my_raised);
+      stage_flush(S, N, a.ctrl, kStage - kChunk);  // also a barrier
+    }
+  } else {
+    const uint32_t incl = wave_incl_sum(len);
+    if (lane() == 63) L.scan[wave_id()] = incl;
+    L.base[t] = b1;
+    if (MIRROR) {
+      L.split[t] = l1;
+      L.base2[t] = b2;
+    }
+    if (COLOR) L.col[t] = own;
+    __syncthreads();
+    uint32_t carry = 0, T = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      if (w < wave_id()) carry += L.scan[w];
+      T += L.scan[w];
+    }
+    L.own[t] = incl - len + carry;  // exclusive prefix; slots past nb have len 0
+    __syncthreads();
+    for (uint32_t e0 = 0; e0 < T; e0 += kChunk) {
+      uint32_t w[kPer];
+      int32_t col[kPer];
+      bool ok[kPer];
+#pragma unroll
+      for (int r = 0; r < kPer; ++r) {
+        const uint32_t e = e0 + (uint32_t)t + (uint32_t)r * kBlock;
+        ok[r] = e < T;
+        w[r] = 0u;
+        col[r] = 0;
+        if (ok[r]) {
+          // the last slot whose list starts at or before e (own[0] = 0 <= e < T)
+          uint32_t lo = 0;
+#pragma unroll
+          for (uint32_t step = kBlock / 2; step >= 1; step >>= 1)
+            if (lo + step < nb && L.own[lo + step] <= e) lo += step;
+          const uint32_t i = e - L.own[lo];
+          w[r] = (uint32_t)((MIRROR && i >= L.split[lo]) ? a.down_adj2[L.base2[lo] + (i - L.split[lo])]
+                                                         : a.down_adj[L.base[lo] + i]);
+          if (COLOR) col[r] = L.col[lo];
+        }
+      }
+      td_claim<OP, MIRROR>(a, inq_nxt, w, col, ok, S, my_raised);
+      stage_flush(S, N, a.ctrl, kStage - kChunk);  // also a barrier
+    }
+  }
+  __syncthreads();  // the slots' LDS is reused by the next block
+}
+
 // The rest [j, end) of a bottom-up candidate's up-list after its first probe missed:
 // MSC_BU_SCAN entries at a time, their adjacency loads and then their seen probes in flight
 // together, instead of one dependent load pair per entry (a wave's round lasts as long as
@@ -1062,6 +1194,19 @@ __device__ __forceinline__ bool bu_scan(const int32_t* __restrict__ adj, const u
   return false;
 }
 
+// One bottom-up word group of a reach search (multistep/scc_fwbw.cpp:118-148): every set bit
+// of cand words [wb, wb + kBW) is a candidate, claimed when some vertex of its up-lists is
+// visited (allowed and no longer claimable, the reference's `fw[in] && valid[in]`).  The
+// workgroup takes the candidates in rounds of kR per thread; each candidate's first up-list
+// entry is checked with all loads of the thread's candidates in flight together, and most
+// candidates are settled there (on R-MAT the first in-neighbour is already visited ~90% of
+// the time); the rest scan on, MSC_BU_SCAN entries at a time.  Claims collect in LDS word
+// masks and are written back with one plain store per word: no other workgroup clears bits
+// of these words during a bottom-up level.  Claims are not queued (queueing them cost R-MAT
+// s24 1.5%, also when only levels with few claims left did): a following bottom-up level
+// needs no frontier, and a following top-down level expands the claims from the group's
+// words of a.fresh, which every group writes in full (zero words included).  The thread's claims and their
+// down-list lengths are counted into my_next / my_nedges.  Every thread calls.
 template <bool MIRROR, int WPT>
 __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIterT<WPT>& B, uint32_t* s_claim,
                                          unsigned long long& my_examined, unsigned long long& my_inspected,
@@ -1699,7 +1844,7 @@ __device__ __forceinline__ void pull_group(const LevelArgs& a, const Frontier& N
 // workgroups without a chunk exit at once (the closer sums only the rows of the ones that
 // ran); then, with a.loc, the workgroup continues on its own claims (local_rounds).
 // part row = (vertices and edges expanded locally, raised, 0).
-template <int OP, bool MIRROR>
+template <int OP, bool MIRROR, bool SLOTS = false>
 __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   static_assert(OP == OP_PUSH || OP == OP_CBW, "colour ops only; searches run in k_reach");
   __shared__ Stage S;
@@ -1746,8 +1891,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   }
   const uint32_t cur = c->cur;
   const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-  const uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
-  if (blockIdx.x >= nchunks) return;  // uniform: no chunk for this workgroup
+  const uint32_t units = td_units(a.balance, q_size, q_edges);
+  if (blockIdx.x >= units) return;  // uniform: no unit for this workgroup
   const Frontier F = cur ? a.fr1 : a.fr0;
   const Frontier N = cur ? a.fr0 : a.fr1;
   uint32_t* inq_cur = cur ? a.inq1 : a.inq0;
@@ -1765,8 +1910,19 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   // cleared with a plain store instead of an atomic on its bit
   // (chunks dealt out XCD-aware, as pull groups are, made no difference on the grid or
   // the batch)
-  for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
-    td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, nchunks, q_size, q_edges, S, L, my_raised);
+  if (!SLOTS) {
+    for (uint32_t ch = blockIdx.x; ch < units; ch += gridDim.x)
+      td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, units, q_size, q_edges, S, L, my_raised);
+  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceGlobal) {
+    for (uint32_t ch = blockIdx.x; ch < units; ch += gridDim.x)
+      td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, units, q_size, q_edges, S, L, my_raised);
+  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceLocal) {
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x)
+      td_slots<OP, MIRROR, kBalanceLocal>(a, F, N, inq_cur, inq_nxt, u, q_size, S, L, my_raised);
+  } else {
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x)
+      td_slots<OP, MIRROR, kBalanceVertex>(a, F, N, inq_cur, inq_nxt, u, q_size, S, L, my_raised);
+  }
   unsigned long long loc_n = 0, loc_e = 0;
   if (a.loc && a.local_rounds > 0)
     local_rounds<OP, MIRROR>(a, N, inq_nxt, S, L, Q, my_raised, loc_n, loc_e, t_begin);
@@ -1783,18 +1939,18 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
 
 // Work units of a reach search's coming level: bitmap word groups (bottom-up) or
 // kChunk-edge chunks of the frontier (top-down); none once the search is done.
-__device__ __forceinline__ uint32_t reach_units(const Ctrl* c, int64_t nwords) {
+__device__ __forceinline__ uint32_t reach_units(const Ctrl* c, const LevelArgs& a) {
   if (c->done) return 0u;
   if (c->bottom_up || c->from_bits) {
     const int64_t gw = (c->bu_wide || !c->bottom_up) ? kBWide : kBW;
-    return (uint32_t)((nwords + gw - 1) / gw);
+    return (uint32_t)((a.nwords + gw - 1) / gw);
   }
-  return (c->q_edges + kChunk - 1) / kChunk;
+  return td_units(a.balance, c->q_size, c->q_edges);
 }
 
 // The units [first, units) of one reach search, stepping by the grid, then a flush of the
 // claims staged for it.  Every thread calls (uniform arguments).
-template <bool MIRROR>
+template <bool MIRROR, bool SLOTS>
 __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first, uint32_t units, Stage& S,
                                              ChunkLds<OP_FW, MIRROR>& L, BitIter& B, uint32_t* s_claim,
                                              BitIterT<kWPTWide>& BW, uint32_t* s_claimw, TdBitsLds& T,
@@ -1823,9 +1979,18 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
   }
   if (c->from_bits) {
     for (uint32_t lu = first; lu < units; lu += gridDim.x) td_bits_group<MIRROR>(a, N, (int64_t)lu * kBWide, S, T);
-  } else {
+  } else if (!SLOTS) {
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
+  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceGlobal) {
+    for (uint32_t lu = first; lu < units; lu += gridDim.x)
+      td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
+  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceLocal) {
+    for (uint32_t lu = first; lu < units; lu += gridDim.x)
+      td_slots<OP_FW, MIRROR, kBalanceLocal>(a, F, N, nullptr, nullptr, lu, q_size, S, L, unused);
+  } else {
+    for (uint32_t lu = first; lu < units; lu += gridDim.x)
+      td_slots<OP_FW, MIRROR, kBalanceVertex>(a, F, N, nullptr, nullptr, lu, q_size, S, L, unused);
   }
   stage_flush(S, N, c, 0u);
 }
@@ -1839,7 +2004,7 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
 // search before it moves on to the other's units.  Workgroups without a unit exit at once;
 // every other one writes one partial row per search, (examined, inspected, claims, their
 // down-list lengths) -- the last two counted by bottom-up levels, which queue nothing.
-template <int NS, bool MIRROR>
+template <int NS, bool MIRROR, bool SLOTS = false>
 __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   __shared__ Stage S;
   // a search's level is either top-down (ChunkLds, or TdBitsLds right after a bottom-up
@@ -1865,8 +2030,8 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
   uint32_t* s_claim = U.bu.claim;
   uint32_t* s_claimw = U.buw.claim;
 
-  const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0].nwords);
-  const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1].nwords) : 0u;
+  const uint32_t u0 = reach_units(p.s[0].ctrl, p.s[0]);
+  const uint32_t u1 = NS == 2 ? reach_units(p.s[1].ctrl, p.s[1]) : 0u;
   // both searches bottom-up over the same word groups: one fused sweep (bu_pair), u0 units
   const bool pair = MSC_BU_PAIR && NS == 2 && !MIRROR && bu_pairable(p.s[0], p.s[NS - 1]);
   const uint32_t b = blockIdx.x, G = gridDim.x;
@@ -1886,13 +2051,13 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
     for (uint32_t lu = b; lu < u0; lu += G) bu_pair(p.s[0], p.s[NS - 1], (int64_t)lu * kBW, U.pair, c0, c1);
     __syncthreads();  // the LDS is reused by the partial sums below
   } else if (b < u0) {
-    reach_search<MIRROR>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, U.T, c0[0], c0[1], c0[2], c0[3]);
+    reach_search<MIRROR, SLOTS>(p.s[0], b, u0, S, L, B, s_claim, BW, s_claimw, U.T, c0[0], c0[1], c0[2], c0[3]);
   }
   if (NS == 2 && !pair) {
     // this workgroup's first unit of the union at or past u0, in search 1's numbering
     const uint32_t first = b >= u0 ? b - u0 : b + (u0 - b + G - 1) / G * G - u0;
     if (first < u1)
-      reach_search<MIRROR>(p.s[1], first, u1, S, L, B, s_claim, BW, s_claimw, U.T, c1[0], c1[1], c1[2], c1[3]);
+      reach_search<MIRROR, SLOTS>(p.s[1], first, u1, S, L, B, s_claim, BW, s_claimw, U.T, c1[0], c1[1], c1[2], c1[3]);
   }
   for (int s = 0; s < NS; ++s) {
     unsigned long long* cs = s == 0 ? c0 : c1;
@@ -1906,10 +2071,9 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
 }
 
 // Rows of the partials array written by the colour level that just ran: those of the
-// workgroups that had a chunk.
-__device__ __forceinline__ int active_rows(int nparts, uint32_t q_edges) {
-  uint32_t nchunks = (q_edges + kChunk - 1) / kChunk;
-  return (int)min((uint32_t)nparts, nchunks);
+// workgroups that had a unit.
+__device__ __forceinline__ int active_rows(int nparts, const LevelArgs& a, uint32_t q_size, uint32_t q_edges) {
+  return (int)min((uint32_t)nparts, td_units(a.balance, q_size, q_edges));
 }
 
 // Direction of the coming colour level: a push level whose frontier has at least
@@ -2161,8 +2325,8 @@ __global__ void __launch_bounds__(kCloseBlock) k_close_reach(ReachArgs p, int np
   __syncthreads();
   Ctrl* c0 = reinterpret_cast<Ctrl*>(s_ctl[0]);
   Ctrl* c1 = reinterpret_cast<Ctrl*>(s_ctl[1]);
-  const uint32_t u0 = reach_units(c0, p.s[0].nwords);
-  const uint32_t u1 = NS == 2 ? reach_units(c1, p.s[1].nwords) : 0u;
+  const uint32_t u0 = reach_units(c0, p.s[0]);
+  const uint32_t u1 = NS == 2 ? reach_units(c1, p.s[1]) : 0u;
   if (u0 + u1 == 0u) return;  // uniform: every search is done
   if (w < NS && (w == 0 ? u0 : u1)) {
     unsigned long long tot[4] = {0ull, 0ull, 0ull, 0ull};
@@ -2202,7 +2366,7 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
     const uint32_t q_size = c->q_size, q_edges = c->q_edges;
     const bool pulled = OP == OP_PUSH && c->bottom_up != 0u;
     const int rows = pulled ? (int)min((uint32_t)nparts, xcd_positions((uint32_t)((a.nwords + kBlock - 1) / kBlock)))
-                            : active_rows(nparts, q_edges);
+                            : active_rows(nparts, a, q_size, q_edges);
     unsigned long long ln = 0, le = 0, ra = 0, rd = 0;
     for (int i = t; i < rows; i += kBlock) {
       ln += a.part[4 * i];
@@ -3220,12 +3384,21 @@ void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelAr
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   dim3 g((unsigned)L.grid), b(kBlock);
+  const bool slots = a.balance != kBalanceGlobal;  // the other balance variants (td_slots)
   switch (op) {
     case OP_PUSH:
-      if (mirror) MSC_LAUNCH(L, (k_level<OP_PUSH, true>), g, b, 0, L.stream, a);
-      else MSC_LAUNCH(L, (k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
+      if (slots) {
+        if (mirror) MSC_LAUNCH(L, (k_level<OP_PUSH, true, true>), g, b, 0, L.stream, a);
+        else MSC_LAUNCH(L, (k_level<OP_PUSH, false, true>), g, b, 0, L.stream, a);
+      } else {
+        if (mirror) MSC_LAUNCH(L, (k_level<OP_PUSH, true>), g, b, 0, L.stream, a);
+        else MSC_LAUNCH(L, (k_level<OP_PUSH, false>), g, b, 0, L.stream, a);
+      }
       break;
-    case OP_CBW: MSC_LAUNCH(L, (k_level<OP_CBW, false>), g, b, 0, L.stream, a); break;
+    case OP_CBW:
+      if (slots) MSC_LAUNCH(L, (k_level<OP_CBW, false, true>), g, b, 0, L.stream, a);
+      else MSC_LAUNCH(L, (k_level<OP_CBW, false>), g, b, 0, L.stream, a);
+      break;
     case OP_TRIM2: MSC_LAUNCH(L, k_trim2, g, b, 0, L.stream, a); break;
     default: break;
   }
@@ -3246,6 +3419,12 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
 
 void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns) {
   dim3 g((unsigned)L.grid), b(kBlock);
+  if (p.s[0].balance != kBalanceGlobal) {  // the other balance variants (td_slots)
+    if (ns == 2) MSC_LAUNCH(L, (k_reach<2, false, true>), g, b, 0, L.stream, p);
+    else if (mirror) MSC_LAUNCH(L, (k_reach<1, true, true>), g, b, 0, L.stream, p);
+    else MSC_LAUNCH(L, (k_reach<1, false, true>), g, b, 0, L.stream, p);
+    return;
+  }
   if (ns == 2) MSC_LAUNCH(L, (k_reach<2, false>), g, b, 0, L.stream, p);
   else if (mirror) MSC_LAUNCH(L, (k_reach<1, true>), g, b, 0, L.stream, p);
   else MSC_LAUNCH(L, (k_reach<1, false>), g, b, 0, L.stream, p);

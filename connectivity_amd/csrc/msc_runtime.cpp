@@ -66,7 +66,7 @@ struct msc_ctx : WsView {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2, 65536};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2, 65536, 3};
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
   float* mlp = nullptr;
@@ -397,6 +397,7 @@ LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opt
   a.mlp = (o->policy == MSC_POLICY_LEARNED && c->has_mlp) ? c->mlp : nullptr;
   a.sched = o->schedule ? c->sched : nullptr;
   a.clock_mhz = c->clock_mhz;
+  a.balance = c->tune.balance;
   return a;
 }
 
@@ -1403,8 +1404,8 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
       t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
-      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024 || t->tail_vertices < 0 ||
-      t->tail_vertices > (int32_t)kTailV)
+      t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024 ||
+      t->tail_vertices < 0 || t->tail_vertices > (int32_t)kTailV || t->balance < 0 || t->balance > 3)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;

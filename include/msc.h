@@ -227,6 +227,14 @@ typedef struct msc_tuning {
   int32_t tail_vertices;    /* colouring round 0 with at most this many vertices is settled by
                                one workgroup (joining edges coloured in LDS) instead of rounds of
                                level launches, when its joining edges fit (0..65536; 0: never) */
+  int32_t balance;          /* top-down load balance of the grid-wide search and colour levels,
+                               the Kokkos driver's variants (kokkos_scc/scc_fwbw.cpp:337-364,
+                               scc_color.cpp:579-655): 0 a thread per frontier vertex (fwbw_baseline),
+                               1 a workgroup deals the edges of its 256 frontier vertices
+                               (fwbw_manhattan_local), 2 fixed-size chunks of the whole
+                               frontier's edges (fwbw_manhattan_global), 3 per level: 1 while
+                               the frontier's lists average at most 4 entries, else 2 (the
+                               default).  Labels do not depend on it. */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);

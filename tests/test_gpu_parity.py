@@ -218,7 +218,7 @@ def test_tuning_rejects_bad_values(msc, gpu_ctx):
                 dict(first_batch=0), dict(first_batch=8, max_batch=4), dict(max_batch=257),
                 dict(concurrent_fwbw=2), dict(search_batch=0), dict(search_batch=257), dict(local_rounds=-1),
                 dict(local_rounds=257), dict(local_us=0), dict(pull_div=-1), dict(pull_div=1025),
-                dict(tail_vertices=-1), dict(tail_vertices=65537)):
+                dict(tail_vertices=-1), dict(tail_vertices=65537), dict(balance=-1), dict(balance=4)):
         with pytest.raises(msc.MscError):
             gpu_ctx.set_tuning(**bad)
         assert gpu_ctx.tuning() == defaults
