@@ -1058,17 +1058,20 @@ enum Balance : int32_t {
 
 // The variant a level of frontier (q_size vertices, q_edges list entries) runs.  With short
 // lists a chunk's owner table (an LDS scatter and max-scan over kChunk entries) costs more
-// than the per-slot prefix of a workgroup's own 256 vertices.
-__device__ __forceinline__ int32_t td_variant(int32_t balance, uint32_t q_size, uint32_t q_edges) {
-  if (balance != kBalanceAuto) return balance;
+// than the per-slot prefix of a workgroup's own 256 vertices (grid 4096^2 SCC 14.4 ->
+// 13.6 ms).  Mirrored levels (WCC: in- and out-list per vertex) keep the chunks: there the
+// slot form was slower (grid WCC 5.9 -> 6.2 ms, batch 21.5 -> 22.1 ms).
+__device__ __forceinline__ int32_t td_variant(const LevelArgs& a, uint32_t q_size, uint32_t q_edges) {
+  if (a.balance != kBalanceAuto) return a.balance;
+  if (a.down_off2) return kBalanceGlobal;
   return (unsigned long long)q_edges <= (unsigned long long)MSC_AUTO_DEG * q_size ? kBalanceLocal : kBalanceGlobal;
 }
 
 // Work units of a queue-based top-down level: chunks of the frontier's edges, or blocks of
 // kBlock frontier vertices.
-__device__ __forceinline__ uint32_t td_units(int32_t balance, uint32_t q_size, uint32_t q_edges) {
-  return td_variant(balance, q_size, q_edges) == kBalanceGlobal ? (q_edges + kChunk - 1) / kChunk
-                                                                : (q_size + kBlock - 1) / kBlock;
+__device__ __forceinline__ uint32_t td_units(const LevelArgs& a, uint32_t q_size, uint32_t q_edges) {
+  return td_variant(a, q_size, q_edges) == kBalanceGlobal ? (q_edges + kChunk - 1) / kChunk
+                                                          : (q_size + kBlock - 1) / kBlock;
 }
 
 // Expands frontier slots [blk * kBlock, blk * kBlock + kBlock) with the whole workgroup, as
@@ -1891,7 +1894,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   }
   const uint32_t cur = c->cur;
   const uint32_t q_size = c->q_size, q_edges = c->q_edges;
-  const uint32_t units = td_units(a.balance, q_size, q_edges);
+  const uint32_t units = td_units(a, q_size, q_edges);
   if (blockIdx.x >= units) return;  // uniform: no unit for this workgroup
   const Frontier F = cur ? a.fr1 : a.fr0;
   const Frontier N = cur ? a.fr0 : a.fr1;
@@ -1913,10 +1916,10 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   if (!SLOTS) {
     for (uint32_t ch = blockIdx.x; ch < units; ch += gridDim.x)
       td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, units, q_size, q_edges, S, L, my_raised);
-  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceGlobal) {
+  } else if (td_variant(a, q_size, q_edges) == kBalanceGlobal) {
     for (uint32_t ch = blockIdx.x; ch < units; ch += gridDim.x)
       td_chunk<OP, MIRROR, false, true>(a, F, N, inq_cur, inq_nxt, ch, units, q_size, q_edges, S, L, my_raised);
-  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceLocal) {
+  } else if (td_variant(a, q_size, q_edges) == kBalanceLocal) {
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x)
       td_slots<OP, MIRROR, kBalanceLocal>(a, F, N, inq_cur, inq_nxt, u, q_size, S, L, my_raised);
   } else {
@@ -1945,7 +1948,7 @@ __device__ __forceinline__ uint32_t reach_units(const Ctrl* c, const LevelArgs& 
     const int64_t gw = (c->bu_wide || !c->bottom_up) ? kBWide : kBW;
     return (uint32_t)((a.nwords + gw - 1) / gw);
   }
-  return td_units(a.balance, c->q_size, c->q_edges);
+  return td_units(a, c->q_size, c->q_edges);
 }
 
 // The units [first, units) of one reach search, stepping by the grid, then a flush of the
@@ -1982,10 +1985,10 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
   } else if (!SLOTS) {
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
-  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceGlobal) {
+  } else if (td_variant(a, q_size, q_edges) == kBalanceGlobal) {
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       td_chunk<OP_FW, MIRROR>(a, F, N, nullptr, nullptr, lu, units, q_size, q_edges, S, L, unused);
-  } else if (td_variant(a.balance, q_size, q_edges) == kBalanceLocal) {
+  } else if (td_variant(a, q_size, q_edges) == kBalanceLocal) {
     for (uint32_t lu = first; lu < units; lu += gridDim.x)
       td_slots<OP_FW, MIRROR, kBalanceLocal>(a, F, N, nullptr, nullptr, lu, q_size, S, L, unused);
   } else {
@@ -2073,7 +2076,7 @@ __global__ void __launch_bounds__(kBlock) k_reach(ReachArgs p) {
 // Rows of the partials array written by the colour level that just ran: those of the
 // workgroups that had a unit.
 __device__ __forceinline__ int active_rows(int nparts, const LevelArgs& a, uint32_t q_size, uint32_t q_edges) {
-  return (int)min((uint32_t)nparts, td_units(a.balance, q_size, q_edges));
+  return (int)min((uint32_t)nparts, td_units(a, q_size, q_edges));
 }
 
 // Direction of the coming colour level: a push level whose frontier has at least
@@ -3384,7 +3387,9 @@ void launch_search_init_pair(const Launch& L, const LevelArgs& a0, const LevelAr
 
 void launch_level(const Launch& L, int op, bool mirror, const LevelArgs& a) {
   dim3 g((unsigned)L.grid), b(kBlock);
-  const bool slots = a.balance != kBalanceGlobal;  // the other balance variants (td_slots)
+  // the instantiations with the other balance variants (td_slots) only where a level may
+  // use one: mirrored levels under kBalanceAuto always take the chunks (td_variant)
+  const bool slots = a.balance != kBalanceGlobal && !(a.balance == kBalanceAuto && mirror);
   switch (op) {
     case OP_PUSH:
       if (slots) {
@@ -3419,7 +3424,8 @@ void launch_close(const Launch& L, int op, bool mirror, const LevelArgs& a) {
 
 void launch_reach(const Launch& L, bool mirror, const ReachArgs& p, int ns) {
   dim3 g((unsigned)L.grid), b(kBlock);
-  if (p.s[0].balance != kBalanceGlobal) {  // the other balance variants (td_slots)
+  // the other balance variants (td_slots), as in launch_level
+  if (p.s[0].balance != kBalanceGlobal && !(p.s[0].balance == kBalanceAuto && mirror)) {
     if (ns == 2) MSC_LAUNCH(L, (k_reach<2, false, true>), g, b, 0, L.stream, p);
     else if (mirror) MSC_LAUNCH(L, (k_reach<1, true, true>), g, b, 0, L.stream, p);
     else MSC_LAUNCH(L, (k_reach<1, false, true>), g, b, 0, L.stream, p);

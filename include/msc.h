@@ -233,8 +233,9 @@ typedef struct msc_tuning {
                                1 a workgroup deals the edges of its 256 frontier vertices
                                (fwbw_manhattan_local), 2 fixed-size chunks of the whole
                                frontier's edges (fwbw_manhattan_global), 3 per level: 1 while
-                               the frontier's lists average at most 4 entries, else 2 (the
-                               default).  Labels do not depend on it. */
+                               the frontier's lists average at most 4 entries, else 2; WCC
+                               levels, which walk two lists per vertex, 2 (the default).
+                               Labels do not depend on it. */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);
