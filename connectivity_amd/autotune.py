@@ -9,7 +9,8 @@ against each other differently on shallow power-law graphs and on deep meshes:
   small_cbw_edges   the same for the per-colour backward searches;
   first_batch       levels enqueued before a loop first reads its `done` word back;
   max_batch         levels enqueued between later reads (batches double up to this);
-  concurrent_fwbw   backward search beside the forward one in the same launches (1) or after it;
+  concurrent_fwbw   backward search beside the forward one in the same launches (1), after it (0),
+                    or beside it and restricted to the forward set once that is done (2);
   search_batch      levels of a FW-BW / CC search enqueued before its first host check;
   local_rounds      levels a workgroup of a colour launch continues on its own claims;
   local_us          ... while that launch has run for less than this many microseconds;
@@ -47,7 +48,7 @@ CANDIDATES = {
     "small_cbw_edges": [0, 1024, 2048, 4096, 8192, 16384],
     "first_batch": [2, 4, 8],
     "max_batch": [16, 32, 64],
-    "concurrent_fwbw": [0, 1],
+    "concurrent_fwbw": [0, 1, 2],
     "search_batch": [2, 4, 6, 8],
     "local_rounds": [0, 4, 16, 32, 64],
     "local_us": [40, 80, 160, 240, 320],

@@ -97,7 +97,8 @@ struct Ctrl {
   uint32_t bu_wide;                // reach: the coming bottom-up level walks kBWide-word groups
   uint32_t from_bits;              // reach: the coming top-down level expands the vertices the
                                    // bottom-up level before it claimed (LevelArgs::fresh), not a queue
-  uint32_t pad_ctrl_;
+  uint32_t within;                  // reach: claims only vertices the other search of the pivot
+                                   // visited (LevelArgs::within_cand), set once that one finished
   // counts the host reads with the next control-block copy instead of a round trip each
   unsigned long long red_trim[4];    // col 0: trim survivors
   unsigned long long red_commit[4];  // col 0: vertices committed by FW-BW / the CC search
@@ -191,6 +192,9 @@ struct LevelArgs {
   uint32_t pull_min;          // ... and over at least this many edges
   TailSeed tail;              // colouring round 0's seed: what the colour tail needs (tail.slots null: off)
   int32_t balance;            // top-down load balance of grid-wide levels (msc_tuning.balance)
+  const uint32_t* within_cand;  // backward search beside the forward one: the forward search's
+                                // claimable bitmap; once ctrl->within is set, only vertices it no
+                                // longer holds (the forward set) may be claimed (null: never)
 };
 
 // The reach searches one k_reach launch advances by a level: one (FW, BW or CC alone) or

@@ -186,21 +186,6 @@ __device__ __forceinline__ uint32_t bits_scan(BitIterT<WPT>& B, const uint32_t* 
   return B.pref[kW];
 }
 
-// bits_load that also loads this thread's words of a second bitmap into keep[].
-template <int WPT>
-__device__ __forceinline__ uint32_t bits_load_keep(BitIterT<WPT>& B, const uint32_t* bits, const uint32_t* other,
-                                                   int64_t nwords, int64_t wb, uint32_t* keep) {
-  const int t = threadIdx.x;
-  const int64_t w0 = wb + (int64_t)t * WPT;
-  uint32_t w[WPT];
-#pragma unroll
-  for (int j = 0; j < WPT; ++j) {
-    w[j] = w0 + j < nwords ? bits[w0 + j] : 0u;
-    keep[j] = w0 + j < nwords ? other[w0 + j] : 0u;
-  }
-  return bits_scan(B, w);
-}
-
 // Vertex id of the k-th set bit (k < total) of the loaded words.
 template <int WPT>
 __device__ __forceinline__ uint32_t bits_pick(const BitIterT<WPT>& B, int64_t wb, uint32_t k) {
@@ -690,6 +675,7 @@ __global__ void __launch_bounds__(kInitBlock) k_search_init(ReachArgs p, int32_t
     c->found = 0;
     c->phase = (uint32_t)(phase0 + s);
     c->abandoned = 0u;
+    c->within = 0u;
     c->t_start = ~0ull;
     uint32_t m = 1u << ((uint32_t)root & 31u);
     a.cand[(uint32_t)root >> 5] &= ~m;
@@ -718,6 +704,7 @@ __device__ __forceinline__ void ctrl_fresh(Ctrl* c, int32_t phase) {
   c->found = 0;
   c->phase = (uint32_t)phase;
   c->abandoned = 0u;
+  c->within = 0u;
   c->t_start = ~0ull;
 }
 
@@ -847,6 +834,7 @@ __global__ void k_ctrl_reset(Ctrl* c, int32_t phase) {
     c->found = 0;
     c->phase = (uint32_t)phase;
     c->abandoned = 0u;
+    c->within = 0u;
     c->t_start = ~0ull;
   }
 }
@@ -884,17 +872,24 @@ __device__ __forceinline__ void td_claim(const LevelArgs& a, uint32_t* inq_nxt, 
                                          unsigned long long& my_raised) {
   constexpr bool REACH = OP == OP_FW || OP == OP_BW || OP == OP_CC;
   constexpr bool COLOR = OP == OP_PUSH || OP == OP_CBW;
-  uint32_t word[kPer];
+  uint32_t word[kPer], excl[kPer];
   int32_t wcol[kPer];
+  // a backward search restricted to the finished forward set: its claimable bitmap still
+  // covers the whole valid set, so a probe also reads the forward search's
+  const uint32_t* within = (REACH && a.within_cand && a.ctrl->within) ? a.within_cand : nullptr;
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     word[r] = 0;
+    excl[r] = 0;
     wcol[r] = 0;
     if (ok[r]) {
       word[r] = (REACH ? a.cand : a.valid)[w[r] >> 5];
+      if (within) excl[r] = within[w[r] >> 5];
       if (COLOR) wcol[r] = a.colors[w[r]];
     }
   }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) word[r] &= ~excl[r];
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     bool in = ok[r] && ((word[r] >> (w[r] & 31u)) & 1u);
@@ -1218,9 +1213,22 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
   const int t = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < WPT; ++j) s_claim[t * WPT + j] = 0u;
-  // candidates: the claimable bits; this thread's seen words are kept for the write-back
-  uint32_t sw[WPT];
-  const uint32_t total = bits_load_keep(B, a.cand, a.seen, a.nwords, wb, sw);  // barrier: s_claim cleared
+  // candidates: the claimable bits (restricted to the forward set for a backward search
+  // once ctrl->within is set); this thread's claimable and seen words are kept for the
+  // write-back
+  uint32_t sw[WPT], cw[WPT], w[WPT];
+  const uint32_t* within = (a.within_cand && a.ctrl->within) ? a.within_cand : nullptr;
+  {
+    const int64_t w0 = wb + (int64_t)t * WPT;
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+      const bool in = w0 + j < a.nwords;
+      cw[j] = in ? a.cand[w0 + j] : 0u;
+      sw[j] = in ? a.seen[w0 + j] : 0u;
+      w[j] = cw[j] & ~((in && within) ? within[w0 + j] : 0u);
+    }
+  }
+  const uint32_t total = bits_scan(B, w);  // barrier: s_claim cleared
   for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kR) {
     uint32_t v[kR], beg[kR], end[kR], beg2[kR], end2[kR], u[kR], wc[kR];
     bool has[kR], hit[kR], first[kR];
@@ -1293,7 +1301,7 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
     const uint32_t c = s_claim[i];
     a.fresh[wb + i] = c;
     if (c) {
-      a.cand[wb + i] = B.word[i] & ~c;
+      a.cand[wb + i] = cw[j] & ~c;
       a.seen[wb + i] = sw[j] & ~c;
     }
   }
@@ -2340,6 +2348,12 @@ __global__ void __launch_bounds__(kCloseBlock) k_close_reach(ReachArgs p, int np
     else
       close_search(p.s[1], c1, s_nv[1], tot[0], tot[1], tot[2], tot[3], learned1, m, s_feat[1], s_z[1], s_h[1]);
   }
+  __syncthreads();
+  // once the forward search has finished (and was not abandoned), a backward search beside
+  // it claims only vertices of the forward set: a path from an SCC member to the pivot never
+  // leaves the SCC, so the SCC (forward set & backward set) is unchanged, and the backward
+  // search no longer explores what reaches the pivot from outside the forward set
+  if (NS == 2 && t == 0 && p.s[1].within_cand && c0->done && !c0->abandoned && !c1->done) c1->within = 1u;
   __syncthreads();
   if (t < NS * kCW && (t < kCW ? u0 : u1))
     reinterpret_cast<unsigned long long*>(p.s[t / kCW].ctrl)[t % kCW] = s_ctl[t / kCW][t % kCW];

@@ -66,7 +66,7 @@ struct msc_ctx : WsView {
   bool own_stream = false;
   int grid = 1024;
   int cus = 256;
-  msc_tuning tune{5, 4096, 4096, 2, 32, 1, 6, 64, 240, 2, 65536, 3};
+  msc_tuning tune{5, 4096, 4096, 2, 32, 2, 6, 64, 240, 2, 65536, 3};
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
   float* mlp = nullptr;
@@ -373,6 +373,13 @@ int run_loop(msc_ctx* c, int op, bool mirror, LevelArgs* as, int k, int kcls, ms
 
 int run_levels(msc_ctx* c, int op, bool mirror, LevelArgs a, int kcls, msc_stats* s, int first_batch) {
   return run_loop(c, op, mirror, &a, 1, kcls, s, first_batch);
+}
+
+// Whether a pivot's forward and backward searches advance side by side (msc_tuning
+// concurrent_fwbw: 1 always, 2 for one graph; a batch's merged searches then run in turn,
+// which measured faster: the backward search stays inside the forward sets).
+bool side_by_side(const msc_ctx* c, bool batch) {
+  return c->tune.concurrent_fwbw == 1 || (c->tune.concurrent_fwbw == 2 && !batch);
 }
 
 LevelArgs base_args(msc_ctx* c, const msc_csr* g, int32_t* labels, const msc_opts* o, double avg) {
@@ -696,7 +703,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   const bool replay_ok = o->replay && !bt && !noalloc && g_in->where == MSC_DEVICE &&
                          o->labels_where == MSC_DEVICE && pivot >= 0 && o->trim_level == 1 && !o->validate &&
                          !o->timing && !o->record && !o->schedule && o->fwbw_rounds <= 1 &&
-                         (cc || c->tune.concurrent_fwbw);
+                         (cc || side_by_side(c, false));
   std::vector<uint64_t> key;
   if (replay_ok) {
     uint64_t avg_bits;
@@ -837,8 +844,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       bw.op = OP_BW;
       bw.down_off = g.in_off; bw.down_adj = g.in_adj;
       bw.up_off = g.out_off; bw.up_adj = g.out_adj;
-      if (c->tune.concurrent_fwbw) {
+      if (side_by_side(c, bt != nullptr)) {
         bw.allow = c->valid; bw.cand = c->b3; bw.seen = c->s3; bw.fresh = c->f3;
+        // the automatic mode restricts BW to the forward set once FW has finished
+        if (c->tune.concurrent_fwbw == 2) bw.within_cand = c->b1;
         bw.ctrl = &cs[1]; bw.part = c->part2;
         bw.fr0 = c->fr2; bw.fr1 = c->fr3;
         bw.recs = o->record ? c->recs + kRecCap / 2 : nullptr;
@@ -1403,7 +1412,7 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
   if (!c || !t) return MSC_E_INVALID;
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 8 || t->small_push_edges < 0 || t->small_cbw_edges < 0 ||
       t->first_batch < 1 || t->max_batch < t->first_batch || t->max_batch > 256 || t->concurrent_fwbw < 0 ||
-      t->concurrent_fwbw > 1 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
+      t->concurrent_fwbw > 2 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
       t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024 ||
       t->tail_vertices < 0 || t->tail_vertices > (int32_t)kTailV || t->balance < 0 || t->balance > 3)
     return MSC_E_INVALID;

@@ -214,7 +214,10 @@ typedef struct msc_tuning {
   int32_t max_batch;        /* levels enqueued between later host checks */
   int32_t concurrent_fwbw;  /* 1: the backward search advances beside the forward one, both in
                                one launch per level, over the valid set (SCC = FW set & BW set);
-                               0: after it, restricted to the FW set, the reference's order */
+                               0: after it, restricted to the FW set, the reference's order;
+                               2 (the default): as 1 for one graph, but once the forward search
+                               has finished the backward one claims only vertices of the FW set;
+                               as 0 for a batch */
   int32_t search_batch;     /* levels of a FW-BW / CC search enqueued before its first host
                                check (colour loops start with first_batch) */
   int32_t local_rounds;     /* levels a workgroup of a colour launch continues on its own claims

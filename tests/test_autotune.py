@@ -32,7 +32,7 @@ def test_candidates_are_accepted_values():
 
     # msc_ctx_set_tuning's ranges (include/msc.h): every candidate value lies inside them
     lim = {"blocks_per_cu": (1, 8), "small_push_edges": (0, 1 << 30), "small_cbw_edges": (0, 1 << 30),
-           "first_batch": (1, 256), "max_batch": (1, 256), "concurrent_fwbw": (0, 1), "search_batch": (1, 256),
+           "first_batch": (1, 256), "max_batch": (1, 256), "concurrent_fwbw": (0, 2), "search_batch": (1, 256),
            "local_rounds": (0, 256), "local_us": (1, 100000), "pull_div": (0, 1024),
            "tail_vertices": (0, 65536), "balance": (0, 3)}
     for field, values in autotune.CANDIDATES.items():

@@ -212,11 +212,45 @@ def test_tuning_keeps_labels(msc, orc, gpu_ctx, tuning):
         gpu_ctx.set_tuning(**defaults)
 
 
+@pytest.mark.parametrize("kind", ["grid", "planted", "rmat", "random"])
+def test_backward_search_within_forward_set(msc, orc, gpu_ctx, kind):
+    """concurrent_fwbw 2: beside the forward search, the backward one claims only vertices
+    of the forward set once that search has finished.  Raw and canonical labels equal the
+    oracle's in every mode, and the restricted search never takes more levels than the
+    unrestricted one (mode 1)."""
+    if kind == "grid":
+        g = msc.gen_grid(512, 512, seed=5)
+    elif kind == "planted":
+        g = msc.gen_planted(1 << 15, max_size=256, seed=6)[0]
+    elif kind == "rmat":
+        g = msc.gen_rmat(15, 8, seed=7)
+    else:
+        rng = np.random.default_rng(8)
+        g = msc.create_csr(20000, rng.integers(0, 20000, 30000), rng.integers(0, 20000, 30000))
+    piv, avg = orc.pivot_scc(g)
+    want = orc.scc(g, piv, avg, vert_cutoff=0)
+    defaults = gpu_ctx.tuning()
+    levels = {}
+    try:
+        for mode in (0, 1, 2):
+            gpu_ctx.set_tuning(concurrent_fwbw=mode)
+            _say(f"case: {kind} concurrent_fwbw={mode} n={g.n} m={g.m}")
+            got, st = msc.run_scc(g, piv, avg, labels=msc.LABELS_RAW, ctx=gpu_ctx, stats=True)
+            bad = np.flatnonzero(got != want)
+            assert bad.size == 0, f"mode {mode}: raw labels differ at {bad.size} vertices, first {bad[:5]}"
+            assert np.array_equal(msc.run_scc(g, piv, avg, ctx=gpu_ctx), orc.canonical(want)), mode
+            levels[mode] = (st.fw_levels, st.bw_levels)
+    finally:
+        gpu_ctx.set_tuning(**defaults)
+    assert levels[2][0] == levels[1][0], levels  # the forward search is not touched
+    assert levels[2][1] <= levels[1][1], levels
+
+
 def test_tuning_rejects_bad_values(msc, gpu_ctx):
     defaults = gpu_ctx.tuning()
     for bad in (dict(blocks_per_cu=0), dict(blocks_per_cu=9), dict(small_push_edges=-1),
                 dict(first_batch=0), dict(first_batch=8, max_batch=4), dict(max_batch=257),
-                dict(concurrent_fwbw=2), dict(search_batch=0), dict(search_batch=257), dict(local_rounds=-1),
+                dict(concurrent_fwbw=3), dict(search_batch=0), dict(search_batch=257), dict(local_rounds=-1),
                 dict(local_rounds=257), dict(local_us=0), dict(pull_div=-1), dict(pull_div=1025),
                 dict(tail_vertices=-1), dict(tail_vertices=65537), dict(balance=-1), dict(balance=4)):
         with pytest.raises(msc.MscError):

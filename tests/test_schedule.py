@@ -79,7 +79,7 @@ def _cases(msc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("concurrent", [0, 1])
+@pytest.mark.parametrize("concurrent", [0, 1, 2])
 def test_plain_schedule_matches_oracle(msc, orc, gpu_ctx, concurrent):
     defaults = gpu_ctx.tuning()
     try:
@@ -91,7 +91,8 @@ def test_plain_schedule_matches_oracle(msc, orc, gpu_ctx, concurrent):
             want, lv, _ = orc.scc(g, piv, avg, vert_cutoff=0, trace_cap=1 << 14)
             assert np.array_equal(got, want), name
             # beside the forward search, the backward one searches the whole valid set
-            # (SCC = FW set & BW set), so only FW follows the reference's schedule there
+            # (SCC = FW set & BW set; with 2 until the forward search has finished), so
+            # only FW follows the reference's schedule there
             for phase in ((0, 1) if concurrent == 0 else (0,)):
                 dev = [(r.phase, r.level, r.bottom_up, r.next_size) for r in recs if r.phase == phase]
                 ref = [x for x in lv if x[0] == phase]
