@@ -16,7 +16,9 @@ The line also carries:
   cpu_baseline  -- the reference's own multistep run_scc (oracle/_ref, compiled from its
                    sources) or, where that was not built, the oracle's restatement, timed on
                    this host's cores on the same graph;
-  wcc_ms        -- msc_cc_run on the same graph (not part of `value`).
+  wcc_ms        -- msc_cc_run on the same graph with the pivot and average degree cc_main's
+                   create_csr gives (not part of `value`); wcc_parity: its canonical labels
+                   against the oracle's.
 
 Workloads: rmat (scale 24, the metric's), grid (4096 x 4096 grid digraph), batch (8192
 packed graphs, msc_scc_run_batch).
@@ -231,14 +233,25 @@ def main() -> int:
             return 1
     counts = msc.scc_verify(labels) if base is None else None
 
+    # WCC with run_cc's call shape: the pivot and average degree come with the graph
+    # (cc_main.cpp's create_csr computes them, untimed), as they do for SCC
+    cc_piv = msc.cc_pivot(dg, ctx=ctx) if base is None else (None, None)
     cc_ms = []
     for _ in range(max(2, args.steps // 2)):
         t1 = time.perf_counter()
         if base is not None:
             msc.run_cc_batch(dg, base, ctx=ctx, out=out, policy=policy)
         else:
-            msc.run_cc(dg, ctx=ctx, out=out, policy=policy)
+            msc.run_cc(dg, cc_piv[0], cc_piv[1], ctx=ctx, out=out, policy=policy)
         cc_ms.append((time.perf_counter() - t1) * 1e3)
+    cc_check = "skipped"
+    if not args.no_check and base is None:
+        import oracle
+
+        cc_check = "identical" if np.array_equal(out.to_host(), oracle.canonical(oracle.cc(g))) else "MISMATCH"
+        log(f"WCC parity vs oracle (canonical labels): {cc_check}")
+        if cc_check != "identical":
+            return 1
 
     cpu = None
     if not args.no_cpu:
@@ -275,6 +288,7 @@ def main() -> int:
                        if counts else None),
         "parity": check,
         "wcc_ms": round(statistics.median(cc_ms), 4),
+        "wcc_parity": cc_check,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -25,7 +25,7 @@ import numpy as np
 __all__ = [
     "MscError", "Graph", "DeviceGraph", "DeviceLabels", "Context", "Stats", "LevelRecord",
     "read_edge", "create_csr", "gen_rmat", "gen_grid", "gen_planted",
-    "run_scc", "run_scc_noalloc", "run_cc", "scc_pivot", "batch_pivots", "scc_verify", "cc_verify", "verify_line",
+    "run_scc", "run_scc_noalloc", "run_cc", "scc_pivot", "cc_pivot", "batch_pivots", "scc_verify", "cc_verify", "verify_line",
     "canonicalize", "write_labels", "device_count", "lib_path",
     "POLICY_PLAIN", "POLICY_LEARNED", "LABELS_CANONICAL", "LABELS_RAW", "schedule_array",
     "set_policy", "load_policy", "mlp_forward", "mlp_grad", "mlp_train", "POLICY_WEIGHTS",
@@ -159,6 +159,8 @@ def _lib():
                                     C.POINTER(C.POINTER(_HostGraph)), P, P]),
         "msc_scc_pivot": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.c_int32),
                                     C.POINTER(C.c_double)]),
+        "msc_cc_pivot": (C.c_int, [C.POINTER(_Csr), C.POINTER(_Opts), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double)]),
         "msc_scc_verify": (C.c_int, [C.c_int32, P, C.POINTER(_Verify)]),
         "msc_cc_verify": (C.c_int, [C.c_int32, P, C.POINTER(_Verify)]),
         "msc_verify_format": (C.c_int, [C.POINTER(_Verify), C.c_int32, C.c_char_p, C.c_int32]),
@@ -821,6 +823,16 @@ def scc_pivot(g, ctx: Context | None = None) -> tuple[int, float]:
     piv, avg = C.c_int32(), C.c_double()
     csr = g.csr()
     _check(_lib().msc_scc_pivot(C.byref(csr), C.byref(o), C.byref(piv), C.byref(avg)), "msc_scc_pivot")
+    return piv.value, avg.value
+
+
+def cc_pivot(g, ctx: Context | None = None) -> tuple[int, float]:
+    """cc_main's create_csr pivot (largest degree in the mirrored graph) and average degree
+    (2m/n), computed on device: the max_deg_vert / avg_degree run_cc takes."""
+    o = _opts(POLICY_PLAIN, LABELS_CANONICAL, False, False, False, ctx)
+    piv, avg = C.c_int32(), C.c_double()
+    csr = g.csr()
+    _check(_lib().msc_cc_pivot(C.byref(csr), C.byref(o), C.byref(piv), C.byref(avg)), "msc_cc_pivot")
     return piv.value, avg.value
 
 

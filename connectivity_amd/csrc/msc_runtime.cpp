@@ -1297,6 +1297,30 @@ int ctx_set_policy(msc_ctx* c, const float* blob) {
 
 }  // namespace msc
 
+namespace {
+// create_csr's pivot and average degree on device: SCC (in x out product) or CC (degree in
+// the mirrored graph, average over its 2m entries).
+int pivot_call(const msc_csr* g, const msc_opts* o, bool cc, int32_t* max_deg_vert, double* avg_degree) {
+  if (!g || !o || !max_deg_vert || !avg_degree || g->n < 0) return MSC_E_INVALID;
+  if (g->n == 0) {
+    *max_deg_vert = -1;
+    *avg_degree = 0.0;
+    return MSC_OK;
+  }
+  int rc = MSC_OK;
+  msc_ctx* c = o->ctx ? o->ctx : default_ctx(o->device, &rc);
+  if (!c) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIP_TRY(hipSetDevice(c->device));
+  if ((rc = ensure_workspace(c, g->n, g->m))) return rc;
+  msc_csr d;
+  if ((rc = stage_graph(c, g, &d))) return rc;
+  if ((rc = device_pivot(c, &d, cc, max_deg_vert))) return rc;
+  *avg_degree = (cc ? 2.0 : 1.0) * (double)g->m / (double)g->n;
+  return MSC_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* msc_version(void) { return "msc 0.1 (gfx950)"; }
@@ -1510,23 +1534,11 @@ int msc_cc_run_batch(const msc_csr* g, int32_t num_graphs, const int64_t* vert_b
 }
 
 int msc_scc_pivot(const msc_csr* g, const msc_opts* o, int32_t* max_deg_vert, double* avg_degree) {
-  if (!g || !o || !max_deg_vert || !avg_degree || g->n < 0) return MSC_E_INVALID;
-  if (g->n == 0) {
-    *max_deg_vert = -1;
-    *avg_degree = 0.0;
-    return MSC_OK;
-  }
-  int rc = MSC_OK;
-  msc_ctx* c = o->ctx ? o->ctx : default_ctx(o->device, &rc);
-  if (!c) return rc;
-  std::lock_guard<std::mutex> lk(c->mu);
-  HIP_TRY(hipSetDevice(c->device));
-  if ((rc = ensure_workspace(c, g->n, g->m))) return rc;
-  msc_csr d;
-  if ((rc = stage_graph(c, g, &d))) return rc;
-  if ((rc = device_pivot(c, &d, false, max_deg_vert))) return rc;
-  *avg_degree = (double)g->m / (double)g->n;
-  return MSC_OK;
+  return pivot_call(g, o, false, max_deg_vert, avg_degree);
+}
+
+int msc_cc_pivot(const msc_csr* g, const msc_opts* o, int32_t* max_deg_vert, double* avg_degree) {
+  return pivot_call(g, o, true, max_deg_vert, avg_degree);
 }
 
 int msc_graph_ingest_text(msc_ctx* c, const char* text, int64_t bytes, msc_csr* dev, int32_t* max_deg_vert,

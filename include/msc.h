@@ -310,6 +310,10 @@ int msc_cc_run_batch(const msc_csr* g, int32_t num_graphs, const int64_t* vert_b
  * (multistep/scc_main.cpp:169-183; kokkos_scc/scc_main.cpp:228-247). */
 int msc_scc_pivot(const msc_csr* g, const msc_opts* opts, int32_t* max_deg_vert,
                   double* avg_degree);
+/* The same for CC/WCC: the first vertex of largest degree in the mirrored graph (out + in)
+ * and the mirrored graph's average degree 2m/n.  Replaces the loop inside cc_main's
+ * create_csr (multistep/cc_main.cpp:148-160), whose results run_cc takes as arguments. */
+int msc_cc_pivot(const msc_csr* g, const msc_opts* opts, int32_t* max_deg_vert, double* avg_degree);
 
 /* ---- verification ---------------------------------------------------------------- */
 typedef struct msc_verify {

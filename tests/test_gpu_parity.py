@@ -169,6 +169,10 @@ def test_device_pivot(msc, orc, gpu_ctx):
     for seed in range(3):
         g = msc.gen_rmat(12, 8, seed=seed)
         assert msc.scc_pivot(g, ctx=gpu_ctx) == orc.pivot_scc(g)
+        # cc_main.cpp:148-160: largest out + in degree, first such vertex; avg over 2m
+        assert msc.cc_pivot(g, ctx=gpu_ctx) == orc.pivot_cc(g)
+        piv, avg = msc.cc_pivot(g, ctx=gpu_ctx)
+        assert np.array_equal(msc.run_cc(g, piv, avg, labels=msc.LABELS_RAW, ctx=gpu_ctx), orc.cc(g))
 
 
 def test_stats_and_levels(msc, orc, gpu_ctx):
