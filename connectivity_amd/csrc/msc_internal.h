@@ -329,7 +329,7 @@ void launch_seed_pivots(const Launch& L, bool mirror, const LevelArgs& a, const 
 // launch_commit with labels[v] = piv[graph of v]
 int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
                         const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
-                        const LevelArgs* seed, int32_t* list);
+                        const LevelArgs* seed, int32_t* list, int32_t* cmin);
 // labels[v] = (mins ? mins[labels[v]] : labels[v]) - base of v's graph, once *gate != 0
 void launch_localise(const Launch& L, int32_t n, int32_t* labels, const int32_t* mins, const int64_t* base,
                      uint32_t ng, const uint32_t* gate);

@@ -3290,13 +3290,17 @@ __global__ void __launch_bounds__(kBlock) k_commit(MemberSrc m, uint32_t* __rest
 // 64-vertex chunks, four at a time with their bitmap loads in flight together, and finds
 // the graph of its first vertex with one binary search over `base`; from there every lane
 // steps forward over the graph boundaries its vertices cross, as k_localise does (a binary
-// search per chunk made this pass 0.68 ms on the 8192-graph batch).  Part row col 0: the
-// vertices committed by this workgroup.
+// search per chunk made this pass 0.68 ms on the 8192-graph batch).  With cmin (canonical
+// labels; cmin[v] = v beforehand) each graph's smallest member is folded into cmin[pivot]:
+// the wave keeps a running minimum for the graph its lanes are in and issues one atomicMin
+// when it moves on to the next graph or ends.  Part row col 0: the vertices committed by
+// this workgroup.
 __global__ void __launch_bounds__(kBlock) k_commit_batch(MemberSrc m, uint32_t* __restrict__ valid, int32_t n,
                                                          int32_t* __restrict__ labels,
                                                          const int32_t* __restrict__ piv,
                                                          const int64_t* __restrict__ base, uint32_t ng,
-                                                         unsigned long long* __restrict__ part, const RoundCtl* rc) {
+                                                         unsigned long long* __restrict__ part, const RoundCtl* rc,
+                                                         int32_t* __restrict__ cmin) {
   __shared__ unsigned long long s_red[kBlock / 64];
   if (rc && !rc->go) return;  // uniform: the searches have not finished
   const uint32_t keep = member_keep(m);
@@ -3309,6 +3313,7 @@ __global__ void __launch_bounds__(kBlock) k_commit_batch(MemberSrc m, uint32_t* 
   unsigned long long cnt = 0;
   if (c0 < c1) {  // per wave
     uint32_t g = graph_of(base, 0u, ng, c0 * 64);
+    uint32_t run_g = 0xffffffffu, run_min = 0xffffffffu;  // the wave's pending minimum
     for (int64_t c = c0; c < c1; c += kU) {
       uint32_t vw[kU], w[kU];
 #pragma unroll
@@ -3319,17 +3324,39 @@ __global__ void __launch_bounds__(kBlock) k_commit_batch(MemberSrc m, uint32_t* 
       }
 #pragma unroll
       for (int r = 0; r < kU; ++r) {
+        if (c + r >= c1) break;  // uniform
         const int64_t v = (c + r) * 64 + lane();
-        if (c + r >= c1 || v >= n) break;
-        while (base[g + 1] <= v) ++g;  // base[ng] = n > v ends it
-        const bool in = (w[r] >> (v & 31)) & 1u;
-        labels[v] = in ? piv[g] : (int32_t)v;
-        if ((lane() & 31) == 0 && w[r]) {
+        const bool here = v < n;  // all lanes stay for the wave-wide steps below
+        if (here)
+          while (base[g + 1] <= v) ++g;  // base[ng] = n > v ends it
+        const bool in = here && ((w[r] >> (v & 31)) & 1u);
+        if (here) labels[v] = in ? piv[g] : (int32_t)v;
+        if (here && (lane() & 31) == 0 && w[r]) {
           valid[v >> 5] = vw[r] & ~w[r];
           cnt += (uint32_t)__popc(w[r]);
         }
+        if (cmin) {
+          // the members' graphs, in increasing order across the lanes and the run
+          bool active = in;
+          for (;;) {
+            const unsigned long long mk = __ballot(active);
+            if (mk == 0ull) break;
+            const uint32_t key = __shfl(g, __ffsll((long long)mk) - 1, 64);
+            const bool mine = active && g == key;
+            const uint32_t mn = wave_min(mine ? (uint32_t)v : 0xffffffffu);
+            if (key != run_g) {
+              if (run_g != 0xffffffffu && lane() == 0) atomicMin(&cmin[piv[run_g]], (int32_t)run_min);
+              run_g = key;
+              run_min = mn;
+            } else {
+              run_min = min(run_min, mn);
+            }
+            active = active && !mine;
+          }
+        }
       }
     }
+    if (cmin && run_g != 0xffffffffu && lane() == 0) atomicMin(&cmin[piv[run_g]], (int32_t)run_min);
   }
   cnt = block_sum64(cnt, s_red);
   if (threadIdx.x == 0) {
@@ -3540,11 +3567,14 @@ void launch_seed_bits(const Launch& L, const LevelArgs& a, int32_t n, int32_t* l
 
 int launch_commit_batch(const Launch& L, const MemberSrc& m, uint32_t* valid, int32_t n, int32_t* labels,
                         const int32_t* piv, const int64_t* base, uint32_t ng, unsigned long long* part,
-                        const LevelArgs* seed, int32_t* list) {
+                        const LevelArgs* seed, int32_t* list, int32_t* cmin) {
   int g = grid_for(n, kBlock * 16, L.grid * 2);
   LevelArgs a = seed ? *seed : LevelArgs{};
+  if (cmin)  // cmin = identity, when the commit itself runs (the round's go)
+    MSC_LAUNCH(L, k_iota, dim3(grid_for(n, kBlock, L.grid * 4)), dim3(kBlock), 0, L.stream, cmin, (int64_t)n,
+               a.rc ? &a.rc->go : nullptr);
   MSC_LAUNCH(L, k_commit_batch, dim3(g), dim3(kBlock), 0, L.stream, member_src(m, valid), valid, n, labels, piv,
-             base, ng, part, a.rc);
+             base, ng, part, a.rc, cmin);
   if (seed)
     MSC_LAUNCH(L, k_seed_bits, dim3(grid_for((n + 31) / 32, BitIterT<4>::kWords, L.grid)), dim3(kBlock), 0,
                        L.stream, a, list);

@@ -809,6 +809,9 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // takes its smallest member, colour SCCs track theirs in cmin (CC finds them after its
   // fixpoint, over the colouring's vertex list); the batch paths relabel at the end instead
   const bool fast_canon = !bt && o->label_mode == MSC_LABELS_CANONICAL;
+  // batch SCC: each graph's pivot component takes its minimum in the commit and every colour
+  // SCC in its backward search (into cmin), applied by the final localising pass
+  const bool batch_canon = bt && !cc && o->label_mode == MSC_LABELS_CANONICAL;
   // control blocks: [0] forward (or CC) search, [1] backward search, [2] colour push,
   // [3] colour-BW; rcd: the colouring rounds' device state
   Ctrl* cs = c->ctrl;
@@ -1007,7 +1010,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.pull_div = (uint32_t)c->tune.pull_div;
     p.pull_min = floor_edges > 0xffffffffll ? 0xffffffffu : (uint32_t)floor_edges;
   }
-  if (fast_canon && !cc) p.cmin = c->cmin;  // tracked by the colour-BW searches
+  if ((fast_canon || batch_canon) && !cc) p.cmin = c->cmin;  // tracked by the colour-BW searches
   // the colour tail (batches colour their union in rounds); its slot map borrows cmin,
   // whose entries the rounds only read after writing them, and its edge list frontier 0's
   // queue and offsets, which nothing reads before round 0's first push level
@@ -1057,7 +1060,8 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       // ended), which every commit workgroup reduces for itself
       const bool want_min = fast_canon && member.ca;
       const int npm = want_min ? launch_min_member(L, member, c->valid, nw, c->part2, rcd) : 0;
-      npc = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p, list_of(0))
+      npc = bt ? launch_commit_batch(L, member, c->valid, n, labels, c->bpiv, c->bbase, ng, c->part, &p, list_of(0),
+                                     batch_canon ? c->cmin : nullptr)
                : launch_commit(L, member, c->valid, n, labels, pivot, nullptr, c->part, &p, list_of(0), nullptr, false,
                                want_min ? c->part2 : nullptr, npm);
       if (!tail) {
@@ -1127,9 +1131,11 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       if (fast_canon) launch_canon_list(L, list_of(0), rcd, n, labels, c->cmin);
       // batches: the minimum member of every label class, then one pass that applies it
       // and turns union ids into each graph's own ids
-      const bool gen_canon = o->label_mode == MSC_LABELS_CANONICAL && !fast_canon;
+      const bool gen_canon = o->label_mode == MSC_LABELS_CANONICAL && !fast_canon && !batch_canon;
       if (gen_canon) launch_canon(L, n, labels, c->colors, done_gate, !bt);
-      if (bt) launch_localise(L, n, labels, gen_canon ? c->colors : nullptr, c->bbase, ng, done_gate);
+      if (bt)
+        launch_localise(L, n, labels, gen_canon ? c->colors : batch_canon ? c->cmin : nullptr, c->bbase, ng,
+                        done_gate);
     }
     HIP_TRY(hipGetLastError());
     return MSC_OK;
