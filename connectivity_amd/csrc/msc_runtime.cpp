@@ -927,7 +927,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
       s0 = &cs[0];
       member = MemberSrc{c->b1, nullptr, &cs[0], nullptr};  // valid & visited
     }
-    if ((rc = enqueue_levels(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, c->tune.search_batch))) return rc;
+    // (none left when the searches ran in turn and the forward one was abandoned: nothing
+    // is committed and the colouring takes every vertex)
+    if (nsearch > 0 && (rc = enqueue_levels(c, sop, smirror, sa, nsearch, MSC_K_LEVEL, c->tune.search_batch)))
+      return rc;
   }
   flush_trim();  // no search (no pivot)
 

@@ -122,6 +122,37 @@ def test_forced_schedules_keep_labels(msc, orc, gpu_ctx, graph):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("concurrent", [0, 1, 2])
+def test_abandoned_searches_every_mode(msc, orc, gpu_ctx, concurrent):
+    """An abandoned search (the policy's third action) leaves its vertices to colouring with
+    the searches side by side and in turn, for one graph and for a batch (which runs its
+    searches in turn under mode 2): canonical labels stay the oracle's.  Abandoning the
+    forward search in turn leaves no backward search to run."""
+    g = msc.gen_rmat(13, 16, seed=5)
+    b, base, _ = msc.gen_batch(24, seed=6)
+    want = orc.canonical(orc.scc(g, vert_cutoff=0))
+    defaults = gpu_ctx.tuning()
+    try:
+        gpu_ctx.set_tuning(concurrent_fwbw=concurrent)
+        for fw, bw in (([2], None), (None, [2]), ([0, 2], [2]), ([1, 1, 2], None)):
+            print(f"case: abandon fw={fw} bw={bw} concurrent_fwbw={concurrent}", flush=True)
+            sched = msc.schedule_array(fw=fw, bw=bw)
+            assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx, schedule=sched), want), (fw, bw)
+            got = msc.run_scc_batch(b, base, ctx=gpu_ctx, schedule=sched)
+            for i in range(len(base) - 1):
+                b0, b1 = int(base[i]), int(base[i + 1])
+                e0, e1 = int(b.out_off[b0]), int(b.out_off[b1])
+                f0, f1 = int(b.in_off[b0]), int(b.in_off[b1])
+                h = msc.Graph(n=b1 - b0, out_off=(b.out_off[b0:b1 + 1] - b.out_off[b0]).astype(np.uint32),
+                              out_adj=(b.out_adj[e0:e1] - b0).astype(np.int32),
+                              in_off=(b.in_off[b0:b1 + 1] - b.in_off[b0]).astype(np.uint32),
+                              in_adj=(b.in_adj[f0:f1] - b0).astype(np.int32))
+                assert np.array_equal(got[b0:b1], orc.canonical(orc.scc(h, vert_cutoff=0))), (fw, bw, i)
+    finally:
+        gpu_ctx.set_tuning(**defaults)
+
+
+@pytest.mark.gpu
 def test_forced_actions_are_recorded(msc, gpu_ctx):
     g = msc.gen_rmat(14, 16, seed=3)
     sched = msc.schedule_array(fw=[1, 1, 0, 0, 0, 0, 0, 0])
