@@ -355,8 +355,9 @@ __device__ __forceinline__ void stage_write(Stage& S, uint32_t n, uint32_t total
 // Workgroup-wide (every thread calls it): when more than `keep_below` claims are staged,
 // reserves their queue slots and edge offsets with one 64-bit atomic and writes ids,
 // offsets and chunk starts to frontier `nx` (and the ids to `list` at the same slots, when
-// given).  Always a barrier.
-__device__ void stage_flush(Stage& S, const Frontier& nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr,
+// given).  Always a barrier.  The frontier comes by value: where the call is not inlined,
+// a reference would make the caller keep its (runtime-selected) frontier in scratch memory.
+__device__ void stage_flush(Stage& S, Frontier nx, Ctrl* c, uint32_t keep_below, int32_t* list = nullptr,
                             const TailSeed* ts = nullptr) {
   __syncthreads();
   const uint32_t n = S.cnt;
@@ -918,10 +919,14 @@ __device__ __forceinline__ void td_claim(const LevelArgs& a, uint32_t* inq_nxt, 
   }
 }
 
+static_assert(kChunk < 65536u, "a chunk's owner table holds 16-bit slot indices");
 template <int OP, bool MIRROR>
 struct ChunkLds {
   static constexpr bool kColor = OP == OP_PUSH || OP == OP_CBW;
-  uint32_t own[kChunk];
+  union {
+    uint16_t own[kChunk];  // td_chunk: the slot that owns each edge of the chunk
+    uint32_t pre[kBlock];  // td_slots: exclusive prefix of the slots' list lengths
+  };
   uint32_t base[kChunk + 1];
   int32_t col[kColor ? kChunk + 1 : 1];
   uint32_t split[MIRROR ? kChunk + 1 : 1];
@@ -959,7 +964,11 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
     uint32_t v = (uint32_t)F.q[k];
     uint32_t s = qo > e0 ? qo - e0 : 0u;
     if (s < len) {
-      atomicMax(&L.own[s], j);
+      // slot j owns the edges from its start on; of slots starting at one offset (all but
+      // the last would have empty lists) the last one writes.  Slot 0 holds edge e0, so no
+      // later slot starts at offset 0 and its entry is the cleared one.
+      const uint32_t qn = k + 1u < q_size ? F.qo[k + 1u] : q_edges;
+      if (j > 0u && qn > qo) L.own[s] = (uint16_t)j;
       if (OP == OP_PUSH) {
         if (PLAIN_CLEAR) inq_cur[v >> 5] = 0u;
         else bit_clear(inq_cur, v);
@@ -1006,7 +1015,7 @@ __device__ __forceinline__ void td_chunk(const LevelArgs& a, const Frontier& F, 
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
       uint32_t idx = (uint32_t)t * kPer + r;
-      if (idx < len) L.own[idx] = max(vals[r], excl);
+      if (idx < len) L.own[idx] = (uint16_t)max(vals[r], excl);
     }
   }
   __syncthreads();
@@ -1136,7 +1145,7 @@ __device__ __forceinline__ void td_slots(const LevelArgs& a, const Frontier& F, 
       if (w < wave_id()) carry += L.scan[w];
       T += L.scan[w];
     }
-    L.own[t] = incl - len + carry;  // exclusive prefix; slots past nb have len 0
+    L.pre[t] = incl - len + carry;  // exclusive prefix; slots past nb have len 0
     __syncthreads();
     for (uint32_t e0 = 0; e0 < T; e0 += kChunk) {
       uint32_t w[kPer];
@@ -1149,12 +1158,12 @@ __device__ __forceinline__ void td_slots(const LevelArgs& a, const Frontier& F, 
         w[r] = 0u;
         col[r] = 0;
         if (ok[r]) {
-          // the last slot whose list starts at or before e (own[0] = 0 <= e < T)
+          // the last slot whose list starts at or before e (pre[0] = 0 <= e < T)
           uint32_t lo = 0;
 #pragma unroll
           for (uint32_t step = kBlock / 2; step >= 1; step >>= 1)
-            if (lo + step < nb && L.own[lo + step] <= e) lo += step;
-          const uint32_t i = e - L.own[lo];
+            if (lo + step < nb && L.pre[lo + step] <= e) lo += step;
+          const uint32_t i = e - L.pre[lo];
           w[r] = (uint32_t)((MIRROR && i >= L.split[lo]) ? a.down_adj2[L.base2[lo] + (i - L.split[lo])]
                                                          : a.down_adj[L.base[lo] + i]);
           if (COLOR) col[r] = L.col[lo];
