@@ -1730,6 +1730,10 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t pos, uint32_t groups) {
 // together).
 constexpr int kPullPer = 8;
 constexpr uint32_t kPullEdges = (uint32_t)kBlock * kPullPer;
+// Bitmap words of a pull level's word group: a.pull_words, at most kBlock (one per thread).
+__device__ __forceinline__ int64_t pull_words(const LevelArgs& a) {
+  return a.pull_words >= 1u && a.pull_words <= (uint32_t)kBlock ? (int64_t)a.pull_words : (int64_t)kBlock;
+}
 
 // LDS of a pull level: one bitmap word group and kBlock of its candidates' in-lists.
 template <bool MIRROR>
@@ -1743,7 +1747,7 @@ struct PullLds {
   uint32_t scan[kBlock / 64];
 };
 
-// One word group [wb, wb + kBlock) of a pull level: a dense push level of max-label
+// One word group [wb, wb + pull_words(a)) of a pull level: a dense push level of max-label
 // propagation (multistep/scc_color.cpp:67-163) turned around.  Every valid vertex of the
 // group takes the largest colour among its valid in-neighbours (in- and out-neighbours
 // when mirrored) and, when that exceeds its own, stores it and is queued for frontier N.
@@ -1763,8 +1767,9 @@ __device__ __forceinline__ void pull_group(const LevelArgs& a, const Frontier& N
                                            unsigned long long& cands, unsigned long long& edges,
                                            unsigned long long& reads) {
   const int t = threadIdx.x;
-  if (wb + t < a.nwords) inq_cur[wb + t] = 0u;
-  const uint32_t total = bits_load(P.B, a.valid, a.nwords, wb);  // ends with a barrier
+  const int64_t wend = wb + pull_words(a) < a.nwords ? wb + pull_words(a) : a.nwords;
+  if (wb + t < wend) inq_cur[wb + t] = 0u;
+  const uint32_t total = bits_load(P.B, a.valid, wend, wb);  // ends with a barrier
   if (t == 0) cands += total;
   for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
     const uint32_t nb = min(total - k0, (uint32_t)kBlock);
@@ -1882,7 +1887,8 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   if (OP == OP_PUSH && c->bottom_up != 0u) {
     // pull level: every word group of the bitmap; part row = (candidates, edges, raised,
     // colour reads)
-    const uint32_t groups = (uint32_t)((a.nwords + kBlock - 1) / kBlock), pos = xcd_positions(groups);
+    const int64_t pw = pull_words(a);
+    const uint32_t groups = (uint32_t)((a.nwords + pw - 1) / pw), pos = xcd_positions(groups);
     if (blockIdx.x >= pos) return;  // uniform
     const Frontier N = c->cur ? a.fr0 : a.fr1;
     uint32_t* inq_cur = c->cur ? a.inq1 : a.inq0;
@@ -1895,7 +1901,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
     unsigned long long ra = 0, cn = 0, ed = 0, rd = 0;
     for (uint32_t p = blockIdx.x; p < pos; p += gridDim.x) {
       const uint32_t g = xcd_group(p, groups);
-      if (g < groups) pull_group<MIRROR>(a, N, inq_cur, (int64_t)g * kBlock, ids, S, lds.P, ra, cn, ed, rd);
+      if (g < groups) pull_group<MIRROR>(a, N, inq_cur, (int64_t)g * pw, ids, S, lds.P, ra, cn, ed, rd);
     }
     stage_flush(S, N, c, 0u);
     ra = block_sum64(ra, s_red);

@@ -1,7 +1,8 @@
 """Seeded random parity sweep: graphs of every generator (R-MAT scales 8-14, grids with random
 edge probabilities, planted SCCs, uniform random digraphs with duplicates and self-loops) and
 small packed batches, each under a random tuning (searches side by side, in turn or
-restricted, every load balance, colour tail on or off, workgroups per CU) and policy.
+restricted, every load balance, colour tail on or off, workgroups per CU, pull group width)
+and policy.
 SCC and WCC labels must equal the oracle's: raw labels under the plain rule (multistep/
 scc_main.cpp:210-317, cc_main.cpp:184-245), canonical ones under the learned policy, whose
 abandoned searches leave the pivot's component to colouring (labelled with its largest
@@ -47,7 +48,7 @@ def test_random_graphs_and_tunings(msc, orc, gpu_ctx, seed):
             gseed = int(rng.integers(1, 1 << 30))
             tune = dict(concurrent_fwbw=int(rng.integers(0, 3)), balance=int(rng.choice([0, 1, 2, 3])),
                         tail_vertices=int(rng.choice([0, 65536])), blocks_per_cu=int(rng.choice([1, 2, 5, 8])),
-                        first_batch=int(rng.choice([1, 2, 16])))
+                        first_batch=int(rng.choice([1, 2, 16])), pull_words=int(rng.choice([0, 1, 8, 64, 256])))
             pol = msc.POLICY_LEARNED if rng.integers(0, 2) else msc.POLICY_PLAIN
             gpu_ctx.set_tuning(**dict(defaults, **tune))
             what = f"case {case} kind {kind} seed {gseed} tuning {tune} policy {pol}"
