@@ -108,12 +108,12 @@ class _Tuning(C.Structure):
                 ("small_cbw_edges", C.c_int32), ("first_batch", C.c_int32), ("max_batch", C.c_int32),
                 ("concurrent_fwbw", C.c_int32), ("search_batch", C.c_int32), ("local_rounds", C.c_int32),
                 ("local_us", C.c_int32), ("pull_div", C.c_int32),
-                ("tail_vertices", C.c_int32), ("balance", C.c_int32), ("pull_words", C.c_int32)]
+                ("tail_vertices", C.c_int32), ("balance", C.c_int32), ("group_words", C.c_int32)]
 
 
 TUNING_FIELDS = ("blocks_per_cu", "small_push_edges", "small_cbw_edges", "first_batch", "max_batch",
                  "concurrent_fwbw", "search_batch", "local_rounds", "local_us", "pull_div",
-                 "tail_vertices", "balance", "pull_words")
+                 "tail_vertices", "balance", "group_words")
 
 
 class _TrainOpts(C.Structure):

@@ -21,8 +21,8 @@ against each other differently on shallow power-law graphs and on deep meshes:
                     frontier vertices' edges, 2 chunks of the whole frontier's edges, 3 one or
                     the other per level by the frontier's mean list length (0, a thread per
                     frontier vertex, is left out: it walks a hub's list on one thread);
-  pull_words        bitmap words a pull level's workgroup takes at a time (0: 64 for a batch,
-                    256 for one graph).
+  group_words       bitmap words a pull or bottom-up level's workgroup takes at a time (0: 64
+                    for a batch, 256 for one graph).
 
 None of them changes a label: every candidate's canonical labels are compared with the
 default tuning's and a candidate that differs is reported as an error, not skipped.
@@ -57,7 +57,7 @@ CANDIDATES = {
     "pull_div": [0, 2, 4, 8, 16],
     "tail_vertices": [0, 4096, 65536],
     "balance": [1, 2, 3],
-    "pull_words": [0, 32, 64, 128, 256],
+    "group_words": [0, 32, 64, 128, 256],
 }
 
 

@@ -190,7 +190,7 @@ struct LevelArgs {
   uint32_t pull_div;          // colour push: a level over at least 1/pull_div of the round's
                               // edges runs as a pull level (0: never)
   uint32_t pull_min;          // ... and over at least this many edges
-  uint32_t pull_words;        // bitmap words of a pull level's word group (1..kBlock)
+  uint32_t group_words;       // bitmap words of a pull or (narrow) bottom-up level's word group (1..kBlock)
   TailSeed tail;              // colouring round 0's seed: what the colour tail needs (tail.slots null: off)
   int32_t balance;            // top-down load balance of grid-wide levels (msc_tuning.balance)
   const uint32_t* within_cand;  // backward search beside the forward one: the forward search's

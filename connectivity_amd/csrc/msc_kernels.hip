@@ -110,6 +110,15 @@ __device__ __forceinline__ bool bit_clear(uint32_t* b, uint32_t v) {
   return (atomicAnd(&b[v >> 5], ~m) & m) != 0u;
 }
 
+// Bitmap words of a pull or bottom-up level's word group: a.group_words, at most kBlock (one
+// word per thread).
+__device__ __forceinline__ int64_t group_words(const LevelArgs& a) {
+  return a.group_words >= 1u && a.group_words <= (uint32_t)kBlock ? (int64_t)a.group_words : (int64_t)kBlock;
+}
+
+// Words of a (not wide) bottom-up level's group: narrow groups need one word per thread.
+__device__ __forceinline__ int64_t bu_words(const LevelArgs& a) { return kWPT == 1 ? group_words(a) : (int64_t)kBW; }
+
 __device__ __forceinline__ uint32_t list_len(const uint32_t* off, const uint32_t* off2, uint32_t v) {
   uint32_t d = off[v + 1] - off[v];
   if (off2) d += off2[v + 1] - off2[v];
@@ -1220,6 +1229,10 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
                                          unsigned long long& my_next, unsigned long long& my_nedges) {
   constexpr int kR = MSC_BU_KR;
   const int t = threadIdx.x;
+  // the group: kBlock * WPT words, or group_words(a) of them for a narrow (WPT == 1) group
+  const int64_t gw = WPT == 1 ? group_words(a) : (int64_t)kBlock * WPT;
+  static_assert(WPT != kWPT || kWPT > 1 || kBW == kBlock, "a one-word-per-thread group is at most kBlock words");
+  const int64_t wend = wb + gw < a.nwords ? wb + gw : a.nwords;
 #pragma unroll
   for (int j = 0; j < WPT; ++j) s_claim[t * WPT + j] = 0u;
   // candidates: the claimable bits (restricted to the forward set for a backward search
@@ -1231,7 +1244,7 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
     const int64_t w0 = wb + (int64_t)t * WPT;
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
-      const bool in = w0 + j < a.nwords;
+      const bool in = w0 + j < wend;
       cw[j] = in ? a.cand[w0 + j] : 0u;
       sw[j] = in ? a.seen[w0 + j] : 0u;
       w[j] = cw[j] & ~((in && within) ? within[w0 + j] : 0u);
@@ -1303,10 +1316,12 @@ __device__ __forceinline__ void bu_group(const LevelArgs& a, int64_t wb, BitIter
     __syncthreads();  // every claim of the round is in s_claim before the next round's picks
   }
   // every claim of this group is in s_claim; each thread reads only its own words and
-  // masks, so the next group's loads need no barrier first
+  // masks, so the next group's loads need no barrier first.  Words past a narrow group are
+  // the next group's.
 #pragma unroll
   for (int j = 0; j < WPT; ++j) {
     const int i = t * WPT + j;
+    if ((int64_t)i >= gw) continue;
     const uint32_t c = s_claim[i];
     a.fresh[wb + i] = c;
     if (c) {
@@ -1333,7 +1348,8 @@ __device__ __forceinline__ bool bu_pairable(const LevelArgs& f, const LevelArgs&
   const Ctrl* cf = f.ctrl;
   const Ctrl* cb = b.ctrl;
   return !cf->done && !cb->done && cf->bottom_up && cb->bottom_up && !cf->bu_wide && !cb->bu_wide &&
-         f.nwords == b.nwords && f.up_off == b.down_off && f.down_off == b.up_off && f.up_off2 == nullptr;
+         f.nwords == b.nwords && f.up_off == b.down_off && f.down_off == b.up_off && f.up_off2 == nullptr &&
+         bu_words(f) == (int64_t)kBW && bu_words(b) == (int64_t)kBW;
 }
 
 // One bottom-up word group [wb, wb + kBW) of a forward search f and a backward search b
@@ -1730,10 +1746,7 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t pos, uint32_t groups) {
 // together).
 constexpr int kPullPer = 8;
 constexpr uint32_t kPullEdges = (uint32_t)kBlock * kPullPer;
-// Bitmap words of a pull level's word group: a.pull_words, at most kBlock (one per thread).
-__device__ __forceinline__ int64_t pull_words(const LevelArgs& a) {
-  return a.pull_words >= 1u && a.pull_words <= (uint32_t)kBlock ? (int64_t)a.pull_words : (int64_t)kBlock;
-}
+
 
 // LDS of a pull level: one bitmap word group and kBlock of its candidates' in-lists.
 template <bool MIRROR>
@@ -1747,7 +1760,7 @@ struct PullLds {
   uint32_t scan[kBlock / 64];
 };
 
-// One word group [wb, wb + pull_words(a)) of a pull level: a dense push level of max-label
+// One word group [wb, wb + group_words(a)) of a pull level: a dense push level of max-label
 // propagation (multistep/scc_color.cpp:67-163) turned around.  Every valid vertex of the
 // group takes the largest colour among its valid in-neighbours (in- and out-neighbours
 // when mirrored) and, when that exceeds its own, stores it and is queued for frontier N.
@@ -1767,7 +1780,7 @@ __device__ __forceinline__ void pull_group(const LevelArgs& a, const Frontier& N
                                            unsigned long long& cands, unsigned long long& edges,
                                            unsigned long long& reads) {
   const int t = threadIdx.x;
-  const int64_t wend = wb + pull_words(a) < a.nwords ? wb + pull_words(a) : a.nwords;
+  const int64_t wend = wb + group_words(a) < a.nwords ? wb + group_words(a) : a.nwords;
   if (wb + t < wend) inq_cur[wb + t] = 0u;
   const uint32_t total = bits_load(P.B, a.valid, wend, wb);  // ends with a barrier
   if (t == 0) cands += total;
@@ -1887,7 +1900,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
   if (OP == OP_PUSH && c->bottom_up != 0u) {
     // pull level: every word group of the bitmap; part row = (candidates, edges, raised,
     // colour reads)
-    const int64_t pw = pull_words(a);
+    const int64_t pw = group_words(a);
     const uint32_t groups = (uint32_t)((a.nwords + pw - 1) / pw), pos = xcd_positions(groups);
     if (blockIdx.x >= pos) return;  // uniform
     const Frontier N = c->cur ? a.fr0 : a.fr1;
@@ -1968,7 +1981,7 @@ __global__ void __launch_bounds__(kBlock) k_level(LevelArgs a) {
 __device__ __forceinline__ uint32_t reach_units(const Ctrl* c, const LevelArgs& a) {
   if (c->done) return 0u;
   if (c->bottom_up || c->from_bits) {
-    const int64_t gw = (c->bu_wide || !c->bottom_up) ? kBWide : kBW;
+    const int64_t gw = (c->bu_wide || !c->bottom_up) ? kBWide : bu_words(a);
     return (uint32_t)((a.nwords + gw - 1) / gw);
   }
   return td_units(a, c->q_size, c->q_edges);
@@ -1998,7 +2011,7 @@ __device__ __forceinline__ void reach_search(const LevelArgs& a, uint32_t first,
         bu_group<MIRROR, kWPTWide>(a, (int64_t)lu * kBWide, BW, s_claimw, ex, in, nx, ne);
     } else {
       for (uint32_t lu = first; lu < units; lu += gridDim.x)
-        bu_group<MIRROR, kWPT>(a, (int64_t)lu * kBW, B, s_claim, ex, in, nx, ne);
+        bu_group<MIRROR, kWPT>(a, (int64_t)lu * bu_words(a), B, s_claim, ex, in, nx, ne);
     }
     __syncthreads();  // nothing staged; the workgroup may move on to the other search's LDS
     return;

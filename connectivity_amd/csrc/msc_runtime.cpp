@@ -805,6 +805,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
 
   LevelArgs a = base_args(c, &g, labels, o, avg);
   a.nvalid = &c->ctrl->red_trim[0];
+  // the word groups of pull and bottom-up levels: narrower for a batch, whose graphs keep
+  // their probes local, so an XCD's groups in flight then share its L2 (batch SCC 91.6 ->
+  // 83-85 ms with 64-word pull groups; the 4096^2 grid, one graph, is 1% faster with 256)
+  a.group_words = (uint32_t)(c->tune.group_words > 0 ? c->tune.group_words : bt ? 64 : kBlock);
   // canonical labels of one graph are written as the phases go: the pivot's component
   // takes its smallest member, colour SCCs track theirs in cmin (CC finds them after its
   // fixpoint, over the colouring's vertex list); the batch paths relabel at the end instead
@@ -1007,10 +1011,6 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     p.down_off2 = g.in_off; p.down_adj2 = g.in_adj;
     p.up_off2 = g.out_off; p.up_adj2 = g.out_adj;
   }
-  // a pull level's word groups: narrower for a batch, whose graphs keep their colour reads
-  // local, so an XCD's groups in flight then share its L2 (batch SCC 91.6 -> 83-85 ms with 64
-  // words; the 4096^2 grid, one graph, is 1% faster with 256)
-  p.pull_words = (uint32_t)(c->tune.pull_words > 0 ? c->tune.pull_words : bt ? 64 : kBlock);
   if (c->tune.pull_div > 0) {
     // a pull level sweeps every bitmap word and never runs inside the closing kernel
     const int64_t floor_edges = nw > (int64_t)c->tune.small_push_edges ? nw : (int64_t)c->tune.small_push_edges + 1;
@@ -1457,7 +1457,7 @@ int msc_ctx_set_tuning(msc_ctx* c, const msc_tuning* t) {
       t->concurrent_fwbw > 2 || t->search_batch < 1 || t->search_batch > 256 || t->local_rounds < 0 ||
       t->local_rounds > 256 || t->local_us < 1 || t->local_us > 100000 || t->pull_div < 0 || t->pull_div > 1024 ||
       t->tail_vertices < 0 || t->tail_vertices > (int32_t)kTailV || t->balance < 0 || t->balance > 3 ||
-      t->pull_words < 0 || t->pull_words > kBlock)
+      t->group_words < 0 || t->group_words > kBlock)
     return MSC_E_INVALID;
   std::lock_guard<std::mutex> lk(c->mu);
   c->tune = *t;

@@ -239,10 +239,11 @@ typedef struct msc_tuning {
                                the frontier's lists average at most 4 entries, else 2; WCC
                                levels, which walk two lists per vertex, 2 (the default).
                                Labels do not depend on it. */
-  int32_t pull_words;       /* bitmap words (32 vertices each) a pull level's workgroup takes at a
-                               time (1..256), or 0 (the default): 64 for a batch, whose colour
-                               reads stay inside each graph, so narrower groups keep an XCD's
-                               working set in its L2, and 256 for one graph */
+  int32_t group_words;      /* bitmap words (32 vertices each) a workgroup of a dense bitmap sweep
+                               (a pull level, a bottom-up search level) takes at a time (1..256),
+                               or 0 (the default): 64 for a batch, whose probes stay inside each
+                               graph, so narrower groups keep an XCD's working set in its L2,
+                               and 256 for one graph */
 } msc_tuning;
 int msc_ctx_get_tuning(msc_ctx* ctx, msc_tuning* out);
 int msc_ctx_set_tuning(msc_ctx* ctx, const msc_tuning* t);

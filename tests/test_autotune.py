@@ -34,7 +34,7 @@ def test_candidates_are_accepted_values():
     lim = {"blocks_per_cu": (1, 8), "small_push_edges": (0, 1 << 30), "small_cbw_edges": (0, 1 << 30),
            "first_batch": (1, 256), "max_batch": (1, 256), "concurrent_fwbw": (0, 2), "search_batch": (1, 256),
            "local_rounds": (0, 256), "local_us": (1, 100000), "pull_div": (0, 1024),
-           "tail_vertices": (0, 65536), "balance": (0, 3), "pull_words": (0, 256)}
+           "tail_vertices": (0, 65536), "balance": (0, 3), "group_words": (0, 256)}
     for field, values in autotune.CANDIDATES.items():
         lo, hi = lim[field]
         assert all(lo <= v <= hi for v in values), field

@@ -48,7 +48,7 @@ def test_random_graphs_and_tunings(msc, orc, gpu_ctx, seed):
             gseed = int(rng.integers(1, 1 << 30))
             tune = dict(concurrent_fwbw=int(rng.integers(0, 3)), balance=int(rng.choice([0, 1, 2, 3])),
                         tail_vertices=int(rng.choice([0, 65536])), blocks_per_cu=int(rng.choice([1, 2, 5, 8])),
-                        first_batch=int(rng.choice([1, 2, 16])), pull_words=int(rng.choice([0, 1, 8, 64, 256])))
+                        first_batch=int(rng.choice([1, 2, 16])), group_words=int(rng.choice([0, 1, 8, 64, 256])))
             pol = msc.POLICY_LEARNED if rng.integers(0, 2) else msc.POLICY_PLAIN
             gpu_ctx.set_tuning(**dict(defaults, **tune))
             what = f"case {case} kind {kind} seed {gseed} tuning {tune} policy {pol}"
