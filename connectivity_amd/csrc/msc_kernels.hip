@@ -2474,6 +2474,15 @@ __global__ void __launch_bounds__(kBlock) k_close_color(LevelArgs a, int nparts)
   }
 }
 
+// Vertices per thread per step of the seeding and root sweeps: up to kListR, fewer while
+// the grid has a step of one per thread for each.
+constexpr int kListR = 4;
+static_assert(kListR * kBlock <= (int)kStage, "a step's claims fit the stage");
+__device__ __forceinline__ uint32_t list_steps(uint32_t n) {
+  const uint32_t per = (n + gridDim.x * kBlock - 1) / (gridDim.x * kBlock);
+  return per < 1u ? 1u : per > (uint32_t)kListR ? (uint32_t)kListR : per;
+}
+
 // Colour initialisation of a later colouring round (multistep/scc_color.cpp:56-63): the
 // previous round's vertex list (length a.ctrl->list_len, which k_ctrl_reset keeps) filtered
 // to the vertices still valid; each gets colors[v] = v and is queued, and the queue is also
@@ -2486,20 +2495,40 @@ __global__ void __launch_bounds__(kBlock) k_seed_list(LevelArgs a, const int32_t
   const uint32_t in_n = a.ctrl->list_len;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
-  for (uint32_t i0 = blockIdx.x * kBlock; i0 < in_n; i0 += gridDim.x * kBlock) {
-    const uint32_t i = i0 + threadIdx.x;
-    bool take = false;
-    uint32_t v = 0, deg = 0;
-    if (i < in_n) {
-      v = (uint32_t)in[i];
-      if (bit_test(a.valid, v)) {
-        take = true;
-        a.colors[v] = (int32_t)v;
-        deg = list_len(a.down_off, a.down_off2, v);
+  // up to kListR entries per thread per step, each step's loads in flight together: one entry
+  // at a time made a long sweep a chain of dependent round trips (grid 4096^2, round 1:
+  // 107 us); a short list keeps one per thread, spread over more workgroups
+  const uint32_t rr = list_steps(in_n);
+  for (uint32_t i0 = blockIdx.x * kBlock * rr; i0 < in_n; i0 += gridDim.x * kBlock * rr) {
+    uint32_t v[kListR], w[kListR], b1[kListR], e1[kListR], b2[kListR], e2[kListR];
+    bool take[kListR];
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) {
+      const uint32_t i = i0 + (uint32_t)r * kBlock + threadIdx.x;
+      v[r] = (r < (int)rr && i < in_n) ? (uint32_t)in[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) {
+      const uint32_t i = i0 + (uint32_t)r * kBlock + threadIdx.x;
+      w[r] = (r < (int)rr && i < in_n) ? a.valid[v[r] >> 5] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) {
+      take[r] = (w[r] >> (v[r] & 31u)) & 1u;
+      b1[r] = e1[r] = b2[r] = e2[r] = 0u;
+      if (take[r]) {
+        a.colors[v[r]] = (int32_t)v[r];
+        b1[r] = a.down_off[v[r]];
+        e1[r] = a.down_off[v[r] + 1];
+        if (a.down_off2) {
+          b2[r] = a.down_off2[v[r]];
+          e2[r] = a.down_off2[v[r] + 1];
+        }
       }
     }
-    stage_push(S, take, v, deg);
-    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, out);  // also the barrier before the next round
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) stage_push(S, take[r], v[r], (e1[r] - b1[r]) + (e2[r] - b2[r]));
+    stage_flush(S, a.fr1, a.ctrl, kStage - kListR * kBlock, out);  // also the barrier before the next step
   }
   stage_flush(S, a.fr1, a.ctrl, 0u, out);
 }
@@ -2519,17 +2548,32 @@ __global__ void __launch_bounds__(kBlock) k_seed_bits(LevelArgs a, int32_t* __re
   constexpr int kW = BitIterT<4>::kWords;
   for (int64_t wb = (int64_t)blockIdx.x * kW; wb < a.nwords; wb += (int64_t)gridDim.x * kW) {
     const uint32_t total = bits_load(B, a.valid, a.nwords, wb);
-    for (uint32_t k0 = 0; k0 < total; k0 += kBlock) {
-      const uint32_t k = k0 + threadIdx.x;
-      const bool take = k < total;
-      uint32_t v = 0, deg = 0;
-      if (take) {
-        v = bits_pick(B, wb, k);
-        a.colors[v] = (int32_t)v;
-        deg = list_len(a.down_off, a.down_off2, v);
+    // kListR vertices per thread per step, their loads in flight together
+    for (uint32_t k0 = 0; k0 < total; k0 += kBlock * kListR) {
+      uint32_t v[kListR], b1[kListR], e1[kListR], b2[kListR], e2[kListR];
+      bool take[kListR];
+#pragma unroll
+      for (int r = 0; r < kListR; ++r) {
+        const uint32_t k = k0 + (uint32_t)r * kBlock + threadIdx.x;
+        take[r] = k < total;
+        v[r] = take[r] ? bits_pick(B, wb, k) : 0u;
       }
-      stage_push(S, take, v, deg);
-      stage_flush(S, a.fr1, a.ctrl, kStage - kBlock, list, a.tail.slots ? &a.tail : nullptr);  // also a barrier
+#pragma unroll
+      for (int r = 0; r < kListR; ++r) {
+        b1[r] = e1[r] = b2[r] = e2[r] = 0u;
+        if (take[r]) {
+          a.colors[v[r]] = (int32_t)v[r];
+          b1[r] = a.down_off[v[r]];
+          e1[r] = a.down_off[v[r] + 1];
+          if (a.down_off2) {
+            b2[r] = a.down_off2[v[r]];
+            e2[r] = a.down_off2[v[r] + 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kListR; ++r) stage_push(S, take[r], v[r], (e1[r] - b1[r]) + (e2[r] - b2[r]));
+      stage_flush(S, a.fr1, a.ctrl, kStage - kListR * kBlock, list, a.tail.slots ? &a.tail : nullptr);  // a barrier
     }
     __syncthreads();  // B is reloaded next
   }
@@ -2548,21 +2592,38 @@ __global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_
   const uint32_t nl = gate->list_len;
   if (threadIdx.x == 0) S.cnt = 0u;
   __syncthreads();
-  for (uint32_t i0 = blockIdx.x * kBlock; i0 < nl; i0 += gridDim.x * kBlock) {
-    const uint32_t i = i0 + threadIdx.x;
-    bool take = false;
-    uint32_t v = 0, deg = 0;
-    if (i < nl) {
-      v = (uint32_t)list[i];
-      if (a.colors[v] == (int32_t)v && bit_clear(a.valid, v)) {
-        take = true;
-        a.labels[v] = (int32_t)v;
-        if (a.cmin) a.cmin[v] = (int32_t)v;
-        deg = a.down_off[v + 1] - a.down_off[v];
+  // up to kListR entries per thread per step (as k_seed_list), their loads and claims in
+  // flight together
+  const uint32_t rr = list_steps(nl);
+  for (uint32_t i0 = blockIdx.x * kBlock * rr; i0 < nl; i0 += gridDim.x * kBlock * rr) {
+    uint32_t v[kListR], b[kListR], e[kListR];
+    int32_t col[kListR];
+    bool take[kListR];
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) {
+      const uint32_t i = i0 + (uint32_t)r * kBlock + threadIdx.x;
+      v[r] = (r < (int)rr && i < nl) ? (uint32_t)list[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) {
+      const uint32_t i = i0 + (uint32_t)r * kBlock + threadIdx.x;
+      col[r] = (r < (int)rr && i < nl) ? a.colors[v[r]] : -1;
+    }
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) take[r] = col[r] == (int32_t)v[r] && bit_clear(a.valid, v[r]);
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) {
+      b[r] = e[r] = 0u;
+      if (take[r]) {
+        a.labels[v[r]] = (int32_t)v[r];
+        if (a.cmin) a.cmin[v[r]] = (int32_t)v[r];
+        b[r] = a.down_off[v[r]];
+        e[r] = a.down_off[v[r] + 1];
       }
     }
-    stage_push(S, take, v, deg);
-    stage_flush(S, a.fr1, a.ctrl, kStage - kBlock);  // also the barrier before the next round
+#pragma unroll
+    for (int r = 0; r < kListR; ++r) stage_push(S, take[r], v[r], e[r] - b[r]);
+    stage_flush(S, a.fr1, a.ctrl, kStage - kListR * kBlock);  // also the barrier before the next step
   }
   stage_flush(S, a.fr1, a.ctrl, 0u);
 }
