@@ -765,19 +765,24 @@ def run_cc_batch(g, vert_base, pivots=None, *, policy: int = POLICY_PLAIN, label
 POLICY_WEIGHTS = os.path.join(_HERE, "policy_weights.npz")
 
 
-def set_policy(ctx: Context, blob: np.ndarray) -> None:
-    """Loads MLP weights + standardisation (MLP_BLOB float32) into the context."""
+def set_policy(ctx: Context, blob: np.ndarray, blob_cc: np.ndarray | None = None) -> None:
+    """Loads MLP weights + standardisation (MLP_BLOB float32) into the context: one network
+    for every search, or with ``blob_cc`` a second one for the CC search."""
     b = np.ascontiguousarray(blob, dtype=np.float32)
-    if b.shape != (MLP_BLOB,):
-        raise ValueError(f"policy blob must hold {MLP_BLOB} floats")
-    _check(_lib().msc_policy_set_weights(ctx.handle, _ptr(b), MLP_BLOB), "msc_policy_set_weights")
+    if b.shape != (MLP_BLOB,) or (blob_cc is not None and np.shape(blob_cc) != (MLP_BLOB,)):
+        raise ValueError(f"a policy blob must hold {MLP_BLOB} floats")
+    if blob_cc is not None:
+        b = np.ascontiguousarray(np.concatenate([b, np.asarray(blob_cc, dtype=np.float32)]))
+    _check(_lib().msc_policy_set_weights(ctx.handle, _ptr(b), b.shape[0]), "msc_policy_set_weights")
 
 
 def load_policy(ctx: Context, path: str = POLICY_WEIGHTS) -> np.ndarray:
-    """Loads the shipped (or given) trained weights into ctx; returns the blob."""
+    """Loads the shipped (or given) trained weights into ctx (`blob`, and `blob_cc` for the CC
+    search when the file has one); returns the first blob."""
     with np.load(path, allow_pickle=False) as z:
         blob = z["blob"].astype(np.float32)
-    set_policy(ctx, blob)
+        blob_cc = z["blob_cc"].astype(np.float32) if "blob_cc" in z.files else None
+    set_policy(ctx, blob, blob_cc)
     return blob
 
 

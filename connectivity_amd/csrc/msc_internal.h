@@ -212,7 +212,7 @@ struct Launch {
 
 // ---- context accessors for the policy code (msc_runtime.cpp) ----
 int ctx_stream(msc_ctx* ctx, hipStream_t* stream, int* device);
-int ctx_set_policy(msc_ctx* ctx, const float* blob);  // kMlpBlob floats, validated by the caller
+int ctx_set_policy(msc_ctx* ctx, const float* blob, int nets);  // nets x kMlpBlob floats, validated by the caller
 
 // ---- kernel launchers (msc_kernels.hip) ----
 // Launchers that leave per-workgroup partials return how many workgroups wrote them.

@@ -234,8 +234,11 @@ static bool blob_ok(const float* blob) {
 }
 
 int msc_policy_set_weights(msc_ctx* ctx, const float* blob, int32_t count) {
-  if (!ctx || !blob || count != kMlpBlob || !blob_ok(blob)) return MSC_E_INVALID;
-  return msc::ctx_set_policy(ctx, blob);
+  if (!ctx || !blob || (count != kMlpBlob && count != 2 * kMlpBlob)) return MSC_E_INVALID;
+  const int nets = count / kMlpBlob;
+  for (int i = 0; i < nets; ++i)
+    if (!blob_ok(blob + (size_t)i * kMlpBlob)) return MSC_E_INVALID;
+  return msc::ctx_set_policy(ctx, blob, nets);
 }
 
 int msc_mlp_forward(msc_ctx* ctx, const float* blob, const float* x, int64_t count, float* logits_out) {

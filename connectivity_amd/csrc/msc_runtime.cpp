@@ -70,7 +70,8 @@ struct msc_ctx : WsView {
   msc_tuning tune{5, 4096, 4096, 16, 32, 2, 6, 64, 240, 2, 65536, 3, 0};
   // persistent across workspace regrowth: policy weights and forced-schedule buffer
   void* small = nullptr;
-  float* mlp = nullptr;
+  float* mlp = nullptr;     // the SCC searches' network (kMlpBlob floats)
+  float* mlp_cc = nullptr;  // the CC search's: mlp, or a second network after it
   int8_t* sched = nullptr;
   int8_t* sched_host = nullptr;  // pinned staging
   bool has_mlp = false;
@@ -908,6 +909,7 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     } else {
       LevelArgs bf = a;
       bf.op = OP_CC;
+      if (bf.mlp) bf.mlp = c->mlp_cc;  // the CC search's own network, when one is loaded
       if (!cand_ready) {  // cand = allow, nothing seen
         launch_andnot(L, c->b1, nullptr, c->valid, nullptr, nw);
         launch_fill_u32(L, c->s1, 0xffffffffu, nw);
@@ -1299,11 +1301,12 @@ int ctx_stream(msc_ctx* c, hipStream_t* stream, int* device) {
   return MSC_OK;
 }
 
-int ctx_set_policy(msc_ctx* c, const float* blob) {
+int ctx_set_policy(msc_ctx* c, const float* blob, int nets) {
   std::lock_guard<std::mutex> lk(c->mu);
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipMemcpyAsync(c->mlp, blob, sizeof(float) * kMlpBlob, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->mlp, blob, sizeof(float) * kMlpBlob * (size_t)nets, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  c->mlp_cc = nets == 2 ? c->mlp + kMlpBlob : c->mlp;
   c->has_mlp = true;
   return MSC_OK;
 }
@@ -1396,7 +1399,7 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
       hipHostMalloc((void**)&c->sched_host, 3 * MSC_SCHED_LEVELS, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->prog_host, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void**)&c->prog_dev, c->prog_host, 0) != hipSuccess ||
-      hipMalloc(&c->small, align256((size_t)kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
+      hipMalloc(&c->small, align256((size_t)2 * kMlpBlob * 4) + 3 * MSC_SCHED_LEVELS) != hipSuccess) {
     (void)hipGetLastError();
     msc_ctx_destroy(c);
     return MSC_E_OOM;
@@ -1407,7 +1410,8 @@ int msc_ctx_create(int32_t device, void* stream, msc_ctx** out) {
   c->rctl_host = (RoundCtl*)(c->ctrl_host + kCtrls);
   *c->prog_host = 0ull;
   c->mlp = (float*)c->small;
-  c->sched = (int8_t*)c->small + align256((size_t)kMlpBlob * 4);
+  c->mlp_cc = c->mlp;
+  c->sched = (int8_t*)c->small + align256((size_t)2 * kMlpBlob * 4);
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
     c->clock_mhz = (float)khz / 1000.0f;

@@ -20,6 +20,7 @@ kernels, msc_mlp.hip).
 
 CLI:  python -m connectivity_amd.policy all --graphs 210 --reps 2 --out connectivity_amd/policy_weights.npz
       python -m connectivity_amd.policy train --data <collected.npz> --seed 2 --epochs 600 --out ...
+      python -m connectivity_amd.policy combine --scc <a.npz> --cc <b.npz> --out ...  (a network per search kind)
 """
 from __future__ import annotations
 
@@ -65,16 +66,17 @@ def init_blob(mean: np.ndarray, std: np.ndarray, seed: int = 0) -> np.ndarray:
 # ---- training graphs ---------------------------------------------------------------------
 def training_graphs(count: int, seed: int = 7):
     """`count` seeded specs: R-MAT (scale 14-20, edge factor 8-32), grid digraphs (side
-    128-1024), planted-SCC graphs and packed batches (128-1024 R-MAT / planted graphs, whose
+    128-1024), planted-SCC graphs and packed batches (128-8192 R-MAT / planted graphs, whose
     merged searches meet states no single graph shows: most unvisited vertices lie in graphs
-    whose search has ended), in a fixed interleaved order."""
+    whose search has ended; the largest make the bottom-up sweeps of the CC search pay where
+    the small ones do not), in a fixed interleaved order."""
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(count):
         kind = ("rmat", "rmat", "grid", "rmat", "grid", "planted", "batch")[i % 7]
         s = int(rng.integers(1, 1 << 30))
         if kind == "batch":
-            specs.append(("batch", {"graphs": int(2 ** rng.integers(7, 11)), "median_n": int(2 ** rng.integers(10, 13)),
+            specs.append(("batch", {"graphs": int(2 ** rng.integers(7, 14)), "median_n": int(2 ** rng.integers(10, 13)),
                                     "seed": s}))
         elif kind == "rmat":
             specs.append(("rmat", {"scale": int(rng.integers(14, 21)), "edge_factor": int(rng.choice([8, 12, 16, 24, 32])),
@@ -242,9 +244,9 @@ def fit(ctx: Context, X, Y, W, epochs=300, batch=256, lr=3e-3, seed=1):
     return blob, curve
 
 
-def evaluate(ctx: Context, blob, specs, reps: int = 3, kinds=("scc", "cc")):
+def evaluate(ctx: Context, blob, specs, reps: int = 3, kinds=("scc", "cc"), blob_cc=None):
     """Plain vs learned wall time (and identical labels) on each spec."""
-    set_policy(ctx, blob)
+    set_policy(ctx, blob, blob_cc)
     rows = []
     for spec in specs:
         g = build(spec)
@@ -267,7 +269,9 @@ def evaluate(ctx: Context, blob, specs, reps: int = 3, kinds=("scc", "cc")):
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m connectivity_amd.policy")
-    ap.add_argument("cmd", choices=["collect", "train", "eval", "all"])
+    ap.add_argument("cmd", choices=["collect", "train", "eval", "all", "combine"])
+    ap.add_argument("--scc", help="combine: weights whose network decides the SCC searches")
+    ap.add_argument("--cc", help="combine: weights whose network decides the CC search")
     ap.add_argument("--graphs", type=int, default=120)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--data", default="gpurun_out/policy_data.npz")
@@ -276,7 +280,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=1, help="initialisation and minibatch order of the training")
     ap.add_argument("--eval-specs", default="heldout")
     args = ap.parse_args(argv)
-    ctx = Context(0)
+    ctx = Context(0) if args.cmd != "combine" else None  # combining two files needs no GPU
     if args.cmd in ("collect", "all"):
         specs = training_graphs(args.graphs)
         X, Y, W, meta = collect(ctx, specs, reps=args.reps)
@@ -295,14 +299,23 @@ def main(argv=None) -> int:
             f"train accuracy {acc:.3f}")
         np.savez(args.out, blob=blob, curve=curve, features=np.array(FEATURES))
         log(f"wrote {args.out}")
+    if args.cmd == "combine":
+        # one file, two networks: msc_policy_set_weights then takes the first for the SCC
+        # searches and the second for the CC search
+        with np.load(args.scc, allow_pickle=False) as a, np.load(args.cc, allow_pickle=False) as b:
+            np.savez(args.out, blob=a["blob"], curve=a["curve"], blob_cc=b["blob"], curve_cc=b["curve"],
+                     features=np.array(FEATURES))
+        log(f"wrote {args.out}: SCC network from {args.scc}, CC network from {args.cc}")
     if args.cmd in ("eval", "all"):
         with np.load(args.out, allow_pickle=False) as z:
             blob = z["blob"]
+            blob_cc = z["blob_cc"] if "blob_cc" in z.files else None
         specs = heldout_specs()
-        rows = evaluate(ctx, blob, specs, reps=args.reps)
+        rows = evaluate(ctx, blob, specs, reps=args.reps, blob_cc=blob_cc)
         with open(os.path.splitext(args.data)[0] + "_eval.json", "w") as f:
             json.dump(rows, f, indent=1)
-    ctx.close()
+    if ctx is not None:
+        ctx.close()
     return 0
 
 

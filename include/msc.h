@@ -182,6 +182,9 @@ int msc_ctx_levels(msc_ctx* ctx, msc_level_rec* out, int32_t cap, int32_t* count
 #define MSC_MLP_OUT 3
 #define MSC_MLP_PARAMS 1571
 #define MSC_MLP_BLOB 1595
+/* count = MSC_MLP_BLOB: one network for every search; count = 2 * MSC_MLP_BLOB: the first
+ * blob decides the SCC searches (FW, BW) and the second the CC search, whose mirrored
+ * sweeps trade off differently. */
 int msc_policy_set_weights(msc_ctx* ctx, const float* blob, int32_t count);
 /* Batched inference on device: logits_out[count][3] from raw features x[count][12]. */
 int msc_mlp_forward(msc_ctx* ctx, const float* blob, const float* x, int64_t count, float* logits_out);

@@ -184,6 +184,40 @@ def test_learned_runs_keep_labels_and_match_batched_mlp(msc, orc, gpu_ctx, seed)
 
 
 @pytest.mark.gpu
+def test_cc_network_decides_the_cc_search(msc, orc, gpu_ctx):
+    """With two networks loaded (msc_policy_set_weights, 2 x MLP_BLOB floats) the SCC
+    searches act on the first one's logits and the CC search on the second one's; a wrong
+    float count is refused."""
+    nets = []
+    for seed, act in ((21, 0), (22, 1)):  # the first always top-down, the second bottom-up
+        b = random_blob(seed, scale=0.5)
+        b[-24:-12] = 0.0
+        b[-12:] = 3.0
+        b[mlp_ref.PARAMS - 3 + act] = 50.0  # output bias b3
+        nets.append(b)
+    msc.set_policy(gpu_ctx, nets[0], nets[1])
+    g = msc.gen_rmat(13, 8, seed=5)
+    assert np.array_equal(msc.run_scc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED, record=True),
+                          orc.canonical(orc.scc(g, vert_cutoff=0)))
+    recs = [r for r in gpu_ctx.levels() if r.phase in (0, 1)]
+    feats = np.array([r.feat for r in recs], np.float32)
+    assert np.array_equal(np.argmax(msc.mlp_forward(gpu_ctx, nets[0], feats), 1), [r.action for r in recs])
+    assert np.array_equal(msc.run_cc(g, ctx=gpu_ctx, policy=msc.POLICY_LEARNED, record=True),
+                          orc.canonical(orc.cc(g)))
+    assert recs and all(r.action == 0 for r in recs)
+    recs = [r for r in gpu_ctx.levels() if r.phase == 2]
+    feats = np.array([r.feat for r in recs], np.float32)
+    acts = [r.action for r in recs]
+    assert np.array_equal(np.argmax(msc.mlp_forward(gpu_ctx, nets[1], feats), 1), acts)
+    assert acts and all(a == 1 for a in acts)
+    import ctypes as C
+
+    three = np.concatenate(nets + nets[:1]).astype(np.float32)
+    rc = msc._lib().msc_policy_set_weights(gpu_ctx.handle, three.ctypes.data_as(C.c_void_p), three.shape[0])
+    assert rc != 0
+
+
+@pytest.mark.gpu
 def test_shipped_weights_keep_labels(msc, orc, gpu_ctx):
     import os
 
