@@ -18,7 +18,7 @@ Training data come from the library's own runs:
 The network is trained on the GPU by msc_mlp_train (hand-written forward / backward / Adam
 kernels, msc_mlp.hip).
 
-CLI:  python -m connectivity_amd.policy all --graphs 210 --reps 2 --out connectivity_amd/policy_weights.npz
+CLI:  python -m connectivity_amd.policy all --graphs 210 --reps 2 [--max-batch-graphs 1024] --out ...
       python -m connectivity_amd.policy train --data <collected.npz> --seed 2 --epochs 600 --out ...
       python -m connectivity_amd.policy combine --scc <a.npz> --cc <b.npz> --out ...  (a network per search kind)
 """
@@ -64,19 +64,21 @@ def init_blob(mean: np.ndarray, std: np.ndarray, seed: int = 0) -> np.ndarray:
 
 
 # ---- training graphs ---------------------------------------------------------------------
-def training_graphs(count: int, seed: int = 7):
+def training_graphs(count: int, seed: int = 7, max_batch_graphs: int = 8192):
     """`count` seeded specs: R-MAT (scale 14-20, edge factor 8-32), grid digraphs (side
     128-1024), planted-SCC graphs and packed batches (128-8192 R-MAT / planted graphs, whose
     merged searches meet states no single graph shows: most unvisited vertices lie in graphs
     whose search has ended; the largest make the bottom-up sweeps of the CC search pay where
-    the small ones do not), in a fixed interleaved order."""
+    the small ones do not), in a fixed interleaved order.  ``max_batch_graphs`` caps the
+    batches at a power of two (the shipped SCC network's data: 1024)."""
+    top = int(math.log2(max_batch_graphs)) + 1
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(count):
         kind = ("rmat", "rmat", "grid", "rmat", "grid", "planted", "batch")[i % 7]
         s = int(rng.integers(1, 1 << 30))
         if kind == "batch":
-            specs.append(("batch", {"graphs": int(2 ** rng.integers(7, 14)), "median_n": int(2 ** rng.integers(10, 13)),
+            specs.append(("batch", {"graphs": int(2 ** rng.integers(7, top)), "median_n": int(2 ** rng.integers(10, 13)),
                                     "seed": s}))
         elif kind == "rmat":
             specs.append(("rmat", {"scale": int(rng.integers(14, 21)), "edge_factor": int(rng.choice([8, 12, 16, 24, 32])),
@@ -273,6 +275,7 @@ def main(argv=None) -> int:
     ap.add_argument("--scc", help="combine: weights whose network decides the SCC searches")
     ap.add_argument("--cc", help="combine: weights whose network decides the CC search")
     ap.add_argument("--graphs", type=int, default=120)
+    ap.add_argument("--max-batch-graphs", type=int, default=8192, help="largest training batch (a power of two)")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--data", default="gpurun_out/policy_data.npz")
     ap.add_argument("--out", default=POLICY_WEIGHTS)
@@ -282,7 +285,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     ctx = Context(0) if args.cmd != "combine" else None  # combining two files needs no GPU
     if args.cmd in ("collect", "all"):
-        specs = training_graphs(args.graphs)
+        specs = training_graphs(args.graphs, max_batch_graphs=args.max_batch_graphs)
         X, Y, W, meta = collect(ctx, specs, reps=args.reps)
         os.makedirs(os.path.dirname(os.path.abspath(args.data)), exist_ok=True)
         np.savez(args.data, X=X, Y=Y, W=W)
