@@ -1630,6 +1630,9 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
                                                                                                            : 0) != 0;
       if (stop) {
         bool take = l < fn;
+        // the last hop's raises are performed before their worklist bits are set: a local
+        // expansion that clears a bit and then reads the colour must see the raise
+        if (OP == OP_PUSH) __builtin_amdgcn_s_waitcnt(0);
         if (OP == OP_PUSH && take) take = bit_set(inq, v);
         stage_push(S, take, v, len);
         break;
@@ -1656,8 +1659,14 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
         we = a.down_off[w + 1];
         const bool in = (word >> (w & 31u)) & 1u;
         if (OP == OP_PUSH) {
-          got = in && wc < jc && atomicMax(&a.colors[w], jc) < jc;
-          if (got) ++raised;
+          // the raise is not waited for: w is walked on with jc at the next hop whether or
+          // not a larger colour reached w first (then that walk is redundant, never wrong),
+          // which takes one of a hop's three dependent round trips off the chain
+          got = in && wc < jc;
+          if (got) {
+            atomicMax(&a.colors[w], jc);
+            ++raised;
+          }
         } else {
           got = in && wc == jc && bit_clear(a.valid, w);
           if (got) {
