@@ -303,9 +303,6 @@ void launch_andnot(const Launch& L, uint32_t* out, uint32_t* out2, const uint32_
 // (largest id; smallest with `canonical`), or declines (rc->tail = 2)
 void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, const int32_t* list, int32_t* labels,
                  bool canonical, const unsigned long long* commit_part, int commit_np, unsigned long long* commit_red);
-// canonical labels (minimum member per component); gate: only once *gate != 0 (null: always).
-// Without `apply` only the minima: scratch[l] = smallest v with labels[v] == l.
-void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply);
 // pivot: part col 3 = max key; results via launch_reduce then launch_pivot_first; with a
 // mask, among the vertices it holds only
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* out_off, const uint32_t* in_off,

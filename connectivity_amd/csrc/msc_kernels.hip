@@ -2938,38 +2938,6 @@ __global__ void __launch_bounds__(kTailBlock) k_tail_solve(RoundCtl* rc, Ctrl* p
   }
 }
 
-// Canonical labels: every component relabelled to its minimum vertex id.  mins[] starts
-// as the identity; a raw label is always a member of its component, so only vertices
-// below their label can lower it.  Lanes of a wave sharing a label combine into one
-// atomic (the pivot's SCC holds ~35% of all vertices at s24).
-__global__ void k_canon_min(int32_t n, const int32_t* __restrict__ labels, int32_t* __restrict__ mins,
-                            const uint32_t* gate) {
-  if (gate && !*gate) return;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t v = base + threadIdx.x;
-    int32_t l = v < n ? labels[v] : -1;
-    bool active = l >= 0 && (int64_t)l > v;
-    for (;;) {
-      unsigned long long m = __ballot(active);
-      if (m == 0ull) break;
-      int leader = __ffsll((long long)m) - 1;
-      int32_t key = __shfl(l, leader, 64);
-      bool mine = active && l == key;
-      uint32_t x = wave_min(mine ? (uint32_t)v : 0xffffffffu);
-      if (lane() == leader) atomicMin(&mins[key], (int32_t)x);
-      active = active && !mine;
-    }
-  }
-}
-__global__ void k_canon_apply(int32_t n, int32_t* __restrict__ labels, const int32_t* __restrict__ mins,
-                              const uint32_t* gate) {
-  if (gate && !*gate) return;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    int32_t l = labels[v];
-    if (l >= 0) labels[v] = mins[l];
-  }
-}
-
 // Pivot of create_csr (multistep/scc_main.cpp:169-183): largest (double)out*(double)in,
 // first vertex on ties.  Pass 1: per-workgroup max of the product's bit pattern
 // (non-negative doubles order like their bits), reduced by k_reduce; pass 2: smallest
@@ -3815,14 +3783,6 @@ void launch_tail(const Launch& L, bool cc, const LevelArgs& a, RoundCtl* rc, con
   else
     MSC_LAUNCH(L, k_tail_solve<false>, dim3(1), dim3(kTailBlock), 0, L.stream, rc, a.ctrl, list, ts.src, ts.dst,
                ts.slots, ts.cap, ts.maxv, labels, canonical ? 1 : 0, pull_min, commit_part, commit_np, commit_red);
-}
-
-void launch_canon(const Launch& L, int32_t n, int32_t* labels, int32_t* scratch, const uint32_t* gate, bool apply) {
-  if (n <= 0) return;
-  dim3 g(grid_for(n, kBlock, L.grid * 4));
-  MSC_LAUNCH(L, k_iota, g, dim3(kBlock), 0, L.stream, scratch, (int64_t)n, gate);
-  MSC_LAUNCH(L, k_canon_min, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
-  if (apply) MSC_LAUNCH(L, k_canon_apply, g, dim3(kBlock), 0, L.stream, n, labels, scratch, gate);
 }
 
 int launch_pivot_scc(const Launch& L, int32_t n, const uint32_t* oo, const uint32_t* io, unsigned long long* part,

@@ -814,9 +814,10 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
   // takes its smallest member, colour SCCs track theirs in cmin (CC finds them after its
   // fixpoint, over the colouring's vertex list); the batch paths relabel at the end instead
   const bool fast_canon = !bt && o->label_mode == MSC_LABELS_CANONICAL;
-  // batch SCC: each graph's pivot component takes its minimum in the commit and every colour
-  // SCC in its backward search (into cmin), applied by the final localising pass
-  const bool batch_canon = bt && !cc && o->label_mode == MSC_LABELS_CANONICAL;
+  // batches: each graph's pivot component takes its minimum in the commit and every colour
+  // SCC in its backward search (into cmin), every colour component of WCC after the fixpoint
+  // over the colouring's vertex list; the final localising pass applies cmin
+  const bool batch_canon = bt && o->label_mode == MSC_LABELS_CANONICAL;
   // control blocks: [0] forward (or CC) search, [1] backward search, [2] colour push,
   // [3] colour-BW; rcd: the colouring rounds' device state
   Ctrl* cs = c->ctrl;
@@ -1136,15 +1137,11 @@ int run_path(Kind kind, const msc_csr* g_in, int32_t pivot, double avg, int32_t*
     launch_round_begin(L, rcd, p.ctrl, r.ctrl, nullptr, nullptr, false, cc);
     {
       Timed t(c, MSC_K_OTHER);
-      if (fast_canon && cc) launch_cc_cmin(L, list_of(0), rcd, n, labels, c->cmin);
+      if ((fast_canon || batch_canon) && cc) launch_cc_cmin(L, list_of(0), rcd, n, labels, c->cmin);
       if (fast_canon) launch_canon_list(L, list_of(0), rcd, n, labels, c->cmin);
-      // batches: the minimum member of every label class, then one pass that applies it
-      // and turns union ids into each graph's own ids
-      const bool gen_canon = o->label_mode == MSC_LABELS_CANONICAL && !fast_canon && !batch_canon;
-      if (gen_canon) launch_canon(L, n, labels, c->colors, done_gate, !bt);
-      if (bt)
-        launch_localise(L, n, labels, gen_canon ? c->colors : batch_canon ? c->cmin : nullptr, c->bbase, ng,
-                        done_gate);
+      // batches: one pass that applies the tracked minima (canonical labels) and turns union
+      // ids into each graph's own ids
+      if (bt) launch_localise(L, n, labels, batch_canon ? c->cmin : nullptr, c->bbase, ng, done_gate);
     }
     HIP_TRY(hipGetLastError());
     return MSC_OK;
