@@ -89,8 +89,9 @@ def main() -> int:
                     help="run one device-to-device copy of this size first (PMC traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only (every box here has one); how the path "
-                                   "would shard is a design note in DESIGN.md", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only: the decomposition runs on a single device (a batch's graphs "
+                                   "would be dealt to devices whole; one graph would need cross-device frontiers, "
+                                   "which are not built)", "n_gpus": args.gpus}))
         return 2
 
     import numpy as np

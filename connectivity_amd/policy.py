@@ -277,7 +277,7 @@ def main(argv=None) -> int:
     ap.add_argument("--graphs", type=int, default=120)
     ap.add_argument("--max-batch-graphs", type=int, default=8192, help="largest training batch (a power of two)")
     ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--data", default="gpurun_out/policy_data.npz")
+    ap.add_argument("--data", default="policy_data.npz")
     ap.add_argument("--out", default=POLICY_WEIGHTS)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--seed", type=int, default=1, help="initialisation and minibatch order of the training")

@@ -119,7 +119,8 @@ enum msc_kernel {
 typedef struct msc_kstat {
   int64_t launches;
   double ms;    /* summed HIP-event durations */
-  double bytes; /* summed algorithmic bytes (unit costs in DESIGN.md) */
+  double bytes; /* summed algorithmic bytes: each kernel class books the bytes its work units
+                   must read and write (offsets, list entries, probes, claims), not cache lines */
 } msc_kstat;
 
 typedef struct msc_stats {

@@ -9,7 +9,7 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (run on the MI355X box)")
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (an MI355X)")
     config.addinivalue_line("markers", "slow: large inputs (seconds of CPU work)")
 
 
@@ -31,7 +31,7 @@ def orc():
 @pytest.fixture(scope="session")
 def gpu_ctx(msc):
     if msc.device_count() < 1:
-        pytest.fail("no gfx950 device visible: GPU tests must run on the MI355X box")
+        pytest.fail("no gfx950 device visible: GPU tests need an MI355X")
     ctx = msc.Context(0)
     yield ctx
     ctx.close()
