@@ -1615,6 +1615,9 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
     const uint32_t l = (uint32_t)lane();
     uint32_t v = 0, beg = 0, len = 0;
     int32_t col = 0;
+    // colour-BW: the valid word a frontier vertex's claim returned (its bit set: the claim
+    // won); the staged vertices were claimed already
+    uint32_t cold = 0xffffffffu;
     if (l < n) {
       v = (uint32_t)S.v[l];
       if (OP == OP_PUSH) {
@@ -1633,7 +1636,8 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
       const uint32_t incl = wave_incl_sum(len);
       const uint32_t total = __shfl(incl, 63, 64);
       if (total == 0u) {
-        if (l == 0) loc_n += fn;
+        const uint32_t won = OP == OP_CBW ? (uint32_t)__popcll(__ballot(l < fn && ((cold >> (v & 31u)) & 1u))) : fn;
+        if (l == 0) loc_n += won;
         break;
       }
       const bool stop = __builtin_amdgcn_readfirstlane(
@@ -1645,13 +1649,12 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
         // expansion that clears a bit and then reads the colour must see the raise
         if (OP == OP_PUSH) __builtin_amdgcn_s_waitcnt(0);
         if (OP == OP_PUSH && take) take = bit_set(inq, v);
+        if (OP == OP_CBW) take = take && ((cold >> (v & 31u)) & 1u);  // only the claims it won
         stage_push(S, take, v, len);
         break;
       }
-      if (l == 0) {
-        loc_n += fn;
-        loc_e += total;
-      }
+      if (l == 0) loc_e += total;
+      if (OP == OP_PUSH && l == 0) loc_n += fn;
       // edge l of this hop belongs to the first lane whose inclusive prefix exceeds l
       uint32_t j = 0;
       for (uint32_t step = 32; step >= 1; step >>= 1)
@@ -1659,10 +1662,16 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
       const uint32_t jb = __shfl(beg, (int)j, 64);
       const uint32_t jx = __shfl(incl - len, (int)j, 64);
       const int32_t jc = __shfl(col, (int)j, 64);
-      uint32_t w = 0, wb = 0, we = 0;
+      uint32_t w = 0, wb = 0, we = 0, ow = 0u;
       bool got = false;
+      if (l < total) w = (uint32_t)a.down_adj[jb + (l - jx)];
+      if (OP == OP_CBW) {
+        // the claims of the last hop, issued then and read only now (while this hop's
+        // adjacency load is in flight): a claimed vertex counts once, for the walk that won it
+        const unsigned long long won = __ballot(l < fn && ((cold >> (v & 31u)) & 1u));
+        if (l == 0) loc_n += (uint32_t)__popcll(won);
+      }
       if (l < total) {
-        w = (uint32_t)a.down_adj[jb + (l - jx)];
         const uint32_t word = a.valid[w >> 5];
         const int32_t wc = a.colors[w];
         // the target's list bounds with its probes: the next hop needs no load to start
@@ -1679,8 +1688,12 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
             ++raised;
           }
         } else {
-          got = in && wc == jc && bit_clear(a.valid, w);
+          // the claim is not waited for either: w is walked on at once, and if another walk
+          // of the same colour won w, expanding it again claims nothing more (each claim is
+          // an atomic); the label and the colour's minimum are the same whoever writes them
+          got = in && wc == jc;
           if (got) {
+            ow = atomicAnd(&a.valid[w >> 5], ~(1u << (w & 31u)));
             a.labels[w] = jc;
             if (a.cmin && (int32_t)w < a.cmin[jc]) atomicMin(&a.cmin[jc], (int32_t)w);
           }
@@ -1698,15 +1711,18 @@ __device__ void wave_walk(const LevelArgs& a, uint32_t* inq, Stage& S, uint32_t 
       const int32_t nc = __shfl(jc, (int)src, 64);
       const uint32_t nb = __shfl(wb, (int)src, 64);
       const uint32_t ne = __shfl(we, (int)src, 64);
+      const uint32_t no = OP == OP_CBW ? __shfl(ow, (int)src, 64) : 0u;
       v = 0;
       beg = 0;
       len = 0;
       col = 0;
+      cold = 0u;
       if (l < fn) {
         v = nv;
         col = nc;
         beg = nb;
         len = ne - nb;
+        cold = no;
       }
     }
   }
