@@ -2645,8 +2645,15 @@ __global__ void __launch_bounds__(kBlock) k_roots_list(LevelArgs a, const int32_
       const uint32_t i = i0 + (uint32_t)r * kBlock + threadIdx.x;
       col[r] = (r < (int)rr && i < nl) ? a.colors[v[r]] : -1;
     }
+    // every list entry is still valid here (the seed listed valid vertices, and nothing but
+    // the roots and the colour-BW search after them removes any), so a root's removal needs
+    // no result: the atomic is not waited for
 #pragma unroll
-    for (int r = 0; r < kListR; ++r) take[r] = col[r] == (int32_t)v[r] && bit_clear(a.valid, v[r]);
+    for (int r = 0; r < kListR; ++r) {
+      take[r] = col[r] == (int32_t)v[r];
+      if (take[r]) __hip_atomic_fetch_and(&a.valid[v[r] >> 5], ~(1u << (v[r] & 31u)), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+    }
 #pragma unroll
     for (int r = 0; r < kListR; ++r) {
       b[r] = e[r] = 0u;
